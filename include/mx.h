@@ -1,0 +1,217 @@
+/* C ABI of the MI355X perturbation path (libmxhip.so).
+ *
+ * The reference engine has no C ABI for this path (its only C-ABI header,
+ * src/engine/dll_interface.h:7-11, is declared but never defined); its boundary is
+ * the C++ backend plugin IComputeBackend (src/engine/compute_backend.h:102-115) over
+ * the tiled frame API Mandel::Compute (src/engine/mandel_perturbation.h:72-75).
+ * These entry points are what that plugin binds.  Plain pointers and sizes only.
+ *
+ * Ownership: host buffers are the caller's; device buffers belong to the library.
+ * One context per host thread; a context runs one call at a time.
+ * Every int-returning call returns MX_OK or a negative MX_E_* code, and the message
+ * for the last failure on a context is available from mx_last_error().
+ */
+#ifndef MX_H
+#define MX_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    MX_OK = 0,
+    MX_E_INVALID = -1,     /* request fails validation (compute_backend.cpp:263-272) */
+    MX_E_NODEVICE = -2,    /* no HIP device */
+    MX_E_HIP = -3,         /* a HIP runtime call failed */
+    MX_E_CANCELLED = -4,   /* mx_cancel() was called (compute_backend.cpp:116) */
+    MX_E_BUSY = -5,        /* the context is already computing (compute_backend.cpp:48) */
+    MX_E_NOMEM = -6,
+    MX_E_UNSUPPORTED = -7  /* a request outside this path (coloring, Julia, ...) */
+};
+
+/* Kernel forms. */
+enum {
+    MX_FORM_FE64 = 0,  /* plain: IEEE FP64 + FP64-mantissa floatexp, engine op order */
+    MX_FORM_FE32 = 1,  /* mixed: FP32-mantissa floatexp delta loop, FP64 decisions */
+    MX_FORM_DOUBLE = 2 /* plain double perturbation (scale 1e6..1e280) */
+};
+
+/* ColoringMethod bits accepted here (mandel_perturbation.h:30-49). */
+enum { MX_SUPER_SAMPLING = 1, MX_EXTERIOR_DIST_EST = 2 };
+
+#define MX_EMPTYPIXEL (-10.0f) /* mandel_perturbation.h:23 */
+
+typedef struct mx_ctx mx_ctx;
+typedef struct mx_ref mx_ref;
+typedef struct mx_plan mx_plan;
+typedef struct mx_frame mx_frame;
+
+/* ---- context ------------------------------------------------------------ */
+
+/* Library ABI version. */
+int mx_version(void);
+/* Number of visible HIP devices (0 when there is none). */
+int mx_device_count(void);
+/* Open device `device`; replaces createComputeBackend("hip") device set-up
+ * (the D3D11 precedent: compute_backend_d3d11.cpp:213-279). */
+int mx_ctx_create(int device, mx_ctx** out);
+void mx_ctx_destroy(mx_ctx* ctx);
+const char* mx_last_error(const mx_ctx* ctx);
+/* IComputeBackend::cancel / resetCancellation (compute_backend.cpp:119-128).
+ * cancel() may be called from any thread; running kernels stop taking samples. */
+void mx_cancel(mx_ctx* ctx);
+void mx_reset_cancellation(mx_ctx* ctx);
+
+/* ---- reference upload (once per reference orbit) -------------------------- */
+
+/* Host arrays describing one reference: the orbit shadows filled by createRef /
+ * calCoefficient (mandel_perturbation.cpp:4113-4249) and the BLA table of buildBLA
+ * (2613-2709).  zf[2k], zf[2k+1] = X_{k+1}.  BLA level p has lev_size[p] entries
+ * starting at lev_off[p], each skipping 2^p iterations. */
+typedef struct {
+    int32_t len;             /* entries = mx_ref_it + 1 */
+    const double* zf;        /* [2*len] */
+    const double* zfe_m;     /* [2*len] FloatExp mantissas (re, im) */
+    const int32_t* zfe_e;    /* [2*len] FloatExp exponents (re, im) */
+    int32_t nlev;            /* 0: no BLA table */
+    const int32_t* lev_off;  /* [nlev] */
+    const int32_t* lev_size; /* [nlev] */
+    const double* bla_r2;    /* [total] validity radius^2 */
+    const double* bla_coef;  /* [4*total] ar, ai, br, bi */
+    double rmax2;            /* largest radius^2 */
+} mx_ref_desc;
+
+int mx_ref_upload(mx_ctx* ctx, const mx_ref_desc* desc, mx_ref** out);
+void mx_ref_release(mx_ref* ref);
+
+/* ---- one tile of samples against one reference --------------------------- */
+
+/* The per-pixel inputs of stepParallel's floatexp path (mandel_perturbation.cpp:
+ * 2150-2168): dc is separable, Dcr = dcol[column], Dci = drow[row]. */
+typedef struct {
+    int32_t width, height;   /* sample grid; outputs are width*height */
+    int32_t stride;          /* 1: all samples; k: only rows/cols multiple of k (probe) */
+    int32_t max_iter;
+    double esc2;             /* escape radius squared (Mandel::escapeRadiusSquared) */
+    int32_t form;            /* MX_FORM_* */
+    int32_t ede;             /* exterior distance estimate output */
+    int32_t interior;        /* periodicity interior detection (_use_interior) */
+    int32_t int_rep;         /* exact state-repeat interior detection */
+    int32_t bla;             /* apply BLA skips */
+    int32_t bla_minlevel;    /* only skips of 2^minlevel or more */
+    double interior_eps2;
+    int32_t interior_confirm;
+    const double* dcol_m;    /* [width] */
+    const int32_t* dcol_e;
+    const double* drow_m;    /* [height] */
+    const int32_t* drow_e;
+    double dx_m;             /* pixel spacing (EDE normalisation) */
+    int32_t dx_e;
+} mx_tile;
+
+/* A plan holds the device copy of a tile's inputs and outputs so that launches
+ * can be repeated (and timed) with everything resident in HBM. */
+int mx_plan_create(mx_ctx* ctx, const mx_ref* ref, const mx_tile* tile, mx_plan** out);
+int mx_plan_launch(mx_plan* plan);                      /* asynchronous */
+int mx_plan_wait(mx_plan* plan, double* kernel_ms);     /* synchronise; kernel time of the last launch */
+/* Copy results to host buffers (any may be NULL).  n_esc: escape iteration, -1 =
+ * reached max_iter, -2 = interior detected.  value: the double the engine casts to
+ * float.  out: the engine's float buffer value (-2 interior).  stats4: executed
+ * steps, BLA applies, BLA-skipped iterations, rebases. */
+int mx_plan_read(mx_plan* plan, int32_t* n_esc, double* value, float* out, uint64_t* stats4);
+void mx_plan_destroy(mx_plan* plan);
+/* create + launch + wait + read + destroy */
+int mx_tile_render(mx_ctx* ctx, const mx_ref* ref, const mx_tile* tile, int32_t* n_esc, double* value, float* out, uint64_t* stats4);
+
+/* ---- whole frame: Mandel::Compute (mandel_perturbation.cpp:1388-1788) ----- */
+
+typedef struct {
+    const char* center_re;   /* decimal strings (ComputeRequest::centerRe/centerIm/scale) */
+    const char* center_im;
+    const char* scale;
+    int32_t width, height;   /* ComputeRequest::width/height */
+    int32_t full_height;     /* strip export: 0 = the whole image (mandel_perturbation.cpp:1399-1424) */
+    int32_t row_base;
+    int32_t sub;             /* must be 1 on this path */
+    int32_t max_iterations;
+    int32_t coloring_method; /* 0 or MX_EXTERIOR_DIST_EST */
+    float escape_radius;     /* 0 => 1e8f (MANDEL_BAILOUT) */
+    int32_t precision_bits;  /* 0 => digits(scale)*log2(10)+64 (render.cpp:170) */
+    int32_t form;            /* MX_FORM_FE64 or MX_FORM_FE32; floatexp path only */
+    int32_t use_bla;         /* -1 default (on) */
+    int32_t bla_minskip;     /* 0 => 8 (MANDEL_BLA_MINSKIP) */
+    int32_t interior;        /* -1 default (on, gated by the probe) */
+    float* iterations;       /* [width*height] engine float buffer, caller-owned */
+    double* values;          /* optional [width*height] */
+    int32_t* n_esc;          /* optional [width*height] */
+} mx_request;
+
+typedef struct {
+    int32_t path;            /* 2 = floatexp rescaled path */
+    int32_t ref_it;          /* reference length - 1 */
+    int32_t ref_escaped;
+    int32_t ref_virtual;     /* exact view-centre reference */
+    double ref_x, ref_y;     /* reference position in pixels */
+    int32_t interior_used;   /* interior detection after the probe gate */
+    int32_t cutoff_sensitive;/* reference escapes within 16 of mxit (flag only) */
+    int32_t precision_bits;
+    int32_t bla_levels;
+    double ref_seconds;      /* host GMP orbit */
+    double bla_seconds;      /* host BLA build */
+    double upload_seconds;
+    double probe_ms;         /* probe kernel */
+    double kernel_ms;        /* main pass kernel */
+    double total_seconds;
+    uint64_t steps, bla_applies, bla_skipped, rebases;
+    uint64_t nominal_iterations; /* sum of n_esc, max_iterations for interior */
+} mx_frame_info;
+
+int mx_compute(mx_ctx* ctx, const mx_request* req, mx_frame_info* info);
+
+/* ---- host-only frame preparation (no GPU needed) --------------------------- */
+
+/* Everything Mandel::Compute prepares on the host before the delta loop: grid,
+ * reference orbit (GMP), BLA table and dc tables.  View pointers stay valid until
+ * mx_frame_free. */
+typedef struct {
+    int32_t len;
+    const double* zf;
+    const double* zfe_m;
+    const int32_t* zfe_e;
+    int32_t nlev;
+    const int32_t* lev_off;
+    const int32_t* lev_size;
+    int64_t bla_total;
+    const double* bla_r2;
+    const double* bla_coef;
+    double rmax2;
+    double dcmax;
+    const double* dcol_m;
+    const int32_t* dcol_e;
+    const double* drow_m;
+    const int32_t* drow_e;
+    double dx_m;
+    int32_t dx_e;
+    double dy_m;
+    int32_t dy_e;
+    double esc2;
+    int32_t bla_minlevel;
+    int32_t ref_it, ref_escaped, ref_virtual;
+    double ref_x, ref_y;
+    int32_t ref_pixel_n;     /* n_esc of the reference pixel (non-virtual references) */
+    double ref_pixel_value;  /* its value (getEscapeTime / accuratePointCompute) */
+    int32_t cutoff_sensitive;
+    int32_t precision_bits;
+    double ref_seconds, bla_seconds;
+} mx_frame_view;
+
+int mx_frame_prepare(const mx_request* req, mx_frame** out, mx_frame_view* view);
+void mx_frame_free(mx_frame* frame);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,67 @@
+// Device-side layout shared by the host launcher and the HIP kernels.
+//
+// HBM layout of one reference (uploaded once per reference, read by every pixel):
+//   zf        double2[reflen]      X_{k+1} double shadow, k = 0..mx_ref_it (16 B/entry)
+//   zfe_m     double2[reflen]      FloatExp mantissas of the same orbit (rebase, underflow)
+//   zfe_e     int2[reflen]         FloatExp exponents
+//   zf32      float2[reflen]       FP32 shadow (mixed form only)
+//   bla_r2    double[total]        BLA validity radius^2, all levels concatenated
+//   bla_coef  double4[total]       (ar, ai, br, bi) per entry
+//   level p holds lev_size[p] entries starting at lev_off[p]; entry i of level p
+//   starts at reference index 1 + i*2^p and skips 2^p iterations.
+// One tile (frame or sub-frame) adds separable dc tables (W + H FloatExp values) and
+// the per-sample outputs (n_esc int32, value double, compat float).
+#pragma once
+
+#include "mx_fe.h"
+
+#define MX_MAXLEV 32
+#define MX_NQUEUE 8 // one work queue per XCD
+
+enum MxFlags {
+    MX_F_EDE = 1,        // exterior distance estimate output (carry dz/dc)
+    MX_F_INTERIOR = 2,   // periodicity-based interior detection
+    MX_F_INT_REP = 4,    // exact state-repeat interior detection
+    MX_F_BLA = 8,        // BLA skipping
+};
+
+struct MxRefDev {
+    const double2* zf;
+    const double2* zfe_m;
+    const int2* zfe_e;
+    const float2* zf32;
+    const double* bla_r2;
+    const double4* bla_coef;
+    const float* bla_r2f;   // mixed form: radius^2 in float (rounded down)
+    const float4* bla_coeff; // mixed form: coefficients in float
+    int mx_ref_it;
+    int nlev;
+    int minlevel;
+    int pad0;
+    double rmax2;
+    int lev_off[MX_MAXLEV];
+    int lev_size[MX_MAXLEV];
+};
+
+struct MxTileDev {
+    const Fe* dcol; // per column dc (real part)
+    const Fe* drow; // per row dc (imag part)
+    int W, H;       // sample grid of the tile
+    int stride;     // 1: every sample; 8: the interior-gate probe grid
+    int nx, ny;     // samples in this launch (after stride)
+    int tiles_x;    // 8x8 lane tiles per launch row
+    int ntiles;
+    int mxit;
+    int flags;
+    double esc2;
+    double interior_eps2;
+    int interior_confirm;
+    int pad1;
+    Fe dxfe;
+    double* val;         // W*H
+    int* nesc;           // W*H
+    float* out;          // W*H (reference-compatible float buffer), may be null
+    unsigned* queue;     // MX_NQUEUE counters, zeroed before launch
+    unsigned long long* stats; // steps, bla applies, bla skipped iterations, rebases
+    const volatile int* cancel; // host-mapped; nonzero => lanes stop taking work
+};

@@ -1,0 +1,483 @@
+// Device side of libmxhip.so: contexts, reference upload, tile plans and the frame
+// driver that mirrors Mandel::Compute's pass structure
+// (src/engine/mandel_perturbation.cpp:1656-1761): interior-gate probe on the
+// 8x8-strided grid, then the main pass over every sample, then the reference
+// pixel's own value.
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <climits>
+#include <cmath>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "mx_internal.h"
+
+namespace {
+
+double now_s() { return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count(); }
+
+#define MX_HIP(call)                                                                 \
+    do {                                                                             \
+        hipError_t e_ = (call);                                                      \
+        if (e_ != hipSuccess) {                                                      \
+            if (ctx) ctx->set_err(std::string(#call) + ": " + hipGetErrorString(e_)); \
+            return MX_E_HIP;                                                         \
+        }                                                                            \
+    } while (0)
+
+size_t align_up(size_t x) { return (x + 255) & ~(size_t)255; }
+
+__global__ void mx_reset_kernel(unsigned* queue, unsigned long long* stats, double* val, int* nesc, float* out, size_t n) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < MX_NQUEUE) queue[i] = 0u;
+    if (i < 4) stats[i] = 0ull;
+    for (size_t k = i; k < n; k += (size_t)gridDim.x * blockDim.x) {
+        val[k] = -10.0;
+        nesc[k] = INT_MIN;
+        if (out) out[k] = -10.0f;
+    }
+}
+
+struct BusyGuard {
+    mx_ctx* ctx;
+    bool ok;
+    explicit BusyGuard(mx_ctx* c) : ctx(c) {
+        bool expected = false;
+        ok = ctx->busy.compare_exchange_strong(expected, true);
+    }
+    ~BusyGuard() {
+        if (ok) ctx->busy.store(false);
+    }
+};
+
+} // namespace
+
+extern "C" int mx_version(void) { return 1; }
+
+extern "C" int mx_device_count(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+extern "C" int mx_ctx_create(int device, mx_ctx** out) {
+    if (!out) return MX_E_INVALID;
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return MX_E_NODEVICE;
+    if (device < 0 || device >= n) return MX_E_INVALID;
+    auto* ctx = new (std::nothrow) mx_ctx();
+    if (!ctx) return MX_E_NOMEM;
+    ctx->device = device;
+    auto fail = [&](int rc) {
+        mx_ctx_destroy(ctx);
+        return rc;
+    };
+    if (hipSetDevice(device) != hipSuccess) return fail(MX_E_HIP);
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) != hipSuccess) return fail(MX_E_HIP);
+    ctx->cu_count = prop.multiProcessorCount;
+    if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) return fail(MX_E_HIP);
+    if (hipHostMalloc((void**)&ctx->cancel_host, sizeof(int), hipHostMallocMapped) != hipSuccess) return fail(MX_E_HIP);
+    *ctx->cancel_host = 0;
+    if (hipHostGetDevicePointer((void**)&ctx->cancel_dev, ctx->cancel_host, 0) != hipSuccess) return fail(MX_E_HIP);
+    *out = ctx;
+    return MX_OK;
+}
+
+extern "C" void mx_ctx_destroy(mx_ctx* ctx) {
+    if (!ctx) return;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->stream) {
+        (void)hipStreamSynchronize(ctx->stream);
+        (void)hipStreamDestroy(ctx->stream);
+    }
+    if (ctx->cancel_host) (void)hipHostFree(ctx->cancel_host);
+    delete ctx;
+}
+
+extern "C" const char* mx_last_error(const mx_ctx* ctx) { return ctx ? ctx->err.c_str() : "no context"; }
+
+extern "C" void mx_cancel(mx_ctx* ctx) {
+    if (!ctx || !ctx->cancel_host) return;
+    __atomic_store_n(ctx->cancel_host, 1, __ATOMIC_RELEASE);
+}
+
+extern "C" void mx_reset_cancellation(mx_ctx* ctx) {
+    if (!ctx || !ctx->cancel_host) return;
+    if (!ctx->busy.load()) __atomic_store_n(ctx->cancel_host, 0, __ATOMIC_RELEASE);
+}
+
+extern "C" int mx_ref_upload(mx_ctx* ctx, const mx_ref_desc* d, mx_ref** out) {
+    if (!ctx || !d || !out || d->len < 1 || !d->zf || !d->zfe_m || !d->zfe_e || d->nlev < 0 || d->nlev > MX_MAXLEV) return MX_E_INVALID;
+    *out = nullptr;
+    int64_t total = 0;
+    for (int p = 0; p < d->nlev; ++p) {
+        if (!d->lev_off || !d->lev_size || d->lev_off[p] != total || d->lev_size[p] < 0) return MX_E_INVALID;
+        total += d->lev_size[p];
+    }
+    if (d->nlev > 0 && (!d->bla_r2 || !d->bla_coef)) return MX_E_INVALID;
+    if (total > INT_MAX / 4 || (int64_t)d->len > INT_MAX / 2) return MX_E_INVALID;
+    MX_HIP(hipSetDevice(ctx->device));
+    const size_t len = (size_t)d->len;
+    size_t o_zf = 0;
+    size_t o_zfm = align_up(o_zf + 16 * len);
+    size_t o_zfe = align_up(o_zfm + 16 * len);
+    size_t o_zf32 = align_up(o_zfe + 8 * len);
+    size_t o_r2 = align_up(o_zf32 + 8 * len);
+    size_t o_coef = align_up(o_r2 + 8 * (size_t)total);
+    size_t o_lev = align_up(o_coef + 32 * (size_t)total);
+    size_t bytes = align_up(o_lev + 8 * MX_MAXLEV);
+    auto* ref = new (std::nothrow) mx_ref();
+    if (!ref) return MX_E_NOMEM;
+    ref->ctx = ctx;
+    ref->bytes = bytes;
+    if (hipMalloc(&ref->block, bytes) != hipSuccess) {
+        delete ref;
+        ctx->set_err("hipMalloc of the reference failed");
+        return MX_E_NOMEM;
+    }
+    char* b = (char*)ref->block;
+    std::vector<float> zf32(2 * len);
+    for (size_t k = 0; k < 2 * len; ++k) zf32[k] = (float)d->zf[k];
+    std::vector<int2> lev(MX_MAXLEV, make_int2(0, 0));
+    for (int p = 0; p < d->nlev; ++p) lev[p] = make_int2(d->lev_off[p], d->lev_size[p]);
+    hipError_t e = hipSuccess;
+    auto cp = [&](size_t off, const void* src, size_t n) {
+        if (e == hipSuccess && n) e = hipMemcpyAsync(b + off, src, n, hipMemcpyHostToDevice, ctx->stream);
+    };
+    cp(o_zf, d->zf, 16 * len);
+    cp(o_zfm, d->zfe_m, 16 * len);
+    cp(o_zfe, d->zfe_e, 8 * len);
+    cp(o_zf32, zf32.data(), 8 * len);
+    if (d->nlev > 0) {
+        cp(o_r2, d->bla_r2, 8 * (size_t)total);
+        cp(o_coef, d->bla_coef, 32 * (size_t)total);
+    }
+    cp(o_lev, lev.data(), 8 * MX_MAXLEV);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) {
+        ctx->set_err(std::string("reference upload: ") + hipGetErrorString(e));
+        (void)hipFree(ref->block);
+        delete ref;
+        return MX_E_HIP;
+    }
+    MxRefDev& R = ref->dev;
+    std::memset(&R, 0, sizeof R);
+    R.zf = (const double2*)(b + o_zf);
+    R.zfe_m = (const double2*)(b + o_zfm);
+    R.zfe_e = (const int2*)(b + o_zfe);
+    R.zf32 = (const float2*)(b + o_zf32);
+    R.bla_r2 = (const double*)(b + o_r2);
+    R.bla_coef = (const double4*)(b + o_coef);
+    R.mx_ref_it = d->len - 1;
+    R.nlev = d->nlev;
+    R.rmax2 = d->rmax2;
+    for (int p = 0; p < d->nlev; ++p) {
+        R.lev_off[p] = d->lev_off[p];
+        R.lev_size[p] = d->lev_size[p];
+    }
+    ref->lev = (int2*)(b + o_lev);
+    *out = ref;
+    return MX_OK;
+}
+
+extern "C" void mx_ref_release(mx_ref* ref) {
+    if (!ref) return;
+    if (ref->block) {
+        (void)hipSetDevice(ref->ctx->device);
+        (void)hipStreamSynchronize(ref->ctx->stream);
+        (void)hipFree(ref->block);
+    }
+    delete ref;
+}
+
+extern "C" int mx_plan_create(mx_ctx* ctx, const mx_ref* ref, const mx_tile* t, mx_plan** out) {
+    if (!ctx || !ref || !t || !out || t->width < 1 || t->height < 1 || t->stride < 1 || t->max_iter < 1) return MX_E_INVALID;
+    if (!t->dcol_m || !t->dcol_e || !t->drow_m || !t->drow_e) return MX_E_INVALID;
+    if (t->form != MX_FORM_FE64) {
+        ctx->set_err("this kernel form is not built");
+        return MX_E_UNSUPPORTED;
+    }
+    if (t->bla && (ref->dev.nlev == 0 || t->bla_minlevel < 0)) return MX_E_INVALID;
+    if ((int64_t)t->width * t->height > (int64_t)INT_MAX / 2) return MX_E_INVALID;
+    *out = nullptr;
+    MX_HIP(hipSetDevice(ctx->device));
+    auto* plan = new (std::nothrow) mx_plan();
+    if (!plan) return MX_E_NOMEM;
+    plan->ctx = ctx;
+    plan->ref = ref;
+    plan->form = t->form;
+    const size_t W = (size_t)t->width, H = (size_t)t->height, n = W * H;
+    plan->n = n;
+    size_t o_col = 0;
+    size_t o_row = align_up(o_col + sizeof(Fe) * W);
+    size_t o_val = align_up(o_row + sizeof(Fe) * H);
+    size_t o_nesc = align_up(o_val + 8 * n);
+    size_t o_out = align_up(o_nesc + 4 * n);
+    size_t o_q = align_up(o_out + 4 * n);
+    size_t o_st = align_up(o_q + 4 * MX_NQUEUE);
+    size_t bytes = align_up(o_st + 8 * 4);
+    if (hipMalloc(&plan->block, bytes) != hipSuccess) {
+        delete plan;
+        ctx->set_err("hipMalloc of the tile failed");
+        return MX_E_NOMEM;
+    }
+    plan->bytes = bytes;
+    char* b = (char*)plan->block;
+    std::vector<Fe> col(W), row(H);
+    for (size_t j = 0; j < W; ++j) col[j] = fe_make(t->dcol_m[j], t->dcol_e[j]);
+    for (size_t i = 0; i < H; ++i) row[i] = fe_make(t->drow_m[i], t->drow_e[i]);
+    hipError_t e = hipMemcpyAsync(b + o_col, col.data(), sizeof(Fe) * W, hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(b + o_row, row.data(), sizeof(Fe) * H, hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e == hipSuccess) e = hipEventCreate(&plan->ev0);
+    if (e == hipSuccess) e = hipEventCreate(&plan->ev1);
+    if (e != hipSuccess) {
+        ctx->set_err(std::string("tile set-up: ") + hipGetErrorString(e));
+        mx_plan_destroy(plan);
+        return MX_E_HIP;
+    }
+    MxTileDev& T = plan->tile;
+    std::memset(&T, 0, sizeof T);
+    T.dcol = (const Fe*)(b + o_col);
+    T.drow = (const Fe*)(b + o_row);
+    T.W = t->width;
+    T.H = t->height;
+    T.stride = t->stride;
+    T.nx = (t->width + t->stride - 1) / t->stride;
+    T.ny = (t->height + t->stride - 1) / t->stride;
+    T.tiles_x = (T.nx + 7) / 8;
+    T.ntiles = T.tiles_x * ((T.ny + 7) / 8);
+    T.mxit = t->max_iter;
+    T.flags = (t->ede ? MX_F_EDE : 0) | (t->interior ? MX_F_INTERIOR : 0) | (t->int_rep ? MX_F_INT_REP : 0) | (t->bla ? MX_F_BLA : 0);
+    T.esc2 = t->esc2;
+    T.interior_eps2 = t->interior_eps2;
+    T.interior_confirm = t->interior_confirm;
+    T.dxfe = fe_make(t->dx_m, t->dx_e);
+    T.val = (double*)(b + o_val);
+    T.nesc = (int*)(b + o_nesc);
+    T.out = (float*)(b + o_out);
+    T.queue = (unsigned*)(b + o_q);
+    T.stats = (unsigned long long*)(b + o_st);
+    T.cancel = ctx->cancel_dev;
+    plan->threads = 256;
+    plan->minlevel = t->bla_minlevel;
+    // persistent grid: 8 waves per SIMD (32 per CU); never more workgroups than
+    // there are 8x8 tiles to hand out
+    const int waves = ctx->cu_count * 32;
+    plan->blocks = std::max(1, std::min(waves / 4, T.ntiles));
+    *out = plan;
+    return MX_OK;
+}
+
+extern "C" int mx_plan_launch(mx_plan* plan) {
+    if (!plan) return MX_E_INVALID;
+    mx_ctx* ctx = plan->ctx;
+    MX_HIP(hipSetDevice(ctx->device));
+    MxTileDev& T = plan->tile;
+    MX_HIP(hipEventRecord(plan->ev0, ctx->stream));
+    hipLaunchKernelGGL(mx_reset_kernel, dim3(1024), dim3(256), 0, ctx->stream, T.queue, T.stats, T.val, T.nesc, T.out, plan->n);
+    MX_HIP(hipGetLastError());
+    MxRefDev R = plan->ref->dev;
+    R.minlevel = plan->minlevel;
+    MX_HIP(mx_launch_fe64(R, plan->ref->lev, T, plan->blocks, plan->threads, ctx->stream));
+    MX_HIP(hipEventRecord(plan->ev1, ctx->stream));
+    return MX_OK;
+}
+
+extern "C" int mx_plan_wait(mx_plan* plan, double* kernel_ms) {
+    if (!plan) return MX_E_INVALID;
+    mx_ctx* ctx = plan->ctx;
+    MX_HIP(hipStreamSynchronize(ctx->stream));
+    if (kernel_ms) {
+        float ms = 0.f;
+        MX_HIP(hipEventElapsedTime(&ms, plan->ev0, plan->ev1));
+        *kernel_ms = ms;
+    }
+    if (__atomic_load_n(ctx->cancel_host, __ATOMIC_ACQUIRE)) {
+        ctx->set_err("cancelled");
+        return MX_E_CANCELLED;
+    }
+    return MX_OK;
+}
+
+extern "C" int mx_plan_read(mx_plan* plan, int32_t* n_esc, double* value, float* out, uint64_t* stats4) {
+    if (!plan) return MX_E_INVALID;
+    mx_ctx* ctx = plan->ctx;
+    MxTileDev& T = plan->tile;
+    if (n_esc) MX_HIP(hipMemcpyAsync(n_esc, T.nesc, 4 * plan->n, hipMemcpyDeviceToHost, ctx->stream));
+    if (value) MX_HIP(hipMemcpyAsync(value, T.val, 8 * plan->n, hipMemcpyDeviceToHost, ctx->stream));
+    if (out) MX_HIP(hipMemcpyAsync(out, T.out, 4 * plan->n, hipMemcpyDeviceToHost, ctx->stream));
+    if (stats4) MX_HIP(hipMemcpyAsync(stats4, T.stats, 32, hipMemcpyDeviceToHost, ctx->stream));
+    MX_HIP(hipStreamSynchronize(ctx->stream));
+    return MX_OK;
+}
+
+extern "C" void mx_plan_destroy(mx_plan* plan) {
+    if (!plan) return;
+    (void)hipSetDevice(plan->ctx->device);
+    (void)hipStreamSynchronize(plan->ctx->stream);
+    if (plan->ev0) (void)hipEventDestroy(plan->ev0);
+    if (plan->ev1) (void)hipEventDestroy(plan->ev1);
+    if (plan->block) (void)hipFree(plan->block);
+    delete plan;
+}
+
+extern "C" int mx_tile_render(mx_ctx* ctx, const mx_ref* ref, const mx_tile* tile, int32_t* n_esc, double* value, float* out, uint64_t* stats4) {
+    if (!ctx) return MX_E_INVALID;
+    BusyGuard busy(ctx);
+    if (!busy.ok) return MX_E_BUSY;
+    mx_plan* plan = nullptr;
+    int rc = mx_plan_create(ctx, ref, tile, &plan);
+    if (rc != MX_OK) return rc;
+    rc = mx_plan_launch(plan);
+    if (rc == MX_OK) rc = mx_plan_wait(plan, nullptr);
+    if (rc == MX_OK || rc == MX_E_CANCELLED) {
+        int rc2 = mx_plan_read(plan, n_esc, value, out, stats4);
+        if (rc == MX_OK) rc = rc2;
+    }
+    mx_plan_destroy(plan);
+    return rc;
+}
+
+extern "C" int mx_compute(mx_ctx* ctx, const mx_request* req, mx_frame_info* info) {
+    if (!ctx || !req || !req->iterations) return MX_E_INVALID;
+    BusyGuard busy(ctx);
+    if (!busy.ok) return MX_E_BUSY;
+    if (__atomic_load_n(ctx->cancel_host, __ATOMIC_ACQUIRE)) return MX_E_CANCELLED;
+    const double t0 = now_s();
+    mx_frame frame;
+    mx::Frame& F = frame.impl;
+    std::string err;
+    int rc = F.prepare(*req, err);
+    if (rc != MX_OK) {
+        ctx->set_err(err);
+        return rc;
+    }
+    if (req->form != MX_FORM_FE64) {
+        ctx->set_err("this kernel form is not built");
+        return MX_E_UNSUPPORTED;
+    }
+    mx_frame_view v;
+    F.fill_view(v);
+    const double t_up = now_s();
+    mx_ref_desc d;
+    std::memset(&d, 0, sizeof d);
+    d.len = v.len;
+    d.zf = v.zf;
+    d.zfe_m = v.zfe_m;
+    d.zfe_e = v.zfe_e;
+    d.nlev = v.nlev;
+    d.lev_off = v.lev_off;
+    d.lev_size = v.lev_size;
+    d.bla_r2 = v.bla_r2;
+    d.bla_coef = v.bla_coef;
+    d.rmax2 = v.rmax2;
+    mx_ref* ref = nullptr;
+    rc = mx_ref_upload(ctx, &d, &ref);
+    if (rc != MX_OK) return rc;
+    const double upload_s = now_s() - t_up;
+    mx_tile t;
+    std::memset(&t, 0, sizeof t);
+    t.width = F.W;
+    t.height = F.H;
+    t.stride = 1;
+    t.max_iter = F.max_iter;
+    t.esc2 = F.esc2;
+    t.form = req->form;
+    t.ede = F.ede;
+    t.interior = F.interior;
+    t.int_rep = F.interior;
+    t.bla = F.use_bla && v.nlev > 0;
+    t.bla_minlevel = F.bla_minlevel;
+    t.interior_eps2 = 3e-9 * 3e-9;
+    t.interior_confirm = 4;
+    t.dcol_m = v.dcol_m;
+    t.dcol_e = v.dcol_e;
+    t.drow_m = v.drow_m;
+    t.drow_e = v.drow_e;
+    t.dx_m = v.dx_m;
+    t.dx_e = v.dx_e;
+    const size_t n = (size_t)F.W * F.H;
+    std::vector<int32_t> nesc(n);
+    std::vector<double> val(n);
+    uint64_t st[4] = {0, 0, 0, 0};
+    double probe_ms = 0.0, kernel_ms = 0.0;
+    bool interior_used = F.interior;
+    // interior auto-gate probe on the 8x8-strided grid (mandel_perturbation.cpp:1668-1700)
+    if (F.interior) {
+        mx_tile pt = t;
+        pt.stride = 8;
+        mx_plan* plan = nullptr;
+        rc = mx_plan_create(ctx, ref, &pt, &plan);
+        if (rc == MX_OK) rc = mx_plan_launch(plan);
+        if (rc == MX_OK) rc = mx_plan_wait(plan, &probe_ms);
+        if (rc == MX_OK) rc = mx_plan_read(plan, nesc.data(), nullptr, nullptr, nullptr);
+        mx_plan_destroy(plan);
+        if (rc != MX_OK) {
+            mx_ref_release(ref);
+            return rc;
+        }
+        int nint = 0;
+        for (int i = 0; i < F.H; i += 8)
+            for (int j = 0; j < F.W; j += 8)
+                if (nesc[(size_t)i * F.W + j] < 0 && nesc[(size_t)i * F.W + j] != INT_MIN) ++nint;
+        if (nint == 0) {
+            interior_used = false;
+            t.interior = 0;
+            t.int_rep = 0;
+        }
+    }
+    {
+        mx_plan* plan = nullptr;
+        rc = mx_plan_create(ctx, ref, &t, &plan);
+        if (rc == MX_OK) rc = mx_plan_launch(plan);
+        if (rc == MX_OK) rc = mx_plan_wait(plan, &kernel_ms);
+        if (rc == MX_OK) rc = mx_plan_read(plan, nesc.data(), val.data(), nullptr, st);
+        mx_plan_destroy(plan);
+    }
+    mx_ref_release(ref);
+    if (rc != MX_OK) return rc;
+    if (!F.ref_virtual) {
+        const size_t c = (size_t)(F.H / 2) * F.W + (F.W / 2);
+        nesc[c] = F.ref_pixel_n;
+        val[c] = F.ref_pixel_value;
+    }
+    uint64_t nominal = 0;
+    for (size_t k = 0; k < n; ++k) {
+        const int32_t ne = nesc[k];
+        req->iterations[k] = ne < 0 ? -2.0f : (float)val[k];
+        nominal += ne > 0 ? (uint64_t)ne : (uint64_t)F.max_iter;
+    }
+    if (req->values) std::memcpy(req->values, val.data(), 8 * n);
+    if (req->n_esc) std::memcpy(req->n_esc, nesc.data(), 4 * n);
+    if (info) {
+        std::memset(info, 0, sizeof *info);
+        info->path = 2;
+        info->ref_it = F.ref_it;
+        info->ref_escaped = F.ref_escaped;
+        info->ref_virtual = F.ref_virtual;
+        info->ref_x = F.ref_x;
+        info->ref_y = F.ref_y;
+        info->interior_used = interior_used;
+        info->cutoff_sensitive = F.cutoff_sensitive;
+        info->precision_bits = (int32_t)F.prec;
+        info->bla_levels = v.nlev;
+        info->ref_seconds = F.ref_seconds;
+        info->bla_seconds = F.bla_seconds;
+        info->upload_seconds = upload_s;
+        info->probe_ms = probe_ms;
+        info->kernel_ms = kernel_ms;
+        info->total_seconds = now_s() - t0;
+        info->steps = st[0];
+        info->bla_applies = st[1];
+        info->bla_skipped = st[2];
+        info->rebases = st[3];
+        info->nominal_iterations = nominal;
+    }
+    return MX_OK;
+}

@@ -1,0 +1,536 @@
+// Plain form of the per-pixel rescaled perturbation kernel for gfx950: IEEE FP64
+// with the engine's operation order and no FP contraction (this file is built with
+// -ffp-contract=off), so every escape count equals the CPU engine's.
+//
+// Per-pixel algorithm: Mandel::pixelRescaled (src/engine/mandel_perturbation.cpp:
+// 3287-3618) with tryBLAfe (2883-2970) and the cold underflow recoveries (2794-2876):
+// dz = S*w with w an O(1) double and S a FloatExp, fused step
+// w' = (2X + s w) w + d, BLA skips tested in FloatExp, rescale of |w| into S,
+// Zhuoran rebase when |z| < |dz|, and the carried derivative J for the exterior
+// distance estimate.
+//
+// Execution model: a persistent grid; each wavefront lane owns one sample.  A lane
+// whose sample finishes takes the next one from a per-XCD atomic work queue (one
+// queue per XCD, wave-aggregated fetch-add; an XCD that drains its queue moves on
+// to the others), the GPU analogue of the engine's AVX2 lane refill
+// (mandel_perturbation.cpp:2978-3282).  Samples are handed out in 8x8 tiles so the
+// 64 lanes of a wave start on neighbouring pixels, which follow similar orbit
+// indices and BLA decisions.
+#include <climits>
+
+#include "mx_dev.h"
+
+namespace {
+
+struct Px {
+    Fe dcr, dci;
+    Fe S, S2, SJ, sS;
+    double wr, wi, dr, di, s;
+    double zsr, zsi, conf_D2, conf_zr, conf_zi;
+    double swr, swi;
+    double jr, ji, invSJd;
+    int m, rm, iter;
+    int save_iter, period_win, conf_P, conf_next, conf_count, conf_giveup;
+    int sm, save_iter2, twin2r;
+    int item;
+};
+
+struct Counters {
+    unsigned long long steps, applies, skipped, rebases;
+};
+
+__device__ __forceinline__ void init_pixel(Px& p, Fe dcr, Fe dci) {
+    p.dcr = dcr;
+    p.dci = dci;
+    p.S = fe_sqrt(fe_add(fe_mul(dcr, dcr), fe_mul(dci, dci)));
+    if (p.S.m == 0.0) {
+        p.S = fe_make(1.0, 0);
+        p.wr = p.wi = p.dr = p.di = 0.0;
+    } else {
+        p.wr = fe_to_double(fe_div(dcr, p.S));
+        p.wi = fe_to_double(fe_div(dci, p.S));
+        p.dr = p.wr;
+        p.di = p.wi;
+    }
+    p.s = fe_to_double(p.S);
+    p.S2 = fe_mul(p.S, p.S);
+    p.m = 1;
+    p.iter = 1;
+    p.rm = 0;
+    p.zsr = 1e30;
+    p.zsi = 1e30;
+    p.save_iter = 1;
+    p.period_win = 1;
+    p.conf_P = p.conf_next = p.conf_count = p.conf_giveup = 0;
+    p.conf_D2 = p.conf_zr = p.conf_zi = 0.0;
+    p.SJ = fe_make(1.0, 0);
+    p.jr = 1.0;
+    p.ji = 0.0;
+    p.invSJd = 1.0;
+    p.swr = 1e300;
+    p.swi = 1e300;
+    p.sS = fe_make(1e300, 0);
+    p.sm = -1;
+    p.save_iter2 = 1;
+    p.twin2r = 1;
+}
+
+// feRescaledStepUnderflow (mandel_perturbation.cpp:2794-2835)
+__device__ __noinline__ void step_underflow(Fe Xr, Fe Xi, Fe dcr, Fe dci, Fe& S, double& wr, double& wi) {
+    const double sm = S.m, sm2 = sm * sm;
+    const int se = S.e;
+    int e = INT_MIN;
+    const bool has_w = wr != 0.0 || wi != 0.0;
+    if (has_w && Xr.m != 0.0) e = max(e, Xr.e + se);
+    if (has_w && Xi.m != 0.0) e = max(e, Xi.e + se);
+    if (has_w) e = max(e, se + se);
+    if (dcr.m != 0.0) e = max(e, dcr.e);
+    if (dci.m != 0.0) e = max(e, dci.e);
+    if (e == INT_MIN) {
+        S = fe_make(1.0, 0);
+        wr = wi = 0.0;
+        return;
+    }
+    auto align = [e](double mm, int te) {
+        int d = te - e;
+        if (mm == 0.0 || d < -54) return 0.0;
+        return mm * fe_pow2(d);
+    };
+    const double xs_r = align(Xr.m * sm, Xr.e + se);
+    const double xs_i = align(Xi.m * sm, Xi.e + se);
+    const double s2 = align(sm2, se + se);
+    const double cr = align(dcr.m, dcr.e);
+    const double ci = align(dci.m, dci.e);
+    const double nr = 2.0 * (xs_r * wr - xs_i * wi) + s2 * (wr * wr - wi * wi) + cr;
+    const double ni = 2.0 * (xs_r * wi + xs_i * wr) + 2.0 * s2 * wr * wi + ci;
+    const double mag2 = nr * nr + ni * ni;
+    const double mag = mag2 > 0.0 ? sqrt(mag2) : hypot(nr, ni);
+    if (mag == 0.0) {
+        S = fe_make(1.0, 0);
+        wr = wi = 0.0;
+        return;
+    }
+    S = fe_renorm(mag, e);
+    wr = nr / mag;
+    wi = ni / mag;
+}
+
+// feDerivativeStepUnderflow (mandel_perturbation.cpp:2842-2876)
+__device__ __noinline__ void deriv_underflow(Fe zr, Fe zi, Fe& SJ, double& jr, double& ji, double& invSJd) {
+    double im = 0.5 / SJ.m;
+    int ie = 1 - SJ.e;
+    if (im >= 1.0) {
+        im *= 0.5;
+        ++ie;
+    }
+    Fe invSJ = fe_make(im, ie);
+    int e = invSJ.e;
+    if (zr.m != 0.0) e = max(e, zr.e);
+    if (zi.m != 0.0) e = max(e, zi.e);
+    auto align = [e](Fe x) {
+        if (x.m == 0.0) return 0.0;
+        int d = x.e - e;
+        if (d < -54) return 0.0;
+        return x.m * fe_pow2(d);
+    };
+    double azr = align(zr), azi = align(zi);
+    double ar = 2.0 * (azr * jr - azi * ji) + align(invSJ);
+    double ai = 2.0 * (azr * ji + azi * jr);
+    double mag = sqrt(ar * ar + ai * ai);
+    if (mag == 0.0) {
+        SJ = fe_make(1.0, 0);
+        jr = ji = 0.0;
+        invSJd = 1.0;
+        return;
+    }
+    SJ = fe_mul(SJ, fe_renorm(mag, e));
+    jr = ar / mag;
+    ji = ai / mag;
+    invSJd = 1.0 / fe_to_double(SJ);
+}
+
+// Overflow-safe FloatExp landing of a BLA skip (mandel_perturbation.cpp:2943-2957).
+// Returns false when the landing escapes.
+__device__ __noinline__ bool bla_land_fe(const double2* zf, int land, double esc2, double4 bc, Fe& S, double& wr, double& wi, double dr, double di) {
+    Fe dzr = fe_mul_d(S, wr), dzi = fe_mul_d(S, wi);
+    Fe dcrfe = fe_mul_d(S, dr), dcife = fe_mul_d(S, di);
+    Fe nzr = fe_add(fe_sub(fe_mul_d(dzr, bc.x), fe_mul_d(dzi, bc.y)), fe_sub(fe_mul_d(dcrfe, bc.z), fe_mul_d(dcife, bc.w)));
+    Fe nzi = fe_add(fe_add(fe_mul_d(dzr, bc.y), fe_mul_d(dzi, bc.x)), fe_add(fe_mul_d(dcrfe, bc.w), fe_mul_d(dcife, bc.z)));
+    double2 X = zf[land];
+    double fr = X.x + fe_to_double(nzr), fi = X.y + fe_to_double(nzi);
+    if (fr * fr + fi * fi > esc2) return false;
+    Fe Snew = fe_sqrt(fe_add(fe_mul(nzr, nzr), fe_mul(nzi, nzi)));
+    if (Snew.m == 0.0) {
+        S = fe_make(1.0, 0);
+        wr = wi = 0.0;
+    } else {
+        S = Snew;
+        wr = fe_to_double(fe_div(nzr, S));
+        wi = fe_to_double(fe_div(nzi, S));
+    }
+    return true;
+}
+
+// tryBLAfe (mandel_perturbation.cpp:2883-2970).  s = m-1, reflen = mx_ref_it+1.
+__device__ __forceinline__ int try_bla(const MxRefDev& R, const int2* lev, int s, Fe& S, Fe S2, double& wr, double& wi, double dr, double di, int reflen, double esc2, double4& ab) {
+    if (s < 1) return 0;
+    const int maxp = R.nlev - 1;
+    int startp = (s == 1) ? maxp : __builtin_ctz((unsigned)(s - 1));
+    if (startp > maxp) startp = maxp;
+    if (startp < R.minlevel) return 0;
+    const Fe dz2 = fe_mul(S2, fe_from(wr * wr + wi * wi));
+    if (R.rmax2 <= 0.0 || !fe_abs_less(dz2, fe_from(R.rmax2))) return 0;
+    for (int p = startp; p >= R.minlevel; --p) {
+        const int2 ls = lev[p];
+        const int i = (s - 1) >> p;
+        if (i >= ls.y) continue;
+        const int k = ls.x + i;
+        const double r2 = R.bla_r2[k];
+        if (r2 <= 0 || !fe_abs_less(dz2, fe_from(r2))) continue;
+        const int land = s + (1 << p);
+        if (land >= reflen - 1) continue;
+        const double4 bc = R.bla_coef[k];
+        const double nwr = (bc.x * wr - bc.y * wi) + (bc.z * dr - bc.w * di);
+        const double nwi = (bc.x * wi + bc.y * wr) + (bc.z * di + bc.w * dr);
+        const double BIG = 1e150;
+        if (isfinite(nwr) && isfinite(nwi) && fabs(nwr) < BIG && fabs(nwi) < BIG) {
+            const double2 X = R.zf[land];
+            const double fr = X.x + fe_to_double(fe_mul_d(S, nwr));
+            const double fi = X.y + fe_to_double(fe_mul_d(S, nwi));
+            if (fr * fr + fi * fi > esc2) return 0;
+            const double wm2 = nwr * nwr + nwi * nwi;
+            if (wm2 == 0.0) return 0; // full reference: unrepresentable landing rejected
+            const double wmag = sqrt(wm2);
+            S = fe_mul(S, fe_from(wmag));
+            const double inv = 1.0 / wmag;
+            wr = nwr * inv;
+            wi = nwi * inv;
+        } else {
+            if (!bla_land_fe(R.zf, land, esc2, bc, S, wr, wi, dr, di)) return 0;
+        }
+        ab = bc;
+        return 1 << p;
+    }
+    return 0;
+}
+
+// One trip of the pixel loop.  Returns true when the sample is finished (value and
+// n_esc written to v/n).  n: escape iteration, -1 = reached mxit, -2 = detected interior.
+template <bool EDE>
+__device__ __forceinline__ bool trip(const MxRefDev& R, const int2* lev, const MxTileDev& T, Px& p, double& v, int& n, Counters& c) {
+    const double ESC2 = T.esc2;
+    const int reflen = R.mx_ref_it + 1;
+    if (p.iter >= T.mxit) {
+        n = -1;
+        v = -2.0;
+        return true;
+    }
+    if (T.flags & MX_F_INT_REP) {
+        if (p.m == p.sm && p.wr == p.swr && p.wi == p.swi && p.S.m == p.sS.m && p.S.e == p.sS.e) {
+            n = -2;
+            v = -2.0;
+            return true;
+        }
+        if (p.iter - p.save_iter2 >= p.twin2r) {
+            p.swr = p.wr;
+            p.swi = p.wi;
+            p.sS = p.S;
+            p.sm = p.m;
+            p.save_iter2 = p.iter;
+            p.twin2r += p.twin2r;
+        }
+    }
+    if ((T.flags & MX_F_BLA) && p.m >= 2) {
+        double4 ab;
+        const int skip = try_bla(R, lev, p.rm, p.S, p.S2, p.wr, p.wi, p.dr, p.di, reflen, ESC2, ab);
+        if (skip > 0) {
+            if constexpr (EDE) {
+                Fe Jr = fe_mul_d(p.SJ, p.jr), Ji = fe_mul_d(p.SJ, p.ji);
+                Fe nJr = fe_add(fe_sub(fe_mul_d(Jr, ab.x), fe_mul_d(Ji, ab.y)), fe_from(ab.z));
+                Fe nJi = fe_add(fe_add(fe_mul_d(Jr, ab.y), fe_mul_d(Ji, ab.x)), fe_from(ab.w));
+                p.SJ = fe_sqrt(fe_add(fe_mul(nJr, nJr), fe_mul(nJi, nJi)));
+                if (p.SJ.m == 0.0) {
+                    p.SJ = fe_make(1.0, 0);
+                    p.jr = p.ji = 0.0;
+                    p.invSJd = 1.0;
+                } else {
+                    p.jr = fe_to_double(fe_div(nJr, p.SJ));
+                    p.ji = fe_to_double(fe_div(nJi, p.SJ));
+                    p.invSJd = 1.0 / fe_to_double(p.SJ);
+                }
+            }
+            p.m += skip;
+            p.iter += skip;
+            p.rm += skip;
+            c.applies++;
+            c.skipped += skip;
+            p.s = fe_to_double(p.S);
+            p.S2 = fe_mul(p.S, p.S);
+            p.dr = fe_to_double(fe_div(p.dcr, p.S));
+            p.di = fe_to_double(fe_div(p.dci, p.S));
+            p.zsr = 1e30;
+            p.zsi = 1e30;
+            p.save_iter = p.iter;
+            p.period_win = 1;
+            p.conf_P = 0;
+            p.conf_giveup = 0;
+            return false;
+        }
+    }
+    double2 X = p.m ? R.zf[p.rm] : make_double2(0.0, 0.0);
+    double zmr = 0.0, zmi = 0.0;
+    bool d_underflow = false;
+    Fe zmrfe = fe_make(0.0, 0), zmife = fe_make(0.0, 0);
+    if constexpr (EDE) {
+        zmr = X.x + p.s * p.wr;
+        zmi = X.y + p.s * p.wi;
+        d_underflow = zmr == 0.0 && zmi == 0.0 && p.invSJd == 0.0;
+        if (d_underflow) {
+            Fe Xrfe = fe_make(0.0, 0), Xife = fe_make(0.0, 0);
+            if (p.m) {
+                const double2 mm = R.zfe_m[p.rm];
+                const int2 ee = R.zfe_e[p.rm];
+                Xrfe = fe_make(mm.x, ee.x);
+                Xife = fe_make(mm.y, ee.y);
+            }
+            zmrfe = fe_add(Xrfe, fe_mul_d(p.S, p.wr));
+            zmife = fe_add(Xife, fe_mul_d(p.S, p.wi));
+        }
+    }
+    const double fR = X.x + X.x + p.s * p.wr;
+    const double fI = X.y + X.y + p.s * p.wi;
+    const double nwr = fR * p.wr - fI * p.wi + p.dr;
+    const double nwi = fR * p.wi + fI * p.wr + p.di;
+    const bool su = fabs(nwr) < 0x1p-1056 && fabs(nwi) < 0x1p-1056;
+    if (su && (nwr != 0.0 || nwi != 0.0 || p.wr != 0.0 || p.wi != 0.0 || p.dcr.m != 0.0 || p.dci.m != 0.0)) {
+        Fe Xrfe = fe_make(0.0, 0), Xife = fe_make(0.0, 0);
+        if (p.m) {
+            const double2 mm = R.zfe_m[p.rm];
+            const int2 ee = R.zfe_e[p.rm];
+            Xrfe = fe_make(mm.x, ee.x);
+            Xife = fe_make(mm.y, ee.y);
+        }
+        step_underflow(Xrfe, Xife, p.dcr, p.dci, p.S, p.wr, p.wi);
+        p.s = fe_to_double(p.S);
+        p.S2 = fe_mul(p.S, p.S);
+        p.dr = fe_to_double(fe_div(p.dcr, p.S));
+        p.di = fe_to_double(fe_div(p.dci, p.S));
+    } else {
+        p.wr = nwr;
+        p.wi = nwi;
+    }
+    ++p.m;
+    ++p.iter;
+    ++p.rm;
+    c.steps++;
+    if constexpr (EDE) {
+        if (d_underflow) {
+            deriv_underflow(zmrfe, zmife, p.SJ, p.jr, p.ji, p.invSJd);
+        } else {
+            const double a = 2.0 * (zmr * p.jr - zmi * p.ji) + p.invSJd;
+            const double b = 2.0 * (zmr * p.ji + zmi * p.jr);
+            p.jr = a;
+            p.ji = b;
+            const double jm2 = p.jr * p.jr + p.ji * p.ji;
+            if (jm2 > 1e16 || (jm2 > 0.0 && jm2 < 1e-16)) {
+                const Fe jmag = fe_sqrt(fe_from(jm2));
+                p.SJ = fe_mul(p.SJ, jmag);
+                const double inv = 1.0 / fe_to_double(jmag);
+                p.jr *= inv;
+                p.ji *= inv;
+                p.invSJd = 1.0 / fe_to_double(p.SJ);
+            }
+        }
+    }
+    X = p.m ? R.zf[p.rm] : make_double2(0.0, 0.0);
+    const double zr = X.x + p.s * p.wr, zi = X.y + p.s * p.wi;
+    const double zrad = zr * zr + zi * zi;
+    if (zrad > ESC2) {
+        n = p.iter;
+        if constexpr (EDE) {
+            const double num = sqrt(zrad) * log(zrad);
+            const Fe Jmag = fe_mul_d(p.SJ, sqrt(p.jr * p.jr + p.ji * p.ji));
+            v = fe_to_double(fe_div(fe_from(num), fe_mul(T.dxfe, Jmag)));
+        } else {
+            const double LG2 = 0.69314718055994530942; // log(2.0), exactly the double libm returns
+            v = (double)p.iter - log(log(zrad) / 2.0 / LG2) / LG2;
+        }
+        return true;
+    }
+    if (T.flags & MX_F_INTERIOR) {
+        if (p.conf_P > 0) {
+            p.conf_D2 *= 4.0 * zrad;
+            if (p.conf_D2 > 1e18) p.conf_D2 = 1e18;
+            if (p.iter >= p.conf_next) {
+                const double pr = zr - p.conf_zr, pi = zi - p.conf_zi;
+                if (pr * pr + pi * pi < T.interior_eps2 * zrad && p.conf_D2 < 1.0) {
+                    if (++p.conf_count >= T.interior_confirm) {
+                        n = -2;
+                        v = -2.0;
+                        return true;
+                    }
+                    p.conf_zr = zr;
+                    p.conf_zi = zi;
+                    p.conf_D2 = 1;
+                    p.conf_next = p.iter + p.conf_P;
+                } else {
+                    p.conf_P = 0;
+                    ++p.conf_giveup;
+                }
+            }
+        } else if ((p.iter & 15) == 0 && p.conf_giveup < 3) {
+            const double pr = zr - p.zsr, pi = zi - p.zsi;
+            if (pr * pr + pi * pi < T.interior_eps2 * zrad) {
+                p.conf_P = p.iter - p.save_iter;
+                if (p.conf_P < 1) p.conf_P = 1;
+                p.conf_zr = zr;
+                p.conf_zi = zi;
+                p.conf_D2 = 1;
+                p.conf_next = p.iter + p.conf_P;
+                p.conf_count = 0;
+            }
+            if (p.iter - p.save_iter >= p.period_win) {
+                p.zsr = zr;
+                p.zsi = zi;
+                p.save_iter = p.iter;
+                p.period_win += p.period_win;
+            }
+        }
+    }
+    const double wmag2 = p.wr * p.wr + p.wi * p.wi;
+    if (wmag2 > 1e16 || (wmag2 < 1e-16 && wmag2 > 0.0)) {
+        const Fe wmag = fe_sqrt(fe_from(wmag2));
+        p.S = fe_mul(p.S, wmag);
+        p.s = fe_to_double(p.S);
+        p.S2 = fe_mul(p.S, p.S);
+        const double inv = 1.0 / fe_to_double(wmag);
+        p.wr *= inv;
+        p.wi *= inv;
+        p.dr = fe_to_double(fe_div(p.dcr, p.S));
+        p.di = fe_to_double(fe_div(p.dci, p.S));
+    }
+    const double dzmag2 = p.s * p.s * (p.wr * p.wr + p.wi * p.wi);
+    if (zrad < 1e-8 || zrad < dzmag2 || p.m >= reflen) {
+        Fe Xmr = fe_make(0.0, 0), Xmi = fe_make(0.0, 0);
+        if (p.m) {
+            const double2 mm = R.zfe_m[p.rm];
+            const int2 ee = R.zfe_e[p.rm];
+            Xmr = fe_make(mm.x, ee.x);
+            Xmi = fe_make(mm.y, ee.y);
+        }
+        const Fe Swr = fe_mul_d(p.S, p.wr), Swi = fe_mul_d(p.S, p.wi);
+        const Fe zrfe = fe_add(Xmr, Swr), zife = fe_add(Xmi, Swi);
+        const Fe zradfe = fe_add(fe_mul(zrfe, zrfe), fe_mul(zife, zife));
+        const Fe dzfe = fe_add(fe_mul(Swr, Swr), fe_mul(Swi, Swi));
+        if (p.m >= reflen || fe_abs_less(zradfe, dzfe)) {
+            const Fe Snew = fe_sqrt(zradfe);
+            if (Snew.m == 0.0) {
+                p.wr = p.wi = 0.0;
+            } else {
+                p.S = Snew;
+                p.s = fe_to_double(p.S);
+                p.S2 = fe_mul(p.S, p.S);
+                p.wr = fe_to_double(fe_div(zrfe, p.S));
+                p.wi = fe_to_double(fe_div(zife, p.S));
+                p.dr = fe_to_double(fe_div(p.dcr, p.S));
+                p.di = fe_to_double(fe_div(p.dci, p.S));
+            }
+            p.m = 0;
+            p.rm = -1;
+            c.rebases++;
+        }
+    }
+    return false;
+}
+
+__device__ __forceinline__ unsigned long long wave_sum(unsigned long long x) {
+    for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o);
+    return x;
+}
+
+// Map a queue item to a sample.  Items are 8x8 lane tiles laid row-major over the
+// launch grid; returns false for the padding lanes of edge tiles.
+__device__ __forceinline__ bool item_to_sample(const MxTileDev& T, unsigned item, int& si, int& sj) {
+    const unsigned tile = item >> 6, l = item & 63;
+    const int ty = (int)(tile / (unsigned)T.tiles_x), tx = (int)(tile % (unsigned)T.tiles_x);
+    const int gi = ty * 8 + (int)(l >> 3), gj = tx * 8 + (int)(l & 7);
+    if (gi >= T.ny || gj >= T.nx) return false;
+    si = gi * T.stride;
+    sj = gj * T.stride;
+    return true;
+}
+
+template <bool EDE>
+__global__ __launch_bounds__(256) void mx_fe64_kernel(MxRefDev R, const int2* __restrict__ lev, MxTileDev T) {
+    const int lane = __lane_id();
+    const unsigned long long lanemask_lt = (1ull << lane) - 1ull;
+    int q = blockIdx.x % MX_NQUEUE;
+    int exhausted = 0;
+    bool alive = true, active = false;
+    Px p;
+    Counters c = {0, 0, 0, 0};
+    int sidx = 0;
+    unsigned trips = 0;
+    const unsigned ntiles = (unsigned)T.ntiles;
+    while (true) {
+        bool need = alive && !active;
+        unsigned long long needmask = __ballot(need);
+        while (needmask != 0ull && exhausted < MX_NQUEUE) {
+            const unsigned t0 = ntiles * (unsigned)q / MX_NQUEUE, t1 = ntiles * (unsigned)(q + 1) / MX_NQUEUE;
+            const unsigned nitems = (t1 - t0) * 64u;
+            const int k = __popcll(needmask);
+            const int leader = __ffsll((long long)needmask) - 1;
+            unsigned base = 0;
+            if (lane == leader) base = atomicAdd(&T.queue[q], (unsigned)k);
+            base = __shfl(base, leader);
+            if (need) {
+                const unsigned it = base + (unsigned)__popcll(needmask & lanemask_lt);
+                if (it < nitems) {
+                    int si, sj;
+                    if (item_to_sample(T, t0 * 64u + it, si, sj)) {
+                        sidx = si * T.W + sj;
+                        init_pixel(p, T.dcol[sj], T.drow[si]);
+                        need = false;
+                        active = true;
+                    }
+                }
+            }
+            if (base + (unsigned)k >= nitems) { // this queue is drained
+                q = (q + 1) % MX_NQUEUE;
+                ++exhausted;
+            }
+            needmask = __ballot(need);
+        }
+        if (need) alive = false;
+        if (__ballot(active) == 0ull) break;
+        if (((++trips) & 4095u) == 0u && T.cancel && *T.cancel) break;
+        if (active) {
+            double v;
+            int n;
+            if (trip<EDE>(R, lev, T, p, v, n, c)) {
+                T.val[sidx] = v;
+                T.nesc[sidx] = n;
+                if (T.out) T.out[sidx] = n < 0 ? -2.0f : (float)v;
+                active = false;
+            }
+        }
+    }
+    const unsigned long long s0 = wave_sum(c.steps), s1 = wave_sum(c.applies), s2 = wave_sum(c.skipped), s3 = wave_sum(c.rebases);
+    if (lane == 0) {
+        atomicAdd(&T.stats[0], s0);
+        atomicAdd(&T.stats[1], s1);
+        atomicAdd(&T.stats[2], s2);
+        atomicAdd(&T.stats[3], s3);
+    }
+}
+
+} // namespace
+
+// Host-side launcher (called from the engine's C++).
+hipError_t mx_launch_fe64(const MxRefDev& R, const int2* lev, const MxTileDev& T, int blocks, int threads, hipStream_t stream) {
+    if (T.flags & MX_F_EDE)
+        hipLaunchKernelGGL(mx_fe64_kernel<true>, dim3(blocks), dim3(threads), 0, stream, R, lev, T);
+    else
+        hipLaunchKernelGGL(mx_fe64_kernel<false>, dim3(blocks), dim3(threads), 0, stream, R, lev, T);
+    return hipGetLastError();
+}

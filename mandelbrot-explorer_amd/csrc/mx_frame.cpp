@@ -1,0 +1,489 @@
+// Host-side frame preparation for the perturbation path: the serial part of
+// Mandel::Compute (src/engine/mandel_perturbation.cpp:1388-1634) that stays on the
+// CPU.  Grid set-up and the reference orbit run in GMP mpf with the engine's
+// operations, so the double and FloatExp shadows handed to the GPU are the truncated
+// values the engine would hold; the BLA table is built in FP64 like buildBLA.
+#include <gmp.h>
+
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "mx_fe.h"
+#include "mx_internal.h"
+
+namespace mx {
+
+static double now_s() { return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count(); }
+
+// mpf_to_fe / mpf_get_ld (mandel_perturbation.cpp:69-74, float_math.cpp:15-19):
+// truncation toward zero to a 53-bit mantissa.
+static void mpf_shadow(mpf_srcptr x, double& m, long& e) {
+    if (mpf_sgn(x) == 0) {
+        m = 0.0;
+        e = 0;
+        return;
+    }
+    m = mpf_get_d_2exp(&e, x);
+}
+
+static double mpf_get_ld(mpf_srcptr x) {
+    long e;
+    double d = mpf_get_d_2exp(&e, x);
+    return ldexp(d, (int)e);
+}
+
+struct Gmp {
+    mpf_t v;
+    explicit Gmp(mp_bitcnt_t prec) { mpf_init2(v, prec); }
+    ~Gmp() { mpf_clear(v); }
+    Gmp(const Gmp&) = delete;
+    Gmp& operator=(const Gmp&) = delete;
+    operator mpf_ptr() { return v; }
+};
+
+// Reference orbit: createRef + calCoefficient (mandel_perturbation.cpp:4014-4187,
+// 4189-4249) with the mpf Karatsuba square re = (a+b)(a-b), im = 2ab.
+void Frame::build_orbit(mpf_srcptr cre, mpf_srcptr cim) {
+    const int mxit = max_iter;
+    zf.assign(2 * (size_t)(mxit + 1), 0.0);
+    zfe_m.assign(2 * (size_t)(mxit + 1), 0.0);
+    zfe_e.assign(2 * (size_t)(mxit + 1), 0);
+    Gmp zr(prec), zi(prec), t1(prec), t2(prec), nzr(prec), nzi(prec);
+    mpf_set(ref_re.v, cre);
+    mpf_set(ref_im.v, cim);
+    mpf_set(zr, cre);
+    mpf_set(zi, cim);
+    auto store = [&](int i) {
+        double m;
+        long e;
+        mpf_shadow(zr, m, e);
+        zfe_m[2 * i] = m;
+        zfe_e[2 * i] = (int32_t)e;
+        zf[2 * i] = m == 0.0 ? 0.0 : ldexp(m, (int)e);
+        mpf_shadow(zi, m, e);
+        zfe_m[2 * i + 1] = m;
+        zfe_e[2 * i + 1] = (int32_t)e;
+        zf[2 * i + 1] = m == 0.0 ? 0.0 : ldexp(m, (int)e);
+    };
+    store(0);
+    ref_escaped = false;
+    ref_it = mxit;
+    const double R = (double)escape_radius;
+    for (int i = 1; i <= mxit; ++i) {
+        mpf_add(t1, zr, zi);
+        mpf_sub(t2, zr, zi);
+        mpf_mul(nzi, zr, zi);
+        mpf_mul(nzr, t1, t2);
+        mpf_mul_ui(nzi, nzi, 2);
+        mpf_add(zr, nzr, cre);
+        mpf_add(zi, nzi, cim);
+        store(i);
+        if (std::hypot(zf[2 * i], zf[2 * i + 1]) > R) { // std::abs(Comp) > R
+            ref_escaped = true;
+            ref_it = i;
+            break;
+        }
+    }
+    mpf_set(tail_re.v, zr);
+    mpf_set(tail_im.v, zi);
+    zf.resize(2 * (size_t)(ref_it + 1));
+    zfe_m.resize(2 * (size_t)(ref_it + 1));
+    zfe_e.resize(2 * (size_t)(ref_it + 1));
+}
+
+// refTailEscapes (mandel_perturbation.cpp:927-957)
+bool Frame::tail_escapes(int extra) {
+    Gmp zr(prec), zi(prec), t1(prec), t2(prec);
+    mpf_set(zr, tail_re.v);
+    mpf_set(zi, tail_im.v);
+    bool esc = false;
+    for (int k = 0; k < extra && !esc; ++k) {
+        mpf_mul(t1, zr, zi);
+        mpf_mul(zr, zr, zr);
+        mpf_mul(t2, zi, zi);
+        mpf_sub(zr, zr, t2);
+        mpf_add(zr, zr, ref_re.v);
+        mpf_mul_ui(zi, t1, 2);
+        mpf_add(zi, zi, ref_im.v);
+        mpf_mul(t1, zr, zr);
+        mpf_mul(t2, zi, zi);
+        mpf_add(t1, t1, t2);
+        if (mpf_get_d(t1) > esc2) esc = true;
+    }
+    return esc;
+}
+
+// accuratePointCompute (mandel_perturbation.cpp:719-894) for the reference pixel's
+// own value under a coloring method; smooth or pixel-normalised EDE, -2 interior.
+double Frame::point_value(mpf_srcptr c_re, mpf_srcptr c_im, int mxit, bool ede, int& n_out) {
+    Gmp t1(prec), t2(prec), zr(prec), zi(prec), dr(prec), di(prec), er(prec), ei(prec), e1(prec), e2(prec);
+    mpf_set(zr, c_re);
+    mpf_set(zi, c_im);
+    mpf_set_ui(dr, 2);
+    mpf_set_ui(di, 0);
+    mpf_set_ui(er, 1);
+    mpf_set_ui(ei, 0);
+    double res = -2.0;
+    n_out = -1;
+    for (int i = 1; i < mxit; ++i) {
+        mpf_mul(t1, dr, zi);
+        mpf_mul(dr, dr, zr);
+        mpf_mul(t2, di, zi);
+        mpf_mul(di, di, zr);
+        mpf_sub(dr, dr, t2);
+        mpf_mul_ui(dr, dr, 2);
+        mpf_add(di, di, t1);
+        mpf_mul_ui(di, di, 2);
+        if (ede) {
+            mpf_mul(e1, er, zi);
+            mpf_mul(e2, ei, zr);
+            mpf_add(e2, e1, e2);
+            mpf_mul(e1, er, zr);
+            mpf_mul(t2, ei, zi);
+            mpf_sub(e1, e1, t2);
+            mpf_mul_ui(er, e1, 2);
+            mpf_add_ui(er, er, 1);
+            mpf_mul_ui(ei, e2, 2);
+        }
+        mpf_mul(t1, zr, zi);
+        mpf_mul(zr, zr, zr);
+        mpf_mul(zi, zi, zi);
+        mpf_sub(zr, zr, zi);
+        mpf_add(zr, zr, c_re);
+        mpf_mul_ui(zi, t1, 2);
+        mpf_add(zi, zi, c_im);
+        mpf_mul(t1, zr, zr);
+        mpf_mul(t2, zi, zi);
+        mpf_add(t1, t1, t2);
+        double rad = mpf_get_d(t1);
+        if (rad > esc2) {
+            n_out = i + 1;
+            if (ede) {
+                mpf_mul(e1, er, er);
+                mpf_mul(e2, ei, ei);
+                mpf_add(e1, e1, e2);
+                mpf_sqrt(e1, e1);
+                mpf_abs(t1, dx.v);
+                mpf_mul(e1, e1, t1);
+                double num = std::isfinite(rad) && rad > 0.0 ? std::sqrt(rad) * std::log(rad) : 0.0;
+                mpf_set_d(e2, num);
+                if (mpf_sgn(e1.v) != 0) {
+                    mpf_div(e1, e2, e1);
+                    res = mpf_get_d(e1);
+                } else
+                    res = 0.0;
+            } else {
+                res = (i + 1 - std::log(std::log(rad) / 2 / std::log(2)) / std::log(2));
+            }
+            break;
+        }
+        mpf_mul(t1, dr, dr);
+        mpf_mul(t2, di, di);
+        mpf_add(t1, t1, t2);
+        if (mpf_get_d(t1) < 0.000000001) {
+            n_out = -2;
+            break;
+        }
+    }
+    return res;
+}
+
+// buildBLA (mandel_perturbation.cpp:2613-2709) without the EDE derivative couplings,
+// which the rescaled path does not read (it carries J through A and B).
+void Frame::build_bla(double dcmax_in) {
+    lev_off.clear();
+    lev_size.clear();
+    bla_r2.clear();
+    bla_coef.clear();
+    rmax2 = 0.0;
+    dcmax = dcmax_in;
+    const int reflen = ref_it + 1;
+    if (reflen < 3) return;
+    const double eps = std::ldexp(1.0, -53);
+    int64_t n = reflen - 1, tot = 0;
+    for (;;) {
+        lev_off.push_back((int32_t)tot);
+        lev_size.push_back((int32_t)n);
+        tot += n;
+        if (n <= 1) break;
+        n /= 2;
+    }
+    bla_r2.assign((size_t)tot, 0.0);
+    bla_coef.assign(4 * (size_t)tot, 0.0);
+    for (int s = 1; s < reflen; ++s) {
+        const double zr = zf[2 * s], zi = zf[2 * s + 1];
+        const double Zmag = std::sqrt(zr * zr + zi * zi);
+        const double Amag = 2.0 * Zmag;
+        const double R = eps * (Zmag - dcmax) / (Amag + 1.0);
+        const double r2 = R > 0 ? R * R : 0.0;
+        if (r2 > rmax2) rmax2 = r2;
+        const size_t k = (size_t)(s - 1);
+        bla_r2[k] = r2;
+        bla_coef[4 * k + 0] = 2.0 * zr;
+        bla_coef[4 * k + 1] = 2.0 * zi;
+        bla_coef[4 * k + 2] = 1.0;
+        bla_coef[4 * k + 3] = 0.0;
+    }
+    for (size_t p = 1; p < lev_off.size(); ++p) {
+        const size_t po = (size_t)lev_off[p - 1], no = (size_t)lev_off[p];
+        for (size_t i = 0; i < (size_t)lev_size[p]; ++i) {
+            const size_t x = po + 2 * i, y = x + 1, z = no + i;
+            const double* xc = &bla_coef[4 * x];
+            const double* yc = &bla_coef[4 * y];
+            double* zc = &bla_coef[4 * z];
+            zc[0] = yc[0] * xc[0] - yc[1] * xc[1];
+            zc[1] = yc[0] * xc[1] + yc[1] * xc[0];
+            zc[2] = yc[0] * xc[2] - yc[1] * xc[3] + yc[2];
+            zc[3] = yc[0] * xc[3] + yc[1] * xc[2] + yc[3];
+            const double Rx = std::sqrt(bla_r2[x]), Ry = std::sqrt(bla_r2[y]);
+            const double Axmag = std::sqrt(xc[0] * xc[0] + xc[1] * xc[1]);
+            const double Bxmag = std::sqrt(xc[2] * xc[2] + xc[3] * xc[3]);
+            const double cand = Axmag > 0 ? (Ry - Bxmag * dcmax) / Axmag : 0.0;
+            const double R = std::min(Rx, cand);
+            bla_r2[z] = R > 0 ? R * R : 0.0;
+        }
+    }
+}
+
+static bool parse_decimal(mpf_ptr x, const char* s) {
+    if (!s || !*s) return false;
+    return mpf_set_str(x, s, 10) == 0;
+}
+
+int Frame::prepare(const mx_request& r, std::string& err) {
+    const double t_start = now_s();
+    const int full_h = r.full_height > 0 ? r.full_height : r.height;
+    if (!r.center_re || !r.center_im || !r.scale || r.width < 2 || r.height < 1 || full_h < 2 || r.row_base < 0 ||
+        r.row_base > full_h - r.height || r.sub < 1 || r.max_iterations < 1) {
+        err = "invalid request";
+        return MX_E_INVALID;
+    }
+    if (r.sub != 1) {
+        err = "sub > 1 (adaptive supersampling) is not on this path";
+        return MX_E_UNSUPPORTED;
+    }
+    if ((r.coloring_method & ~(MX_EXTERIOR_DIST_EST | MX_SUPER_SAMPLING)) != 0) {
+        err = "coloring method not on this path (smooth and exterior DE only)";
+        return MX_E_UNSUPPORTED;
+    }
+    W = r.width;
+    H = r.height;
+    max_iter = r.max_iterations;
+    ede = (r.coloring_method & MX_EXTERIOR_DIST_EST) != 0;
+    escape_radius = r.escape_radius > 0 ? r.escape_radius : 1e8f;
+    const float R2f = escape_radius * escape_radius; // Mandel::escapeRadiusSquared (float product)
+    esc2 = (double)R2f;
+    const std::string sc = r.scale;
+    prec = r.precision_bits > 0 ? (mp_bitcnt_t)r.precision_bits : (mp_bitcnt_t)((int)(sc.size() * std::log(10) / std::log(2)) + 64);
+    if (prec > (1u << 24)) {
+        err = "precision too large";
+        return MX_E_INVALID;
+    }
+    init_mpf();
+    Gmp cre(prec), cim(prec), scale(prec);
+    if (!parse_decimal(cre, r.center_re) || !parse_decimal(cim, r.center_im) || !parse_decimal(scale, r.scale) || mpf_sgn(scale.v) <= 0) {
+        err = "could not parse centre/scale";
+        return MX_E_INVALID;
+    }
+    if (mpf_cmp_d(scale, 1e280) <= 0) {
+        err = "scale <= 1e280: the plain double path is not in this build";
+        return MX_E_UNSUPPORTED;
+    }
+    // depth guard for the int32 exponent of the device FloatExp: |log2 scale| < 2^26
+    {
+        long se;
+        (void)mpf_get_d_2exp(&se, scale);
+        if (se > (1L << 26)) {
+            err = "scale out of range";
+            return MX_E_INVALID;
+        }
+    }
+    // grid (mandel_perturbation.cpp:1403-1424)
+    {
+        Gmp dw(prec), dh(prec);
+        mpf_set_ui(dw, 2);
+        mpf_div(dw, dw, scale);
+        mpf_set(dh, dw);
+        mpf_div_ui(dh, dh, W);
+        mpf_mul_ui(dh, dh, full_h);
+        mpf_sub(c0_re.v, cre, dw);
+        mpf_sub(c0_im.v, cim, dh);
+        mpf_div_ui(dx.v, dw, W - 1);
+        mpf_mul_ui(dx.v, dx.v, 2);
+        mpf_div_ui(dy.v, dh, full_h - 1);
+        mpf_mul_ui(dy.v, dy.v, 2);
+        if (r.row_base > 0) {
+            Gmp off(prec);
+            mpf_mul_ui(off, dy.v, r.row_base);
+            mpf_add(c0_im.v, c0_im.v, off);
+        }
+    }
+    double m;
+    long e;
+    mpf_shadow(dx.v, m, e);
+    dxfe = fe_make(m, (int)e);
+    mpf_shadow(dy.v, m, e);
+    dyfe = fe_make(m, (int)e);
+
+    // reference: the centre pixel, then the exact view centre when the pixel
+    // reference is bounded-but-sensitive (mandel_perturbation.cpp:1326-1373)
+    const double t_ref = now_s();
+    {
+        Gmp pre(prec), pim(prec), t(prec);
+        const int pi = H / 2, pj = W / 2;
+        mpf_mul_ui(t, dx.v, pj);
+        mpf_add(pre, c0_re.v, t);
+        mpf_mul_ui(t, dy.v, pi);
+        mpf_add(pim, c0_im.v, t);
+        ref_virtual = false;
+        ref_x = pj;
+        ref_y = pi;
+        build_orbit(pre, pim);
+        if (!ref_escaped || ref_it >= max_iter - 16) {
+            const bool sensitive = ref_escaped || tail_escapes(80);
+            if (sensitive) {
+                mpf_mul_ui(t, dx.v, W - 1);
+                mpf_div_ui(t, t, 2);
+                mpf_add(pre, c0_re.v, t);
+                mpf_mul_ui(t, dy.v, H - 1);
+                mpf_div_ui(t, t, 2);
+                mpf_add(pim, c0_im.v, t);
+                ref_virtual = true;
+                ref_x = (W - 1) * 0.5;
+                ref_y = (H - 1) * 0.5;
+                build_orbit(pre, pim);
+                cutoff_sensitive = ref_escaped && ref_it >= max_iter - 16;
+            }
+        }
+        // value of the reference pixel itself (mandel_perturbation.cpp:4145-4160, 4179-4185)
+        if (!ref_virtual) {
+            if (!ref_escaped) {
+                ref_pixel_n = -1;
+                ref_pixel_value = -2.0;
+            } else if (ede) {
+                ref_pixel_value = point_value(ref_re.v, ref_im.v, max_iter, true, ref_pixel_n);
+            } else {
+                Gmp t1(prec), t2(prec);
+                mpf_mul(t1, tail_re.v, tail_re.v);
+                mpf_mul(t2, tail_im.v, tail_im.v);
+                mpf_add(t1, t1, t2);
+                const double rad = mpf_get_d(t1);
+                ref_pixel_n = ref_it + 1;
+                ref_pixel_value = (ref_it + 1 - std::log(std::log(rad) / 2 / std::log(2)) / std::log(2));
+            }
+        }
+    }
+    ref_seconds = now_s() - t_ref;
+
+    use_bla = r.use_bla != 0;
+    int ms = r.bla_minskip > 0 ? r.bla_minskip : 8;
+    bla_minlevel = 0;
+    while ((1 << (bla_minlevel + 1)) <= ms) ++bla_minlevel;
+    const double t_bla = now_s();
+    if (use_bla) {
+        // dcmax from the farthest image corner (mandel_perturbation.cpp:2626-2629)
+        const double dxf = std::fabs(mpf_get_ld(dx.v)), dyf = std::fabs(mpf_get_ld(dy.v));
+        const double mxx = std::max(ref_x, (double)(W - 1) - ref_x) + 0.5;
+        const double mxy = std::max(ref_y, (double)(H - 1) - ref_y) + 0.5;
+        build_bla(std::sqrt(dxf * dxf * mxx * mxx + dyf * dyf * mxy * mxy));
+    } else {
+        build_bla(0.0);
+        lev_off.clear();
+        lev_size.clear();
+        bla_r2.clear();
+        bla_coef.clear();
+        rmax2 = 0.0;
+    }
+    bla_seconds = now_s() - t_bla;
+
+    // per-column / per-row dc (mandel_perturbation.cpp:2160-2163)
+    dcol_m.resize(W);
+    dcol_e.resize(W);
+    drow_m.resize(H);
+    drow_e.resize(H);
+    for (int j = 0; j < W; ++j) {
+        Fe f = fe_mul_d(dxfe, (double)j - ref_x);
+        dcol_m[j] = f.m;
+        dcol_e[j] = f.e;
+    }
+    for (int i = 0; i < H; ++i) {
+        Fe f = fe_mul_d(dyfe, (double)i - ref_y);
+        drow_m[i] = f.m;
+        drow_e[i] = f.e;
+    }
+    interior = r.interior != 0;
+    prepare_seconds = now_s() - t_start;
+    return MX_OK;
+}
+
+void Frame::init_mpf() {
+    for (mpf_ptr x : {c0_re.v, c0_im.v, dx.v, dy.v, ref_re.v, ref_im.v, tail_re.v, tail_im.v}) mpf_set_prec(x, prec);
+}
+
+Frame::Frame() {
+    for (mpf_ptr x : {c0_re.v, c0_im.v, dx.v, dy.v, ref_re.v, ref_im.v, tail_re.v, tail_im.v}) mpf_init2(x, 64);
+}
+
+Frame::~Frame() {
+    for (mpf_ptr x : {c0_re.v, c0_im.v, dx.v, dy.v, ref_re.v, ref_im.v, tail_re.v, tail_im.v}) mpf_clear(x);
+}
+
+void Frame::fill_view(mx_frame_view& v) const {
+    std::memset(&v, 0, sizeof v);
+    v.len = ref_it + 1;
+    v.zf = zf.data();
+    v.zfe_m = zfe_m.data();
+    v.zfe_e = zfe_e.data();
+    v.nlev = (int32_t)lev_off.size();
+    v.lev_off = lev_off.data();
+    v.lev_size = lev_size.data();
+    v.bla_total = (int64_t)bla_r2.size();
+    v.bla_r2 = bla_r2.data();
+    v.bla_coef = bla_coef.data();
+    v.rmax2 = rmax2;
+    v.dcmax = dcmax;
+    v.dcol_m = dcol_m.data();
+    v.dcol_e = dcol_e.data();
+    v.drow_m = drow_m.data();
+    v.drow_e = drow_e.data();
+    v.dx_m = dxfe.m;
+    v.dx_e = dxfe.e;
+    v.dy_m = dyfe.m;
+    v.dy_e = dyfe.e;
+    v.esc2 = esc2;
+    v.bla_minlevel = bla_minlevel;
+    v.ref_it = ref_it;
+    v.ref_escaped = ref_escaped;
+    v.ref_virtual = ref_virtual;
+    v.ref_x = ref_x;
+    v.ref_y = ref_y;
+    v.ref_pixel_n = ref_pixel_n;
+    v.ref_pixel_value = ref_pixel_value;
+    v.cutoff_sensitive = cutoff_sensitive;
+    v.precision_bits = (int32_t)prec;
+    v.ref_seconds = ref_seconds;
+    v.bla_seconds = bla_seconds;
+}
+
+} // namespace mx
+
+extern "C" int mx_frame_prepare(const mx_request* req, mx_frame** out, mx_frame_view* view) {
+    if (!req || !out) return MX_E_INVALID;
+    *out = nullptr;
+    auto* f = new (std::nothrow) mx_frame();
+    if (!f) return MX_E_NOMEM;
+    std::string err;
+    int rc = f->impl.prepare(*req, err);
+    if (rc != MX_OK) {
+        delete f;
+        return rc;
+    }
+    if (view) f->impl.fill_view(*view);
+    *out = f;
+    return MX_OK;
+}
+
+extern "C" void mx_frame_free(mx_frame* f) { delete f; }

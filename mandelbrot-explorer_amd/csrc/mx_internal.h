@@ -1,0 +1,115 @@
+// Internal host-side types of libmxhip.so (not part of the C ABI).
+#pragma once
+
+#include <gmp.h>
+#include <hip/hip_runtime.h>
+
+#include <atomic>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/mx.h"
+#include "mx_dev.h"
+
+namespace mx {
+
+struct Mpf {
+    mpf_t v;
+};
+
+// Host state of one frame: grid, reference orbit and BLA table in the engine's
+// formats (mandel_perturbation.h:193-206, 295-341).
+struct Frame {
+    Frame();
+    ~Frame();
+    Frame(const Frame&) = delete;
+    Frame& operator=(const Frame&) = delete;
+
+    int prepare(const mx_request& r, std::string& err);
+    void fill_view(mx_frame_view& v) const;
+
+    // inputs
+    int W = 0, H = 0, max_iter = 0;
+    bool ede = false, use_bla = true, interior = true;
+    float escape_radius = 1e8f;
+    double esc2 = 0.0;
+    mp_bitcnt_t prec = 64;
+    int bla_minlevel = 3;
+
+    // grid
+    Mpf c0_re, c0_im, dx, dy;
+    Fe dxfe{0.0, 0}, dyfe{0.0, 0};
+
+    // reference
+    Mpf ref_re, ref_im, tail_re, tail_im;
+    int ref_it = 0;
+    bool ref_escaped = false, ref_virtual = false, cutoff_sensitive = false;
+    double ref_x = 0.0, ref_y = 0.0;
+    int ref_pixel_n = -1;
+    double ref_pixel_value = -2.0;
+    std::vector<double> zf, zfe_m;
+    std::vector<int32_t> zfe_e;
+
+    // BLA
+    std::vector<int32_t> lev_off, lev_size;
+    std::vector<double> bla_r2, bla_coef;
+    double rmax2 = 0.0, dcmax = 0.0;
+
+    // per-pixel dc
+    std::vector<double> dcol_m, drow_m;
+    std::vector<int32_t> dcol_e, drow_e;
+
+    double ref_seconds = 0.0, bla_seconds = 0.0, prepare_seconds = 0.0;
+
+  private:
+    void init_mpf();
+    void build_orbit(mpf_srcptr cre, mpf_srcptr cim);
+    bool tail_escapes(int extra);
+    double point_value(mpf_srcptr c_re, mpf_srcptr c_im, int mxit, bool ede, int& n_out);
+    void build_bla(double dcmax);
+};
+
+} // namespace mx
+
+struct mx_frame {
+    mx::Frame impl;
+};
+
+struct mx_ctx {
+    int device = 0;
+    int cu_count = 0;
+    hipStream_t stream = nullptr;
+    int* cancel_host = nullptr;     // pinned, host-mapped flag
+    int* cancel_dev = nullptr;      // device view of the same flag
+    std::atomic<bool> busy{false};
+    std::string err;
+    std::mutex err_mu;
+    void set_err(const std::string& e) {
+        std::lock_guard<std::mutex> g(err_mu);
+        err = e;
+    }
+};
+
+struct mx_ref {
+    mx_ctx* ctx = nullptr;
+    MxRefDev dev{};
+    int2* lev = nullptr;        // device level table (off, size)
+    void* block = nullptr;      // one device allocation for every array
+    size_t bytes = 0;
+};
+
+struct mx_plan {
+    mx_ctx* ctx = nullptr;
+    const mx_ref* ref = nullptr;
+    MxTileDev tile{};
+    int form = MX_FORM_FE64;
+    int blocks = 0, threads = 256;
+    int minlevel = 0;
+    void* block = nullptr;      // dc tables, outputs, queue counters, stats
+    size_t bytes = 0;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    size_t n = 0;
+};
+
+hipError_t mx_launch_fe64(const MxRefDev& R, const int2* lev, const MxTileDev& T, int blocks, int threads, hipStream_t stream);

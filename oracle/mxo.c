@@ -1,0 +1,1274 @@
+/* ORACLE — test infrastructure only.
+ *
+ * CPU restatement of the reference engine's per-pixel perturbation path
+ * (src/engine/mandel_perturbation.cpp, "MP.cpp" below) used as the checker for the
+ * HIP kernels.  Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg
+ * may load it; the product never links it.
+ *
+ * Contents (each function cites the reference lines it follows):
+ *   - frame grid and reference point in MPFR            MP.cpp:1403-1424, 4020-4074
+ *   - reference orbit (Z_{i+1} = Z_i^2 + C) in MPFR      MP.cpp:4014-4187, 4189-4271
+ *   - BLA table build                                    MP.cpp:2613-2709
+ *   - floatexp BLA apply (tryBLAfe)                      MP.cpp:2883-2970
+ *   - cold underflow helpers                             MP.cpp:2770-2876
+ *   - per-pixel rescaled perturbation (pixelRescaled)    MP.cpp:3287-3618
+ *   - plain-double perturbation loop + tryBLA + SA seed  MP.cpp:2291-2584, 2714-2755
+ *   - brute-force MPFR pixel (accuratePointCompute)      MP.cpp:719-894
+ *
+ * Where each piece is pinned (tests/test_oracle_*.py):
+ *   FloatExp ops       exact rational arithmetic on random inputs; the reference header
+ *                      itself compiled into oracle/_ref where g++ builds it
+ *   orbit              shadows equal the correctly-truncated MPFR value at 4x precision
+ *   BLA tables         one merged entry == composition of its single steps (A,B) in
+ *                      exact rationals; radius formula on hand-computed cases
+ *   pixel loops        invariant z = X_m + S*w tracked against the MPFR brute force;
+ *                      escape counts/classes vs the reference's golden frames
+ *                      (tests/golden f32 frames) and vs the MPFR brute force on small frames
+ *   brute force        closed forms: c=i Misiurewicz frame all-exterior, c in main
+ *                      cardioid interior, c=0.25+e escape count vs sqrt law
+ */
+#include <gmp.h>
+#include <mpfr.h>
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+#include <omp.h>
+
+#include "mxo_fe.h"
+
+#define MXO_EXPORT __attribute__((visibility("default")))
+
+/* ------------------------------------------------------------------------- */
+/* exact conversions                                                          */
+/* ------------------------------------------------------------------------- */
+
+/* mpf_to_fe / mpf_get_ld (MP.cpp:69-74, float_math.cpp:15-19): truncate the
+ * high-precision value toward zero to a 53-bit mantissa in [0.5,1). */
+static mxo_fe mpfr_to_fe_trunc(mpfr_srcptr x) {
+    if (mpfr_zero_p(x)) return mxo_fe_make(0.0, 0);
+    long ex;
+    double d = mpfr_get_d_2exp(&ex, x, MPFR_RNDZ);
+    /* mpfr_get_d_2exp returns |d| in [0.5,1) */
+    return mxo_fe_make(d, (int64_t)ex);
+}
+
+static double mpfr_to_ld_trunc(mpfr_srcptr x) {
+    if (mpfr_zero_p(x)) return 0.0;
+    long ex;
+    double d = mpfr_get_d_2exp(&ex, x, MPFR_RNDZ);
+    return ldexp(d, (int)ex);
+}
+
+/* ------------------------------------------------------------------------- */
+/* frame grid (MP.cpp:1403-1424)                                              */
+/* ------------------------------------------------------------------------- */
+
+typedef struct {
+    mpfr_t c0_re, c0_im, dx, dy;
+    mpfr_prec_t prec;
+} mxo_grid;
+
+static void grid_init(mxo_grid* g, const char* cre, const char* cim, const char* scale, int W, int H, int full_h, int row_base, mpfr_prec_t prec) {
+    g->prec = prec;
+    mpfr_inits2(prec, g->c0_re, g->c0_im, g->dx, g->dy, (mpfr_ptr)0);
+    mpfr_t c_re, c_im, sc, dw, dh;
+    mpfr_inits2(prec, c_re, c_im, sc, dw, dh, (mpfr_ptr)0);
+    mpfr_set_str(c_re, cre, 10, MPFR_RNDN);
+    mpfr_set_str(c_im, cim, 10, MPFR_RNDN);
+    mpfr_set_str(sc, scale, 10, MPFR_RNDN);
+    int rows = full_h > 0 ? full_h : H;
+    mpfr_ui_div(dw, 2, sc, MPFR_RNDN);   /* dw = 2/scale */
+    mpfr_div_ui(dh, dw, W, MPFR_RNDN);   /* dh = dw/W*rows */
+    mpfr_mul_ui(dh, dh, rows, MPFR_RNDN);
+    mpfr_sub(g->c0_re, c_re, dw, MPFR_RNDN);
+    mpfr_sub(g->c0_im, c_im, dh, MPFR_RNDN);
+    mpfr_div_ui(g->dx, dw, W - 1, MPFR_RNDN);
+    mpfr_mul_ui(g->dx, g->dx, 2, MPFR_RNDN);
+    mpfr_div_ui(g->dy, dh, rows - 1, MPFR_RNDN);
+    mpfr_mul_ui(g->dy, g->dy, 2, MPFR_RNDN);
+    if (row_base > 0) {
+        mpfr_t off;
+        mpfr_init2(off, prec);
+        mpfr_mul_ui(off, g->dy, row_base, MPFR_RNDN);
+        mpfr_add(g->c0_im, g->c0_im, off, MPFR_RNDN);
+        mpfr_clear(off);
+    }
+    mpfr_clears(c_re, c_im, sc, dw, dh, (mpfr_ptr)0);
+}
+
+static void grid_clear(mxo_grid* g) { mpfr_clears(g->c0_re, g->c0_im, g->dx, g->dy, (mpfr_ptr)0); }
+
+/* c of pixel (row i, col j) with sub-sample offsets (u,v); MP.cpp:896-915 */
+static void grid_pixel_c(const mxo_grid* g, int i, int j, int u, int v, int sub, mpfr_ptr cre, mpfr_ptr cim) {
+    mpfr_t t;
+    mpfr_init2(t, g->prec);
+    mpfr_mul_ui(t, g->dx, j, MPFR_RNDN);
+    mpfr_add(cre, g->c0_re, t, MPFR_RNDN);
+    mpfr_mul_si(t, g->dx, v, MPFR_RNDN);
+    mpfr_div_ui(t, t, sub, MPFR_RNDN);
+    mpfr_add(cre, cre, t, MPFR_RNDN);
+    mpfr_mul_ui(t, g->dy, i, MPFR_RNDN);
+    mpfr_add(cim, g->c0_im, t, MPFR_RNDN);
+    mpfr_mul_si(t, g->dy, u, MPFR_RNDN);
+    mpfr_div_ui(t, t, sub, MPFR_RNDN);
+    mpfr_add(cim, cim, t, MPFR_RNDN);
+    mpfr_clear(t);
+}
+
+/* exact geometric centre c0 + d*(n-1)/2 (MP.cpp:4024-4036) */
+static void grid_view_center(const mxo_grid* g, int W, int H, mpfr_ptr cre, mpfr_ptr cim) {
+    mpfr_t t;
+    mpfr_init2(t, g->prec);
+    mpfr_mul_ui(t, g->dx, W - 1, MPFR_RNDN);
+    mpfr_div_ui(t, t, 2, MPFR_RNDN);
+    mpfr_add(cre, g->c0_re, t, MPFR_RNDN);
+    mpfr_mul_ui(t, g->dy, H - 1, MPFR_RNDN);
+    mpfr_div_ui(t, t, 2, MPFR_RNDN);
+    mpfr_add(cim, g->c0_im, t, MPFR_RNDN);
+    mpfr_clear(t);
+}
+
+/* ------------------------------------------------------------------------- */
+/* reference orbit (MP.cpp:4014-4187 createRef, 4189-4271 calCoefficient)     */
+/* ------------------------------------------------------------------------- */
+
+typedef struct {
+    int len;      /* entries filled: 0..ref_it (ref_it+1 entries) */
+    int ref_it;   /* escape index i (Z[i] escaped) or mxit */
+    int escaped;  /* reference escaped before mxit */
+    double* zf;   /* interleaved (re,im) double shadow, zf[2k..2k+1] = X_{k+1} */
+    mxo_fe* zre_fe;
+    mxo_fe* zim_fe;
+    double* dfr;  /* reference derivative shadow (EDE) */
+    double* dfi;
+    double* z_m3; /* |Z|^2/1e6 (double path glitch metric) */
+    mpfr_t last_re, last_im; /* full-precision tail Z[ref_it] */
+    mpfr_t c_re, c_im;
+    mpfr_prec_t prec;
+} mxo_orbit;
+
+/* Serial full-precision orbit. Escape test on the double shadow with hypot, as
+ * std::abs(Comp) > R (MP.cpp:2047-2049, 4272).  The derivative shadow follows the
+ * floatexp branch of MP.cpp:4255-4263 when fe_deriv, else the double branch
+ * MP.cpp:4265-4267. */
+static void orbit_build(mxo_orbit* o, mpfr_srcptr cre, mpfr_srcptr cim, mpfr_prec_t prec, int mxit, double escape_radius, int deriv, int fe_deriv) {
+    o->prec = prec;
+    o->zf = (double*)malloc(sizeof(double) * 2 * (size_t)(mxit + 1));
+    o->zre_fe = (mxo_fe*)malloc(sizeof(mxo_fe) * (size_t)(mxit + 1));
+    o->zim_fe = (mxo_fe*)malloc(sizeof(mxo_fe) * (size_t)(mxit + 1));
+    o->dfr = (double*)calloc((size_t)(mxit + 1), sizeof(double));
+    o->dfi = (double*)calloc((size_t)(mxit + 1), sizeof(double));
+    o->z_m3 = (double*)malloc(sizeof(double) * (size_t)(mxit + 1));
+    mpfr_inits2(prec, o->last_re, o->last_im, o->c_re, o->c_im, (mpfr_ptr)0);
+    mpfr_set(o->c_re, cre, MPFR_RNDN);
+    mpfr_set(o->c_im, cim, MPFR_RNDN);
+    mpfr_t zr, zi, t1, t2, nzr, nzi;
+    mpfr_inits2(prec, zr, zi, t1, t2, nzr, nzi, (mpfr_ptr)0);
+    mpfr_set(zr, cre, MPFR_RNDN);
+    mpfr_set(zi, cim, MPFR_RNDN);
+    o->zf[0] = mpfr_to_ld_trunc(zr);
+    o->zf[1] = mpfr_to_ld_trunc(zi);
+    o->zre_fe[0] = mpfr_to_fe_trunc(zr);
+    o->zim_fe[0] = mpfr_to_fe_trunc(zi);
+    o->dfr[0] = 1.0;
+    o->dfi[0] = 0.0;
+    o->z_m3[0] = 0.0;
+    mxo_fe der = mxo_fe_make(1.0, 0), dei = mxo_fe_make(0.0, 0);
+    o->escaped = 0;
+    o->ref_it = mxit;
+    int i;
+    for (i = 1; i <= mxit; ++i) {
+        mpfr_add(t1, zr, zi, MPFR_RNDN); /* a+b */
+        mpfr_sub(t2, zr, zi, MPFR_RNDN); /* a-b */
+        mpfr_mul(nzi, zr, zi, MPFR_RNDN);
+        mpfr_mul(nzr, t1, t2, MPFR_RNDN);
+        mpfr_mul_2ui(nzi, nzi, 1, MPFR_RNDN);
+        mpfr_add(zr, nzr, cre, MPFR_RNDN);
+        mpfr_add(zi, nzi, cim, MPFR_RNDN);
+        double dr = mpfr_to_ld_trunc(zr), di = mpfr_to_ld_trunc(zi);
+        o->zf[2 * i] = dr;
+        o->zf[2 * i + 1] = di;
+        o->zre_fe[i] = mpfr_to_fe_trunc(zr);
+        o->zim_fe[i] = mpfr_to_fe_trunc(zi);
+        o->z_m3[i] = (dr * dr + di * di) / 1000000;
+        if (deriv) {
+            if (fe_deriv) {
+                mxo_fe a = der, b = dei, ar = o->zre_fe[i - 1], ai = o->zim_fe[i - 1];
+                mxo_fe re = mxo_fe_add(mxo_fe_scale2(mxo_fe_sub(mxo_fe_mul(a, ar), mxo_fe_mul(b, ai)), 1), mxo_fe_make(1.0, 0));
+                mxo_fe im = mxo_fe_scale2(mxo_fe_add(mxo_fe_mul(a, ai), mxo_fe_mul(b, ar)), 1);
+                der = re;
+                dei = im;
+                o->dfr[i] = mxo_fe_to_double(re);
+                o->dfi[i] = mxo_fe_to_double(im);
+            } else {
+                double a = o->dfr[i - 1], b = o->dfi[i - 1], xr = o->zf[2 * (i - 1)], xi = o->zf[2 * (i - 1) + 1];
+                o->dfr[i] = 2.0 * (a * xr - b * xi) + 1.0;
+                o->dfi[i] = 2.0 * (a * xi + b * xr);
+            }
+        }
+        if (hypot(dr, di) > escape_radius) {
+            o->escaped = 1;
+            o->ref_it = i;
+            break;
+        }
+    }
+    o->len = o->ref_it + 1;
+    mpfr_set(o->last_re, zr, MPFR_RNDN);
+    mpfr_set(o->last_im, zi, MPFR_RNDN);
+    mpfr_clears(zr, zi, t1, t2, nzr, nzi, (mpfr_ptr)0);
+}
+
+static void orbit_free(mxo_orbit* o) {
+    free(o->zf);
+    free(o->zre_fe);
+    free(o->zim_fe);
+    free(o->dfr);
+    free(o->dfi);
+    free(o->z_m3);
+    mpfr_clears(o->last_re, o->last_im, o->c_re, o->c_im, (mpfr_ptr)0);
+}
+
+/* refTailEscapes (MP.cpp:927-957): continue the orbit `extra` steps from its tail */
+static int orbit_tail_escapes(const mxo_orbit* o, int extra, double esc2) {
+    mpfr_t zr, zi, t1, t2;
+    mpfr_inits2(o->prec, zr, zi, t1, t2, (mpfr_ptr)0);
+    mpfr_set(zr, o->last_re, MPFR_RNDN);
+    mpfr_set(zi, o->last_im, MPFR_RNDN);
+    int esc = 0;
+    for (int k = 0; k < extra && !esc; ++k) {
+        mpfr_mul(t1, zr, zi, MPFR_RNDN);
+        mpfr_mul(zr, zr, zr, MPFR_RNDN);
+        mpfr_mul(t2, zi, zi, MPFR_RNDN);
+        mpfr_sub(zr, zr, t2, MPFR_RNDN);
+        mpfr_add(zr, zr, o->c_re, MPFR_RNDN);
+        mpfr_mul_2ui(zi, t1, 1, MPFR_RNDN);
+        mpfr_add(zi, zi, o->c_im, MPFR_RNDN);
+        mpfr_mul(t1, zr, zr, MPFR_RNDN);
+        mpfr_mul(t2, zi, zi, MPFR_RNDN);
+        mpfr_add(t1, t1, t2, MPFR_RNDN);
+        if (mpfr_get_d(t1, MPFR_RNDZ) > esc2) esc = 1;
+    }
+    mpfr_clears(zr, zi, t1, t2, (mpfr_ptr)0);
+    return esc;
+}
+
+/* ------------------------------------------------------------------------- */
+/* brute-force pixel (accuratePointCompute, MP.cpp:719-894)                    */
+/* ------------------------------------------------------------------------- */
+
+/* Returns the value the reference writes (smooth, EDE, or -2 interior) as double;
+ * *n_out = escape index (i+1) or -1 (ran to mxit) / -2 (|d|^2 < 1e-9 early out). */
+static double point_mpfr(mpfr_srcptr c_re, mpfr_srcptr c_im, mpfr_prec_t wp, int mxit, int ede, int track_d, double esc2, mpfr_srcptr dx, int* n_out) {
+    mpfr_t zr, zi, dr, di, t1, t2, er, ei, e1, e2;
+    mpfr_inits2(wp, zr, zi, dr, di, t1, t2, er, ei, e1, e2, (mpfr_ptr)0);
+    mpfr_set(zr, c_re, MPFR_RNDN);
+    mpfr_set(zi, c_im, MPFR_RNDN);
+    mpfr_set_ui(dr, 2, MPFR_RNDN);
+    mpfr_set_ui(di, 0, MPFR_RNDN);
+    mpfr_set_ui(er, 1, MPFR_RNDN);
+    mpfr_set_ui(ei, 0, MPFR_RNDN);
+    const double LG = log(2);
+    double res = -2.0;
+    int nres = -1;
+    int i = 1;
+    while (i < mxit) {
+        if (track_d) { /* d = 2 d z */
+            mpfr_mul(t1, dr, zi, MPFR_RNDN);
+            mpfr_mul(dr, dr, zr, MPFR_RNDN);
+            mpfr_mul(t2, di, zi, MPFR_RNDN);
+            mpfr_mul(di, di, zr, MPFR_RNDN);
+            mpfr_sub(dr, dr, t2, MPFR_RNDN);
+            mpfr_mul_2ui(dr, dr, 1, MPFR_RNDN);
+            mpfr_add(di, di, t1, MPFR_RNDN);
+            mpfr_mul_2ui(di, di, 1, MPFR_RNDN);
+        }
+        if (ede) { /* dc = 2 dc z + 1 */
+            mpfr_mul(e1, er, zi, MPFR_RNDN);
+            mpfr_mul(e2, ei, zr, MPFR_RNDN);
+            mpfr_add(e2, e1, e2, MPFR_RNDN);
+            mpfr_mul(e1, er, zr, MPFR_RNDN);
+            mpfr_mul(t2, ei, zi, MPFR_RNDN);
+            mpfr_sub(e1, e1, t2, MPFR_RNDN);
+            mpfr_mul_2ui(er, e1, 1, MPFR_RNDN);
+            mpfr_add_ui(er, er, 1, MPFR_RNDN);
+            mpfr_mul_2ui(ei, e2, 1, MPFR_RNDN);
+        }
+        /* z = z^2 + c */
+        mpfr_mul(t1, zr, zi, MPFR_RNDN);
+        mpfr_mul(zr, zr, zr, MPFR_RNDN);
+        mpfr_mul(zi, zi, zi, MPFR_RNDN);
+        mpfr_sub(zr, zr, zi, MPFR_RNDN);
+        mpfr_add(zr, zr, c_re, MPFR_RNDN);
+        mpfr_mul_2ui(zi, t1, 1, MPFR_RNDN);
+        mpfr_add(zi, zi, c_im, MPFR_RNDN);
+        mpfr_mul(t1, zr, zr, MPFR_RNDN);
+        mpfr_mul(t2, zi, zi, MPFR_RNDN);
+        mpfr_add(t1, t1, t2, MPFR_RNDN);
+        double rad = mpfr_get_d(t1, MPFR_RNDZ); /* mpf_get_d truncates */
+        if (rad > esc2) {
+            nres = i + 1;
+            if (ede) {
+                mpfr_mul(e1, er, er, MPFR_RNDN);
+                mpfr_mul(e2, ei, ei, MPFR_RNDN);
+                mpfr_add(e1, e1, e2, MPFR_RNDN);
+                mpfr_sqrt(e1, e1, MPFR_RNDN);
+                mpfr_abs(t1, dx, MPFR_RNDN);
+                mpfr_mul(e1, e1, t1, MPFR_RNDN);
+                double num = isfinite(rad) && rad > 0.0 ? sqrt(rad) * log(rad) : 0.0;
+                mpfr_set_d(e2, num, MPFR_RNDN);
+                if (!mpfr_zero_p(e1)) {
+                    mpfr_div(e1, e2, e1, MPFR_RNDN);
+                    res = mpfr_get_d(e1, MPFR_RNDZ);
+                } else
+                    res = 0.0;
+            } else {
+                res = (double)(i + 1) - log(log(rad) / 2 / LG) / LG;
+            }
+            break;
+        }
+        if (track_d) {
+            mpfr_mul(t1, dr, dr, MPFR_RNDN);
+            mpfr_mul(t2, di, di, MPFR_RNDN);
+            mpfr_add(t1, t1, t2, MPFR_RNDN);
+            double r = mpfr_get_d(t1, MPFR_RNDZ);
+            if (r < 0.000000001) {
+                nres = -2;
+                break;
+            }
+        }
+        ++i;
+    }
+    mpfr_clears(zr, zi, dr, di, t1, t2, er, ei, e1, e2, (mpfr_ptr)0);
+    *n_out = nres;
+    return res;
+}
+
+/* ------------------------------------------------------------------------- */
+/* BLA table (MP.cpp:2613-2709)                                               */
+/* ------------------------------------------------------------------------- */
+
+#define MXO_MAXLEV 40
+
+typedef struct {
+    int nlev;
+    int64_t off[MXO_MAXLEV];
+    int64_t size[MXO_MAXLEV];
+    int64_t total;
+    double* r2;   /* per entry */
+    int64_t* l;   /* per entry, skip length */
+    double* coef; /* 4 per entry: ar ai br bi */
+    double* der;  /* 4 per entry: cr ci er ei (ede only) */
+    double rmax2;
+    double dcmax2;
+} mxo_bla;
+
+/* zfr[s] = zf[2s], s = 1..reflen-1 */
+static void bla_build(mxo_bla* b, const double* zf, const double* dfr, const double* dfi, int reflen, double dcmax, double eps, int ede) {
+    memset(b, 0, sizeof(*b));
+    if (reflen < 3) return;
+    /* sizes: n0 = reflen-1, n_{p+1} = n_p/2 until one entry */
+    int64_t n = reflen - 1, tot = 0;
+    int nl = 0;
+    for (;;) {
+        b->off[nl] = tot;
+        b->size[nl] = n;
+        tot += n;
+        ++nl;
+        if (n <= 1) break;
+        n = n / 2;
+    }
+    b->nlev = nl;
+    b->total = tot;
+    b->r2 = (double*)malloc(sizeof(double) * tot);
+    b->l = (int64_t*)malloc(sizeof(int64_t) * tot);
+    b->coef = (double*)malloc(sizeof(double) * 4 * tot);
+    b->der = ede ? (double*)malloc(sizeof(double) * 4 * tot) : NULL;
+    b->dcmax2 = dcmax * dcmax;
+    b->rmax2 = 0.0;
+    for (int s = 1; s < reflen; ++s) {
+        double zr = zf[2 * s], zi = zf[2 * s + 1];
+        double Zmag = sqrt(zr * zr + zi * zi);
+        double ar = 2.0 * zr, ai = 2.0 * zi;
+        double Amag = 2.0 * Zmag;
+        double R = eps * (Zmag - dcmax) / (Amag + 1.0);
+        double r2 = R > 0 ? R * R : 0.0;
+        if (r2 > b->rmax2) b->rmax2 = r2;
+        int64_t k = s - 1;
+        b->r2[k] = r2;
+        b->l[k] = 1;
+        b->coef[4 * k + 0] = ar;
+        b->coef[4 * k + 1] = ai;
+        b->coef[4 * k + 2] = 1.0;
+        b->coef[4 * k + 3] = 0.0;
+        if (ede) {
+            b->der[4 * k + 0] = 2.0 * dfr[s];
+            b->der[4 * k + 1] = 2.0 * dfi[s];
+            b->der[4 * k + 2] = 0.0;
+            b->der[4 * k + 3] = 0.0;
+        }
+    }
+    for (int p = 1; p < nl; ++p) {
+        int64_t po = b->off[p - 1], no = b->off[p];
+        for (int64_t i = 0; i < b->size[p]; ++i) {
+            int64_t x = po + 2 * i, y = po + 2 * i + 1, z = no + i;
+            const double* xc = b->coef + 4 * x;
+            const double* yc = b->coef + 4 * y;
+            double* zc = b->coef + 4 * z;
+            zc[0] = yc[0] * xc[0] - yc[1] * xc[1];
+            zc[1] = yc[0] * xc[1] + yc[1] * xc[0];
+            zc[2] = yc[0] * xc[2] - yc[1] * xc[3] + yc[2];
+            zc[3] = yc[0] * xc[3] + yc[1] * xc[2] + yc[3];
+            if (ede) {
+                const double* xd = b->der + 4 * x;
+                const double* yd = b->der + 4 * y;
+                double* zd = b->der + 4 * z;
+                zd[0] = (yd[0] * xc[0] - yd[1] * xc[1]) + (yc[0] * xd[0] - yc[1] * xd[1]);
+                zd[1] = (yd[0] * xc[1] + yd[1] * xc[0]) + (yc[0] * xd[1] + yc[1] * xd[0]);
+                zd[2] = (yd[0] * xc[2] - yd[1] * xc[3]) + (yc[0] * xd[2] - yc[1] * xd[3]) + yd[2];
+                zd[3] = (yd[0] * xc[3] + yd[1] * xc[2]) + (yc[0] * xd[3] + yc[1] * xd[2]) + yd[3];
+            }
+            double Rx = sqrt(b->r2[x]), Ry = sqrt(b->r2[y]);
+            double Axmag = sqrt(xc[0] * xc[0] + xc[1] * xc[1]);
+            double Bxmag = sqrt(xc[2] * xc[2] + xc[3] * xc[3]);
+            double cand = Axmag > 0 ? (Ry - Bxmag * dcmax) / Axmag : 0.0;
+            double R = (cand < Rx) ? cand : Rx; /* std::min(Rx, cand): returns Rx unless cand < Rx */
+            b->r2[z] = R > 0 ? R * R : 0.0;
+            b->l[z] = b->l[x] + b->l[y];
+        }
+    }
+}
+
+static void bla_free(mxo_bla* b) {
+    free(b->r2);
+    free(b->l);
+    free(b->coef);
+    free(b->der);
+}
+
+/* ------------------------------------------------------------------------- */
+/* pixel context                                                              */
+/* ------------------------------------------------------------------------- */
+
+typedef struct {
+    const double* zf;
+    const mxo_fe* zre_fe;
+    const mxo_fe* zim_fe;
+    int mx_ref_it;
+    const mxo_bla* bla; /* NULL => BLA off */
+    int bla_minlevel;
+    double esc2;
+    int use_interior;
+    double interior_eps2;
+    int interior_confirm;
+    int int_rep;
+    int ede;
+    mxo_fe dxfe;
+} mxo_ctx;
+
+typedef struct {
+    long long steps, bla_applies, bla_skipped, rebases, rescales, underflow_steps;
+} mxo_stats;
+
+static inline int tzcnt32(unsigned x) { return __builtin_ctz(x); }
+
+/* blaReRescaleUnderflow (MP.cpp:2770-2781) */
+static void bla_rerescale_underflow(mxo_fe* S, double* wr, double* wi, double nwr, double nwi) {
+    if (nwr == 0.0 && nwi == 0.0) {
+        *S = mxo_fe_make(1.0, 0);
+        *wr = *wi = 0.0;
+        return;
+    }
+    mxo_fe wmag = mxo_fe_sqrt(mxo_fe_add(mxo_fe_mul(mxo_fe_from(nwr), mxo_fe_from(nwr)), mxo_fe_mul(mxo_fe_from(nwi), mxo_fe_from(nwi))));
+    *S = mxo_fe_mul(*S, wmag);
+    double inv = 1.0 / mxo_fe_to_double(wmag);
+    *wr = nwr * inv;
+    *wi = nwi * inv;
+}
+
+static inline double pow2_exact(int64_t d) {
+    uint64_t bits = (uint64_t)(d + 1023) << 52;
+    double s;
+    memcpy(&s, &bits, sizeof s);
+    return s;
+}
+
+/* feRescaledStepUnderflow (MP.cpp:2794-2835) */
+static void fe_step_underflow(mxo_fe Xr, mxo_fe Xi, mxo_fe dcr, mxo_fe dci, mxo_fe* S, double* wr, double* wi) {
+    const double sm = S->m, sm2 = sm * sm;
+    const int64_t se = S->e;
+    int64_t e = INT64_MIN;
+    const int has_w = *wr != 0.0 || *wi != 0.0;
+    if (has_w && Xr.m != 0.0 && Xr.e + se > e) e = Xr.e + se;
+    if (has_w && Xi.m != 0.0 && Xi.e + se > e) e = Xi.e + se;
+    if (has_w && se + se > e) e = se + se;
+    if (dcr.m != 0.0 && dcr.e > e) e = dcr.e;
+    if (dci.m != 0.0 && dci.e > e) e = dci.e;
+    if (e == INT64_MIN) {
+        *S = mxo_fe_make(1.0, 0);
+        *wr = *wi = 0.0;
+        return;
+    }
+#define ALIGN_(mm, te) (((mm) == 0.0 || ((te) - e) < -54) ? 0.0 : (mm) * pow2_exact((te) - e))
+    const double xs_r = ALIGN_(Xr.m * sm, Xr.e + se);
+    const double xs_i = ALIGN_(Xi.m * sm, Xi.e + se);
+    const double s2 = ALIGN_(sm2, se + se);
+    const double cr = ALIGN_(dcr.m, dcr.e);
+    const double ci = ALIGN_(dci.m, dci.e);
+#undef ALIGN_
+    const double w_r = *wr, w_i = *wi;
+    const double nr = 2.0 * (xs_r * w_r - xs_i * w_i) + s2 * (w_r * w_r - w_i * w_i) + cr;
+    const double ni = 2.0 * (xs_r * w_i + xs_i * w_r) + 2.0 * s2 * w_r * w_i + ci;
+    const double mag2 = nr * nr + ni * ni;
+    const double mag = mag2 > 0.0 ? sqrt(mag2) : hypot(nr, ni);
+    if (mag == 0.0) {
+        *S = mxo_fe_make(1.0, 0);
+        *wr = *wi = 0.0;
+        return;
+    }
+    *S = mxo_fe_renorm(mag, e);
+    *wr = nr / mag;
+    *wi = ni / mag;
+}
+
+/* feDerivativeStepUnderflow (MP.cpp:2842-2876) */
+static void fe_deriv_underflow(mxo_fe zr, mxo_fe zi, mxo_fe* SJ, double* jr, double* ji, double* invSJd) {
+    double im = 0.5 / SJ->m;
+    int64_t ie = 1 - SJ->e;
+    if (im >= 1.0) {
+        im *= 0.5;
+        ++ie;
+    }
+    mxo_fe invSJ = mxo_fe_make(im, ie);
+    int64_t e = invSJ.e;
+    if (zr.m != 0.0 && zr.e > e) e = zr.e;
+    if (zi.m != 0.0 && zi.e > e) e = zi.e;
+#define ALIGNF_(x) (((x).m == 0.0 || ((x).e - e) < -54) ? 0.0 : (x).m * pow2_exact((x).e - e))
+    double azr = ALIGNF_(zr), azi = ALIGNF_(zi);
+    double ar = 2.0 * (azr * *jr - azi * *ji) + ALIGNF_(invSJ);
+    double ai = 2.0 * (azr * *ji + azi * *jr);
+#undef ALIGNF_
+    double mag = sqrt(ar * ar + ai * ai);
+    if (mag == 0.0) {
+        *SJ = mxo_fe_make(1.0, 0);
+        *jr = *ji = 0.0;
+        *invSJd = 1.0;
+        return;
+    }
+    *SJ = mxo_fe_mul(*SJ, mxo_fe_renorm(mag, e));
+    *jr = ar / mag;
+    *ji = ai / mag;
+    *invSJd = 1.0 / mxo_fe_to_double(*SJ);
+}
+
+/* tryBLAfe (MP.cpp:2883-2970); mx_ref_it here is the caller's reflen. */
+static int try_bla_fe(const mxo_ctx* c, int s, mxo_fe* S, mxo_fe S2, double* wr, double* wi, double dr, double di, int mx_ref_it, double* outAB) {
+    const mxo_bla* b = c->bla;
+    if (s < 1 || !b || b->nlev == 0) return 0;
+    int maxp = b->nlev - 1;
+    int startp = (s == 1) ? maxp : tzcnt32((unsigned)(s - 1));
+    if (startp > maxp) startp = maxp;
+    if (startp < c->bla_minlevel) return 0;
+    mxo_fe dz2 = mxo_fe_mul(S2, mxo_fe_from(*wr * *wr + *wi * *wi));
+    if (b->rmax2 <= 0.0 || !mxo_fe_abs_less(dz2, mxo_fe_from(b->rmax2))) return 0;
+    for (int p = startp; p >= c->bla_minlevel; --p) {
+        int64_t i = (int64_t)((s - 1) >> p);
+        if (i >= b->size[p]) continue;
+        int64_t k = b->off[p] + i;
+        double r2 = b->r2[k];
+        if (r2 <= 0 || !mxo_fe_abs_less(dz2, mxo_fe_from(r2))) continue;
+        int land = s + (int)b->l[k];
+        if (land >= mx_ref_it - 1) continue;
+        const double* bc = b->coef + 4 * k;
+        double nwr = (bc[0] * *wr - bc[1] * *wi) + (bc[2] * dr - bc[3] * di);
+        double nwi = (bc[0] * *wi + bc[1] * *wr) + (bc[2] * di + bc[3] * dr);
+        const double BIG = 1e150;
+        if (isfinite(nwr) && isfinite(nwi) && fabs(nwr) < BIG && fabs(nwi) < BIG) {
+            double fr = c->zf[2 * land] + mxo_fe_to_double(mxo_fe_mul_d(*S, nwr));
+            double fi = c->zf[2 * land + 1] + mxo_fe_to_double(mxo_fe_mul_d(*S, nwi));
+            if (fr * fr + fi * fi > c->esc2) return 0;
+            double wm2 = nwr * nwr + nwi * nwi;
+            if (wm2 == 0.0) {
+                /* non-periodic reference: reject the unrepresentable landing */
+                return 0;
+            } else {
+                double wmag = sqrt(wm2);
+                *S = mxo_fe_mul(*S, mxo_fe_from(wmag));
+                double inv = 1.0 / wmag;
+                *wr = nwr * inv;
+                *wi = nwi * inv;
+            }
+        } else {
+            mxo_fe dzr = mxo_fe_mul_d(*S, *wr), dzi = mxo_fe_mul_d(*S, *wi);
+            mxo_fe dcrfe = mxo_fe_mul_d(*S, dr), dcife = mxo_fe_mul_d(*S, di);
+            mxo_fe nzr = mxo_fe_add(mxo_fe_sub(mxo_fe_mul_d(dzr, bc[0]), mxo_fe_mul_d(dzi, bc[1])), mxo_fe_sub(mxo_fe_mul_d(dcrfe, bc[2]), mxo_fe_mul_d(dcife, bc[3])));
+            mxo_fe nzi = mxo_fe_add(mxo_fe_add(mxo_fe_mul_d(dzr, bc[1]), mxo_fe_mul_d(dzi, bc[0])), mxo_fe_add(mxo_fe_mul_d(dcrfe, bc[3]), mxo_fe_mul_d(dcife, bc[2])));
+            double fr = c->zf[2 * land] + mxo_fe_to_double(nzr), fi = c->zf[2 * land + 1] + mxo_fe_to_double(nzi);
+            if (fr * fr + fi * fi > c->esc2) return 0;
+            mxo_fe Snew = mxo_fe_sqrt(mxo_fe_add(mxo_fe_mul(nzr, nzr), mxo_fe_mul(nzi, nzi)));
+            if (Snew.m == 0.0) {
+                *S = mxo_fe_make(1.0, 0);
+                *wr = *wi = 0.0;
+            } else {
+                *S = Snew;
+                *wr = mxo_fe_to_double(mxo_fe_div(nzr, *S));
+                *wi = mxo_fe_to_double(mxo_fe_div(nzi, *S));
+            }
+        }
+        if (outAB) {
+            outAB[0] = bc[0];
+            outAB[1] = bc[1];
+            outAB[2] = bc[2];
+            outAB[3] = bc[3];
+        }
+        return (int)b->l[k];
+    }
+    return 0;
+}
+
+/* pixelRescaled (MP.cpp:3287-3618), smooth (c_method 0) and EDE outputs.
+ * Returns the double the reference casts to float; *n_out = escape iteration, or
+ * -1 (reached mxit) / -2 (interior detector). */
+static double pixel_rescaled(const mxo_ctx* c, mxo_fe dcr, mxo_fe dci, int mxit, int* n_out, mxo_stats* st) {
+    const double ESC2 = c->esc2;
+    const double LG2 = log(2.0);
+    const double* zfp = c->zf;
+    const int reflen = c->mx_ref_it + 1;
+    int rm = 0;
+    mxo_fe S = mxo_fe_sqrt(mxo_fe_add(mxo_fe_mul(dcr, dcr), mxo_fe_mul(dci, dci)));
+    double wr, wi, dr, di;
+    if (S.m == 0.0) {
+        S = mxo_fe_make(1.0, 0);
+        wr = wi = dr = di = 0.0;
+    } else {
+        wr = mxo_fe_to_double(mxo_fe_div(dcr, S));
+        wi = mxo_fe_to_double(mxo_fe_div(dci, S));
+        dr = wr;
+        di = wi;
+    }
+    double s = mxo_fe_to_double(S);
+    mxo_fe S2 = mxo_fe_mul(S, S);
+    int m = 1, iter = 1;
+    double zsr = 1e30, zsi = 1e30;
+    int save_iter = 1, period_win = 1;
+    int conf_P = 0, conf_next = 0, conf_count = 0, conf_giveup = 0;
+    double conf_D2 = 0, conf_zr = 0, conf_zi = 0;
+    const int ede = c->ede;
+    const int deriv = ede;
+    mxo_fe SJ = mxo_fe_make(1.0, 0);
+    double jr = 1.0, ji = 0.0, invSJd = 1.0;
+    const int detect_interior = c->use_interior;
+    const int int_rep = c->int_rep && detect_interior;
+    double swr = 1e300, swi = 1e300;
+    mxo_fe sS = mxo_fe_make(1e300, 0);
+    int sm = -1, save_iter2 = 1, twin2r = 1;
+
+    while (iter < mxit) {
+        if (int_rep) {
+            if (m == sm && wr == swr && wi == swi && S.m == sS.m && S.e == sS.e) {
+                *n_out = -2;
+                return -2.0;
+            }
+            if (iter - save_iter2 >= twin2r) {
+                swr = wr;
+                swi = wi;
+                sS = S;
+                sm = m;
+                save_iter2 = iter;
+                twin2r += twin2r;
+            }
+        }
+        if (c->bla && m >= 2) {
+            double ab[4];
+            int skip = try_bla_fe(c, rm, &S, S2, &wr, &wi, dr, di, reflen, deriv ? ab : NULL);
+            if (skip > 0) {
+                if (deriv) {
+                    mxo_fe Jr = mxo_fe_mul_d(SJ, jr), Ji = mxo_fe_mul_d(SJ, ji);
+                    mxo_fe nJr = mxo_fe_add(mxo_fe_sub(mxo_fe_mul_d(Jr, ab[0]), mxo_fe_mul_d(Ji, ab[1])), mxo_fe_from(ab[2]));
+                    mxo_fe nJi = mxo_fe_add(mxo_fe_add(mxo_fe_mul_d(Jr, ab[1]), mxo_fe_mul_d(Ji, ab[0])), mxo_fe_from(ab[3]));
+                    SJ = mxo_fe_sqrt(mxo_fe_add(mxo_fe_mul(nJr, nJr), mxo_fe_mul(nJi, nJi)));
+                    if (SJ.m == 0.0) {
+                        SJ = mxo_fe_make(1.0, 0);
+                        jr = ji = 0.0;
+                        invSJd = 1.0;
+                    } else {
+                        jr = mxo_fe_to_double(mxo_fe_div(nJr, SJ));
+                        ji = mxo_fe_to_double(mxo_fe_div(nJi, SJ));
+                        invSJd = 1.0 / mxo_fe_to_double(SJ);
+                    }
+                }
+                m += skip;
+                iter += skip;
+                rm += skip;
+                st->bla_applies++;
+                st->bla_skipped += skip;
+                s = mxo_fe_to_double(S);
+                S2 = mxo_fe_mul(S, S);
+                dr = mxo_fe_to_double(mxo_fe_div(dcr, S));
+                di = mxo_fe_to_double(mxo_fe_div(dci, S));
+                zsr = 1e30;
+                zsi = 1e30;
+                save_iter = iter;
+                period_win = 1;
+                conf_P = 0;
+                conf_giveup = 0;
+                continue;
+            }
+        }
+        double Xr = m ? zfp[2 * rm] : 0.0, Xi = m ? zfp[2 * rm + 1] : 0.0;
+        double zmr = Xr + s * wr, zmi = Xi + s * wi;
+        const int deriv_underflow = deriv && zmr == 0.0 && zmi == 0.0 && invSJd == 0.0;
+        mxo_fe zmrfe = mxo_fe_make(0.0, 0), zmife = mxo_fe_make(0.0, 0);
+        if (deriv_underflow) {
+            mxo_fe Xrfe = m ? c->zre_fe[rm] : mxo_fe_make(0.0, 0);
+            mxo_fe Xife = m ? c->zim_fe[rm] : mxo_fe_make(0.0, 0);
+            zmrfe = mxo_fe_add(Xrfe, mxo_fe_mul_d(S, wr));
+            zmife = mxo_fe_add(Xife, mxo_fe_mul_d(S, wi));
+        }
+        double nwr, nwi;
+        {
+            const double fR = Xr + Xr + s * wr;
+            const double fI = Xi + Xi + s * wi;
+            nwr = fR * wr - fI * wi + dr;
+            nwi = fR * wi + fI * wr + di;
+        }
+        const int step_underflow = fabs(nwr) < 0x1p-1056 && fabs(nwi) < 0x1p-1056;
+        if (step_underflow && (nwr != 0.0 || nwi != 0.0 || wr != 0.0 || wi != 0.0 || dcr.m != 0.0 || dci.m != 0.0)) {
+            mxo_fe Xrfe = m ? c->zre_fe[rm] : mxo_fe_make(0.0, 0);
+            mxo_fe Xife = m ? c->zim_fe[rm] : mxo_fe_make(0.0, 0);
+            fe_step_underflow(Xrfe, Xife, dcr, dci, &S, &wr, &wi);
+            s = mxo_fe_to_double(S);
+            S2 = mxo_fe_mul(S, S);
+            dr = mxo_fe_to_double(mxo_fe_div(dcr, S));
+            di = mxo_fe_to_double(mxo_fe_div(dci, S));
+            st->underflow_steps++;
+        } else {
+            wr = nwr;
+            wi = nwi;
+        }
+        ++m;
+        ++iter;
+        ++rm;
+        st->steps++;
+        if (deriv) {
+            if (deriv_underflow) {
+                fe_deriv_underflow(zmrfe, zmife, &SJ, &jr, &ji, &invSJd);
+            } else {
+                double a = 2.0 * (zmr * jr - zmi * ji) + invSJd;
+                double bb = 2.0 * (zmr * ji + zmi * jr);
+                jr = a;
+                ji = bb;
+                double jm2 = jr * jr + ji * ji;
+                if (jm2 > 1e16 || (jm2 > 0.0 && jm2 < 1e-16)) {
+                    mxo_fe jmag = mxo_fe_sqrt(mxo_fe_from(jm2));
+                    SJ = mxo_fe_mul(SJ, jmag);
+                    double inv = 1.0 / mxo_fe_to_double(jmag);
+                    jr *= inv;
+                    ji *= inv;
+                    invSJd = 1.0 / mxo_fe_to_double(SJ);
+                }
+            }
+        }
+        Xr = m ? zfp[2 * rm] : 0.0;
+        Xi = m ? zfp[2 * rm + 1] : 0.0;
+        double zr = Xr + s * wr, zi = Xi + s * wi;
+        double zrad = zr * zr + zi * zi;
+        if (zrad > ESC2) {
+            *n_out = iter;
+            if (ede) {
+                double num = sqrt(zrad) * log(zrad);
+                mxo_fe Jmag = mxo_fe_mul_d(SJ, sqrt(jr * jr + ji * ji));
+                return mxo_fe_to_double(mxo_fe_div(mxo_fe_from(num), mxo_fe_mul(c->dxfe, Jmag)));
+            }
+            return (double)iter - log(log(zrad) / 2.0 / LG2) / LG2;
+        }
+        if (detect_interior) {
+            if (conf_P > 0) {
+                conf_D2 *= 4.0 * zrad;
+                if (conf_D2 > 1e18) conf_D2 = 1e18;
+                if (iter >= conf_next) {
+                    double pr = zr - conf_zr, pi = zi - conf_zi;
+                    if (pr * pr + pi * pi < c->interior_eps2 * zrad && conf_D2 < 1.0) {
+                        if (++conf_count >= c->interior_confirm) {
+                            *n_out = -2;
+                            return -2.0;
+                        }
+                        conf_zr = zr;
+                        conf_zi = zi;
+                        conf_D2 = 1;
+                        conf_next = iter + conf_P;
+                    } else {
+                        conf_P = 0;
+                        ++conf_giveup;
+                    }
+                }
+            } else if ((iter & 15) == 0 && conf_giveup < 3) {
+                double pr = zr - zsr, pi = zi - zsi;
+                if (pr * pr + pi * pi < c->interior_eps2 * zrad) {
+                    conf_P = iter - save_iter;
+                    if (conf_P < 1) conf_P = 1;
+                    conf_zr = zr;
+                    conf_zi = zi;
+                    conf_D2 = 1;
+                    conf_next = iter + conf_P;
+                    conf_count = 0;
+                }
+                if (iter - save_iter >= period_win) {
+                    zsr = zr;
+                    zsi = zi;
+                    save_iter = iter;
+                    period_win += period_win;
+                }
+            }
+        }
+        double wmag2 = wr * wr + wi * wi;
+        if (wmag2 > 1e16 || (wmag2 < 1e-16 && wmag2 > 0.0)) {
+            mxo_fe wmag = mxo_fe_sqrt(mxo_fe_from(wmag2));
+            S = mxo_fe_mul(S, wmag);
+            s = mxo_fe_to_double(S);
+            S2 = mxo_fe_mul(S, S);
+            double inv = 1.0 / mxo_fe_to_double(wmag);
+            wr *= inv;
+            wi *= inv;
+            dr = mxo_fe_to_double(mxo_fe_div(dcr, S));
+            di = mxo_fe_to_double(mxo_fe_div(dci, S));
+            st->rescales++;
+        }
+        double dzmag2 = s * s * (wr * wr + wi * wi);
+        if (zrad < 1e-8 || zrad < dzmag2 || m >= reflen) {
+            mxo_fe Xmr = m ? c->zre_fe[rm] : mxo_fe_make(0.0, 0);
+            mxo_fe Xmi = m ? c->zim_fe[rm] : mxo_fe_make(0.0, 0);
+            mxo_fe Swr = mxo_fe_mul_d(S, wr), Swi = mxo_fe_mul_d(S, wi);
+            mxo_fe zrfe = mxo_fe_add(Xmr, Swr), zife = mxo_fe_add(Xmi, Swi);
+            mxo_fe zradfe = mxo_fe_add(mxo_fe_mul(zrfe, zrfe), mxo_fe_mul(zife, zife));
+            mxo_fe dzfe = mxo_fe_add(mxo_fe_mul(Swr, Swr), mxo_fe_mul(Swi, Swi));
+            if (m >= reflen || mxo_fe_abs_less(zradfe, dzfe)) {
+                mxo_fe Snew = mxo_fe_sqrt(zradfe);
+                if (Snew.m == 0.0) {
+                    wr = wi = 0.0;
+                } else {
+                    S = Snew;
+                    s = mxo_fe_to_double(S);
+                    S2 = mxo_fe_mul(S, S);
+                    wr = mxo_fe_to_double(mxo_fe_div(zrfe, S));
+                    wi = mxo_fe_to_double(mxo_fe_div(zife, S));
+                    dr = mxo_fe_to_double(mxo_fe_div(dcr, S));
+                    di = mxo_fe_to_double(mxo_fe_div(dci, S));
+                }
+                m = 0;
+                rm = -1;
+                st->rebases++;
+            }
+        }
+    }
+    *n_out = -1;
+    return -2.0;
+}
+
+/* ------------------------------------------------------------------------- */
+/* frame (floatexp path): Compute -> buildReferenceOrbit -> buildBLA -> probe  */
+/* -> main pass (MP.cpp:1520-1761, 2128-2229)                                  */
+/* ------------------------------------------------------------------------- */
+
+typedef struct {
+    /* inputs */
+    const char* cre;
+    const char* cim;
+    const char* scale;
+    int W, H, full_h, row_base;
+    int mxit;
+    int c_method;        /* 0 smooth, 2 EDE */
+    float escape_radius; /* the engine's float radius, default 1e8f */
+    int prec_bits;
+    int use_bla;
+    int bla_minskip;     /* MANDEL_BLA_MINSKIP, default 8 */
+    double bla_eps;      /* 0 => 2^-53 */
+    int interior;        /* MANDEL_INTERIOR default 1 */
+    int probe;           /* run the 8x8 interior-gate probe (MANDEL_COARSE default 1) */
+    int threads;
+    /* outputs (W*H each) */
+    double* val;
+    int32_t* n_esc;
+    float* out;
+    /* reference info outputs */
+    int ref_it;
+    int ref_escaped;
+    int ref_virtual;
+    double ref_x, ref_y;
+    int interior_used;
+    int cutoff_sensitive_hint;
+    double seconds_pixels;
+    long long steps, bla_applies, bla_skipped, rebases;
+} mxo_frame;
+
+static int bla_minlevel_from_skip(int ms) {
+    int l = 0;
+    while ((1 << (l + 1)) <= ms) ++l;
+    return l;
+}
+
+/* reference pixel value for a pixel (non-virtual) reference (MP.cpp:4145-4160, 4179-4185) */
+static double ref_pixel_value(const mxo_orbit* o, int mxit, int c_method, double esc2, mpfr_srcptr dx, int* n_out) {
+    if (!o->escaped) {
+        *n_out = -1;
+        return -2.0;
+    }
+    if (c_method) return point_mpfr(o->c_re, o->c_im, o->prec, mxit, (c_method & 2) != 0, 1, esc2, dx, n_out);
+    mpfr_t t1, t2;
+    mpfr_inits2(o->prec, t1, t2, (mpfr_ptr)0);
+    mpfr_mul(t1, o->last_re, o->last_re, MPFR_RNDN);
+    mpfr_mul(t2, o->last_im, o->last_im, MPFR_RNDN);
+    mpfr_add(t1, t1, t2, MPFR_RNDN);
+    double rad = mpfr_get_d(t1, MPFR_RNDZ);
+    mpfr_clears(t1, t2, (mpfr_ptr)0);
+    int i = o->ref_it;
+    *n_out = i + 1;
+    return (i + 1 - log(log(rad) / 2 / log(2)) / log(2));
+}
+
+static void run_pixels(const mxo_ctx* ctx, const mxo_frame* f, const mxo_fe* dcol, const mxo_fe* drow, const int* idx, int n, double* val, int32_t* nesc, mxo_stats* tot) {
+    long long steps = 0, ap = 0, sk = 0, rb = 0;
+#pragma omp parallel for schedule(dynamic, 64) num_threads(f->threads) reduction(+ : steps, ap, sk, rb)
+    for (int q = 0; q < n; ++q) {
+        int p = idx[q];
+        int i = p / f->W, j = p % f->W;
+        mxo_stats st = {0, 0, 0, 0, 0, 0};
+        int ne;
+        double v = pixel_rescaled(ctx, dcol[j], drow[i], f->mxit, &ne, &st);
+        val[p] = v;
+        nesc[p] = ne;
+        steps += st.steps;
+        ap += st.bla_applies;
+        sk += st.bla_skipped;
+        rb += st.rebases;
+    }
+    tot->steps += steps;
+    tot->bla_applies += ap;
+    tot->bla_skipped += sk;
+    tot->rebases += rb;
+}
+
+MXO_EXPORT int mxo_compute_fe(mxo_frame* f) {
+    const int W = f->W, H = f->H;
+    const double esc2 = (double)(f->escape_radius * f->escape_radius);
+    const double escR = (double)f->escape_radius;
+    mxo_grid g;
+    grid_init(&g, f->cre, f->cim, f->scale, W, H, f->full_h, f->row_base, f->prec_bits);
+    mxo_fe dxfe = mpfr_to_fe_trunc(g.dx), dyfe = mpfr_to_fe_trunc(g.dy);
+    const int ede = (f->c_method & 2) != 0;
+
+    /* reference at the centre pixel (H/2, W/2) */
+    mpfr_t rre, rim;
+    mpfr_inits2(f->prec_bits, rre, rim, (mpfr_ptr)0);
+    grid_pixel_c(&g, H / 2, W / 2, 0, 0, 1, rre, rim);
+    mxo_orbit o;
+    orbit_build(&o, rre, rim, f->prec_bits, f->mxit, escR, ede, 1);
+    int ref_virtual = 0;
+    double ref_x = W / 2, ref_y = H / 2;
+    f->cutoff_sensitive_hint = 0;
+    if (!o.escaped || o.ref_it >= f->mxit - 16) {
+        int sensitive = o.escaped || orbit_tail_escapes(&o, 80, esc2);
+        if (sensitive) {
+            orbit_free(&o);
+            grid_view_center(&g, W, H, rre, rim);
+            orbit_build(&o, rre, rim, f->prec_bits, f->mxit, escR, ede, 1);
+            ref_virtual = 1;
+            ref_x = (W - 1) * 0.5;
+            ref_y = (H - 1) * 0.5;
+            if (o.escaped && o.ref_it >= f->mxit - 16) f->cutoff_sensitive_hint = 1;
+            if (!o.escaped) f->cutoff_sensitive_hint = 2; /* needs a brute-force probe at mxit+64 */
+        }
+    }
+    f->ref_it = o.ref_it;
+    f->ref_escaped = o.escaped;
+    f->ref_virtual = ref_virtual;
+    f->ref_x = ref_x;
+    f->ref_y = ref_y;
+
+    mxo_bla bla;
+    memset(&bla, 0, sizeof bla);
+    if (f->use_bla) {
+        double dxf = fabs(mpfr_to_ld_trunc(g.dx)), dyf = fabs(mpfr_to_ld_trunc(g.dy));
+        double a = ref_x, bx = (double)(W - 1) - ref_x;
+        double mxx = (a > bx ? a : bx) + 0.5;
+        double ay = ref_y, by = (double)(H - 1) - ref_y;
+        double mxy = (ay > by ? ay : by) + 0.5;
+        double dcmax = sqrt(dxf * dxf * mxx * mxx + dyf * dyf * mxy * mxy);
+        bla_build(&bla, o.zf, o.dfr, o.dfi, o.ref_it + 1, dcmax, f->bla_eps > 0 ? f->bla_eps : ldexp(1.0, -53), 0);
+    }
+
+    mxo_ctx ctx;
+    ctx.zf = o.zf;
+    ctx.zre_fe = o.zre_fe;
+    ctx.zim_fe = o.zim_fe;
+    ctx.mx_ref_it = o.ref_it;
+    ctx.bla = f->use_bla ? &bla : NULL;
+    ctx.bla_minlevel = bla_minlevel_from_skip(f->bla_minskip > 0 ? f->bla_minskip : 8);
+    ctx.esc2 = esc2;
+    ctx.use_interior = f->interior;
+    ctx.interior_eps2 = 3e-9 * 3e-9;
+    ctx.interior_confirm = 4;
+    ctx.int_rep = 1;
+    ctx.ede = ede;
+    ctx.dxfe = dxfe;
+
+    /* per-column / per-row dc (MP.cpp:2160-2163) */
+    mxo_fe* dcol = (mxo_fe*)malloc(sizeof(mxo_fe) * W);
+    mxo_fe* drow = (mxo_fe*)malloc(sizeof(mxo_fe) * H);
+    for (int j = 0; j < W; ++j) dcol[j] = mxo_fe_mul_d(dxfe, (double)j - ref_x);
+    for (int i = 0; i < H; ++i) drow[i] = mxo_fe_mul_d(dyfe, (double)i - ref_y);
+
+    int* idx = (int*)malloc(sizeof(int) * (size_t)W * H);
+    mxo_stats tot = {0, 0, 0, 0, 0, 0};
+    double t0 = omp_get_wtime();
+    if (f->interior && f->probe) {
+        int n = 0;
+        for (int i = 0; i < H; i += 8)
+            for (int j = 0; j < W; j += 8) idx[n++] = i * W + j;
+        mxo_stats pst = {0, 0, 0, 0, 0, 0};
+        run_pixels(&ctx, f, dcol, drow, idx, n, f->val, f->n_esc, &pst);
+        int nint = 0;
+        for (int q = 0; q < n; ++q)
+            if (f->val[idx[q]] == -2.0) ++nint;
+        if (nint == 0) ctx.use_interior = 0;
+    }
+    f->interior_used = ctx.use_interior;
+    int n = 0;
+    const int cidx = (H / 2) * W + (W / 2);
+    for (int p = 0; p < W * H; ++p)
+        if (ref_virtual || p != cidx) idx[n++] = p;
+    run_pixels(&ctx, f, dcol, drow, idx, n, f->val, f->n_esc, &tot);
+    f->seconds_pixels = omp_get_wtime() - t0;
+    if (!ref_virtual) {
+        int ne;
+        f->val[cidx] = ref_pixel_value(&o, f->mxit, f->c_method, esc2, g.dx, &ne);
+        f->n_esc[cidx] = ne;
+    }
+    for (int p = 0; p < W * H; ++p) f->out[p] = (f->n_esc[p] < 0) ? -2.f : (float)f->val[p];
+    f->steps = tot.steps;
+    f->bla_applies = tot.bla_applies;
+    f->bla_skipped = tot.bla_skipped;
+    f->rebases = tot.rebases;
+
+    free(idx);
+    free(dcol);
+    free(drow);
+    if (f->use_bla) bla_free(&bla);
+    orbit_free(&o);
+    mpfr_clears(rre, rim, (mpfr_ptr)0);
+    grid_clear(&g);
+    return 0;
+}
+
+/* ------------------------------------------------------------------------- */
+/* pieces exported for pinning tests                                          */
+/* ------------------------------------------------------------------------- */
+
+/* Direct FloatExp entry points (tests compare them with exact rationals). op: 0 mul,
+ * 1 mul_d, 2 add, 3 sub, 4 div, 5 sqrt(a), 6 from(a.m), 7 to_double(a), 8 abs_less */
+MXO_EXPORT void mxo_fe_op(int op, int n, const double* am, const int64_t* ae, const double* bm, const int64_t* be, double* rm, int64_t* re) {
+    for (int k = 0; k < n; ++k) {
+        mxo_fe a = mxo_fe_make(am[k], ae[k]), b = mxo_fe_make(bm[k], be[k]), r = mxo_fe_make(0, 0);
+        switch (op) {
+        case 0: r = mxo_fe_mul(a, b); break;
+        case 1: r = mxo_fe_mul_d(a, bm[k]); break;
+        case 2: r = mxo_fe_add(a, b); break;
+        case 3: r = mxo_fe_sub(a, b); break;
+        case 4: r = mxo_fe_div(a, b); break;
+        case 5: r = mxo_fe_sqrt(a); break;
+        case 6: r = mxo_fe_from(am[k]); break;
+        case 7: r = mxo_fe_make(mxo_fe_to_double(a), 0); break;
+        case 8: r = mxo_fe_make((double)mxo_fe_abs_less(a, b), 0); break;
+        }
+        rm[k] = r.m;
+        re[k] = r.e;
+    }
+}
+
+/* Reference orbit shadows for a point given as decimal strings. Returns ref_it. */
+MXO_EXPORT int mxo_orbit_shadows(const char* cre, const char* cim, int prec_bits, int mxit, double escape_radius, int deriv, double* zf, double* zre_m, int64_t* zre_e, double* zim_m, int64_t* zim_e, double* dfr, double* dfi) {
+    mpfr_t a, b;
+    mpfr_inits2(prec_bits, a, b, (mpfr_ptr)0);
+    mpfr_set_str(a, cre, 10, MPFR_RNDN);
+    mpfr_set_str(b, cim, 10, MPFR_RNDN);
+    mxo_orbit o;
+    orbit_build(&o, a, b, prec_bits, mxit, escape_radius, deriv, 1);
+    for (int k = 0; k < o.len; ++k) {
+        zf[2 * k] = o.zf[2 * k];
+        zf[2 * k + 1] = o.zf[2 * k + 1];
+        if (zre_m) {
+            zre_m[k] = o.zre_fe[k].m;
+            zre_e[k] = o.zre_fe[k].e;
+            zim_m[k] = o.zim_fe[k].m;
+            zim_e[k] = o.zim_fe[k].e;
+        }
+        if (dfr) {
+            dfr[k] = o.dfr[k];
+            dfi[k] = o.dfi[k];
+        }
+    }
+    int r = o.ref_it;
+    orbit_free(&o);
+    mpfr_clears(a, b, (mpfr_ptr)0);
+    return r;
+}
+
+/* BLA table from a double orbit shadow; writes r2/l/coef (caller sizes them with
+ * mxo_bla_total). Returns the number of levels. */
+MXO_EXPORT int64_t mxo_bla_total(int reflen) {
+    if (reflen < 3) return 0;
+    int64_t n = reflen - 1, tot = 0;
+    for (;;) {
+        tot += n;
+        if (n <= 1) break;
+        n /= 2;
+    }
+    return tot;
+}
+
+MXO_EXPORT int mxo_bla_tables(const double* zf, const double* dfr, const double* dfi, int reflen, double dcmax, double eps, int ede, double* r2, int64_t* l, double* coef, double* der, double* rmax2, int64_t* lev_off, int64_t* lev_size) {
+    mxo_bla b;
+    bla_build(&b, zf, dfr, dfi, reflen, dcmax, eps, ede);
+    if (b.nlev == 0) return 0;
+    memcpy(r2, b.r2, sizeof(double) * b.total);
+    memcpy(l, b.l, sizeof(int64_t) * b.total);
+    memcpy(coef, b.coef, sizeof(double) * 4 * b.total);
+    if (ede && der) memcpy(der, b.der, sizeof(double) * 4 * b.total);
+    *rmax2 = b.rmax2;
+    for (int p = 0; p < b.nlev; ++p) {
+        lev_off[p] = b.off[p];
+        lev_size[p] = b.size[p];
+    }
+    int nl = b.nlev;
+    bla_free(&b);
+    return nl;
+}
+
+/* Per-pixel rescaled loop on caller-supplied orbit/BLA (the same arrays the GPU
+ * consumed), so a parity test isolates the kernel. */
+typedef struct {
+    const double* zf;
+    const double* zre_m;
+    const int64_t* zre_e;
+    const double* zim_m;
+    const int64_t* zim_e;
+    int mx_ref_it;
+    /* BLA (nlev == 0 => off) */
+    int nlev;
+    const int64_t* lev_off;
+    const int64_t* lev_size;
+    const double* r2;
+    const int64_t* l;
+    const double* coef;
+    double rmax2;
+    int bla_minlevel;
+    double esc2;
+    int use_interior;
+    int ede;
+    double dx_m;
+    int64_t dx_e;
+    int mxit;
+    int threads;
+} mxo_pixels_in;
+
+MXO_EXPORT void mxo_pixels_fe(const mxo_pixels_in* in, int n, const double* dcr_m, const int64_t* dcr_e, const double* dci_m, const int64_t* dci_e, double* val, int32_t* nesc, long long* stats4) {
+    int64_t len = in->mx_ref_it + 1;
+    mxo_fe* zre = (mxo_fe*)malloc(sizeof(mxo_fe) * len);
+    mxo_fe* zim = (mxo_fe*)malloc(sizeof(mxo_fe) * len);
+    for (int64_t k = 0; k < len; ++k) {
+        zre[k] = mxo_fe_make(in->zre_m[k], in->zre_e[k]);
+        zim[k] = mxo_fe_make(in->zim_m[k], in->zim_e[k]);
+    }
+    mxo_bla b;
+    memset(&b, 0, sizeof b);
+    if (in->nlev > 0) {
+        b.nlev = in->nlev;
+        for (int p = 0; p < in->nlev; ++p) {
+            b.off[p] = in->lev_off[p];
+            b.size[p] = in->lev_size[p];
+        }
+        b.r2 = (double*)in->r2;
+        b.l = (int64_t*)in->l;
+        b.coef = (double*)in->coef;
+        b.rmax2 = in->rmax2;
+    }
+    mxo_ctx ctx;
+    ctx.zf = in->zf;
+    ctx.zre_fe = zre;
+    ctx.zim_fe = zim;
+    ctx.mx_ref_it = in->mx_ref_it;
+    ctx.bla = in->nlev > 0 ? &b : NULL;
+    ctx.bla_minlevel = in->bla_minlevel;
+    ctx.esc2 = in->esc2;
+    ctx.use_interior = in->use_interior;
+    ctx.interior_eps2 = 3e-9 * 3e-9;
+    ctx.interior_confirm = 4;
+    ctx.int_rep = 1;
+    ctx.ede = in->ede;
+    ctx.dxfe = mxo_fe_make(in->dx_m, in->dx_e);
+    long long steps = 0, ap = 0, sk = 0, rb = 0;
+#pragma omp parallel for schedule(dynamic, 16) num_threads(in->threads > 0 ? in->threads : 1) reduction(+ : steps, ap, sk, rb)
+    for (int q = 0; q < n; ++q) {
+        mxo_stats st = {0, 0, 0, 0, 0, 0};
+        int ne;
+        val[q] = pixel_rescaled(&ctx, mxo_fe_make(dcr_m[q], dcr_e[q]), mxo_fe_make(dci_m[q], dci_e[q]), in->mxit, &ne, &st);
+        nesc[q] = ne;
+        steps += st.steps;
+        ap += st.bla_applies;
+        sk += st.bla_skipped;
+        rb += st.rebases;
+    }
+    if (stats4) {
+        stats4[0] = steps;
+        stats4[1] = ap;
+        stats4[2] = sk;
+        stats4[3] = rb;
+    }
+    free(zre);
+    free(zim);
+}
+
+/* Brute-force pixel grid for a frame (ComputeDirect, MP.cpp:959-990): pixels on a
+ * (step x step) grid; others untouched.  wp_bits = working precision. */
+MXO_EXPORT void mxo_direct(const char* cre, const char* cim, const char* scale, int W, int H, int prec_bits, int wp_bits, int mxit, int c_method, float escape_radius, int step, int threads, int track_d, double* val, int32_t* nesc) {
+    mxo_grid g;
+    grid_init(&g, cre, cim, scale, W, H, 0, 0, prec_bits > wp_bits ? prec_bits : wp_bits);
+    const double esc2 = (double)(escape_radius * escape_radius);
+    int ny = (H + step - 1) / step, nx = (W + step - 1) / step;
+#pragma omp parallel for schedule(dynamic, 1) num_threads(threads > 0 ? threads : 1)
+    for (int q = 0; q < nx * ny; ++q) {
+        int i = (q / nx) * step, j = (q % nx) * step;
+        mpfr_t a, b;
+        mpfr_inits2(g.prec, a, b, (mpfr_ptr)0);
+        grid_pixel_c(&g, i, j, 0, 0, 1, a, b);
+        int ne;
+        val[i * W + j] = point_mpfr(a, b, wp_bits, mxit, (c_method & 2) != 0, track_d, esc2, g.dx, &ne);
+        nesc[i * W + j] = ne;
+        mpfr_clears(a, b, (mpfr_ptr)0);
+    }
+    grid_clear(&g);
+}
+
+/* Brute-force single point from decimal strings. */
+MXO_EXPORT double mxo_point(const char* cre, const char* cim, int wp_bits, int mxit, int ede, int track_d, double esc2, const char* dx, int* nesc) {
+    mpfr_t a, b, d;
+    mpfr_inits2(wp_bits, a, b, d, (mpfr_ptr)0);
+    mpfr_set_str(a, cre, 10, MPFR_RNDN);
+    mpfr_set_str(b, cim, 10, MPFR_RNDN);
+    mpfr_set_str(d, dx ? dx : "1", 10, MPFR_RNDN);
+    double v = point_mpfr(a, b, wp_bits, mxit, ede, track_d, esc2, d, nesc);
+    mpfr_clears(a, b, d, (mpfr_ptr)0);
+    return v;
+}
+
+/* Grid values for tests: truncated dx/dy as FloatExp and c0 as strings. */
+MXO_EXPORT void mxo_grid_info(const char* cre, const char* cim, const char* scale, int W, int H, int full_h, int row_base, int prec_bits, double* dx_m, int64_t* dx_e, double* dy_m, int64_t* dy_e) {
+    mxo_grid g;
+    grid_init(&g, cre, cim, scale, W, H, full_h, row_base, prec_bits);
+    mxo_fe a = mpfr_to_fe_trunc(g.dx), b = mpfr_to_fe_trunc(g.dy);
+    *dx_m = a.m;
+    *dx_e = a.e;
+    *dy_m = b.m;
+    *dy_e = b.e;
+    grid_clear(&g);
+}

@@ -1,0 +1,204 @@
+"""ORACLE — test infrastructure only (tests/, __graft_entry__.smoke(), bench.py's
+cpu_baseline leg).  ctypes bindings for oracle/libmxo.so, the CPU restatement of the
+reference engine's perturbation path (src/engine/mandel_perturbation.cpp)."""
+import ctypes as C
+import math
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+
+class Frame(C.Structure):
+    _fields_ = [
+        ("cre", C.c_char_p), ("cim", C.c_char_p), ("scale", C.c_char_p),
+        ("W", C.c_int), ("H", C.c_int), ("full_h", C.c_int), ("row_base", C.c_int),
+        ("mxit", C.c_int), ("c_method", C.c_int), ("escape_radius", C.c_float),
+        ("prec_bits", C.c_int), ("use_bla", C.c_int), ("bla_minskip", C.c_int),
+        ("bla_eps", C.c_double), ("interior", C.c_int), ("probe", C.c_int), ("threads", C.c_int),
+        ("val", C.POINTER(C.c_double)), ("n_esc", C.POINTER(C.c_int32)), ("out", C.POINTER(C.c_float)),
+        ("ref_it", C.c_int), ("ref_escaped", C.c_int), ("ref_virtual", C.c_int),
+        ("ref_x", C.c_double), ("ref_y", C.c_double), ("interior_used", C.c_int),
+        ("cutoff_sensitive_hint", C.c_int), ("seconds_pixels", C.c_double),
+        ("steps", C.c_longlong), ("bla_applies", C.c_longlong), ("bla_skipped", C.c_longlong),
+        ("rebases", C.c_longlong),
+    ]
+
+
+class PixelsIn(C.Structure):
+    _fields_ = [
+        ("zf", C.c_void_p), ("zre_m", C.c_void_p), ("zre_e", C.c_void_p),
+        ("zim_m", C.c_void_p), ("zim_e", C.c_void_p), ("mx_ref_it", C.c_int),
+        ("nlev", C.c_int), ("lev_off", C.c_void_p), ("lev_size", C.c_void_p),
+        ("r2", C.c_void_p), ("l", C.c_void_p), ("coef", C.c_void_p), ("rmax2", C.c_double),
+        ("bla_minlevel", C.c_int), ("esc2", C.c_double), ("use_interior", C.c_int),
+        ("ede", C.c_int), ("dx_m", C.c_double), ("dx_e", C.c_int64), ("mxit", C.c_int),
+        ("threads", C.c_int),
+    ]
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "libmxo.so")
+        if not os.path.exists(path):
+            raise RuntimeError("oracle/libmxo.so is not built (run `make -C oracle`)")
+        L = C.CDLL(path)
+        L.mxo_compute_fe.argtypes = [C.POINTER(Frame)]
+        L.mxo_fe_op.argtypes = [C.c_int, C.c_int] + [C.c_void_p] * 6
+        L.mxo_orbit_shadows.argtypes = [C.c_char_p, C.c_char_p, C.c_int, C.c_int, C.c_double, C.c_int] + [C.c_void_p] * 7
+        L.mxo_orbit_shadows.restype = C.c_int
+        L.mxo_bla_total.argtypes = [C.c_int]
+        L.mxo_bla_total.restype = C.c_int64
+        L.mxo_bla_tables.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_double, C.c_double, C.c_int] + [C.c_void_p] * 7
+        L.mxo_bla_tables.restype = C.c_int
+        L.mxo_pixels_fe.argtypes = [C.POINTER(PixelsIn), C.c_int] + [C.c_void_p] * 7
+        L.mxo_direct.argtypes = [C.c_char_p, C.c_char_p, C.c_char_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_float, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p]
+        L.mxo_point.argtypes = [C.c_char_p, C.c_char_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, C.c_char_p, C.c_void_p]
+        L.mxo_point.restype = C.c_double
+        L.mxo_grid_info.argtypes = [C.c_char_p, C.c_char_p, C.c_char_p] + [C.c_int] * 5 + [C.c_void_p] * 4
+        _LIB = L
+    return _LIB
+
+
+def _p(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def precision_bits(scale_str):
+    """R.cpp:170 / verify.cpp:157: digits(scale) * log2(10) + 64."""
+    return int(len(scale_str) * math.log(10) / math.log(2)) + 64
+
+
+def compute_fe(cre, cim, scale, W, H, mxit, c_method=0, escape_radius=1e8, prec_bits=None,
+               use_bla=True, bla_minskip=8, interior=True, probe=True, threads=None, full_h=0, row_base=0):
+    """Whole floatexp frame (Compute -> reference -> BLA -> probe -> pass)."""
+    f = Frame()
+    f.cre, f.cim, f.scale = cre.encode(), cim.encode(), scale.encode()
+    f.W, f.H, f.full_h, f.row_base = W, H, full_h, row_base
+    f.mxit, f.c_method, f.escape_radius = mxit, c_method, escape_radius
+    f.prec_bits = prec_bits or precision_bits(scale)
+    f.use_bla, f.bla_minskip, f.bla_eps = int(use_bla), bla_minskip, 0.0
+    f.interior, f.probe = int(interior), int(probe)
+    f.threads = threads or os.cpu_count()
+    val = np.zeros(W * H, np.float64)
+    nesc = np.zeros(W * H, np.int32)
+    out = np.zeros(W * H, np.float32)
+    f.val = val.ctypes.data_as(C.POINTER(C.c_double))
+    f.n_esc = nesc.ctypes.data_as(C.POINTER(C.c_int32))
+    f.out = out.ctypes.data_as(C.POINTER(C.c_float))
+    lib().mxo_compute_fe(C.byref(f))
+    info = {k: getattr(f, k) for k in ("ref_it", "ref_escaped", "ref_virtual", "ref_x", "ref_y", "interior_used",
+                                        "cutoff_sensitive_hint", "seconds_pixels", "steps", "bla_applies",
+                                        "bla_skipped", "rebases")}
+    return val.reshape(H, W), nesc.reshape(H, W), out.reshape(H, W), info
+
+
+def fe_op(op, am, ae, bm=None, be=None):
+    am = np.ascontiguousarray(am, np.float64)
+    ae = np.ascontiguousarray(ae, np.int64)
+    n = am.size
+    bm = np.ascontiguousarray(np.zeros(n) if bm is None else bm, np.float64)
+    be = np.ascontiguousarray(np.zeros(n, np.int64) if be is None else be, np.int64)
+    rm = np.zeros(n)
+    re = np.zeros(n, np.int64)
+    lib().mxo_fe_op(op, n, _p(am), _p(ae), _p(bm), _p(be), _p(rm), _p(re))
+    return rm, re
+
+
+def orbit(cre, cim, prec_bits, mxit, escape_radius=1e8, deriv=False):
+    zf = np.zeros(2 * (mxit + 1))
+    zrm = np.zeros(mxit + 1)
+    zre = np.zeros(mxit + 1, np.int64)
+    zim = np.zeros(mxit + 1)
+    zie = np.zeros(mxit + 1, np.int64)
+    dfr = np.zeros(mxit + 1)
+    dfi = np.zeros(mxit + 1)
+    r = lib().mxo_orbit_shadows(cre.encode(), cim.encode(), prec_bits, mxit, float(np.float32(escape_radius)), int(deriv),
+                                _p(zf), _p(zrm), _p(zre), _p(zim), _p(zie), _p(dfr), _p(dfi))
+    n = r + 1
+    return dict(ref_it=r, zf=zf[:2 * n].copy(), zre_m=zrm[:n].copy(), zre_e=zre[:n].copy(),
+                zim_m=zim[:n].copy(), zim_e=zie[:n].copy(), dfr=dfr[:n].copy(), dfi=dfi[:n].copy())
+
+
+def bla_tables(zf, reflen, dcmax, eps=2.0 ** -53, ede=False, dfr=None, dfi=None):
+    tot = lib().mxo_bla_total(reflen)
+    if tot == 0:
+        return None
+    r2 = np.zeros(tot)
+    l = np.zeros(tot, np.int64)
+    coef = np.zeros(4 * tot)
+    der = np.zeros(4 * tot)
+    rmax2 = C.c_double(0)
+    off = np.zeros(64, np.int64)
+    size = np.zeros(64, np.int64)
+    zf = np.ascontiguousarray(zf, np.float64)
+    dfr = np.ascontiguousarray(np.zeros(reflen) if dfr is None else dfr, np.float64)
+    dfi = np.ascontiguousarray(np.zeros(reflen) if dfi is None else dfi, np.float64)
+    nl = lib().mxo_bla_tables(_p(zf), _p(dfr), _p(dfi), reflen, dcmax, eps, int(ede), _p(r2), _p(l), _p(coef), _p(der),
+                              C.cast(C.byref(rmax2), C.c_void_p), _p(off), _p(size))
+    return dict(nlev=nl, r2=r2, l=l, coef=coef, der=der, rmax2=rmax2.value, off=off[:nl].copy(), size=size[:nl].copy())
+
+
+def pixels_fe(orb, bla, dcr_m, dcr_e, dci_m, dci_e, mxit, esc2, dx_m, dx_e, ede=False, use_interior=True,
+              bla_minlevel=3, threads=None):
+    """Per-pixel rescaled loop over caller-supplied orbit/BLA arrays."""
+    keep = []
+
+    def arr(a, dt):
+        a = np.ascontiguousarray(a, dt)
+        keep.append(a)
+        return _p(a)
+
+    pi = PixelsIn()
+    pi.zf = arr(orb["zf"], np.float64)
+    pi.zre_m, pi.zre_e = arr(orb["zre_m"], np.float64), arr(orb["zre_e"], np.int64)
+    pi.zim_m, pi.zim_e = arr(orb["zim_m"], np.float64), arr(orb["zim_e"], np.int64)
+    pi.mx_ref_it = len(orb["zre_m"]) - 1
+    if bla is None:
+        pi.nlev = 0
+    else:
+        pi.nlev = bla["nlev"]
+        pi.lev_off, pi.lev_size = arr(bla["off"], np.int64), arr(bla["size"], np.int64)
+        pi.r2, pi.l, pi.coef = arr(bla["r2"], np.float64), arr(bla["l"], np.int64), arr(bla["coef"], np.float64)
+        pi.rmax2 = bla["rmax2"]
+    pi.bla_minlevel, pi.esc2, pi.use_interior, pi.ede = bla_minlevel, esc2, int(use_interior), int(ede)
+    pi.dx_m, pi.dx_e, pi.mxit = dx_m, dx_e, mxit
+    pi.threads = threads or os.cpu_count()
+    n = len(dcr_m)
+    val = np.zeros(n)
+    nesc = np.zeros(n, np.int32)
+    st = np.zeros(4, np.int64)
+    lib().mxo_pixels_fe(C.byref(pi), n, arr(dcr_m, np.float64), arr(dcr_e, np.int64), arr(dci_m, np.float64),
+                        arr(dci_e, np.int64), _p(val), _p(nesc), _p(st))
+    return val, nesc, dict(steps=int(st[0]), bla_applies=int(st[1]), bla_skipped=int(st[2]), rebases=int(st[3]))
+
+
+def direct(cre, cim, scale, W, H, mxit, c_method=0, escape_radius=1e8, step=1, prec_bits=None, wp_bits=None,
+           threads=None, track_d=True):
+    """Brute-force MPFR frame (ComputeDirect); unsampled pixels are NaN / 0."""
+    pb = prec_bits or precision_bits(scale)
+    val = np.full(W * H, np.nan)
+    nesc = np.zeros(W * H, np.int32)
+    lib().mxo_direct(cre.encode(), cim.encode(), scale.encode(), W, H, pb, wp_bits or pb, mxit, c_method,
+                     escape_radius, step, threads or os.cpu_count(), int(track_d), _p(val), _p(nesc))
+    return val.reshape(H, W), nesc.reshape(H, W)
+
+
+def point(cre, cim, wp_bits, mxit, esc2=float(np.float32(1e8)) ** 2, ede=False, track_d=True, dx="1"):
+    n = C.c_int(0)
+    v = lib().mxo_point(cre.encode(), cim.encode(), wp_bits, mxit, int(ede), int(track_d), esc2, dx.encode(),
+                        C.cast(C.byref(n), C.c_void_p))
+    return v, n.value
+
+
+def grid_info(cre, cim, scale, W, H, prec_bits=None, full_h=0, row_base=0):
+    dxm, dym = C.c_double(), C.c_double()
+    dxe, dye = C.c_int64(), C.c_int64()
+    lib().mxo_grid_info(cre.encode(), cim.encode(), scale.encode(), W, H, full_h, row_base,
+                        prec_bits or precision_bits(scale), C.cast(C.byref(dxm), C.c_void_p),
+                        C.cast(C.byref(dxe), C.c_void_p), C.cast(C.byref(dym), C.c_void_p),
+                        C.cast(C.byref(dye), C.c_void_p))
+    return (dxm.value, dxe.value), (dym.value, dye.value)
