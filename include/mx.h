@@ -109,6 +109,7 @@ typedef struct {
     const int32_t* drow_e;
     double dx_m;             /* pixel spacing (EDE normalisation) */
     int32_t dx_e;
+    int32_t skip_sample;     /* sample index left untouched (the reference pixel), -1: none */
 } mx_tile;
 
 /* A plan holds the device copy of a tile's inputs and outputs so that launches

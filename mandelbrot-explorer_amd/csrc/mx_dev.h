@@ -39,8 +39,6 @@ struct MxRefDev {
     int minlevel;
     int pad0;
     double rmax2;
-    int lev_off[MX_MAXLEV];
-    int lev_size[MX_MAXLEV];
 };
 
 struct MxTileDev {
@@ -56,7 +54,7 @@ struct MxTileDev {
     double esc2;
     double interior_eps2;
     int interior_confirm;
-    int pad1;
+    int skip_sample; // sample index not computed (the reference pixel), -1 none
     Fe dxfe;
     double* val;         // W*H
     int* nesc;           // W*H

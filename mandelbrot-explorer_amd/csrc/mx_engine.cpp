@@ -176,10 +176,6 @@ extern "C" int mx_ref_upload(mx_ctx* ctx, const mx_ref_desc* d, mx_ref** out) {
     R.mx_ref_it = d->len - 1;
     R.nlev = d->nlev;
     R.rmax2 = d->rmax2;
-    for (int p = 0; p < d->nlev; ++p) {
-        R.lev_off[p] = d->lev_off[p];
-        R.lev_size[p] = d->lev_size[p];
-    }
     ref->lev = (int2*)(b + o_lev);
     *out = ref;
     return MX_OK;
@@ -258,6 +254,7 @@ extern "C" int mx_plan_create(mx_ctx* ctx, const mx_ref* ref, const mx_tile* t, 
     T.interior_eps2 = t->interior_eps2;
     T.interior_confirm = t->interior_confirm;
     T.dxfe = fe_make(t->dx_m, t->dx_e);
+    T.skip_sample = t->skip_sample;
     T.val = (double*)(b + o_val);
     T.nesc = (int*)(b + o_nesc);
     T.out = (float*)(b + o_out);
@@ -402,6 +399,7 @@ extern "C" int mx_compute(mx_ctx* ctx, const mx_request* req, mx_frame_info* inf
     t.drow_e = v.drow_e;
     t.dx_m = v.dx_m;
     t.dx_e = v.dx_e;
+    t.skip_sample = -1;
     const size_t n = (size_t)F.W * F.H;
     std::vector<int32_t> nesc(n);
     std::vector<double> val(n);
@@ -432,6 +430,9 @@ extern "C" int mx_compute(mx_ctx* ctx, const mx_request* req, mx_frame_info* inf
             t.int_rep = 0;
         }
     }
+    // main pass: every sample but the reference pixel, whose value comes from the
+    // reference orbit itself (mandel_perturbation.cpp:1724-1726)
+    if (!F.ref_virtual) t.skip_sample = (F.H / 2) * F.W + (F.W / 2);
     {
         mx_plan* plan = nullptr;
         rc = mx_plan_create(ctx, ref, &t, &plan);

@@ -32,7 +32,8 @@ struct Px {
     int m, rm, iter;
     int save_iter, period_win, conf_P, conf_next, conf_count, conf_giveup;
     int sm, save_iter2, twin2r;
-    int item;
+    int xv;          // 1: Xc = X_m and Xn = X_{m+1} are loaded
+    double2 Xc, Xn;  // orbit entries carried from the previous trip
 };
 
 struct Counters {
@@ -73,10 +74,24 @@ __device__ __forceinline__ void init_pixel(Px& p, Fe dcr, Fe dci) {
     p.sm = -1;
     p.save_iter2 = 1;
     p.twin2r = 1;
+    p.xv = 0;
 }
 
+// Scaled delta (S, w) returned by the cold helpers by value: references into the
+// per-lane state would force that state out of registers into scratch memory.
+struct SW {
+    Fe S;
+    double wr, wi;
+    bool ok;
+};
+
+struct SJW {
+    Fe SJ;
+    double jr, ji, invSJd;
+};
+
 // feRescaledStepUnderflow (mandel_perturbation.cpp:2794-2835)
-__device__ __noinline__ void step_underflow(Fe Xr, Fe Xi, Fe dcr, Fe dci, Fe& S, double& wr, double& wi) {
+__device__ __noinline__ SW step_underflow(Fe Xr, Fe Xi, Fe dcr, Fe dci, Fe S, double wr, double wi) {
     const double sm = S.m, sm2 = sm * sm;
     const int se = S.e;
     int e = INT_MIN;
@@ -86,11 +101,7 @@ __device__ __noinline__ void step_underflow(Fe Xr, Fe Xi, Fe dcr, Fe dci, Fe& S,
     if (has_w) e = max(e, se + se);
     if (dcr.m != 0.0) e = max(e, dcr.e);
     if (dci.m != 0.0) e = max(e, dci.e);
-    if (e == INT_MIN) {
-        S = fe_make(1.0, 0);
-        wr = wi = 0.0;
-        return;
-    }
+    if (e == INT_MIN) return SW{fe_make(1.0, 0), 0.0, 0.0, true};
     auto align = [e](double mm, int te) {
         int d = te - e;
         if (mm == 0.0 || d < -54) return 0.0;
@@ -105,18 +116,12 @@ __device__ __noinline__ void step_underflow(Fe Xr, Fe Xi, Fe dcr, Fe dci, Fe& S,
     const double ni = 2.0 * (xs_r * wi + xs_i * wr) + 2.0 * s2 * wr * wi + ci;
     const double mag2 = nr * nr + ni * ni;
     const double mag = mag2 > 0.0 ? sqrt(mag2) : hypot(nr, ni);
-    if (mag == 0.0) {
-        S = fe_make(1.0, 0);
-        wr = wi = 0.0;
-        return;
-    }
-    S = fe_renorm(mag, e);
-    wr = nr / mag;
-    wi = ni / mag;
+    if (mag == 0.0) return SW{fe_make(1.0, 0), 0.0, 0.0, true};
+    return SW{fe_renorm(mag, e), nr / mag, ni / mag, true};
 }
 
 // feDerivativeStepUnderflow (mandel_perturbation.cpp:2842-2876)
-__device__ __noinline__ void deriv_underflow(Fe zr, Fe zi, Fe& SJ, double& jr, double& ji, double& invSJd) {
+__device__ __noinline__ SJW deriv_underflow(Fe zr, Fe zi, Fe SJ, double jr, double ji) {
     double im = 0.5 / SJ.m;
     int ie = 1 - SJ.e;
     if (im >= 1.0) {
@@ -137,38 +142,24 @@ __device__ __noinline__ void deriv_underflow(Fe zr, Fe zi, Fe& SJ, double& jr, d
     double ar = 2.0 * (azr * jr - azi * ji) + align(invSJ);
     double ai = 2.0 * (azr * ji + azi * jr);
     double mag = sqrt(ar * ar + ai * ai);
-    if (mag == 0.0) {
-        SJ = fe_make(1.0, 0);
-        jr = ji = 0.0;
-        invSJd = 1.0;
-        return;
-    }
-    SJ = fe_mul(SJ, fe_renorm(mag, e));
-    jr = ar / mag;
-    ji = ai / mag;
-    invSJd = 1.0 / fe_to_double(SJ);
+    if (mag == 0.0) return SJW{fe_make(1.0, 0), 0.0, 0.0, 1.0};
+    const Fe nSJ = fe_mul(SJ, fe_renorm(mag, e));
+    return SJW{nSJ, ar / mag, ai / mag, 1.0 / fe_to_double(nSJ)};
 }
 
 // Overflow-safe FloatExp landing of a BLA skip (mandel_perturbation.cpp:2943-2957).
 // Returns false when the landing escapes.
-__device__ __noinline__ bool bla_land_fe(const double2* zf, int land, double esc2, double4 bc, Fe& S, double& wr, double& wi, double dr, double di) {
+__device__ __noinline__ SW bla_land_fe(const double2* zf, int land, double esc2, double4 bc, Fe S, double wr, double wi, double dr, double di) {
     Fe dzr = fe_mul_d(S, wr), dzi = fe_mul_d(S, wi);
     Fe dcrfe = fe_mul_d(S, dr), dcife = fe_mul_d(S, di);
     Fe nzr = fe_add(fe_sub(fe_mul_d(dzr, bc.x), fe_mul_d(dzi, bc.y)), fe_sub(fe_mul_d(dcrfe, bc.z), fe_mul_d(dcife, bc.w)));
     Fe nzi = fe_add(fe_add(fe_mul_d(dzr, bc.y), fe_mul_d(dzi, bc.x)), fe_add(fe_mul_d(dcrfe, bc.w), fe_mul_d(dcife, bc.z)));
     double2 X = zf[land];
     double fr = X.x + fe_to_double(nzr), fi = X.y + fe_to_double(nzi);
-    if (fr * fr + fi * fi > esc2) return false;
+    if (fr * fr + fi * fi > esc2) return SW{S, wr, wi, false};
     Fe Snew = fe_sqrt(fe_add(fe_mul(nzr, nzr), fe_mul(nzi, nzi)));
-    if (Snew.m == 0.0) {
-        S = fe_make(1.0, 0);
-        wr = wi = 0.0;
-    } else {
-        S = Snew;
-        wr = fe_to_double(fe_div(nzr, S));
-        wi = fe_to_double(fe_div(nzi, S));
-    }
-    return true;
+    if (Snew.m == 0.0) return SW{fe_make(1.0, 0), 0.0, 0.0, true};
+    return SW{Snew, fe_to_double(fe_div(nzr, Snew)), fe_to_double(fe_div(nzi, Snew)), true};
 }
 
 // tryBLAfe (mandel_perturbation.cpp:2883-2970).  s = m-1, reflen = mx_ref_it+1.
@@ -206,7 +197,11 @@ __device__ __forceinline__ int try_bla(const MxRefDev& R, const int2* lev, int s
             wr = nwr * inv;
             wi = nwi * inv;
         } else {
-            if (!bla_land_fe(R.zf, land, esc2, bc, S, wr, wi, dr, di)) return 0;
+            const SW r = bla_land_fe(R.zf, land, esc2, bc, S, wr, wi, dr, di);
+            if (!r.ok) return 0;
+            S = r.S;
+            wr = r.wr;
+            wi = r.wi;
         }
         ab = bc;
         return 1 << p;
@@ -216,7 +211,7 @@ __device__ __forceinline__ int try_bla(const MxRefDev& R, const int2* lev, int s
 
 // One trip of the pixel loop.  Returns true when the sample is finished (value and
 // n_esc written to v/n).  n: escape iteration, -1 = reached mxit, -2 = detected interior.
-template <bool EDE>
+template <bool EDE, bool INTERIOR>
 __device__ __forceinline__ bool trip(const MxRefDev& R, const int2* lev, const MxTileDev& T, Px& p, double& v, int& n, Counters& c) {
     const double ESC2 = T.esc2;
     const int reflen = R.mx_ref_it + 1;
@@ -225,7 +220,7 @@ __device__ __forceinline__ bool trip(const MxRefDev& R, const int2* lev, const M
         v = -2.0;
         return true;
     }
-    if (T.flags & MX_F_INT_REP) {
+    if (INTERIOR && (T.flags & MX_F_INT_REP)) {
         if (p.m == p.sm && p.wr == p.swr && p.wi == p.swi && p.S.m == p.sS.m && p.S.e == p.sS.e) {
             n = -2;
             v = -2.0;
@@ -274,10 +269,18 @@ __device__ __forceinline__ bool trip(const MxRefDev& R, const int2* lev, const M
             p.period_win = 1;
             p.conf_P = 0;
             p.conf_giveup = 0;
+            p.xv = 0;
             return false;
         }
     }
-    double2 X = p.m ? R.zf[p.rm] : make_double2(0.0, 0.0);
+    // X_m and X_{m+1} come from the previous trip; X_{m+2} is fetched one trip ahead
+    // (m <= mx_ref_it here, so rm+1 is in range; rm+2 only when m+1 <= mx_ref_it).
+    if (!p.xv) {
+        p.Xc = p.m ? R.zf[p.rm] : make_double2(0.0, 0.0);
+        p.Xn = R.zf[p.rm + 1];
+    }
+    const double2 Xpf = (p.rm + 2 < reflen) ? R.zf[p.rm + 2] : make_double2(0.0, 0.0);
+    double2 X = p.Xc;
     double zmr = 0.0, zmi = 0.0;
     bool d_underflow = false;
     Fe zmrfe = fe_make(0.0, 0), zmife = fe_make(0.0, 0);
@@ -310,7 +313,10 @@ __device__ __forceinline__ bool trip(const MxRefDev& R, const int2* lev, const M
             Xrfe = fe_make(mm.x, ee.x);
             Xife = fe_make(mm.y, ee.y);
         }
-        step_underflow(Xrfe, Xife, p.dcr, p.dci, p.S, p.wr, p.wi);
+        const SW r = step_underflow(Xrfe, Xife, p.dcr, p.dci, p.S, p.wr, p.wi);
+        p.S = r.S;
+        p.wr = r.wr;
+        p.wi = r.wi;
         p.s = fe_to_double(p.S);
         p.S2 = fe_mul(p.S, p.S);
         p.dr = fe_to_double(fe_div(p.dcr, p.S));
@@ -325,7 +331,11 @@ __device__ __forceinline__ bool trip(const MxRefDev& R, const int2* lev, const M
     c.steps++;
     if constexpr (EDE) {
         if (d_underflow) {
-            deriv_underflow(zmrfe, zmife, p.SJ, p.jr, p.ji, p.invSJd);
+            const SJW r = deriv_underflow(zmrfe, zmife, p.SJ, p.jr, p.ji);
+            p.SJ = r.SJ;
+            p.jr = r.jr;
+            p.ji = r.ji;
+            p.invSJd = r.invSJd;
         } else {
             const double a = 2.0 * (zmr * p.jr - zmi * p.ji) + p.invSJd;
             const double b = 2.0 * (zmr * p.ji + zmi * p.jr);
@@ -342,7 +352,10 @@ __device__ __forceinline__ bool trip(const MxRefDev& R, const int2* lev, const M
             }
         }
     }
-    X = p.m ? R.zf[p.rm] : make_double2(0.0, 0.0);
+    X = p.Xn;
+    p.Xc = p.Xn;
+    p.Xn = Xpf;
+    p.xv = 1;
     const double zr = X.x + p.s * p.wr, zi = X.y + p.s * p.wi;
     const double zrad = zr * zr + zi * zi;
     if (zrad > ESC2) {
@@ -357,7 +370,7 @@ __device__ __forceinline__ bool trip(const MxRefDev& R, const int2* lev, const M
         }
         return true;
     }
-    if (T.flags & MX_F_INTERIOR) {
+    if (INTERIOR) {
         if (p.conf_P > 0) {
             p.conf_D2 *= 4.0 * zrad;
             if (p.conf_D2 > 1e18) p.conf_D2 = 1e18;
@@ -437,6 +450,7 @@ __device__ __forceinline__ bool trip(const MxRefDev& R, const int2* lev, const M
             }
             p.m = 0;
             p.rm = -1;
+            p.xv = 0;
             c.rebases++;
         }
     }
@@ -457,10 +471,10 @@ __device__ __forceinline__ bool item_to_sample(const MxTileDev& T, unsigned item
     if (gi >= T.ny || gj >= T.nx) return false;
     si = gi * T.stride;
     sj = gj * T.stride;
-    return true;
+    return si * T.W + sj != T.skip_sample;
 }
 
-template <bool EDE>
+template <bool EDE, bool INTERIOR>
 __global__ __launch_bounds__(256) void mx_fe64_kernel(MxRefDev R, const int2* __restrict__ lev, MxTileDev T) {
     const int lane = __lane_id();
     const unsigned long long lanemask_lt = (1ull << lane) - 1ull;
@@ -507,7 +521,7 @@ __global__ __launch_bounds__(256) void mx_fe64_kernel(MxRefDev R, const int2* __
         if (active) {
             double v;
             int n;
-            if (trip<EDE>(R, lev, T, p, v, n, c)) {
+            if (trip<EDE, INTERIOR>(R, lev, T, p, v, n, c)) {
                 T.val[sidx] = v;
                 T.nesc[sidx] = n;
                 if (T.out) T.out[sidx] = n < 0 ? -2.0f : (float)v;
@@ -528,9 +542,14 @@ __global__ __launch_bounds__(256) void mx_fe64_kernel(MxRefDev R, const int2* __
 
 // Host-side launcher (called from the engine's C++).
 hipError_t mx_launch_fe64(const MxRefDev& R, const int2* lev, const MxTileDev& T, int blocks, int threads, hipStream_t stream) {
-    if (T.flags & MX_F_EDE)
-        hipLaunchKernelGGL(mx_fe64_kernel<true>, dim3(blocks), dim3(threads), 0, stream, R, lev, T);
+    const bool ede = T.flags & MX_F_EDE, interior = T.flags & MX_F_INTERIOR;
+    if (ede && interior)
+        hipLaunchKernelGGL((mx_fe64_kernel<true, true>), dim3(blocks), dim3(threads), 0, stream, R, lev, T);
+    else if (ede)
+        hipLaunchKernelGGL((mx_fe64_kernel<true, false>), dim3(blocks), dim3(threads), 0, stream, R, lev, T);
+    else if (interior)
+        hipLaunchKernelGGL((mx_fe64_kernel<false, true>), dim3(blocks), dim3(threads), 0, stream, R, lev, T);
     else
-        hipLaunchKernelGGL(mx_fe64_kernel<false>, dim3(blocks), dim3(threads), 0, stream, R, lev, T);
+        hipLaunchKernelGGL((mx_fe64_kernel<false, false>), dim3(blocks), dim3(threads), 0, stream, R, lev, T);
     return hipGetLastError();
 }

@@ -33,7 +33,7 @@ class Tile(C.Structure):
                 ("int_rep", C.c_int32), ("bla", C.c_int32), ("bla_minlevel", C.c_int32),
                 ("interior_eps2", C.c_double), ("interior_confirm", C.c_int32),
                 ("dcol_m", C.c_void_p), ("dcol_e", C.c_void_p), ("drow_m", C.c_void_p), ("drow_e", C.c_void_p),
-                ("dx_m", C.c_double), ("dx_e", C.c_int32)]
+                ("dx_m", C.c_double), ("dx_e", C.c_int32), ("skip_sample", C.c_int32)]
 
 
 class Request(C.Structure):
@@ -234,6 +234,7 @@ class Context:
                 (("dcol_m", np.float64), ("dcol_e", np.int32), ("drow_m", np.float64), ("drow_e", np.int32))]
         t.dcol_m, t.dcol_e, t.drow_m, t.drow_e = [_p(x) for x in keep]
         t.dx_m, t.dx_e = fv["dx_m"], fv["dx_e"]
+        t.skip_sample = -1
         t._keep = keep
         return t
 

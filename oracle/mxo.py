@@ -187,7 +187,7 @@ def direct(cre, cim, scale, W, H, mxit, c_method=0, escape_radius=1e8, step=1, p
     return val.reshape(H, W), nesc.reshape(H, W)
 
 
-def point(cre, cim, wp_bits, mxit, esc2=float(np.float32(1e8)) ** 2, ede=False, track_d=True, dx="1"):
+def point(cre, cim, wp_bits, mxit, esc2=float(np.float32(1e8) * np.float32(1e8)), ede=False, track_d=True, dx="1"):
     n = C.c_int(0)
     v = lib().mxo_point(cre.encode(), cim.encode(), wp_bits, mxit, int(ede), int(track_d), esc2, dx.encode(),
                         C.cast(C.byref(n), C.c_void_p))

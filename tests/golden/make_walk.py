@@ -1,0 +1,125 @@
+"""Seeded boundary walk: generates the deep view centres used by the benchmark
+configurations and the large-frame tests; writes tests/golden/walk_seed<S>.json.
+
+Definition (the workload spec's walk, with the two choices it leaves open fixed):
+  start c = -0.75 + 0i, view width w = 3; seed s drives splitmix64.
+  repeat until w <= target:
+    render a 64x64 probe of the view (square, the engine's grid: scale = 4/w) at
+    probe mxit (starts at 1024) with the oracle;
+    rank the pixels that escaped by (escape count desc, index asc) -- pixels that
+    did not escape are not candidates (counting them as the maximum would walk into
+    the interior of the set and end on an all-interior view);
+    if fewer than 41 pixels escaped, or the highest escape count exceeds half the
+    probe mxit (the probe would otherwise follow its own mxit level curve instead of
+    the set), double the probe mxit (up to 2^21) and render again;
+    pick uniformly among the top ceil(0.1 * n_escaped) candidates (splitmix64 mod k);
+    recentre on that pixel and halve w.
+  MPFR precision of each probe = ceil(-log2 w) + 64 bits.
+Centres are recorded at every requested depth; the recorded decimal strings carry
+ceil(-log10 w) + 30 digits.
+
+Usage: python tests/golden/make_walk.py SEED DEPTH10 [DEPTH10 ...]
+e.g.   python tests/golden/make_walk.py 7 300 1000
+"""
+import json
+import math
+import os
+import sys
+import time
+from decimal import Decimal, getcontext
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(HERE, "..", "..", "oracle"))
+import mxo  # noqa: E402
+
+MASK = (1 << 64) - 1
+MXIT_CAP = 1 << 21
+
+
+class SplitMix64:
+    def __init__(self, seed):
+        self.x = seed & MASK
+
+    def next(self):
+        self.x = (self.x + 0x9E3779B97F4A7C15) & MASK
+        z = self.x
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & MASK
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & MASK
+        return z ^ (z >> 31)
+
+
+def walk(seed, depths, log=print, on_depth=None):
+    depths = sorted(depths)
+    rng = SplitMix64(seed)
+    getcontext().prec = max(depths) + 80
+    cx, cy = Decimal("-0.75"), Decimal(0)
+    w = Decimal(3)
+    mxit = 1024
+    out = {}
+    step = 0
+    t0 = time.time()
+    while depths:
+        target = Decimal(10) ** (-depths[0])
+        if w <= target:
+            digits = depths[0] + 30
+            out[str(depths[0])] = dict(re=format(cx.quantize(Decimal(10) ** -digits), "f"),
+                                       im=format(cy.quantize(Decimal(10) ** -digits), "f"),
+                                       width=format(w, ".6e"), steps=step, probe_mxit=mxit)
+            log(f"seed {seed}: reached 1e-{depths[0]} after {step} steps ({time.time() - t0:.0f}s)")
+            if on_depth:
+                on_depth(out)
+            depths.pop(0)
+            continue
+        scale = Decimal(4) / w
+        # the engine reads the scale as a decimal integer magnitude; keep 30 significant digits
+        sc_int = format(scale.to_integral_value(), "f") if scale >= 1 else None
+        prec = int(math.ceil(-math.log2(float(w)) if w > Decimal("1e-300") else (-w.log10() * Decimal(math.log2(10))))) + 64
+        prec = max(prec, 80)
+        W = H = 64
+        while True:
+            if sc_int is not None and scale > 1e6:
+                val, nesc, _, _ = mxo.compute_fe(format(cx, "f"), format(cy, "f"), sc_int, W, H, mxit, prec_bits=prec, threads=os.cpu_count())
+            else:
+                val, nesc = mxo.direct(format(cx, "f"), format(cy, "f"), format(scale, "f"), W, H, mxit, prec_bits=prec, wp_bits=prec, threads=os.cpu_count())
+            flat = nesc.ravel()
+            cand = [(-int(flat[k]), k) for k in range(W * H) if flat[k] > 0]
+            top = -min(cand)[0] if cand else 0
+            if (len(cand) >= 41 and top <= mxit // 2) or mxit >= MXIT_CAP:
+                break
+            mxit *= 2
+        if not cand:
+            raise RuntimeError("walk stalled: no escaping pixel")
+        cand.sort()
+        k = int(math.ceil(0.1 * len(cand)))
+        pick = cand[rng.next() % k][1]
+        i, j = divmod(pick, W)
+        dw = Decimal(2) / scale
+        dx = dw / (W - 1) * 2
+        dh = dw / W * H
+        dy = dh / (H - 1) * 2
+        cx = cx - dw + dx * j
+        cy = cy - dh + dy * i
+        w = w / 2
+        step += 1
+        if step % 50 == 0:
+            log(f"seed {seed}: step {step} width {float(w.log10()):.1f} probe_mxit {mxit} ({time.time() - t0:.0f}s)")
+    return out
+
+
+def main():
+    seed = int(sys.argv[1])
+    depths = [int(x) for x in sys.argv[2:]]
+    path = os.path.join(HERE, f"walk_seed{seed}.json")
+
+    def save(res):
+        old = json.load(open(path)) if os.path.exists(path) else {}
+        old.update(res)
+        old["_definition"] = "tests/golden/make_walk.py (seeded boundary walk)"
+        json.dump(old, open(path, "w"), indent=1)
+        print("wrote", path, flush=True)
+
+    save(walk(seed, depths, log=lambda s: print(s, flush=True), on_depth=save))
+
+
+if __name__ == "__main__":
+    main()
