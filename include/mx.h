@@ -35,7 +35,8 @@ enum {
 enum {
     MX_FORM_FE64 = 0,  /* plain: IEEE FP64 + FP64-mantissa floatexp, engine op order */
     MX_FORM_FE32 = 1,  /* mixed: FP32-mantissa floatexp delta loop, FP64 decisions */
-    MX_FORM_DOUBLE = 2 /* plain double perturbation (scale 1e6..1e280) */
+    MX_FORM_DOUBLE = 2, /* plain double perturbation (scale 1e6..1e280) */
+    MX_FORM_SC64 = 3    /* the mixed form's power-of-two-scaled algorithm with an FP64 delta */
 };
 
 /* ColoringMethod bits accepted here (mandel_perturbation.h:30-49). */

@@ -32,8 +32,8 @@ struct MxRefDev {
     const float2* zf32;
     const double* bla_r2;
     const double4* bla_coef;
-    const float* bla_r2f;   // mixed form: radius^2 in float (rounded down)
-    const float4* bla_coeff; // mixed form: coefficients in float
+    const float4* bla_coeff; // mixed form: coefficient mantissas (ar, ai) and (br, bi), FP32
+    const int2* bla_coefe;   // mixed form: exponents of A and of B
     int mx_ref_it;
     int nlev;
     int minlevel;

@@ -5,6 +5,7 @@
 // pixel's own value.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <chrono>
 #include <climits>
 #include <cmath>
@@ -129,7 +130,9 @@ extern "C" int mx_ref_upload(mx_ctx* ctx, const mx_ref_desc* d, mx_ref** out) {
     size_t o_zf32 = align_up(o_zfe + 8 * len);
     size_t o_r2 = align_up(o_zf32 + 8 * len);
     size_t o_coef = align_up(o_r2 + 8 * (size_t)total);
-    size_t o_lev = align_up(o_coef + 32 * (size_t)total);
+    size_t o_cf = align_up(o_coef + 32 * (size_t)total);
+    size_t o_ce = align_up(o_cf + 16 * (size_t)total);
+    size_t o_lev = align_up(o_ce + 8 * (size_t)total);
     size_t bytes = align_up(o_lev + 8 * MX_MAXLEV);
     auto* ref = new (std::nothrow) mx_ref();
     if (!ref) return MX_E_NOMEM;
@@ -143,6 +146,19 @@ extern "C" int mx_ref_upload(mx_ctx* ctx, const mx_ref_desc* d, mx_ref** out) {
     char* b = (char*)ref->block;
     std::vector<float> zf32(2 * len);
     for (size_t k = 0; k < 2 * len; ++k) zf32[k] = (float)d->zf[k];
+    // FP32-floatexp copy of the coefficients for the mixed form: A and B each get one
+    // exponent (that of their larger component) and FP32 mantissas in [0.5, 1)
+    std::vector<float4> cf((size_t)total);
+    std::vector<int2> ce((size_t)total);
+    for (int64_t q = 0; q < total; ++q) {
+        const double* c = d->bla_coef + 4 * q;
+        int ea = 0, eb = 0;
+        const double a = std::max(std::fabs(c[0]), std::fabs(c[1])), bb = std::max(std::fabs(c[2]), std::fabs(c[3]));
+        if (a != 0.0 && std::isfinite(a)) (void)std::frexp(a, &ea);
+        if (bb != 0.0 && std::isfinite(bb)) (void)std::frexp(bb, &eb);
+        cf[q] = make_float4((float)std::ldexp(c[0], -ea), (float)std::ldexp(c[1], -ea), (float)std::ldexp(c[2], -eb), (float)std::ldexp(c[3], -eb));
+        ce[q] = make_int2(ea, eb);
+    }
     std::vector<int2> lev(MX_MAXLEV, make_int2(0, 0));
     for (int p = 0; p < d->nlev; ++p) lev[p] = make_int2(d->lev_off[p], d->lev_size[p]);
     hipError_t e = hipSuccess;
@@ -156,6 +172,8 @@ extern "C" int mx_ref_upload(mx_ctx* ctx, const mx_ref_desc* d, mx_ref** out) {
     if (d->nlev > 0) {
         cp(o_r2, d->bla_r2, 8 * (size_t)total);
         cp(o_coef, d->bla_coef, 32 * (size_t)total);
+        cp(o_cf, cf.data(), 16 * (size_t)total);
+        cp(o_ce, ce.data(), 8 * (size_t)total);
     }
     cp(o_lev, lev.data(), 8 * MX_MAXLEV);
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
@@ -173,6 +191,8 @@ extern "C" int mx_ref_upload(mx_ctx* ctx, const mx_ref_desc* d, mx_ref** out) {
     R.zf32 = (const float2*)(b + o_zf32);
     R.bla_r2 = (const double*)(b + o_r2);
     R.bla_coef = (const double4*)(b + o_coef);
+    R.bla_coeff = (const float4*)(b + o_cf);
+    R.bla_coefe = (const int2*)(b + o_ce);
     R.mx_ref_it = d->len - 1;
     R.nlev = d->nlev;
     R.rmax2 = d->rmax2;
@@ -194,7 +214,7 @@ extern "C" void mx_ref_release(mx_ref* ref) {
 extern "C" int mx_plan_create(mx_ctx* ctx, const mx_ref* ref, const mx_tile* t, mx_plan** out) {
     if (!ctx || !ref || !t || !out || t->width < 1 || t->height < 1 || t->stride < 1 || t->max_iter < 1) return MX_E_INVALID;
     if (!t->dcol_m || !t->dcol_e || !t->drow_m || !t->drow_e) return MX_E_INVALID;
-    if (t->form != MX_FORM_FE64) {
+    if (t->form != MX_FORM_FE64 && t->form != MX_FORM_FE32 && t->form != MX_FORM_SC64) {
         ctx->set_err("this kernel form is not built");
         return MX_E_UNSUPPORTED;
     }
@@ -281,7 +301,12 @@ extern "C" int mx_plan_launch(mx_plan* plan) {
     MX_HIP(hipGetLastError());
     MxRefDev R = plan->ref->dev;
     R.minlevel = plan->minlevel;
-    MX_HIP(mx_launch_fe64(R, plan->ref->lev, T, plan->blocks, plan->threads, ctx->stream));
+    if (plan->form == MX_FORM_FE32)
+        MX_HIP(mx_launch_fe32(R, plan->ref->lev, T, plan->blocks, plan->threads, ctx->stream));
+    else if (plan->form == MX_FORM_SC64)
+        MX_HIP(mx_launch_sc64(R, plan->ref->lev, T, plan->blocks, plan->threads, ctx->stream));
+    else
+        MX_HIP(mx_launch_fe64(R, plan->ref->lev, T, plan->blocks, plan->threads, ctx->stream));
     MX_HIP(hipEventRecord(plan->ev1, ctx->stream));
     return MX_OK;
 }
@@ -355,7 +380,7 @@ extern "C" int mx_compute(mx_ctx* ctx, const mx_request* req, mx_frame_info* inf
         ctx->set_err(err);
         return rc;
     }
-    if (req->form != MX_FORM_FE64) {
+    if (req->form != MX_FORM_FE64 && req->form != MX_FORM_FE32 && req->form != MX_FORM_SC64) {
         ctx->set_err("this kernel form is not built");
         return MX_E_UNSUPPORTED;
     }

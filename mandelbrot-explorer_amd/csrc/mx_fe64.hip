@@ -40,7 +40,7 @@ struct Counters {
     unsigned long long steps, applies, skipped, rebases;
 };
 
-__device__ __forceinline__ void init_pixel(Px& p, Fe dcr, Fe dci) {
+MX_HD void init_pixel(Px& p, Fe dcr, Fe dci) {
     p.dcr = dcr;
     p.dci = dci;
     p.S = fe_sqrt(fe_add(fe_mul(dcr, dcr), fe_mul(dci, dci)));
@@ -91,7 +91,7 @@ struct SJW {
 };
 
 // feRescaledStepUnderflow (mandel_perturbation.cpp:2794-2835)
-__device__ __noinline__ SW step_underflow(Fe Xr, Fe Xi, Fe dcr, Fe dci, Fe S, double wr, double wi) {
+__host__ __device__ __noinline__ SW step_underflow(Fe Xr, Fe Xi, Fe dcr, Fe dci, Fe S, double wr, double wi) {
     const double sm = S.m, sm2 = sm * sm;
     const int se = S.e;
     int e = INT_MIN;
@@ -121,7 +121,7 @@ __device__ __noinline__ SW step_underflow(Fe Xr, Fe Xi, Fe dcr, Fe dci, Fe S, do
 }
 
 // feDerivativeStepUnderflow (mandel_perturbation.cpp:2842-2876)
-__device__ __noinline__ SJW deriv_underflow(Fe zr, Fe zi, Fe SJ, double jr, double ji) {
+__host__ __device__ __noinline__ SJW deriv_underflow(Fe zr, Fe zi, Fe SJ, double jr, double ji) {
     double im = 0.5 / SJ.m;
     int ie = 1 - SJ.e;
     if (im >= 1.0) {
@@ -149,7 +149,7 @@ __device__ __noinline__ SJW deriv_underflow(Fe zr, Fe zi, Fe SJ, double jr, doub
 
 // Overflow-safe FloatExp landing of a BLA skip (mandel_perturbation.cpp:2943-2957).
 // Returns false when the landing escapes.
-__device__ __noinline__ SW bla_land_fe(const double2* zf, int land, double esc2, double4 bc, Fe S, double wr, double wi, double dr, double di) {
+__host__ __device__ __noinline__ SW bla_land_fe(const double2* zf, int land, double esc2, double4 bc, Fe S, double wr, double wi, double dr, double di) {
     Fe dzr = fe_mul_d(S, wr), dzi = fe_mul_d(S, wi);
     Fe dcrfe = fe_mul_d(S, dr), dcife = fe_mul_d(S, di);
     Fe nzr = fe_add(fe_sub(fe_mul_d(dzr, bc.x), fe_mul_d(dzi, bc.y)), fe_sub(fe_mul_d(dcrfe, bc.z), fe_mul_d(dcife, bc.w)));
@@ -163,7 +163,7 @@ __device__ __noinline__ SW bla_land_fe(const double2* zf, int land, double esc2,
 }
 
 // tryBLAfe (mandel_perturbation.cpp:2883-2970).  s = m-1, reflen = mx_ref_it+1.
-__device__ __forceinline__ int try_bla(const MxRefDev& R, const int2* lev, int s, Fe& S, Fe S2, double& wr, double& wi, double dr, double di, int reflen, double esc2, double4& ab) {
+MX_HD int try_bla(const MxRefDev& R, const int2* lev, int s, Fe& S, Fe S2, double& wr, double& wi, double dr, double di, int reflen, double esc2, double4& ab) {
     if (s < 1) return 0;
     const int maxp = R.nlev - 1;
     int startp = (s == 1) ? maxp : __builtin_ctz((unsigned)(s - 1));
@@ -184,7 +184,7 @@ __device__ __forceinline__ int try_bla(const MxRefDev& R, const int2* lev, int s
         const double nwr = (bc.x * wr - bc.y * wi) + (bc.z * dr - bc.w * di);
         const double nwi = (bc.x * wi + bc.y * wr) + (bc.z * di + bc.w * dr);
         const double BIG = 1e150;
-        if (isfinite(nwr) && isfinite(nwi) && fabs(nwr) < BIG && fabs(nwi) < BIG) {
+        if (__builtin_isfinite(nwr) && __builtin_isfinite(nwi) && fabs(nwr) < BIG && fabs(nwi) < BIG) {
             const double2 X = R.zf[land];
             const double fr = X.x + fe_to_double(fe_mul_d(S, nwr));
             const double fi = X.y + fe_to_double(fe_mul_d(S, nwi));
@@ -212,7 +212,7 @@ __device__ __forceinline__ int try_bla(const MxRefDev& R, const int2* lev, int s
 // One trip of the pixel loop.  Returns true when the sample is finished (value and
 // n_esc written to v/n).  n: escape iteration, -1 = reached mxit, -2 = detected interior.
 template <bool EDE, bool INTERIOR>
-__device__ __forceinline__ bool trip(const MxRefDev& R, const int2* lev, const MxTileDev& T, Px& p, double& v, int& n, Counters& c) {
+MX_HD bool trip(const MxRefDev& R, const int2* lev, const MxTileDev& T, Px& p, double& v, int& n, Counters& c) {
     const double ESC2 = T.esc2;
     const int reflen = R.mx_ref_it + 1;
     if (p.iter >= T.mxit) {
@@ -464,7 +464,7 @@ __device__ __forceinline__ unsigned long long wave_sum(unsigned long long x) {
 
 // Map a queue item to a sample.  Items are 8x8 lane tiles laid row-major over the
 // launch grid; returns false for the padding lanes of edge tiles.
-__device__ __forceinline__ bool item_to_sample(const MxTileDev& T, unsigned item, int& si, int& sj) {
+MX_HD bool item_to_sample(const MxTileDev& T, unsigned item, int& si, int& sj) {
     const unsigned tile = item >> 6, l = item & 63;
     const int ty = (int)(tile / (unsigned)T.tiles_x), tx = (int)(tile % (unsigned)T.tiles_x);
     const int gi = ty * 8 + (int)(l >> 3), gj = tx * 8 + (int)(l & 7);

@@ -113,3 +113,5 @@ struct mx_plan {
 };
 
 hipError_t mx_launch_fe64(const MxRefDev& R, const int2* lev, const MxTileDev& T, int blocks, int threads, hipStream_t stream);
+hipError_t mx_launch_fe32(const MxRefDev& R, const int2* lev, const MxTileDev& T, int blocks, int threads, hipStream_t stream);
+hipError_t mx_launch_sc64(const MxRefDev& R, const int2* lev, const MxTileDev& T, int blocks, int threads, hipStream_t stream);

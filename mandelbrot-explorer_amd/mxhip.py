@@ -10,7 +10,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libmxhip.so")
 
 MX_OK, MX_E_INVALID, MX_E_NODEVICE, MX_E_HIP, MX_E_CANCELLED, MX_E_BUSY, MX_E_NOMEM, MX_E_UNSUPPORTED = 0, -1, -2, -3, -4, -5, -6, -7
-MX_FORM_FE64, MX_FORM_FE32, MX_FORM_DOUBLE = 0, 1, 2
+MX_FORM_FE64, MX_FORM_FE32, MX_FORM_DOUBLE, MX_FORM_SC64 = 0, 1, 2, 3
 MX_EXTERIOR_DIST_EST = 2
 ERRORS = {-1: "invalid", -2: "no device", -3: "hip error", -4: "cancelled", -5: "busy", -6: "out of memory", -7: "unsupported"}
 
