@@ -1,0 +1,99 @@
+"""Host side of the product without a GPU: the C ABI library loads and exports every
+declared symbol; frame preparation (grid, GMP reference orbit, BLA table, dc tables)
+equals the oracle's restatement; requests outside the path are refused."""
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+import cases
+import mxhip
+import mxo
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def test_library_exports_every_declared_symbol():
+    hdr = open(os.path.join(ROOT, "include", "mx.h")).read()
+    declared = sorted(set(re.findall(r"^\s*(?:int|void|const char\*)\s+(mx_\w+)\(", hdr, re.M)))
+    assert declared and set(declared) == set(mxhip.EXPORTS)
+    lib = mxhip.lib()
+    for name in declared:
+        assert hasattr(lib, name), name
+    assert lib.mx_version() == 1
+
+
+@pytest.mark.parametrize("name", ["extref875", "minibrot875", "exterior1000", "subpixel737"])
+def test_prepare_matches_oracle(name):
+    cre, cim, scale, W, H, mxit, cm = cases.FRAMES[name]
+    fv = mxhip.frame_prepare(cre, cim, scale, W, H, mxit, c_method=cm)
+    val, nesc, out, info = mxo.compute_fe(cre, cim, scale, W, H, mxit, c_method=cm)
+    assert fv["ref_it"] == info["ref_it"] and fv["ref_virtual"] == info["ref_virtual"]
+    assert (fv["ref_x"], fv["ref_y"]) == (info["ref_x"], info["ref_y"])
+    # the oracle's orbit at the same point: rebuild it from the grid the oracle uses
+    (dxm, dxe), (dym, dye) = mxo.grid_info(cre, cim, scale, W, H)
+    assert (fv["dx_m"], fv["dx_e"], fv["dy_m"], fv["dy_e"]) == (dxm, dxe, dym, dye)
+    # BLA tables built by the product equal the oracle's build on the same orbit, bit for bit
+    b = mxo.bla_tables(fv["zf"], fv["len"], fv["dcmax"])
+    assert b["nlev"] == fv["nlev"]
+    assert np.array_equal(b["off"], fv["lev_off"]) and np.array_equal(b["size"], fv["lev_size"])
+    assert np.array_equal(b["r2"].view(np.int64), fv["bla_r2"].view(np.int64))
+    assert np.array_equal(b["coef"].view(np.int64), fv["bla_coef"].view(np.int64))
+    assert b["rmax2"] == fv["rmax2"]
+    # dc tables: fe_mul_d(dxfe, j - ref_x) (mandel_perturbation.cpp:2160-2163)
+    am, ae = mxo.fe_op(1, np.full(W, dxm), np.full(W, dxe, np.int64), np.arange(W) - fv["ref_x"], None)
+    assert np.array_equal(am, fv["dcol_m"]) and np.array_equal(ae, fv["dcol_e"])
+    # the oracle's per-pixel loop on the product's orbit reproduces the oracle's own frame
+    orb = dict(zf=fv["zf"], zre_m=fv["zfe_m"][0::2], zre_e=fv["zfe_e"][0::2], zim_m=fv["zfe_m"][1::2], zim_e=fv["zfe_e"][1::2])
+    bla = dict(nlev=b["nlev"], off=b["off"], size=b["size"], r2=b["r2"], l=b["l"], coef=b["coef"], rmax2=b["rmax2"])
+    jj, ii = np.meshgrid(np.arange(W), np.arange(H))
+    v2, n2, _ = mxo.pixels_fe(orb, bla, fv["dcol_m"][jj.ravel()], fv["dcol_e"][jj.ravel()], fv["drow_m"][ii.ravel()],
+                              fv["drow_e"][ii.ravel()], mxit, fv["esc2"], fv["dx_m"], fv["dx_e"], ede=bool(cm & 2),
+                              use_interior=bool(info["interior_used"]), bla_minlevel=fv["bla_minlevel"])
+    mask = np.ones((H, W), bool)
+    if not fv["ref_virtual"]:
+        mask[H // 2, W // 2] = False
+        assert fv["ref_pixel_n"] == nesc[H // 2, W // 2]
+        assert fv["ref_pixel_value"] == val[H // 2, W // 2]
+    assert np.array_equal(n2.reshape(H, W)[mask], nesc[mask])
+    assert np.array_equal(v2.reshape(H, W)[mask], val[mask])
+
+
+def test_orbit_shadows_equal_mpfr_truncation():
+    """The product's GMP orbit shadows equal the oracle's MPFR orbit truncated to 53
+    bits, entry by entry.  For an odd frame the centre pixel (H/2, W/2) is the view
+    centre, c0 + dx*(W-1)/2, up to the truncation residue of the grid arithmetic
+    (|residue| < 2^-(precision-64)); entries that are exactly 0 in the orbit of the
+    exact centre c = i carry that residue instead and are compared as zeros."""
+    cre, cim, scale = "0", "1", cases.pow10(300)
+    W, H, mxit = 65, 33, 5000
+    fv = mxhip.frame_prepare(cre, cim, scale, W, H, mxit)
+    o = mxo.orbit(cre, cim, fv["precision_bits"], mxit)
+    assert o["ref_it"] == fv["ref_it"] == mxit
+    assert np.array_equal(o["zf"], fv["zf"])
+    for comp, (om, oe) in ((0, ("zre_m", "zre_e")), (1, ("zim_m", "zim_e"))):
+        pm, pe = fv["zfe_m"][comp::2], fv["zfe_e"][comp::2]
+        residue = pe < -(fv["precision_bits"] - 80)
+        assert (o[om][residue] == 0).all()
+        assert np.array_equal(o[om][~residue], pm[~residue]) and np.array_equal(o[oe][~residue], pe[~residue])
+
+
+@pytest.mark.parametrize("kw,rc", [(dict(W=1), mxhip.MX_E_INVALID), (dict(mxit=0), mxhip.MX_E_INVALID),
+                                   (dict(scale="1000000"), mxhip.MX_E_UNSUPPORTED),
+                                   (dict(c_method=8), mxhip.MX_E_UNSUPPORTED), (dict(row_base=5), mxhip.MX_E_INVALID),
+                                   (dict(scale="-4" + "0" * 300), mxhip.MX_E_INVALID)])
+def test_invalid_requests_are_refused(kw, rc):
+    args = dict(cre="0", cim="1", scale=cases.pow10(300), W=16, H=8, mxit=1000, c_method=0, row_base=0)
+    args.update(kw)
+    with pytest.raises(mxhip.MxError) as e:
+        mxhip.frame_prepare(args["cre"], args["cim"], args["scale"], args["W"], args["H"], args["mxit"],
+                            c_method=args["c_method"], row_base=args["row_base"])
+    assert e.value.rc == rc
+
+
+def test_backend_factory_refuses_unknown_names():
+    drv = os.path.join(ROOT, "mandelbrot-explorer_amd", "_build", "backend_driver")
+    r = subprocess.run([drv, "names"], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, r.stderr
