@@ -259,7 +259,7 @@ def cpu_baseline(fv, fe64, mxit, ede, interior, seconds, center):
     orb = dict(zf=fv["zf"], zre_m=fv["zfe_m"][0::2], zre_e=fv["zfe_e"][0::2], zim_m=fv["zfe_m"][1::2], zim_e=fv["zfe_e"][1::2])
     bla = None
     if fv["nlev"]:
-        b = mxo.bla_tables(fv["zf"], fv["len"], fv["dcmax"])
+        b = mxo.bla_tables(fv["zf"], fv["len"] - 1, fv["dcmax"])
         bla = dict(nlev=b["nlev"], off=b["off"], size=b["size"], r2=b["r2"], l=b["l"], coef=b["coef"], rmax2=b["rmax2"])
     threads = min(os.cpu_count() or 1, 16)
 

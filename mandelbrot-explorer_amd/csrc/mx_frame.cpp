@@ -200,7 +200,8 @@ void Frame::build_bla(double dcmax_in) {
     bla_coef.clear();
     rmax2 = 0.0;
     dcmax = dcmax_in;
-    const int reflen = ref_it + 1;
+    /* buildBLA(ref_it): level 0 covers reference indices 1..ref_it-1 (MP.cpp:1632, 2618) */
+    const int reflen = ref_it;
     if (reflen < 3) return;
     const double eps = std::ldexp(1.0, -53);
     int64_t n = reflen - 1, tot = 0;

@@ -994,7 +994,7 @@ MXO_EXPORT int mxo_compute_fe(mxo_frame* f) {
         double ay = ref_y, by = (double)(H - 1) - ref_y;
         double mxy = (ay > by ? ay : by) + 0.5;
         double dcmax = sqrt(dxf * dxf * mxx * mxx + dyf * dyf * mxy * mxy);
-        bla_build(&bla, o.zf, o.dfr, o.dfi, o.ref_it + 1, dcmax, f->bla_eps > 0 ? f->bla_eps : ldexp(1.0, -53), 0);
+        bla_build(&bla, o.zf, o.dfr, o.dfi, o.ref_it, dcmax, f->bla_eps > 0 ? f->bla_eps : ldexp(1.0, -53), 0);
     }
 
     mxo_ctx ctx;
@@ -1271,4 +1271,578 @@ MXO_EXPORT void mxo_grid_info(const char* cre, const char* cim, const char* scal
     *dy_m = b.m;
     *dy_e = b.e;
     grid_clear(&g);
+}
+
+/* ========================================================================= */
+/* plain-double perturbation path (scale 1e6..1e280)                          */
+/* ========================================================================= */
+
+#define MXO_SA_N 30
+
+typedef struct {
+    double r, i;
+} mxo_c;
+
+/* The engine uses std::complex<double> here.  Its rounding is library-specific
+ * (MSVC vs libstdc++), so this restatement fixes explicit formulas, shared with the
+ * GPU path: textbook multiplication, and the Smith division of libgcc's __divdc3
+ * for finite operands. */
+static inline mxo_c c_add(mxo_c a, mxo_c b) {
+    mxo_c r = {a.r + b.r, a.i + b.i};
+    return r;
+}
+static inline mxo_c c_mul(mxo_c a, mxo_c b) {
+    mxo_c r = {a.r * b.r - a.i * b.i, a.r * b.i + a.i * b.r};
+    return r;
+}
+static inline mxo_c c_div(mxo_c a, mxo_c b) {
+    mxo_c r;
+    if (fabs(b.r) < fabs(b.i)) {
+        double ratio = b.r / b.i;
+        double denom = (b.r * ratio) + b.i;
+        r.r = ((a.r * ratio) + a.i) / denom;
+        r.i = ((a.i * ratio) - a.r) / denom;
+    } else {
+        double ratio = b.i / b.r;
+        double denom = (b.i * ratio) + b.r;
+        r.r = ((a.i * ratio) + a.r) / denom;
+        r.i = (a.i - (a.r * ratio)) / denom;
+    }
+    return r;
+}
+
+/* get_exp(long double) (float_math.cpp:24-29) */
+static inline int sa_get_exp(double x) {
+    if (x == 0.0) return -0x40000000;
+    int e;
+    (void)frexp(x, &e);
+    return e - 1;
+}
+
+/* SACheckMagnitude (MP.cpp:4307-4332) */
+static int sa_check(const mxo_c* A) {
+    int pre_mn[MXO_SA_N], suf_mx[MXO_SA_N];
+    int re = sa_get_exp(A[0].r), im = sa_get_exp(A[0].i);
+    pre_mn[0] = re < im ? re : im;
+    re = sa_get_exp(A[MXO_SA_N - 1].r);
+    im = sa_get_exp(A[MXO_SA_N - 1].i);
+    suf_mx[MXO_SA_N - 1] = re > im ? re : im;
+    for (int i = 1; i < MXO_SA_N; ++i) {
+        re = sa_get_exp(A[i].r);
+        im = sa_get_exp(A[i].i);
+        int mn = re < im ? re : im;
+        pre_mn[i] = pre_mn[i - 1] > mn ? pre_mn[i - 1] : mn;
+    }
+    for (int i = MXO_SA_N - 2; i >= 0; --i) {
+        re = sa_get_exp(A[i].r);
+        im = sa_get_exp(A[i].i);
+        int mx = re > im ? re : im;
+        suf_mx[i] = suf_mx[i + 1] > mx ? suf_mx[i + 1] : mx;
+    }
+    for (int i = 0; i < MXO_SA_N - 10; ++i)
+        if (pre_mn[i] - suf_mx[i + 1] >= 80) return i > 5 ? i : 5;
+    return -1;
+}
+
+typedef struct {
+    mxo_c A[MXO_SA_N]; /* _Adf_old */
+    mxo_c delta;       /* _SA_delta */
+    int order, it, flag;
+} mxo_sa;
+
+/* One series step (calCoefficient's SA block, MP.cpp:4273-4302): z = _zf[i-1]. */
+static void sa_step(mxo_sa* sa, mxo_c z, int i) {
+    if (!sa->flag) return;
+    mxo_c N[MXO_SA_N];
+    mxo_c two = {2.0, 0.0};
+    mxo_c z2 = c_mul(two, z);
+    for (int j = 0; j < MXO_SA_N; ++j) {
+        N[j] = c_mul(z2, sa->A[j]);
+        for (int k = 0; k < j / 2; ++k) N[j] = c_add(N[j], c_mul(c_mul(two, sa->A[k]), sa->A[j - k - 1]));
+        if (j % 2) N[j] = c_add(N[j], c_mul(sa->A[j / 2], sa->A[j / 2]));
+    }
+    N[0] = c_add(N[0], sa->delta);
+    int order = sa_check(N);
+    if (order < 0) {
+        sa->flag = 0;
+    } else {
+        sa->it = i;
+        sa->order = order;
+        memcpy(sa->A, N, sizeof N);
+    }
+}
+
+/* Horner seed of the double loop (MP.cpp:2319-2323) */
+static mxo_c sa_seed(const mxo_sa* sa, mxo_c dc) {
+    mxo_c dz = {0.0, 0.0};
+    for (int x = sa->order; x >= 0; --x) {
+        dz = c_add(dz, sa->A[x]);
+        dz = c_mul(dz, dc);
+        dz = c_div(dz, sa->delta);
+    }
+    return dz;
+}
+
+/* tryBLA (MP.cpp:2714-2755), no derivative */
+static int try_bla_d(const mxo_bla* b, int minlevel, const double* zf, int s, double* dzr, double* dzi, double dcr, double dci, double esc2, int mx_ref_it) {
+    if (s < 1 || !b || b->nlev == 0) return 0;
+    double zmag2 = *dzr * *dzr + *dzi * *dzi;
+    int maxp = b->nlev - 1;
+    int startp = (s == 1) ? maxp : tzcnt32((unsigned)(s - 1));
+    if (startp > maxp) startp = maxp;
+    if (startp < minlevel) return 0;
+    for (int p = startp; p >= minlevel; --p) {
+        int64_t i = (int64_t)((s - 1) >> p);
+        if (i >= b->size[p]) continue;
+        int64_t k = b->off[p] + i;
+        double r2 = b->r2[k];
+        if (r2 <= 0 || zmag2 >= r2) continue;
+        int land = s + (int)b->l[k];
+        if (land >= mx_ref_it - 1) continue;
+        const double* c = b->coef + 4 * k;
+        double nzr = c[0] * *dzr - c[1] * *dzi + c[2] * dcr - c[3] * dci;
+        double nzi = c[0] * *dzi + c[1] * *dzr + c[2] * dci + c[3] * dcr;
+        double fr = zf[2 * land] + nzr, fi = zf[2 * land + 1] + nzi;
+        if (fr * fr + fi * fi > esc2) return 0;
+        *dzr = nzr;
+        *dzi = nzi;
+        return (int)b->l[k];
+    }
+    return 0;
+}
+
+typedef struct {
+    const double* zf;  /* interleaved double orbit, reflen entries */
+    const double* z_m3; /* |Z|^2/1e6 per entry */
+    int mx_ref_it;
+    const mxo_bla* bla;
+    int bla_minlevel;
+    double esc2;
+    int use_interior;
+    double interior_eps2;
+    int interior_confirm;
+    int int_rep;
+    int sa_it;
+} mxo_dctx;
+
+/* The scalar double loop of stepParallel (MP.cpp:2331-2584), smooth output.
+ * Returns the value; *n_out = escape iteration, -1 reached mxit, -2 interior
+ * detected, -3 unresolved (glitch or end of reference; *glitch_p set). */
+static double pixel_double(const mxo_dctx* c, mxo_c dc, mxo_c dz0, int mxit, int* n_out, double* glitch_p, mxo_stats* st) {
+    const double* Z = c->zf;
+    const int mx_ref_it = c->mx_ref_it;
+    int j = c->sa_it + 1;
+    double dzr = dz0.r, dzi = dz0.i, dcr = dc.r, dci = dc.i;
+    int k = j;
+    double zsr = 1e30, zsi = 1e30;
+    int save_j = j, period_win = 1;
+    int conf_P = 0, conf_next = 0, conf_count = 0, conf_giveup = 0;
+    double conf_D2 = 0, conf_zr = 0, conf_zi = 0;
+    const int detect_interior = c->use_interior;
+    const int int_rep = c->int_rep && detect_interior;
+    double sdzr = 1e300, sdzi = 1e300;
+    int sk = -1, save_j2 = j, twin2 = 1;
+    *glitch_p = -1;
+    const double LG = log(2);
+    while (j < mxit) {
+        if (int_rep) {
+            if (k == sk && dzr == sdzr && dzi == sdzi) {
+                *n_out = -2;
+                return -2.0;
+            }
+            if (j - save_j2 >= twin2) {
+                sdzr = dzr;
+                sdzi = dzi;
+                sk = k;
+                save_j2 = j;
+                twin2 += twin2;
+            }
+        }
+        if (c->bla && dzr * dzr + dzi * dzi < c->bla->rmax2) {
+            int skip = try_bla_d(c->bla, c->bla_minlevel, Z, k - 1, &dzr, &dzi, dcr, dci, c->esc2, mx_ref_it);
+            if (skip > 0) {
+                k += skip;
+                j += skip;
+                st->bla_applies++;
+                st->bla_skipped += skip;
+                zsr = 1e30;
+                zsi = 1e30;
+                save_j = j;
+                period_win = 1;
+                conf_P = 0;
+                conf_giveup = 0;
+                continue;
+            }
+        }
+        double dzr2 = dzr * dzr, dzi2 = dzi * dzi, tmp;
+        if (k == 0) {
+            tmp = dzr2 - dzi2 + dcr;
+            dzi = dzr * dzi * 2 + dci;
+        } else {
+            double xr = Z[2 * (k - 1)], xi = Z[2 * (k - 1) + 1];
+            tmp = (dzr * xr - dzi * xi) * 2 + dzr2 - dzi2 + dcr;
+            dzi = (dzr * xi + dzi * xr) * 2 + dzr * dzi * 2 + dci;
+        }
+        dzr = tmp;
+        double zr = dzr + Z[2 * k], zi = dzi + Z[2 * k + 1];
+        double zrad = zr * zr + zi * zi;
+        ++k;
+        st->steps++;
+        if (zrad > c->esc2) {
+            *n_out = j + 1;
+            return j + 1 - log(log(zrad) / 2 / LG) / LG;
+        }
+        if (detect_interior) {
+            if (conf_P > 0) {
+                conf_D2 *= 4 * zrad;
+                if (conf_D2 > 1e18) conf_D2 = 1e18;
+                if (j >= conf_next) {
+                    double pr = zr - conf_zr, pi = zi - conf_zi;
+                    int ret = (pr * pr + pi * pi < c->interior_eps2 * zrad);
+                    if (ret && conf_D2 < 1.0) {
+                        if (++conf_count >= c->interior_confirm) {
+                            *n_out = -2;
+                            return -2.0;
+                        }
+                        conf_zr = zr;
+                        conf_zi = zi;
+                        conf_D2 = 1;
+                        conf_next = j + conf_P;
+                    } else {
+                        conf_P = 0;
+                        ++conf_giveup;
+                    }
+                }
+            } else if ((j & 15) == 0 && conf_giveup < 3) {
+                double pr = zr - zsr, pi = zi - zsi;
+                if (pr * pr + pi * pi < c->interior_eps2 * zrad) {
+                    conf_P = j - save_j;
+                    if (conf_P < 1) conf_P = 1;
+                    conf_zr = zr;
+                    conf_zi = zi;
+                    conf_D2 = 1;
+                    conf_next = j + conf_P;
+                    conf_count = 0;
+                }
+                if (j - save_j >= period_win) {
+                    zsr = zr;
+                    zsi = zi;
+                    save_j = j;
+                    period_win += period_win;
+                }
+            }
+        }
+        /* The engine reads _z_m3[j] with j the pixel's iteration, which can run past
+         * the entries this reference filled; the index is clamped to the last one. */
+        const int jm = j < mx_ref_it ? j : mx_ref_it;
+        if (zrad < dzr * dzr + dzi * dzi || k == mx_ref_it) {
+            if ((dzr * dzr + dzi * dzi) / zrad > 10000000) {
+                *glitch_p = zrad / c->z_m3[jm];
+                *n_out = -3;
+                return -10.0;
+            }
+            dzr = zr;
+            dzi = zi;
+            k = 0;
+            st->rebases++;
+        }
+        if (k == mx_ref_it) {
+            *glitch_p = (double)zrad / c->z_m3[jm];
+            *n_out = -3;
+            return -10.0;
+        }
+        ++j;
+    }
+    *n_out = -1;
+    return -2.0;
+}
+
+/* Reference orbit of the double path with its series coefficients (createRef +
+ * calCoefficient, MP.cpp:4014-4187, 4189-4305).  sa keeps `order` from the previous
+ * reference, as the engine's _SA_order does. */
+static void orbit_build_sa(mxo_orbit* o, mxo_sa* sa, mpfr_srcptr cre, mpfr_srcptr cim, mpfr_prec_t prec, int mxit, double escape_radius, mxo_c delta) {
+    orbit_build(o, cre, cim, prec, mxit, escape_radius, 0, 0);
+    for (int i = 0; i < MXO_SA_N; ++i) sa->A[i].r = sa->A[i].i = 0.0;
+    sa->A[0] = delta;
+    sa->delta = delta;
+    sa->flag = 1;
+    sa->it = 0;
+    /* calCoefficient(i) runs the SA step after a non-escaping Z[i] */
+    const int last = o->escaped ? o->ref_it - 1 : o->ref_it;
+    for (int i = 1; i <= last && sa->flag; ++i) {
+        mxo_c z = {o->zf[2 * (i - 1)], o->zf[2 * (i - 1) + 1]};
+        sa_step(sa, z, i);
+    }
+}
+
+typedef struct {
+    const char* cre;
+    const char* cim;
+    const char* scale;
+    int W, H, mxit;
+    float escape_radius;
+    int prec_bits;
+    int use_bla, bla_minskip, interior, threads;
+    double* val;
+    int32_t* n_esc;
+    float* out;
+    int passes;          /* reference passes used */
+    int escalate;        /* 1: the frame belongs to the floatexp path (MP.cpp:1336-1355) */
+    int sa_it0, sa_order0;
+    long long steps, bla_applies, bla_skipped, rebases;
+} mxo_dframe;
+
+MXO_EXPORT int mxo_compute_dbl(mxo_dframe* f) {
+    const int W = f->W, H = f->H, mxit = f->mxit;
+    const double esc2 = (double)(f->escape_radius * f->escape_radius);
+    const double escR = (double)f->escape_radius;
+    mxo_grid g;
+    grid_init(&g, f->cre, f->cim, f->scale, W, H, 0, 0, f->prec_bits);
+    mpfr_t t;
+    mpfr_init2(t, f->prec_bits);
+    mxo_c delta;
+    mpfr_mul_ui(t, g.dx, W, MPFR_RNDN);
+    mpfr_div_ui(t, t, 2, MPFR_RNDN);
+    delta.r = mpfr_to_ld_trunc(t);
+    mpfr_mul_ui(t, g.dy, H, MPFR_RNDN);
+    mpfr_div_ui(t, t, 2, MPFR_RNDN);
+    delta.i = mpfr_to_ld_trunc(t);
+    const double dxf = fabs(mpfr_to_ld_trunc(g.dx)), dyf = fabs(mpfr_to_ld_trunc(g.dy));
+    /* per-pixel dc relative to a reference pixel, truncated from full precision */
+    double* dcol = (double*)malloc(sizeof(double) * W);
+    double* drow = (double*)malloc(sizeof(double) * H);
+    int ref_i = H / 2, ref_j = W / 2;
+    mpfr_t rre, rim;
+    mpfr_inits2(f->prec_bits, rre, rim, (mpfr_ptr)0);
+    grid_pixel_c(&g, ref_i, ref_j, 0, 0, 1, rre, rim);
+    mxo_orbit o;
+    mxo_sa sa;
+    memset(&sa, 0, sizeof sa);
+    orbit_build_sa(&o, &sa, rre, rim, f->prec_bits, mxit, escR, delta);
+    f->sa_it0 = sa.it;
+    f->sa_order0 = sa.order;
+    f->escalate = 0;
+    {
+        mpfr_t sc;
+        mpfr_init2(sc, f->prec_bits);
+        mpfr_set_str(sc, f->scale, 10, MPFR_RNDN);
+        if (mpfr_cmp_d(sc, 1e150) > 0) {
+            double refmin2 = 1e300;
+            int uf = 0;
+            for (int q = 1; q <= o.ref_it; ++q) {
+                double m2 = o.z_m3[q] * 1e6;
+                if (m2 <= 0.0) {
+                    uf = 1;
+                    break;
+                }
+                if (m2 < refmin2) refmin2 = m2;
+            }
+            double dcmax = 2.0 / mpfr_get_d(sc, MPFR_RNDZ);
+            if (uf || sqrt(refmin2) * dcmax < 1e-300) f->escalate = 1;
+        }
+        mpfr_clear(sc);
+    }
+    int* idx = (int*)malloc(sizeof(int) * (size_t)W * H);
+    int* rest = (int*)malloc(sizeof(int) * (size_t)W * H);
+    double* gp = (double*)malloc(sizeof(double) * (size_t)W * H);
+    for (int p = 0; p < W * H; ++p) {
+        f->val[p] = -10.0;
+        f->n_esc[p] = INT32_MIN;
+    }
+    mxo_bla bla;
+    memset(&bla, 0, sizeof bla);
+    mxo_dctx ctx;
+    memset(&ctx, 0, sizeof ctx);
+    ctx.esc2 = esc2;
+    ctx.bla_minlevel = bla_minlevel_from_skip(f->bla_minskip > 0 ? f->bla_minskip : 8);
+    ctx.use_interior = f->interior;
+    ctx.interior_eps2 = 3e-9 * 3e-9;
+    ctx.interior_confirm = 4;
+    ctx.int_rep = 1;
+    long long steps = 0, ap = 0, sk = 0, rb = 0;
+
+    /* set-up for the current reference: BLA, dc tables */
+#define MXO_SETUP_REF()                                                                                       \
+    do {                                                                                                      \
+        if (f->use_bla) {                                                                                     \
+            bla_free(&bla);                                                                                   \
+            double mxx = (ref_j > (W - 1) - ref_j ? ref_j : (W - 1) - ref_j) + 0.5;                           \
+            double mxy = (ref_i > (H - 1) - ref_i ? ref_i : (H - 1) - ref_i) + 0.5;                           \
+            double dcmax = sqrt(dxf * dxf * mxx * mxx + dyf * dyf * mxy * mxy);                               \
+            bla_build(&bla, o.zf, o.dfr, o.dfi, o.ref_it, dcmax, ldexp(1.0, -53), 0);                     \
+        }                                                                                                     \
+        ctx.zf = o.zf;                                                                                        \
+        ctx.z_m3 = o.z_m3;                                                                                    \
+        ctx.mx_ref_it = o.ref_it;                                                                             \
+        ctx.bla = f->use_bla ? &bla : NULL;                                                                   \
+        ctx.sa_it = sa.it;                                                                                    \
+        for (int jj = 0; jj < W; ++jj) {                                                                      \
+            mpfr_mul_si(t, g.dx, jj - ref_j, MPFR_RNDN);                                                      \
+            dcol[jj] = mpfr_to_ld_trunc(t);                                                                   \
+        }                                                                                                     \
+        for (int ii = 0; ii < H; ++ii) {                                                                      \
+            mpfr_mul_si(t, g.dy, ii - ref_i, MPFR_RNDN);                                                      \
+            drow[ii] = mpfr_to_ld_trunc(t);                                                                   \
+        }                                                                                                     \
+    } while (0)
+
+    /* value of a (non-virtual) reference pixel (MP.cpp:4145-4160, 4179-4185) */
+#define MXO_REF_PIXEL()                                                                                       \
+    do {                                                                                                      \
+        int ne_;                                                                                              \
+        f->val[ref_i * W + ref_j] = ref_pixel_value(&o, mxit, 0, esc2, g.dx, &ne_);                           \
+        f->n_esc[ref_i * W + ref_j] = ne_;                                                                    \
+    } while (0)
+
+    MXO_SETUP_REF();
+    MXO_REF_PIXEL();
+    const int ref0 = ref_i * W + ref_j;
+
+    /* one pass over idx[0..n): resolved pixels written; unresolved kept in rest */
+#define MXO_PASS(nin, nout)                                                                                   \
+    do {                                                                                                      \
+        _Pragma("omp parallel for schedule(dynamic, 16) num_threads(f->threads) reduction(+ : steps, ap, sk, rb)") \
+        for (int q = 0; q < (nin); ++q) {                                                                     \
+            int p = idx[q];                                                                                   \
+            int i = p / W, jx = p % W;                                                                        \
+            mxo_c dc = {dcol[jx], drow[i]};                                                                   \
+            mxo_c dz = sa_seed(&sa, dc);                                                                      \
+            mxo_stats st = {0, 0, 0, 0, 0, 0};                                                                \
+            int ne;                                                                                           \
+            double gpp;                                                                                       \
+            double v = pixel_double(&ctx, dc, dz, mxit, &ne, &gpp, &st);                                      \
+            gp[p] = gpp;                                                                                      \
+            if (ne != -3) {                                                                                   \
+                f->val[p] = v;                                                                                \
+                f->n_esc[p] = ne;                                                                             \
+            }                                                                                                 \
+            steps += st.steps;                                                                                \
+            ap += st.bla_applies;                                                                             \
+            sk += st.bla_skipped;                                                                             \
+            rb += st.rebases;                                                                                 \
+        }                                                                                                     \
+        (nout) = 0;                                                                                           \
+        for (int q = 0; q < (nin); ++q)                                                                       \
+            if (gp[idx[q]] >= 0) rest[(nout)++] = idx[q];                                                     \
+    } while (0)
+
+    if (f->interior) {
+        int n = 0, nr;
+        for (int i = 0; i < H; i += 8)
+            for (int jx = 0; jx < W; jx += 8) idx[n++] = i * W + jx;
+        long long s0 = steps, a0 = ap, k0 = sk, r0 = rb;
+        MXO_PASS(n, nr);
+        steps = s0, ap = a0, sk = k0, rb = r0;
+        int nint = 0;
+        for (int q = 0; q < n; ++q)
+            if (f->n_esc[idx[q]] == -1 || f->n_esc[idx[q]] == -2) ++nint;
+        /* the probe's own centre value (a pixel reference) is restored afterwards */
+        if (nint == 0) ctx.use_interior = 0;
+        for (int q = 0; q < n; ++q) {
+            f->val[idx[q]] = -10.0;
+            f->n_esc[idx[q]] = INT32_MIN;
+        }
+        MXO_REF_PIXEL();
+    }
+    int n = 0;
+    for (int p = 0; p < W * H; ++p)
+        if (p != ref0) idx[n++] = p;
+    int passes = 0;
+    while (n > 0) {
+        int nr;
+        MXO_PASS(n, nr);
+        ++passes;
+        if (nr == 0) break;
+        /* glitch-guided next reference: the unresolved pixel with the smallest
+         * glitch metric, first in row-major order (MP.cpp:2587-2594, 4046-4056) */
+        int best = -1;
+        double mn = 1e100;
+        for (int q = 0; q < nr; ++q)
+            if (gp[rest[q]] < mn) {
+                mn = gp[rest[q]];
+                best = rest[q];
+            }
+        if (best < 0) best = rest[0];
+        ref_i = best / W;
+        ref_j = best % W;
+        orbit_free(&o);
+        grid_pixel_c(&g, ref_i, ref_j, 0, 0, 1, rre, rim);
+        orbit_build_sa(&o, &sa, rre, rim, f->prec_bits, mxit, escR, delta);
+        MXO_REF_PIXEL();
+        n = 0;
+        for (int q = 0; q < nr; ++q)
+            if (rest[q] != best) idx[n++] = rest[q];
+        if (n == 0) break;
+        MXO_SETUP_REF();
+    }
+    f->passes = passes;
+    for (int p = 0; p < W * H; ++p) f->out[p] = (f->n_esc[p] == INT32_MIN) ? -10.f : (f->n_esc[p] < 0) ? -2.f : (float)f->val[p];
+    f->steps = steps;
+    f->bla_applies = ap;
+    f->bla_skipped = sk;
+    f->rebases = rb;
+#undef MXO_SETUP_REF
+#undef MXO_REF_PIXEL
+#undef MXO_PASS
+    free(idx);
+    free(rest);
+    free(gp);
+    free(dcol);
+    free(drow);
+    bla_free(&bla);
+    orbit_free(&o);
+    mpfr_clears(rre, rim, t, (mpfr_ptr)0);
+    grid_clear(&g);
+    return 0;
+}
+
+/* Series coefficients of the reference at (cre, cim) for a frame half-extent delta:
+ * A[2*x], A[2*x+1] = A_x (x < 30); returns the last accepted iteration (_SA_it). */
+MXO_EXPORT int mxo_sa_coeffs(const char* cre, const char* cim, int prec_bits, int mxit, double escape_radius, double delta_r, double delta_i, double* A, int* order) {
+    mpfr_t re, im;
+    mpfr_inits2(prec_bits, re, im, (mpfr_ptr)0);
+    mpfr_set_str(re, cre, 10, MPFR_RNDN);
+    mpfr_set_str(im, cim, 10, MPFR_RNDN);
+    mxo_orbit o;
+    mxo_sa sa;
+    memset(&sa, 0, sizeof sa);
+    mxo_c delta = {delta_r, delta_i};
+    orbit_build_sa(&o, &sa, re, im, prec_bits, mxit, escape_radius, delta);
+    for (int x = 0; x < MXO_SA_N; ++x) {
+        A[2 * x] = sa.A[x].r;
+        A[2 * x + 1] = sa.A[x].i;
+    }
+    *order = sa.order;
+    orbit_free(&o);
+    mpfr_clears(re, im, (mpfr_ptr)0);
+    return sa.it;
+}
+
+/* The series seed (MP.cpp:2319-2323) for n pixel offsets. */
+MXO_EXPORT void mxo_sa_seed(const double* A, int order, double delta_r, double delta_i, int n, const double* dcr, const double* dci, double* out) {
+    mxo_sa sa;
+    memset(&sa, 0, sizeof sa);
+    for (int x = 0; x < MXO_SA_N; ++x) {
+        sa.A[x].r = A[2 * x];
+        sa.A[x].i = A[2 * x + 1];
+    }
+    sa.order = order;
+    sa.delta.r = delta_r;
+    sa.delta.i = delta_i;
+    for (int q = 0; q < n; ++q) {
+        mxo_c dc = {dcr[q], dci[q]};
+        mxo_c dz = sa_seed(&sa, dc);
+        out[2 * q] = dz.r;
+        out[2 * q + 1] = dz.i;
+    }
+}
+
+/* Complex division as the double path's series seed does it (exposed for tests). */
+MXO_EXPORT void mxo_cdiv(int n, const double* a, const double* b, double* out) {
+    for (int q = 0; q < n; ++q) {
+        mxo_c x = {a[2 * q], a[2 * q + 1]}, y = {b[2 * q], b[2 * q + 1]};
+        mxo_c r = c_div(x, y);
+        out[2 * q] = r.r;
+        out[2 * q + 1] = r.i;
+    }
 }

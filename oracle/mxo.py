@@ -39,6 +39,19 @@ class PixelsIn(C.Structure):
     ]
 
 
+class DFrame(C.Structure):
+    _fields_ = [
+        ("cre", C.c_char_p), ("cim", C.c_char_p), ("scale", C.c_char_p),
+        ("W", C.c_int), ("H", C.c_int), ("mxit", C.c_int), ("escape_radius", C.c_float),
+        ("prec_bits", C.c_int), ("use_bla", C.c_int), ("bla_minskip", C.c_int), ("interior", C.c_int),
+        ("threads", C.c_int),
+        ("val", C.POINTER(C.c_double)), ("n_esc", C.POINTER(C.c_int32)), ("out", C.POINTER(C.c_float)),
+        ("passes", C.c_int), ("escalate", C.c_int), ("sa_it0", C.c_int), ("sa_order0", C.c_int),
+        ("steps", C.c_longlong), ("bla_applies", C.c_longlong), ("bla_skipped", C.c_longlong),
+        ("rebases", C.c_longlong),
+    ]
+
+
 def lib():
     global _LIB
     if _LIB is None:
@@ -47,6 +60,11 @@ def lib():
             raise RuntimeError("oracle/libmxo.so is not built (run `make -C oracle`)")
         L = C.CDLL(path)
         L.mxo_compute_fe.argtypes = [C.POINTER(Frame)]
+        L.mxo_compute_dbl.argtypes = [C.POINTER(DFrame)]
+        L.mxo_sa_coeffs.argtypes = [C.c_char_p, C.c_char_p, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, C.c_void_p, C.POINTER(C.c_int)]
+        L.mxo_sa_coeffs.restype = C.c_int
+        L.mxo_sa_seed.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_double, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.mxo_cdiv.argtypes = [C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
         L.mxo_fe_op.argtypes = [C.c_int, C.c_int] + [C.c_void_p] * 6
         L.mxo_orbit_shadows.argtypes = [C.c_char_p, C.c_char_p, C.c_int, C.c_int, C.c_double, C.c_int] + [C.c_void_p] * 7
         L.mxo_orbit_shadows.restype = C.c_int
@@ -94,6 +112,57 @@ def compute_fe(cre, cim, scale, W, H, mxit, c_method=0, escape_radius=1e8, prec_
                                         "cutoff_sensitive_hint", "seconds_pixels", "steps", "bla_applies",
                                         "bla_skipped", "rebases")}
     return val.reshape(H, W), nesc.reshape(H, W), out.reshape(H, W), info
+
+
+def compute_dbl(cre, cim, scale, W, H, mxit, escape_radius=1e8, prec_bits=None, use_bla=True, bla_minskip=8,
+                interior=True, threads=None):
+    """Whole plain-double frame (Compute's double path: centre-pixel reference with its
+    series approximation, BLA, probe gate, passes re-referenced on glitches)."""
+    f = DFrame()
+    f.cre, f.cim, f.scale = cre.encode(), cim.encode(), scale.encode()
+    f.W, f.H, f.mxit, f.escape_radius = W, H, mxit, escape_radius
+    f.prec_bits = prec_bits or precision_bits(scale)
+    f.use_bla, f.bla_minskip, f.interior = int(use_bla), bla_minskip, int(interior)
+    f.threads = threads or os.cpu_count()
+    val = np.zeros(W * H, np.float64)
+    nesc = np.zeros(W * H, np.int32)
+    out = np.zeros(W * H, np.float32)
+    f.val = val.ctypes.data_as(C.POINTER(C.c_double))
+    f.n_esc = nesc.ctypes.data_as(C.POINTER(C.c_int32))
+    f.out = out.ctypes.data_as(C.POINTER(C.c_float))
+    lib().mxo_compute_dbl(C.byref(f))
+    info = {k: getattr(f, k) for k in ("passes", "escalate", "sa_it0", "sa_order0", "steps", "bla_applies",
+                                        "bla_skipped", "rebases")}
+    return val.reshape(H, W), nesc.reshape(H, W), out.reshape(H, W), info
+
+
+def sa_coeffs(cre, cim, prec_bits, mxit, delta, escape_radius=1e8):
+    """Series coefficients A_0..A_29 (complex) of the reference at (cre, cim), the last
+    accepted iteration and the truncation order (calCoefficient / SACheckMagnitude)."""
+    A = np.zeros(60, np.float64)
+    order = C.c_int(0)
+    it = lib().mxo_sa_coeffs(cre.encode(), cim.encode(), prec_bits, mxit, escape_radius, delta.real, delta.imag,
+                             _p(A), C.byref(order))
+    return A[0::2] + 1j * A[1::2], it, order.value
+
+
+def sa_seed(A, order, delta, dc):
+    """Horner evaluation of the series at pixel offsets dc (complex array)."""
+    Ai = np.zeros(60, np.float64)
+    Ai[0::2], Ai[1::2] = np.real(A), np.imag(A)
+    dc = np.asarray(dc, np.complex128)
+    dr, di = np.ascontiguousarray(dc.real), np.ascontiguousarray(dc.imag)
+    out = np.zeros(2 * dc.size, np.float64)
+    lib().mxo_sa_seed(_p(Ai), order, delta.real, delta.imag, dc.size, _p(dr), _p(di), _p(out))
+    return out[0::2] + 1j * out[1::2]
+
+
+def cdiv(a, b):
+    a = np.ascontiguousarray(np.asarray(a, np.complex128))
+    b = np.ascontiguousarray(np.asarray(b, np.complex128))
+    out = np.zeros(a.size, np.complex128)
+    lib().mxo_cdiv(a.size, _p(a), _p(b), _p(out))
+    return out
 
 
 def fe_op(op, am, ae, bm=None, be=None):

@@ -68,7 +68,7 @@ def test_tile_on_product_arrays(ctx):
     finally:
         ref.release()
     orb = dict(zf=fv["zf"], zre_m=fv["zfe_m"][0::2], zre_e=fv["zfe_e"][0::2], zim_m=fv["zfe_m"][1::2], zim_e=fv["zfe_e"][1::2])
-    b = mxo.bla_tables(fv["zf"], fv["len"], fv["dcmax"])
+    b = mxo.bla_tables(fv["zf"], fv["len"] - 1, fv["dcmax"])
     jj, ii = np.meshgrid(np.arange(W), np.arange(H))
     v2, n2, st2 = mxo.pixels_fe(orb, dict(nlev=b["nlev"], off=b["off"], size=b["size"], r2=b["r2"], l=b["l"], coef=b["coef"], rmax2=b["rmax2"]),
                                 fv["dcol_m"][jj.ravel()], fv["dcol_e"][jj.ravel()], fv["drow_m"][ii.ravel()], fv["drow_e"][ii.ravel()],
