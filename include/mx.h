@@ -82,6 +82,14 @@ typedef struct {
     const double* bla_r2;    /* [total] validity radius^2 */
     const double* bla_coef;  /* [4*total] ar, ai, br, bi */
     double rmax2;            /* largest radius^2 */
+    /* plain-double path only (NULL / 0 otherwise): the glitch metric's orbit
+     * magnitudes _z_m3 (mandel_perturbation.cpp:4249) and the series approximation
+     * of calCoefficient (4273-4303): dz0 = sum A_x (dc/delta)^(x+1), start at sa_it+1 */
+    const double* z_m3;      /* [len] |X|^2 / 1e6 */
+    int32_t sa_it;
+    int32_t sa_order;
+    const double* sa_coef;   /* [60] A_0..A_29 as (re, im) */
+    double sa_delta_re, sa_delta_im;
 } mx_ref_desc;
 
 int mx_ref_upload(mx_ctx* ctx, const mx_ref_desc* desc, mx_ref** out);
@@ -111,6 +119,15 @@ typedef struct {
     double dx_m;             /* pixel spacing (EDE normalisation) */
     int32_t dx_e;
     int32_t skip_sample;     /* sample index left untouched (the reference pixel), -1: none */
+    /* plain-double path (form MX_FORM_DOUBLE): dc = (dcol_d[column], drow_d[row]),
+     * truncated from dx*(column - ref column) as stepParallel does
+     * (mandel_perturbation.cpp:2300-2312); the FloatExp tables above may be NULL */
+    const double* dcol_d;
+    const double* drow_d;
+    /* optional list of sample indices (row-major order) to compute instead of the
+     * grid: a later reference pass over the samples the previous pass left unresolved */
+    const int32_t* list;
+    int32_t list_len;
 } mx_tile;
 
 /* A plan holds the device copy of a tile's inputs and outputs so that launches
@@ -124,6 +141,11 @@ int mx_plan_wait(mx_plan* plan, double* kernel_ms);     /* synchronise; kernel t
  * steps, BLA applies, BLA-skipped iterations, rebases. */
 int mx_plan_read(mx_plan* plan, int32_t* n_esc, double* value, float* out, uint64_t* stats4);
 void mx_plan_destroy(mx_plan* plan);
+/* Plain-double path: the samples the last launch left unresolved (glitch, or the end of
+ * the reference reached; mandel_perturbation.cpp:2556-2577) with their glitch metric
+ * glitch_p = |z|^2 / _z_m3[iteration], in no particular order.  *count is the number of
+ * such samples; at most cap are copied. */
+int mx_plan_unresolved(mx_plan* plan, int32_t* idx, double* glitch, int32_t cap, int32_t* count);
 /* create + launch + wait + read + destroy */
 int mx_tile_render(mx_ctx* ctx, const mx_ref* ref, const mx_tile* tile, int32_t* n_esc, double* value, float* out, uint64_t* stats4);
 
@@ -141,7 +163,8 @@ typedef struct {
     int32_t coloring_method; /* 0 or MX_EXTERIOR_DIST_EST */
     float escape_radius;     /* 0 => 1e8f (MANDEL_BAILOUT) */
     int32_t precision_bits;  /* 0 => digits(scale)*log2(10)+64 (render.cpp:170) */
-    int32_t form;            /* MX_FORM_FE64 or MX_FORM_FE32; floatexp path only */
+    int32_t form;            /* floatexp path (scale > 1e280): MX_FORM_FE64, FE32 or SC64;
+                                the plain-double path (1e6 < scale <= 1e280) ignores it */
     int32_t use_bla;         /* -1 default (on) */
     int32_t bla_minskip;     /* 0 => 8 (MANDEL_BLA_MINSKIP) */
     int32_t interior;        /* -1 default (on, gated by the probe) */
@@ -151,7 +174,7 @@ typedef struct {
 } mx_request;
 
 typedef struct {
-    int32_t path;            /* 2 = floatexp rescaled path */
+    int32_t path;            /* 1 = plain-double path, 2 = floatexp rescaled path */
     int32_t ref_it;          /* reference length - 1 */
     int32_t ref_escaped;
     int32_t ref_virtual;     /* exact view-centre reference */
@@ -168,6 +191,8 @@ typedef struct {
     double total_seconds;
     uint64_t steps, bla_applies, bla_skipped, rebases;
     uint64_t nominal_iterations; /* sum of n_esc, max_iterations for interior */
+    int32_t passes;          /* reference passes (plain-double path re-references on glitches) */
+    int32_t sa_it;           /* first reference's series-approximation skip */
 } mx_frame_info;
 
 int mx_compute(mx_ctx* ctx, const mx_request* req, mx_frame_info* info);
@@ -207,6 +232,14 @@ typedef struct {
     int32_t cutoff_sensitive;
     int32_t precision_bits;
     double ref_seconds, bla_seconds;
+    /* plain-double path (path == 1) */
+    int32_t path;
+    const double* z_m3;      /* [len] */
+    int32_t sa_it, sa_order;
+    const double* sa_coef;   /* [60] */
+    double sa_delta_re, sa_delta_im;
+    const double* dcol_d;    /* [width] */
+    const double* drow_d;    /* [height] */
 } mx_frame_view;
 
 int mx_frame_prepare(const mx_request* req, mx_frame** out, mx_frame_view* view);

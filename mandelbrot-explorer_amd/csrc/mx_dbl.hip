@@ -1,0 +1,339 @@
+// Plain-double perturbation kernel for gfx950 (frames at scale 1e6..1e280): IEEE FP64
+// in the engine's operation order, no FP contraction (-ffp-contract=off), so escape
+// counts equal the CPU engine's.
+//
+// Per-pixel algorithm: the scalar loop of Mandel::stepParallel
+// (src/engine/mandel_perturbation.cpp:2291-2584) with tryBLA (2714-2755): the
+// series-approximation seed dz0 = sum A_x (dc/delta)^(x+1) (2319-2323) starting at
+// iteration sa_it+1, the step dz' = 2 X dz + dz^2 + dc against the double orbit,
+// Zhuoran rebase when |z| < |dz| or at the end of the reference, and glitch
+// detection (|dz|^2/|z|^2 > 1e7 at a rebase) that leaves the sample unresolved with
+// its glitch metric |z|^2/_z_m3[j] for the host's next reference choice.
+//
+// Execution model as in the floatexp kernels: a persistent grid whose lanes refill
+// from per-XCD atomic work queues (wave-aggregated fetch-add), samples handed out
+// in 8x8 tiles, or taken from a list of sample indices for a later reference pass.
+#include <climits>
+
+#include "mx_dbl.h"
+#include "mx_dev.h"
+
+namespace {
+
+struct Pd {
+    double dzr, dzi, dcr, dci;
+    double zsr, zsi, conf_D2, conf_zr, conf_zi;
+    double sdzr, sdzi;
+    int j, k;
+    int save_j, period_win, conf_P, conf_next, conf_count, conf_giveup;
+    int sk, save_j2, twin2;
+    int xv;        // 1: Xc = X[k-1] is loaded
+    double2 Xc;
+};
+
+struct Counters {
+    unsigned long long steps, applies, skipped, rebases;
+};
+
+__device__ __forceinline__ unsigned long long wave_sum(unsigned long long v) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    return v;
+}
+
+// Series seed (the Horner loop of mandel_perturbation.cpp:2319-2323) with the
+// explicit complex formulas of mx_dbl.h.
+__device__ void init_pixel(const MxRefDev& R, Pd& p, double dcr, double dci) {
+    p.dcr = dcr;
+    p.dci = dci;
+    const MxC dc = {dcr, dci}, delta = {R.sa_delta.x, R.sa_delta.y};
+    MxC dz = {0.0, 0.0};
+    for (int x = R.sa_order; x >= 0; --x) {
+        const double2 a = R.sa_A[x];
+        dz = mxc_add(dz, MxC{a.x, a.y});
+        dz = mxc_mul(dz, dc);
+        dz = mxc_div(dz, delta);
+    }
+    p.dzr = dz.r;
+    p.dzi = dz.i;
+    p.j = R.sa_it + 1;
+    p.k = p.j;
+    p.zsr = 1e30;
+    p.zsi = 1e30;
+    p.save_j = p.j;
+    p.period_win = 1;
+    p.conf_P = p.conf_next = p.conf_count = p.conf_giveup = 0;
+    p.conf_D2 = p.conf_zr = p.conf_zi = 0.0;
+    p.sdzr = 1e300;
+    p.sdzi = 1e300;
+    p.sk = -1;
+    p.save_j2 = p.j;
+    p.twin2 = 1;
+    p.xv = 0;
+}
+
+// tryBLA (mandel_perturbation.cpp:2714-2755) without the derivative carry: the
+// largest aligned level whose radius admits |dz|, landing at least two entries
+// before the end of the reference and not beyond the escape radius.
+struct BlaRes {
+    double nzr, nzi;
+    int skip;
+};
+
+__device__ __noinline__ BlaRes try_bla(const double* __restrict__ bla_r2, const double4* __restrict__ bla_coef,
+                                       const double2* __restrict__ zf, const int2* __restrict__ lev, int nlev, int minlevel,
+                                       int mx_ref_it, int s, double dzr, double dzi, double dcr, double dci, double esc2) {
+    BlaRes r = {0.0, 0.0, 0};
+    if (s < 1) return r;
+    const double zmag2 = dzr * dzr + dzi * dzi;
+    const int maxp = nlev - 1;
+    int startp = (s == 1) ? maxp : __builtin_ctz((unsigned)(s - 1));
+    if (startp > maxp) startp = maxp;
+    if (startp < minlevel) return r;
+    for (int p = startp; p >= minlevel; --p) {
+        const int2 L = lev[p];
+        const int i = (s - 1) >> p;
+        if (i >= L.y) continue;
+        const int q = L.x + i;
+        const double r2 = bla_r2[q];
+        if (r2 <= 0 || zmag2 >= r2) continue;
+        const int l = 1 << p;
+        const int land = s + l;
+        if (land >= mx_ref_it - 1) continue;
+        const double4 c = bla_coef[q];
+        const double nzr = c.x * dzr - c.y * dzi + c.z * dcr - c.w * dci;
+        const double nzi = c.x * dzi + c.y * dzr + c.z * dci + c.w * dcr;
+        const double2 Z = zf[land];
+        const double fr = Z.x + nzr, fi = Z.y + nzi;
+        if (fr * fr + fi * fi > esc2) return r;
+        r.nzr = nzr;
+        r.nzi = nzi;
+        r.skip = l;
+        return r;
+    }
+    return r;
+}
+
+// One iteration of the stepParallel loop body for one lane.  Returns 1 when the
+// sample is finished (n, v set; n == INT_MIN: unresolved, *gp set).
+template <bool INTERIOR>
+__device__ __forceinline__ int trip(const MxRefDev& R, const int2* __restrict__ lev, const MxTileDev& T, Pd& p, double& v, int& n,
+                                    double& gp, Counters& c) {
+    if (p.j >= T.mxit) {
+        v = -2.0;
+        n = -1;
+        return 1;
+    }
+    if (INTERIOR && (T.flags & MX_F_INT_REP)) {
+        if (p.k == p.sk && p.dzr == p.sdzr && p.dzi == p.sdzi) {
+            v = -2.0;
+            n = -2;
+            return 1;
+        }
+        if (p.j - p.save_j2 >= p.twin2) {
+            p.sdzr = p.dzr;
+            p.sdzi = p.dzi;
+            p.sk = p.k;
+            p.save_j2 = p.j;
+            p.twin2 += p.twin2;
+        }
+    }
+    if ((T.flags & MX_F_BLA) && p.dzr * p.dzr + p.dzi * p.dzi < R.rmax2) {
+        const BlaRes b = try_bla(R.bla_r2, R.bla_coef, R.zf, lev, R.nlev, R.minlevel, R.mx_ref_it, p.k - 1, p.dzr, p.dzi, p.dcr, p.dci, T.esc2);
+        const int skip = b.skip;
+        if (skip > 0) {
+            p.dzr = b.nzr;
+            p.dzi = b.nzi;
+            p.k += skip;
+            p.j += skip;
+            p.xv = 0;
+            c.applies++;
+            c.skipped += (unsigned long long)skip;
+            p.zsr = 1e30;
+            p.zsi = 1e30;
+            p.save_j = p.j;
+            p.period_win = 1;
+            p.conf_P = 0;
+            p.conf_giveup = 0;
+            return 0;
+        }
+    }
+    const double dzr2 = p.dzr * p.dzr, dzi2 = p.dzi * p.dzi;
+    double tmp, dzi;
+    if (p.k == 0) {
+        tmp = dzr2 - dzi2 + p.dcr;
+        dzi = p.dzr * p.dzi * 2 + p.dci;
+    } else {
+        if (!p.xv) p.Xc = R.zf[p.k - 1];
+        const double xr = p.Xc.x, xi = p.Xc.y;
+        tmp = (p.dzr * xr - p.dzi * xi) * 2 + dzr2 - dzi2 + p.dcr;
+        dzi = (p.dzr * xi + p.dzi * xr) * 2 + p.dzr * p.dzi * 2 + p.dci;
+    }
+    const double dzr = tmp;
+    const double2 Xn = R.zf[p.k];
+    const double zr = dzr + Xn.x, zi = dzi + Xn.y;
+    const double zrad = zr * zr + zi * zi;
+    p.dzr = dzr;
+    p.dzi = dzi;
+    ++p.k;
+    p.Xc = Xn; // X[k-1] of the next step
+    p.xv = 1;
+    c.steps++;
+    if (zrad > T.esc2) {
+        const double LG = 0.69314718055994530942; // log(2) rounded to double
+        v = p.j + 1 - log(log(zrad) / 2 / LG) / LG;
+        n = p.j + 1;
+        return 1;
+    }
+    if (INTERIOR) {
+        if (p.conf_P > 0) {
+            p.conf_D2 *= 4 * zrad;
+            if (p.conf_D2 > 1e18) p.conf_D2 = 1e18;
+            if (p.j >= p.conf_next) {
+                const double pr = zr - p.conf_zr, pi = zi - p.conf_zi;
+                const bool ret = (pr * pr + pi * pi < T.interior_eps2 * zrad);
+                if (ret && p.conf_D2 < 1.0) {
+                    if (++p.conf_count >= T.interior_confirm) {
+                        v = -2.0;
+                        n = -2;
+                        return 1;
+                    }
+                    p.conf_zr = zr;
+                    p.conf_zi = zi;
+                    p.conf_D2 = 1;
+                    p.conf_next = p.j + p.conf_P;
+                } else {
+                    p.conf_P = 0;
+                    ++p.conf_giveup;
+                }
+            }
+        } else if ((p.j & 15) == 0 && p.conf_giveup < 3) {
+            const double pr = zr - p.zsr, pi = zi - p.zsi;
+            if (pr * pr + pi * pi < T.interior_eps2 * zrad) {
+                p.conf_P = p.j - p.save_j;
+                if (p.conf_P < 1) p.conf_P = 1;
+                p.conf_zr = zr;
+                p.conf_zi = zi;
+                p.conf_D2 = 1;
+                p.conf_next = p.j + p.conf_P;
+                p.conf_count = 0;
+            }
+            if (p.j - p.save_j >= p.period_win) {
+                p.zsr = zr;
+                p.zsi = zi;
+                p.save_j = p.j;
+                p.period_win += p.period_win;
+            }
+        }
+    }
+    const double dz2 = dzr * dzr + dzi * dzi;
+    // end of the reference: k >= mx_ref_it (a series skip that ends at the reference's
+    // escape iteration starts at k == mx_ref_it, so k reaches mx_ref_it + 1 here)
+    if (zrad < dz2 || p.k >= R.mx_ref_it) {
+        if (dz2 / zrad > 10000000) {
+            // glitch: the metric reads _z_m3 at the pixel's iteration, clamped to the
+            // reference's last entry
+            const int jm = p.j < R.mx_ref_it ? p.j : R.mx_ref_it;
+            gp = zrad / R.z_m3[jm];
+            n = INT_MIN;
+            return 1;
+        }
+        p.dzr = zr;
+        p.dzi = zi;
+        p.k = 0;
+        p.xv = 0;
+        c.rebases++;
+    }
+    ++p.j;
+    return 0;
+}
+
+// Map a queue item to a sample index; false for the padding lanes of edge tiles.
+__device__ __forceinline__ bool item_to_sample(const MxTileDev& T, unsigned item, int& sidx) {
+    if (T.list) {
+        sidx = T.list[item];
+        return sidx != T.skip_sample;
+    }
+    const unsigned tile = item >> 6, l = item & 63;
+    const int ty = (int)(tile / (unsigned)T.tiles_x), tx = (int)(tile % (unsigned)T.tiles_x);
+    const int gi = ty * 8 + (int)(l >> 3), gj = tx * 8 + (int)(l & 7);
+    if (gi >= T.ny || gj >= T.nx) return false;
+    sidx = gi * T.stride * T.W + gj * T.stride;
+    return sidx != T.skip_sample;
+}
+
+template <bool INTERIOR>
+__global__ __launch_bounds__(256) void mx_dbl_kernel(MxRefDev R, const int2* __restrict__ lev, MxTileDev T) {
+    const int lane = __lane_id();
+    const unsigned long long lanemask_lt = (1ull << lane) - 1ull;
+    int q = blockIdx.x % MX_NQUEUE;
+    int exhausted = 0;
+    bool alive = true, active = false;
+    Pd p;
+    Counters c = {0, 0, 0, 0};
+    int sidx = 0;
+    unsigned trips = 0;
+    const unsigned nitems_all = T.list ? (unsigned)T.list_len : (unsigned)T.ntiles * 64u;
+    while (true) {
+        bool need = alive && !active;
+        unsigned long long needmask = __ballot(need);
+        while (needmask != 0ull && exhausted < MX_NQUEUE) {
+            const unsigned i0 = (unsigned)(((unsigned long long)nitems_all * (unsigned)q) / MX_NQUEUE);
+            const unsigned i1 = (unsigned)(((unsigned long long)nitems_all * (unsigned)(q + 1)) / MX_NQUEUE);
+            const unsigned nitems = i1 - i0;
+            const int k = __popcll(needmask);
+            const int leader = __ffsll((long long)needmask) - 1;
+            unsigned base = 0;
+            if (lane == leader) base = atomicAdd(&T.queue[q], (unsigned)k);
+            base = __shfl(base, leader);
+            if (need) {
+                const unsigned it = base + (unsigned)__popcll(needmask & lanemask_lt);
+                if (it < nitems && item_to_sample(T, i0 + it, sidx)) {
+                    init_pixel(R, p, T.dcol_d[sidx % T.W], T.drow_d[sidx / T.W]);
+                    need = false;
+                    active = true;
+                }
+            }
+            if (base + (unsigned)k >= nitems) { // this queue is drained
+                q = (q + 1) % MX_NQUEUE;
+                ++exhausted;
+            }
+            needmask = __ballot(need);
+        }
+        if (need) alive = false;
+        if (__ballot(active) == 0ull) break;
+        if (((++trips) & 4095u) == 0u && T.cancel && *T.cancel) break;
+        if (active) {
+            double v = 0.0, gp = 0.0;
+            int n = 0;
+            if (trip<INTERIOR>(R, lev, T, p, v, n, gp, c)) {
+                if (n == INT_MIN) {
+                    const unsigned slot = atomicAdd(T.unres_count, 1u);
+                    T.unres_idx[slot] = sidx;
+                    T.unres_gp[slot] = gp;
+                } else {
+                    T.val[sidx] = v;
+                    T.nesc[sidx] = n;
+                    if (T.out) T.out[sidx] = n < 0 ? -2.0f : (float)v;
+                }
+                active = false;
+            }
+        }
+    }
+    const unsigned long long s0 = wave_sum(c.steps), s1 = wave_sum(c.applies), s2 = wave_sum(c.skipped), s3 = wave_sum(c.rebases);
+    if (lane == 0) {
+        atomicAdd(&T.stats[0], s0);
+        atomicAdd(&T.stats[1], s1);
+        atomicAdd(&T.stats[2], s2);
+        atomicAdd(&T.stats[3], s3);
+    }
+}
+
+} // namespace
+
+hipError_t mx_launch_dbl(const MxRefDev& R, const int2* lev, const MxTileDev& T, int blocks, int threads, hipStream_t stream) {
+    if (T.flags & MX_F_INTERIOR)
+        hipLaunchKernelGGL((mx_dbl_kernel<true>), dim3(blocks), dim3(threads), 0, stream, R, lev, T);
+    else
+        hipLaunchKernelGGL((mx_dbl_kernel<false>), dim3(blocks), dim3(threads), 0, stream, R, lev, T);
+    return hipGetLastError();
+}

@@ -39,6 +39,11 @@ struct MxRefDev {
     int minlevel;
     int pad0;
     double rmax2;
+    // plain-double path
+    const double* z_m3;    // |X|^2 / 1e6 per entry (glitch metric)
+    const double2* sa_A;   // series coefficients A_0..A_29
+    double2 sa_delta;
+    int sa_it, sa_order;
 };
 
 struct MxTileDev {
@@ -62,4 +67,12 @@ struct MxTileDev {
     unsigned* queue;     // MX_NQUEUE counters, zeroed before launch
     unsigned long long* stats; // steps, bla applies, bla skipped iterations, rebases
     const volatile int* cancel; // host-mapped; nonzero => lanes stop taking work
+    // plain-double path
+    const double* dcol_d;  // per column dc (double)
+    const double* drow_d;  // per row dc (double)
+    const int* list;       // sample indices to compute (null: the grid)
+    int list_len;
+    int* unres_idx;        // unresolved samples of this launch
+    double* unres_gp;      // their glitch metric
+    unsigned* unres_count;
 };

@@ -31,9 +31,11 @@ double now_s() { return std::chrono::duration<double>(std::chrono::steady_clock:
 
 size_t align_up(size_t x) { return (x + 255) & ~(size_t)255; }
 
-__global__ void mx_reset_kernel(unsigned* queue, unsigned long long* stats, double* val, int* nesc, float* out, size_t n) {
+__global__ void mx_reset_kernel(unsigned* queue, unsigned long long* stats, unsigned* unres_count, double* val, int* nesc, float* out,
+                                size_t n) {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < MX_NQUEUE) queue[i] = 0u;
+    if (i == 0 && unres_count) *unres_count = 0u;
     if (i < 4) stats[i] = 0ull;
     for (size_t k = i; k < n; k += (size_t)gridDim.x * blockDim.x) {
         val[k] = -10.0;
@@ -122,6 +124,7 @@ extern "C" int mx_ref_upload(mx_ctx* ctx, const mx_ref_desc* d, mx_ref** out) {
     }
     if (d->nlev > 0 && (!d->bla_r2 || !d->bla_coef)) return MX_E_INVALID;
     if (total > INT_MAX / 4 || (int64_t)d->len > INT_MAX / 2) return MX_E_INVALID;
+    if (d->z_m3 && (!d->sa_coef || d->sa_order < 0 || d->sa_order >= MX_SA_N || d->sa_it < 0)) return MX_E_INVALID;
     MX_HIP(hipSetDevice(ctx->device));
     const size_t len = (size_t)d->len;
     size_t o_zf = 0;
@@ -133,7 +136,9 @@ extern "C" int mx_ref_upload(mx_ctx* ctx, const mx_ref_desc* d, mx_ref** out) {
     size_t o_cf = align_up(o_coef + 32 * (size_t)total);
     size_t o_ce = align_up(o_cf + 16 * (size_t)total);
     size_t o_lev = align_up(o_ce + 8 * (size_t)total);
-    size_t bytes = align_up(o_lev + 8 * MX_MAXLEV);
+    size_t o_m3 = align_up(o_lev + 8 * MX_MAXLEV);
+    size_t o_sa = align_up(o_m3 + (d->z_m3 ? 8 * len : 0));
+    size_t bytes = align_up(o_sa + 16 * MX_SA_N);
     auto* ref = new (std::nothrow) mx_ref();
     if (!ref) return MX_E_NOMEM;
     ref->ctx = ctx;
@@ -176,6 +181,10 @@ extern "C" int mx_ref_upload(mx_ctx* ctx, const mx_ref_desc* d, mx_ref** out) {
         cp(o_ce, ce.data(), 8 * (size_t)total);
     }
     cp(o_lev, lev.data(), 8 * MX_MAXLEV);
+    if (d->z_m3) {
+        cp(o_m3, d->z_m3, 8 * len);
+        cp(o_sa, d->sa_coef, 16 * MX_SA_N);
+    }
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
     if (e != hipSuccess) {
         ctx->set_err(std::string("reference upload: ") + hipGetErrorString(e));
@@ -196,6 +205,13 @@ extern "C" int mx_ref_upload(mx_ctx* ctx, const mx_ref_desc* d, mx_ref** out) {
     R.mx_ref_it = d->len - 1;
     R.nlev = d->nlev;
     R.rmax2 = d->rmax2;
+    if (d->z_m3) {
+        R.z_m3 = (const double*)(b + o_m3);
+        R.sa_A = (const double2*)(b + o_sa);
+        R.sa_delta = make_double2(d->sa_delta_re, d->sa_delta_im);
+        R.sa_it = d->sa_it;
+        R.sa_order = d->sa_order;
+    }
     ref->lev = (int2*)(b + o_lev);
     *out = ref;
     return MX_OK;
@@ -213,11 +229,21 @@ extern "C" void mx_ref_release(mx_ref* ref) {
 
 extern "C" int mx_plan_create(mx_ctx* ctx, const mx_ref* ref, const mx_tile* t, mx_plan** out) {
     if (!ctx || !ref || !t || !out || t->width < 1 || t->height < 1 || t->stride < 1 || t->max_iter < 1) return MX_E_INVALID;
-    if (!t->dcol_m || !t->dcol_e || !t->drow_m || !t->drow_e) return MX_E_INVALID;
-    if (t->form != MX_FORM_FE64 && t->form != MX_FORM_FE32 && t->form != MX_FORM_SC64) {
-        ctx->set_err("this kernel form is not built");
-        return MX_E_UNSUPPORTED;
+    const bool dbl = t->form == MX_FORM_DOUBLE;
+    if (t->form != MX_FORM_FE64 && t->form != MX_FORM_FE32 && t->form != MX_FORM_SC64 && !dbl) {
+        ctx->set_err("unknown kernel form");
+        return MX_E_INVALID;
     }
+    if (!dbl && (!t->dcol_m || !t->dcol_e || !t->drow_m || !t->drow_e)) return MX_E_INVALID;
+    if (dbl && (!t->dcol_d || !t->drow_d || !ref->dev.z_m3 || t->ede)) {
+        ctx->set_err(t->ede ? "exterior distance estimate on the plain-double path is not built"
+                            : "the plain-double form needs double dc tables and a reference with its series");
+        return t->ede ? MX_E_UNSUPPORTED : MX_E_INVALID;
+    }
+    if (t->list && (t->list_len < 0 || t->list_len > t->width * t->height)) return MX_E_INVALID;
+    if (t->list)
+        for (int32_t q = 0; q < t->list_len; ++q)
+            if (t->list[q] < 0 || (int64_t)t->list[q] >= (int64_t)t->width * t->height) return MX_E_INVALID;
     if (t->bla && (ref->dev.nlev == 0 || t->bla_minlevel < 0)) return MX_E_INVALID;
     if ((int64_t)t->width * t->height > (int64_t)INT_MAX / 2) return MX_E_INVALID;
     *out = nullptr;
@@ -229,6 +255,9 @@ extern "C" int mx_plan_create(mx_ctx* ctx, const mx_ref* ref, const mx_tile* t, 
     plan->form = t->form;
     const size_t W = (size_t)t->width, H = (size_t)t->height, n = W * H;
     plan->n = n;
+    const size_t nlist = t->list ? (size_t)t->list_len : 0;
+    // unresolved samples: at most one per sample of the launch
+    const size_t nunres = dbl ? (t->list ? nlist : n) : 0;
     size_t o_col = 0;
     size_t o_row = align_up(o_col + sizeof(Fe) * W);
     size_t o_val = align_up(o_row + sizeof(Fe) * H);
@@ -236,7 +265,13 @@ extern "C" int mx_plan_create(mx_ctx* ctx, const mx_ref* ref, const mx_tile* t, 
     size_t o_out = align_up(o_nesc + 4 * n);
     size_t o_q = align_up(o_out + 4 * n);
     size_t o_st = align_up(o_q + 4 * MX_NQUEUE);
-    size_t bytes = align_up(o_st + 8 * 4);
+    size_t o_dcd = align_up(o_st + 8 * 4);
+    size_t o_drd = align_up(o_dcd + (dbl ? 8 * W : 0));
+    size_t o_list = align_up(o_drd + (dbl ? 8 * H : 0));
+    size_t o_uidx = align_up(o_list + 4 * nlist);
+    size_t o_ugp = align_up(o_uidx + 4 * nunres);
+    size_t o_ucnt = align_up(o_ugp + 8 * nunres);
+    size_t bytes = align_up(o_ucnt + 4);
     if (hipMalloc(&plan->block, bytes) != hipSuccess) {
         delete plan;
         ctx->set_err("hipMalloc of the tile failed");
@@ -244,11 +279,16 @@ extern "C" int mx_plan_create(mx_ctx* ctx, const mx_ref* ref, const mx_tile* t, 
     }
     plan->bytes = bytes;
     char* b = (char*)plan->block;
-    std::vector<Fe> col(W), row(H);
-    for (size_t j = 0; j < W; ++j) col[j] = fe_make(t->dcol_m[j], t->dcol_e[j]);
-    for (size_t i = 0; i < H; ++i) row[i] = fe_make(t->drow_m[i], t->drow_e[i]);
+    std::vector<Fe> col(W, Fe{0.0, 0}), row(H, Fe{0.0, 0});
+    if (!dbl) {
+        for (size_t j = 0; j < W; ++j) col[j] = fe_make(t->dcol_m[j], t->dcol_e[j]);
+        for (size_t i = 0; i < H; ++i) row[i] = fe_make(t->drow_m[i], t->drow_e[i]);
+    }
     hipError_t e = hipMemcpyAsync(b + o_col, col.data(), sizeof(Fe) * W, hipMemcpyHostToDevice, ctx->stream);
     if (e == hipSuccess) e = hipMemcpyAsync(b + o_row, row.data(), sizeof(Fe) * H, hipMemcpyHostToDevice, ctx->stream);
+    if (dbl && e == hipSuccess) e = hipMemcpyAsync(b + o_dcd, t->dcol_d, 8 * W, hipMemcpyHostToDevice, ctx->stream);
+    if (dbl && e == hipSuccess) e = hipMemcpyAsync(b + o_drd, t->drow_d, 8 * H, hipMemcpyHostToDevice, ctx->stream);
+    if (nlist && e == hipSuccess) e = hipMemcpyAsync(b + o_list, t->list, 4 * nlist, hipMemcpyHostToDevice, ctx->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
     if (e == hipSuccess) e = hipEventCreate(&plan->ev0);
     if (e == hipSuccess) e = hipEventCreate(&plan->ev1);
@@ -281,12 +321,24 @@ extern "C" int mx_plan_create(mx_ctx* ctx, const mx_ref* ref, const mx_tile* t, 
     T.queue = (unsigned*)(b + o_q);
     T.stats = (unsigned long long*)(b + o_st);
     T.cancel = ctx->cancel_dev;
+    if (dbl) {
+        T.dcol_d = (const double*)(b + o_dcd);
+        T.drow_d = (const double*)(b + o_drd);
+        T.unres_idx = (int*)(b + o_uidx);
+        T.unres_gp = (double*)(b + o_ugp);
+        T.unres_count = (unsigned*)(b + o_ucnt);
+    }
+    if (nlist) {
+        T.list = (const int*)(b + o_list);
+        T.list_len = (int)nlist;
+    }
     plan->threads = 256;
     plan->minlevel = t->bla_minlevel;
     // persistent grid: 8 waves per SIMD (32 per CU); never more workgroups than
     // there are 8x8 tiles to hand out
     const int waves = ctx->cu_count * 32;
-    plan->blocks = std::max(1, std::min(waves / 4, T.ntiles));
+    const int work_tiles = t->list ? (int)((nlist + 63) / 64) : T.ntiles;
+    plan->blocks = std::max(1, std::min(waves / 4, work_tiles));
     *out = plan;
     return MX_OK;
 }
@@ -297,11 +349,13 @@ extern "C" int mx_plan_launch(mx_plan* plan) {
     MX_HIP(hipSetDevice(ctx->device));
     MxTileDev& T = plan->tile;
     MX_HIP(hipEventRecord(plan->ev0, ctx->stream));
-    hipLaunchKernelGGL(mx_reset_kernel, dim3(1024), dim3(256), 0, ctx->stream, T.queue, T.stats, T.val, T.nesc, T.out, plan->n);
+    hipLaunchKernelGGL(mx_reset_kernel, dim3(1024), dim3(256), 0, ctx->stream, T.queue, T.stats, T.unres_count, T.val, T.nesc, T.out, plan->n);
     MX_HIP(hipGetLastError());
     MxRefDev R = plan->ref->dev;
     R.minlevel = plan->minlevel;
-    if (plan->form == MX_FORM_FE32)
+    if (plan->form == MX_FORM_DOUBLE)
+        MX_HIP(mx_launch_dbl(R, plan->ref->lev, T, plan->blocks, plan->threads, ctx->stream));
+    else if (plan->form == MX_FORM_FE32)
         MX_HIP(mx_launch_fe32(R, plan->ref->lev, T, plan->blocks, plan->threads, ctx->stream));
     else if (plan->form == MX_FORM_SC64)
         MX_HIP(mx_launch_sc64(R, plan->ref->lev, T, plan->blocks, plan->threads, ctx->stream));
@@ -339,6 +393,23 @@ extern "C" int mx_plan_read(mx_plan* plan, int32_t* n_esc, double* value, float*
     return MX_OK;
 }
 
+extern "C" int mx_plan_unresolved(mx_plan* plan, int32_t* idx, double* glitch, int32_t cap, int32_t* count) {
+    if (!plan || !count || cap < 0) return MX_E_INVALID;
+    mx_ctx* ctx = plan->ctx;
+    const MxTileDev& T = plan->tile;
+    *count = 0;
+    if (!T.unres_count) return MX_OK;
+    unsigned c = 0;
+    MX_HIP(hipMemcpyAsync(&c, T.unres_count, 4, hipMemcpyDeviceToHost, ctx->stream));
+    MX_HIP(hipStreamSynchronize(ctx->stream));
+    *count = (int32_t)c;
+    const size_t k = std::min((size_t)c, (size_t)cap);
+    if (k && idx) MX_HIP(hipMemcpyAsync(idx, T.unres_idx, 4 * k, hipMemcpyDeviceToHost, ctx->stream));
+    if (k && glitch) MX_HIP(hipMemcpyAsync(glitch, T.unres_gp, 8 * k, hipMemcpyDeviceToHost, ctx->stream));
+    MX_HIP(hipStreamSynchronize(ctx->stream));
+    return MX_OK;
+}
+
 extern "C" void mx_plan_destroy(mx_plan* plan) {
     if (!plan) return;
     (void)hipSetDevice(plan->ctx->device);
@@ -366,6 +437,230 @@ extern "C" int mx_tile_render(mx_ctx* ctx, const mx_ref* ref, const mx_tile* til
     return rc;
 }
 
+namespace {
+
+int upload_frame_ref(mx_ctx* ctx, const mx::Frame& F, mx_ref** ref) {
+    mx_frame_view v;
+    F.fill_view(v);
+    mx_ref_desc d;
+    std::memset(&d, 0, sizeof d);
+    d.len = v.len;
+    d.zf = v.zf;
+    d.zfe_m = v.zfe_m;
+    d.zfe_e = v.zfe_e;
+    d.nlev = v.nlev;
+    d.lev_off = v.lev_off;
+    d.lev_size = v.lev_size;
+    d.bla_r2 = v.bla_r2;
+    d.bla_coef = v.bla_coef;
+    d.rmax2 = v.rmax2;
+    d.z_m3 = v.z_m3;
+    d.sa_it = v.sa_it;
+    d.sa_order = v.sa_order;
+    d.sa_coef = v.sa_coef;
+    d.sa_delta_re = v.sa_delta_re;
+    d.sa_delta_im = v.sa_delta_im;
+    return mx_ref_upload(ctx, &d, ref);
+}
+
+// One launch of the double kernel against `ref` over the grid (list == nullptr) or a
+// list of samples; results merged into nesc/val for the samples it resolved, the
+// unresolved ones returned with their glitch metric.
+int dbl_pass(mx_ctx* ctx, const mx::Frame& F, const mx_ref* ref, mx_tile t, const std::vector<int32_t>* list, int stride,
+             std::vector<int32_t>& nesc, std::vector<double>& val, std::vector<int32_t>& un_idx, std::vector<double>& un_gp,
+             uint64_t st[4], double* kernel_ms) {
+    t.stride = stride;
+    t.dcol_d = F.dcol_d.data();
+    t.drow_d = F.drow_d.data();
+    t.list = list ? list->data() : nullptr;
+    t.list_len = list ? (int32_t)list->size() : 0;
+    mx_plan* plan = nullptr;
+    int rc = mx_plan_create(ctx, ref, &t, &plan);
+    if (rc != MX_OK) return rc;
+    double ms = 0.0;
+    const size_t n = (size_t)F.W * F.H;
+    std::vector<int32_t> pn(n);
+    std::vector<double> pv(n);
+    uint64_t ps[4] = {0, 0, 0, 0};
+    rc = mx_plan_launch(plan);
+    if (rc == MX_OK) rc = mx_plan_wait(plan, &ms);
+    if (rc == MX_OK) rc = mx_plan_read(plan, pn.data(), pv.data(), nullptr, ps);
+    int32_t cnt = 0;
+    if (rc == MX_OK) rc = mx_plan_unresolved(plan, nullptr, nullptr, 0, &cnt);
+    if (rc == MX_OK) {
+        un_idx.resize((size_t)cnt);
+        un_gp.resize((size_t)cnt);
+        rc = mx_plan_unresolved(plan, un_idx.data(), un_gp.data(), cnt, &cnt);
+    }
+    mx_plan_destroy(plan);
+    if (rc != MX_OK) return rc;
+    for (size_t k = 0; k < n; ++k)
+        if (pn[k] != INT_MIN) {
+            nesc[k] = pn[k];
+            val[k] = pv[k];
+        }
+    for (int q = 0; q < 4; ++q) st[q] += ps[q];
+    if (kernel_ms) *kernel_ms += ms;
+    // row-major order, as the engine's std::set of pending pixels
+    std::vector<size_t> ord(un_idx.size());
+    for (size_t q = 0; q < ord.size(); ++q) ord[q] = q;
+    std::sort(ord.begin(), ord.end(), [&](size_t a, size_t b) { return un_idx[a] < un_idx[b]; });
+    std::vector<int32_t> si(ord.size());
+    std::vector<double> sg(ord.size());
+    for (size_t q = 0; q < ord.size(); ++q) {
+        si[q] = un_idx[ord[q]];
+        sg[q] = un_gp[ord[q]];
+    }
+    un_idx.swap(si);
+    un_gp.swap(sg);
+    return MX_OK;
+}
+
+// Mandel::Compute's plain-double path (mandel_perturbation.cpp:1668-1761): probe
+// gate on the 8x8-strided grid, then passes until every pixel is resolved; after a
+// pass that leaves glitched pixels the next reference is the pending pixel with the
+// smallest glitch metric (the first one in row-major order among equals; the first
+// pending pixel when none has a finite metric below 1e100), and the next pass runs
+// over the pending pixels only.
+int compute_dbl(mx_ctx* ctx, mx::Frame& F, const mx_request* req, mx_frame_info* info, double t0) {
+    const double t_up = now_s();
+    mx_ref* ref = nullptr;
+    int rc = upload_frame_ref(ctx, F, &ref);
+    if (rc != MX_OK) return rc;
+    double upload_s = now_s() - t_up;
+    const int ref_it0 = F.ref_it, sa_it0 = F.sa_it;
+    const bool ref_escaped0 = F.ref_escaped;
+    const int bla_levels0 = (int)F.lev_off.size();
+    mx_tile t;
+    std::memset(&t, 0, sizeof t);
+    t.width = F.W;
+    t.height = F.H;
+    t.max_iter = F.max_iter;
+    t.esc2 = F.esc2;
+    t.form = MX_FORM_DOUBLE;
+    t.interior = F.interior;
+    t.int_rep = F.interior;
+    t.bla = F.use_bla && !F.lev_off.empty();
+    t.bla_minlevel = F.bla_minlevel;
+    t.interior_eps2 = 3e-9 * 3e-9;
+    t.interior_confirm = 4;
+    t.skip_sample = -1;
+    const size_t n = (size_t)F.W * F.H;
+    std::vector<int32_t> nesc(n, INT_MIN);
+    std::vector<double> val(n, -10.0);
+    std::vector<int32_t> un_idx;
+    std::vector<double> un_gp;
+    uint64_t st[4] = {0, 0, 0, 0};
+    double probe_ms = 0.0, kernel_ms = 0.0;
+    bool interior_used = F.interior;
+    if (F.interior) {
+        std::vector<int32_t> pn(n, INT_MIN);
+        std::vector<double> pv(n, -10.0);
+        uint64_t pst[4] = {0, 0, 0, 0};
+        rc = dbl_pass(ctx, F, ref, t, nullptr, 8, pn, pv, un_idx, un_gp, pst, &probe_ms);
+        if (rc != MX_OK) {
+            mx_ref_release(ref);
+            return rc;
+        }
+        int nint = 0;
+        for (int i = 0; i < F.H; i += 8)
+            for (int j = 0; j < F.W; j += 8) {
+                const int32_t v = pn[(size_t)i * F.W + j];
+                if (v == -1 || v == -2) ++nint;
+            }
+        if (nint == 0) {
+            interior_used = false;
+            t.interior = 0;
+            t.int_rep = 0;
+        }
+    }
+    // the reference pixel's value comes from its own orbit
+    size_t refpix = (size_t)F.ref_i * F.W + F.ref_j;
+    nesc[refpix] = F.ref_pixel_n;
+    val[refpix] = F.ref_pixel_value;
+    t.skip_sample = (int32_t)refpix;
+    int passes = 0;
+    std::vector<int32_t> list;
+    bool first = true;
+    while (true) {
+        rc = dbl_pass(ctx, F, ref, t, first ? nullptr : &list, 1, nesc, val, un_idx, un_gp, st, &kernel_ms);
+        first = false;
+        ++passes;
+        if (rc != MX_OK) break;
+        if (un_idx.empty()) break;
+        if (__atomic_load_n(ctx->cancel_host, __ATOMIC_ACQUIRE)) {
+            ctx->set_err("cancelled");
+            rc = MX_E_CANCELLED;
+            break;
+        }
+        size_t best = 0;
+        double mn = 1e100;
+        bool found = false;
+        for (size_t q = 0; q < un_idx.size(); ++q)
+            if (un_gp[q] >= 0 && un_gp[q] < mn) {
+                mn = un_gp[q];
+                best = q;
+                found = true;
+            }
+        if (!found) best = 0;
+        const int32_t p = un_idx[best];
+        mx_ref_release(ref);
+        ref = nullptr;
+        F.set_pixel_reference(p / F.W, p % F.W);
+        refpix = (size_t)p;
+        nesc[refpix] = F.ref_pixel_n;
+        val[refpix] = F.ref_pixel_value;
+        list.clear();
+        for (size_t q = 0; q < un_idx.size(); ++q)
+            if (q != best) list.push_back(un_idx[q]);
+        if (list.empty()) break;
+        const double tu = now_s();
+        rc = upload_frame_ref(ctx, F, &ref);
+        upload_s += now_s() - tu;
+        if (rc != MX_OK) break;
+        t.bla = F.use_bla && !F.lev_off.empty();
+        t.skip_sample = (int32_t)refpix;
+    }
+    if (ref) mx_ref_release(ref);
+    if (rc != MX_OK) return rc;
+    uint64_t nominal = 0;
+    for (size_t k = 0; k < n; ++k) {
+        const int32_t ne = nesc[k];
+        req->iterations[k] = ne == INT_MIN ? -10.0f : ne < 0 ? -2.0f : (float)val[k];
+        nominal += ne > 0 ? (uint64_t)ne : (uint64_t)F.max_iter;
+    }
+    if (req->values) std::memcpy(req->values, val.data(), 8 * n);
+    if (req->n_esc) std::memcpy(req->n_esc, nesc.data(), 4 * n);
+    if (info) {
+        std::memset(info, 0, sizeof *info);
+        info->path = 1;
+        info->ref_it = ref_it0;
+        info->ref_escaped = ref_escaped0;
+        info->ref_virtual = 0;
+        info->ref_x = F.W / 2;
+        info->ref_y = F.H / 2;
+        info->interior_used = interior_used;
+        info->precision_bits = (int32_t)F.prec;
+        info->bla_levels = bla_levels0;
+        info->ref_seconds = F.ref_seconds;
+        info->bla_seconds = F.bla_seconds;
+        info->upload_seconds = upload_s;
+        info->probe_ms = probe_ms;
+        info->kernel_ms = kernel_ms;
+        info->total_seconds = now_s() - t0;
+        info->steps = st[0];
+        info->bla_applies = st[1];
+        info->bla_skipped = st[2];
+        info->rebases = st[3];
+        info->nominal_iterations = nominal;
+        info->passes = passes;
+        info->sa_it = sa_it0;
+    }
+    return MX_OK;
+}
+
+} // namespace
+
 extern "C" int mx_compute(mx_ctx* ctx, const mx_request* req, mx_frame_info* info) {
     if (!ctx || !req || !req->iterations) return MX_E_INVALID;
     BusyGuard busy(ctx);
@@ -380,9 +675,10 @@ extern "C" int mx_compute(mx_ctx* ctx, const mx_request* req, mx_frame_info* inf
         ctx->set_err(err);
         return rc;
     }
+    if (F.dbl) return compute_dbl(ctx, F, req, info, t0);
     if (req->form != MX_FORM_FE64 && req->form != MX_FORM_FE32 && req->form != MX_FORM_SC64) {
-        ctx->set_err("this kernel form is not built");
-        return MX_E_UNSUPPORTED;
+        ctx->set_err("form must be MX_FORM_FE64, MX_FORM_FE32 or MX_FORM_SC64 on the floatexp path");
+        return MX_E_INVALID;
     }
     mx_frame_view v;
     F.fill_view(v);
@@ -504,6 +800,7 @@ extern "C" int mx_compute(mx_ctx* ctx, const mx_request* req, mx_frame_info* inf
         info->bla_skipped = st[2];
         info->rebases = st[3];
         info->nominal_iterations = nominal;
+        info->passes = 1;
     }
     return MX_OK;
 }

@@ -51,6 +51,7 @@ void Frame::build_orbit(mpf_srcptr cre, mpf_srcptr cim) {
     zf.assign(2 * (size_t)(mxit + 1), 0.0);
     zfe_m.assign(2 * (size_t)(mxit + 1), 0.0);
     zfe_e.assign(2 * (size_t)(mxit + 1), 0);
+    z_m3.assign((size_t)(mxit + 1), 0.0);
     Gmp zr(prec), zi(prec), t1(prec), t2(prec), nzr(prec), nzi(prec);
     mpf_set(ref_re.v, cre);
     mpf_set(ref_im.v, cim);
@@ -67,8 +68,18 @@ void Frame::build_orbit(mpf_srcptr cre, mpf_srcptr cim) {
         zfe_m[2 * i + 1] = m;
         zfe_e[2 * i + 1] = (int32_t)e;
         zf[2 * i + 1] = m == 0.0 ? 0.0 : ldexp(m, (int)e);
+        // the glitch metric's |X|^2 (calCoefficient, mandel_perturbation.cpp:4249)
+        z_m3[i] = i == 0 ? 0.0 : (zf[2 * i] * zf[2 * i] + zf[2 * i + 1] * zf[2 * i + 1]) / 1000000;
     };
     store(0);
+    if (dbl) {
+        // series approximation restarts with every reference; the order persists
+        // (createRef, mandel_perturbation.cpp:4113-4124)
+        for (int x = 0; x < MX_SA_N; ++x) sa_A[x] = MxC{0.0, 0.0};
+        sa_A[0] = sa_delta;
+        sa_flag = true;
+        sa_it = 0;
+    }
     ref_escaped = false;
     ref_it = mxit;
     const double R = (double)escape_radius;
@@ -86,12 +97,78 @@ void Frame::build_orbit(mpf_srcptr cre, mpf_srcptr cim) {
             ref_it = i;
             break;
         }
+        if (dbl && sa_flag) sa_step(i);
     }
     mpf_set(tail_re.v, zr);
     mpf_set(tail_im.v, zi);
     zf.resize(2 * (size_t)(ref_it + 1));
     zfe_m.resize(2 * (size_t)(ref_it + 1));
     zfe_e.resize(2 * (size_t)(ref_it + 1));
+    z_m3.resize((size_t)(ref_it + 1));
+}
+
+// get_exp (float_math.cpp:23-29): floor(log2|x|), zero -> -2^30
+static int sa_get_exp(double x) {
+    if (x == 0.0) return -0x40000000;
+    int e;
+    (void)std::frexp(x, &e);
+    return e - 1;
+}
+
+// SACheckMagnitude (mandel_perturbation.cpp:4307-4332): the first order after which
+// every later coefficient is 2^80 below every earlier one, at least 5; -1 if none.
+static int sa_check(const MxC* A) {
+    int pre_mn[MX_SA_N], suf_mx[MX_SA_N];
+    pre_mn[0] = std::min(sa_get_exp(A[0].r), sa_get_exp(A[0].i));
+    suf_mx[MX_SA_N - 1] = std::max(sa_get_exp(A[MX_SA_N - 1].r), sa_get_exp(A[MX_SA_N - 1].i));
+    for (int i = 1; i < MX_SA_N; ++i) pre_mn[i] = std::max(pre_mn[i - 1], std::min(sa_get_exp(A[i].r), sa_get_exp(A[i].i)));
+    for (int i = MX_SA_N - 2; i >= 0; --i) suf_mx[i] = std::max(suf_mx[i + 1], std::max(sa_get_exp(A[i].r), sa_get_exp(A[i].i)));
+    for (int i = 0; i < MX_SA_N - 10; ++i)
+        if (pre_mn[i] - suf_mx[i + 1] >= 80) return std::max(i, 5);
+    return -1;
+}
+
+// One step of the series coefficients after a non-escaping X_i (calCoefficient's
+// series block, mandel_perturbation.cpp:4273-4302), with X = _zf[i-1]:
+// N_j = 2X A_j + sum_{k<j/2} 2 A_k A_{j-k-1} (+ A_{j/2}^2 for odd j), N_0 += delta.
+void Frame::sa_step(int i) {
+    MxC N[MX_SA_N];
+    const MxC two = {2.0, 0.0};
+    const MxC z2 = mxc_mul(two, MxC{zf[2 * (i - 1)], zf[2 * (i - 1) + 1]});
+    for (int j = 0; j < MX_SA_N; ++j) {
+        N[j] = mxc_mul(z2, sa_A[j]);
+        for (int k = 0; k < j / 2; ++k) N[j] = mxc_add(N[j], mxc_mul(mxc_mul(two, sa_A[k]), sa_A[j - k - 1]));
+        if (j % 2) N[j] = mxc_add(N[j], mxc_mul(sa_A[j / 2], sa_A[j / 2]));
+    }
+    N[0] = mxc_add(N[0], sa_delta);
+    const int order = sa_check(N);
+    if (order < 0) {
+        sa_flag = false;
+        return;
+    }
+    sa_it = i;
+    sa_order = order;
+    for (int j = 0; j < MX_SA_N; ++j) sa_A[j] = N[j];
+}
+
+// Value of a (non-virtual) reference pixel from its own orbit
+// (mandel_perturbation.cpp:4145-4160, 4179-4185): getEscapeTime, or the GMP point
+// under EDE, -2 when the reference does not escape.
+void Frame::ref_pixel() {
+    if (!ref_escaped) {
+        ref_pixel_n = -1;
+        ref_pixel_value = -2.0;
+    } else if (ede) {
+        ref_pixel_value = point_value(ref_re.v, ref_im.v, max_iter, true, ref_pixel_n);
+    } else {
+        Gmp t1(prec), t2(prec);
+        mpf_mul(t1, tail_re.v, tail_re.v);
+        mpf_mul(t2, tail_im.v, tail_im.v);
+        mpf_add(t1, t1, t2);
+        const double rad = mpf_get_d(t1);
+        ref_pixel_n = ref_it + 1;
+        ref_pixel_value = (ref_it + 1 - std::log(std::log(rad) / 2 / std::log(2)) / std::log(2));
+    }
 }
 
 // refTailEscapes (mandel_perturbation.cpp:927-957)
@@ -289,8 +366,14 @@ int Frame::prepare(const mx_request& r, std::string& err) {
         err = "could not parse centre/scale";
         return MX_E_INVALID;
     }
-    if (mpf_cmp_d(scale, 1e280) <= 0) {
-        err = "scale <= 1e280: the plain double path is not in this build";
+    // path selection (mandel_perturbation.cpp:1434, 1569-1576)
+    if (mpf_cmp_d(scale, 1e6) <= 0) {
+        err = "scale <= 1e6: the engine's shallow (non-perturbation) path is not on this path";
+        return MX_E_UNSUPPORTED;
+    }
+    dbl = mpf_cmp_d(scale, 1e280) <= 0;
+    if (dbl && ede) {
+        err = "exterior distance estimate on the plain-double path (scale <= 1e280) is not built";
         return MX_E_UNSUPPORTED;
     }
     // depth guard for the int32 exponent of the device FloatExp: |log2 scale| < 2^26
@@ -329,6 +412,19 @@ int Frame::prepare(const mx_request& r, std::string& err) {
     mpf_shadow(dy.v, m, e);
     dyfe = fe_make(m, (int)e);
 
+    use_bla = r.use_bla != 0;
+    {
+        const int ms = r.bla_minskip > 0 ? r.bla_minskip : 8;
+        bla_minlevel = 0;
+        while ((1 << (bla_minlevel + 1)) <= ms) ++bla_minlevel;
+    }
+    interior = r.interior != 0;
+    if (dbl) {
+        const int rc = prepare_dbl(scale, err);
+        prepare_seconds = now_s() - t_start;
+        return rc;
+    }
+
     // reference: the centre pixel, then the exact view centre when the pixel
     // reference is bounded-but-sensitive (mandel_perturbation.cpp:1326-1373)
     const double t_ref = now_s();
@@ -360,29 +456,10 @@ int Frame::prepare(const mx_request& r, std::string& err) {
             }
         }
         // value of the reference pixel itself (mandel_perturbation.cpp:4145-4160, 4179-4185)
-        if (!ref_virtual) {
-            if (!ref_escaped) {
-                ref_pixel_n = -1;
-                ref_pixel_value = -2.0;
-            } else if (ede) {
-                ref_pixel_value = point_value(ref_re.v, ref_im.v, max_iter, true, ref_pixel_n);
-            } else {
-                Gmp t1(prec), t2(prec);
-                mpf_mul(t1, tail_re.v, tail_re.v);
-                mpf_mul(t2, tail_im.v, tail_im.v);
-                mpf_add(t1, t1, t2);
-                const double rad = mpf_get_d(t1);
-                ref_pixel_n = ref_it + 1;
-                ref_pixel_value = (ref_it + 1 - std::log(std::log(rad) / 2 / std::log(2)) / std::log(2));
-            }
-        }
+        if (!ref_virtual) ref_pixel();
     }
     ref_seconds = now_s() - t_ref;
 
-    use_bla = r.use_bla != 0;
-    int ms = r.bla_minskip > 0 ? r.bla_minskip : 8;
-    bla_minlevel = 0;
-    while ((1 << (bla_minlevel + 1)) <= ms) ++bla_minlevel;
     const double t_bla = now_s();
     if (use_bla) {
         // dcmax from the farthest image corner (mandel_perturbation.cpp:2626-2629)
@@ -415,9 +492,93 @@ int Frame::prepare(const mx_request& r, std::string& err) {
         drow_m[i] = f.m;
         drow_e[i] = f.e;
     }
-    interior = r.interior != 0;
     prepare_seconds = now_s() - t_start;
     return MX_OK;
+}
+
+// The plain-double path's first reference: the centre pixel (h/2, w/2)
+// (mandel_perturbation.cpp:1326-1328) with its series approximation over the frame's
+// half extent _SA_delta = (dx*w/2, dy*h/2) (4114-4119), and the content-aware
+// escalation test of scales above 1e150 (1336-1355).
+int Frame::prepare_dbl(mpf_srcptr scale, std::string& err) {
+    {
+        Gmp t(prec);
+        mpf_mul_ui(t, dx.v, W);
+        mpf_div_ui(t, t, 2);
+        sa_delta.r = mpf_get_ld(t);
+        mpf_mul_ui(t, dy.v, H);
+        mpf_div_ui(t, t, 2);
+        sa_delta.i = mpf_get_ld(t);
+    }
+    sa_order = 0;
+    n_refs = 0;
+    set_pixel_reference(H / 2, W / 2);
+    if (mpf_cmp_d(scale, 1e150) > 0) {
+        double refmin2 = 1e300;
+        bool underflowed = false;
+        for (int q = 1; q <= ref_it; ++q) {
+            const double m2 = z_m3[q] * 1e6;
+            if (m2 <= 0.0) {
+                underflowed = true;
+                break;
+            }
+            if (m2 < refmin2) refmin2 = m2;
+        }
+        const double dcmax_c = 2.0 / mpf_get_d(scale);
+        if (underflowed || std::sqrt(refmin2) * dcmax_c < 1e-300) {
+            err = "the reference passes too close to zero for the plain-double path (the engine escalates such "
+                  "frames to floatexp with an exact-centre reference, which is not built)";
+            return MX_E_UNSUPPORTED;
+        }
+    }
+    return MX_OK;
+}
+
+void Frame::set_pixel_reference(int pi, int pj) {
+    const double t_ref = now_s();
+    Gmp pre(prec), pim(prec), t(prec);
+    mpf_mul_ui(t, dx.v, pj);
+    mpf_add(pre, c0_re.v, t);
+    mpf_mul_ui(t, dy.v, pi);
+    mpf_add(pim, c0_im.v, t);
+    ref_virtual = false;
+    ref_i = pi;
+    ref_j = pj;
+    ref_x = pj;
+    ref_y = pi;
+    build_orbit(pre, pim);
+    ref_pixel();
+    ++n_refs;
+    ref_seconds += now_s() - t_ref;
+    const double t_bla = now_s();
+    if (use_bla) {
+        const double dxf = std::fabs(mpf_get_ld(dx.v)), dyf = std::fabs(mpf_get_ld(dy.v));
+        const double mxx = std::max(ref_x, (double)(W - 1) - ref_x) + 0.5;
+        const double mxy = std::max(ref_y, (double)(H - 1) - ref_y) + 0.5;
+        build_bla(std::sqrt(dxf * dxf * mxx * mxx + dyf * dyf * mxy * mxy));
+    } else {
+        lev_off.clear();
+        lev_size.clear();
+        bla_r2.clear();
+        bla_coef.clear();
+        rmax2 = 0.0;
+    }
+    bla_seconds += now_s() - t_bla;
+    // dc = trunc53(dx * (column - ref column)) from mpf (mandel_perturbation.cpp:2300-2312)
+    dcol_d.resize(W);
+    drow_d.resize(H);
+    for (int j = 0; j < W; ++j) {
+        const int xp = j - pj;
+        mpf_mul_ui(t, dx.v, (unsigned long)std::abs(xp));
+        if (xp < 0) mpf_neg(t, t);
+        dcol_d[j] = mpf_get_ld(t);
+    }
+    for (int i = 0; i < H; ++i) {
+        const int yp = i - pi;
+        mpf_mul_ui(t, dy.v, (unsigned long)std::abs(yp));
+        if (yp < 0) mpf_neg(t, t);
+        drow_d[i] = mpf_get_ld(t);
+    }
 }
 
 void Frame::init_mpf() {
@@ -467,6 +628,17 @@ void Frame::fill_view(mx_frame_view& v) const {
     v.precision_bits = (int32_t)prec;
     v.ref_seconds = ref_seconds;
     v.bla_seconds = bla_seconds;
+    v.path = dbl ? 1 : 2;
+    if (dbl) {
+        v.z_m3 = z_m3.data();
+        v.sa_it = sa_it;
+        v.sa_order = sa_order;
+        v.sa_coef = &sa_A[0].r;
+        v.sa_delta_re = sa_delta.r;
+        v.sa_delta_im = sa_delta.i;
+        v.dcol_d = dcol_d.data();
+        v.drow_d = drow_d.data();
+    }
 }
 
 } // namespace mx

@@ -10,6 +10,7 @@
 #include <vector>
 
 #include "../../include/mx.h"
+#include "mx_dbl.h"
 #include "mx_dev.h"
 
 namespace mx {
@@ -62,9 +63,27 @@ struct Frame {
 
     double ref_seconds = 0.0, bla_seconds = 0.0, prepare_seconds = 0.0;
 
+    // plain-double path (1e6 < scale <= 1e280): pixel references, the series
+    // approximation, the glitch metric's orbit magnitudes and double dc tables
+    bool dbl = false;
+    int ref_i = 0, ref_j = 0;
+    std::vector<double> z_m3;
+    MxC sa_A[MX_SA_N] = {};
+    MxC sa_delta = {0.0, 0.0};
+    int sa_it = 0, sa_order = 0;
+    bool sa_flag = false;
+    std::vector<double> dcol_d, drow_d;
+    int n_refs = 0;
+    // Build the reference at pixel (pi, pj) of the double path with everything a pass
+    // against it needs (createRef(random=false) + buildBLA, MP.cpp:1750-1758, 4014-4187).
+    void set_pixel_reference(int pi, int pj);
+
   private:
     void init_mpf();
     void build_orbit(mpf_srcptr cre, mpf_srcptr cim);
+    void ref_pixel();
+    void sa_step(int i);
+    int prepare_dbl(mpf_srcptr scale, std::string& err);
     bool tail_escapes(int extra);
     double point_value(mpf_srcptr c_re, mpf_srcptr c_im, int mxit, bool ede, int& n_out);
     void build_bla(double dcmax);
@@ -115,3 +134,4 @@ struct mx_plan {
 hipError_t mx_launch_fe64(const MxRefDev& R, const int2* lev, const MxTileDev& T, int blocks, int threads, hipStream_t stream);
 hipError_t mx_launch_fe32(const MxRefDev& R, const int2* lev, const MxTileDev& T, int blocks, int threads, hipStream_t stream);
 hipError_t mx_launch_sc64(const MxRefDev& R, const int2* lev, const MxTileDev& T, int blocks, int threads, hipStream_t stream);
+hipError_t mx_launch_dbl(const MxRefDev& R, const int2* lev, const MxTileDev& T, int blocks, int threads, hipStream_t stream);

@@ -24,7 +24,9 @@ class MxError(RuntimeError):
 class RefDesc(C.Structure):
     _fields_ = [("len", C.c_int32), ("zf", C.c_void_p), ("zfe_m", C.c_void_p), ("zfe_e", C.c_void_p),
                 ("nlev", C.c_int32), ("lev_off", C.c_void_p), ("lev_size", C.c_void_p),
-                ("bla_r2", C.c_void_p), ("bla_coef", C.c_void_p), ("rmax2", C.c_double)]
+                ("bla_r2", C.c_void_p), ("bla_coef", C.c_void_p), ("rmax2", C.c_double),
+                ("z_m3", C.c_void_p), ("sa_it", C.c_int32), ("sa_order", C.c_int32), ("sa_coef", C.c_void_p),
+                ("sa_delta_re", C.c_double), ("sa_delta_im", C.c_double)]
 
 
 class Tile(C.Structure):
@@ -33,7 +35,8 @@ class Tile(C.Structure):
                 ("int_rep", C.c_int32), ("bla", C.c_int32), ("bla_minlevel", C.c_int32),
                 ("interior_eps2", C.c_double), ("interior_confirm", C.c_int32),
                 ("dcol_m", C.c_void_p), ("dcol_e", C.c_void_p), ("drow_m", C.c_void_p), ("drow_e", C.c_void_p),
-                ("dx_m", C.c_double), ("dx_e", C.c_int32), ("skip_sample", C.c_int32)]
+                ("dx_m", C.c_double), ("dx_e", C.c_int32), ("skip_sample", C.c_int32),
+                ("dcol_d", C.c_void_p), ("drow_d", C.c_void_p), ("list", C.c_void_p), ("list_len", C.c_int32)]
 
 
 class Request(C.Structure):
@@ -52,7 +55,8 @@ class FrameInfo(C.Structure):
                 ("ref_seconds", C.c_double), ("bla_seconds", C.c_double), ("upload_seconds", C.c_double),
                 ("probe_ms", C.c_double), ("kernel_ms", C.c_double), ("total_seconds", C.c_double),
                 ("steps", C.c_uint64), ("bla_applies", C.c_uint64), ("bla_skipped", C.c_uint64),
-                ("rebases", C.c_uint64), ("nominal_iterations", C.c_uint64)]
+                ("rebases", C.c_uint64), ("nominal_iterations", C.c_uint64), ("passes", C.c_int32),
+                ("sa_it", C.c_int32)]
 
 
 class FrameView(C.Structure):
@@ -66,14 +70,17 @@ class FrameView(C.Structure):
                 ("esc2", C.c_double), ("bla_minlevel", C.c_int32), ("ref_it", C.c_int32),
                 ("ref_escaped", C.c_int32), ("ref_virtual", C.c_int32), ("ref_x", C.c_double), ("ref_y", C.c_double),
                 ("ref_pixel_n", C.c_int32), ("ref_pixel_value", C.c_double), ("cutoff_sensitive", C.c_int32),
-                ("precision_bits", C.c_int32), ("ref_seconds", C.c_double), ("bla_seconds", C.c_double)]
+                ("precision_bits", C.c_int32), ("ref_seconds", C.c_double), ("bla_seconds", C.c_double),
+                ("path", C.c_int32), ("z_m3", C.POINTER(C.c_double)), ("sa_it", C.c_int32), ("sa_order", C.c_int32),
+                ("sa_coef", C.POINTER(C.c_double)), ("sa_delta_re", C.c_double), ("sa_delta_im", C.c_double),
+                ("dcol_d", C.POINTER(C.c_double)), ("drow_d", C.POINTER(C.c_double))]
 
 
 # every symbol include/mx.h declares
 EXPORTS = ["mx_version", "mx_device_count", "mx_ctx_create", "mx_ctx_destroy", "mx_last_error", "mx_cancel",
            "mx_reset_cancellation", "mx_ref_upload", "mx_ref_release", "mx_plan_create", "mx_plan_launch",
-           "mx_plan_wait", "mx_plan_read", "mx_plan_destroy", "mx_tile_render", "mx_compute", "mx_frame_prepare",
-           "mx_frame_free"]
+           "mx_plan_wait", "mx_plan_read", "mx_plan_unresolved", "mx_plan_destroy", "mx_tile_render", "mx_compute",
+           "mx_frame_prepare", "mx_frame_free"]
 
 _LIB = None
 
@@ -100,6 +107,7 @@ def lib():
         L.mx_plan_wait.argtypes = [vp, C.POINTER(C.c_double)]
         L.mx_plan_read.argtypes = [vp, vp, vp, vp, vp]
         L.mx_plan_destroy.argtypes = [vp]
+        L.mx_plan_unresolved.argtypes = [vp, vp, vp, C.c_int32, C.POINTER(C.c_int32)]
         L.mx_tile_render.argtypes = [vp, vp, C.POINTER(Tile), vp, vp, vp, vp]
         L.mx_compute.argtypes = [vp, C.POINTER(Request), C.POINTER(FrameInfo)]
         L.mx_frame_prepare.argtypes = [C.POINTER(Request), C.POINTER(vp), C.POINTER(FrameView)]
@@ -149,7 +157,11 @@ def frame_prepare(cre, cim, scale, W, H, mxit, **kw):
                  ref_it=v.ref_it, ref_escaped=v.ref_escaped, ref_virtual=v.ref_virtual, ref_x=v.ref_x,
                  ref_y=v.ref_y, ref_pixel_n=v.ref_pixel_n, ref_pixel_value=v.ref_pixel_value,
                  cutoff_sensitive=v.cutoff_sensitive, precision_bits=v.precision_bits,
-                 ref_seconds=v.ref_seconds, bla_seconds=v.bla_seconds, W=W, H=H, mxit=mxit)
+                 ref_seconds=v.ref_seconds, bla_seconds=v.bla_seconds, W=W, H=H, mxit=mxit, path=v.path)
+        if v.path == 1:
+            d.update(z_m3=_arr(v.z_m3, n, np.float64), sa_it=v.sa_it, sa_order=v.sa_order,
+                     sa_coef=_arr(v.sa_coef, 60, np.float64), sa_delta=complex(v.sa_delta_re, v.sa_delta_im),
+                     dcol_d=_arr(v.dcol_d, W, np.float64), drow_d=_arr(v.drow_d, H, np.float64))
     finally:
         lib().mx_frame_free(fr)
     return d
@@ -215,12 +227,17 @@ class Context:
             d.lev_off, d.lev_size = a(fv["lev_off"], np.int32), a(fv["lev_size"], np.int32)
             d.bla_r2, d.bla_coef = a(fv["bla_r2"], np.float64), a(fv["bla_coef"], np.float64)
         d.rmax2 = fv["rmax2"]
+        if fv.get("path") == 1:
+            d.z_m3 = a(fv["z_m3"], np.float64)
+            d.sa_it, d.sa_order = fv["sa_it"], fv["sa_order"]
+            d.sa_coef = a(fv["sa_coef"], np.float64)
+            d.sa_delta_re, d.sa_delta_im = fv["sa_delta"].real, fv["sa_delta"].imag
         ref = C.c_void_p()
         self.check(lib().mx_ref_upload(self.h, C.byref(d), C.byref(ref)), "mx_ref_upload")
         return Ref(ref)
 
     def tile(self, fv, stride=1, form=MX_FORM_FE64, ede=False, interior=True, int_rep=None, bla=True,
-             bla_minlevel=None, mxit=None):
+             bla_minlevel=None, mxit=None, samples=None):
         t = Tile()
         t.width, t.height, t.stride = fv["W"], fv["H"], stride
         t.max_iter = mxit or fv["mxit"]
@@ -230,9 +247,17 @@ class Context:
         t.bla = int(bla and fv["nlev"] > 0)
         t.bla_minlevel = fv["bla_minlevel"] if bla_minlevel is None else bla_minlevel
         t.interior_eps2, t.interior_confirm = 3e-9 * 3e-9, 4
-        keep = [np.ascontiguousarray(fv[k], dt) for k, dt in
-                (("dcol_m", np.float64), ("dcol_e", np.int32), ("drow_m", np.float64), ("drow_e", np.int32))]
-        t.dcol_m, t.dcol_e, t.drow_m, t.drow_e = [_p(x) for x in keep]
+        if form == MX_FORM_DOUBLE:
+            keep = [np.ascontiguousarray(fv[k], np.float64) for k in ("dcol_d", "drow_d")]
+            t.dcol_d, t.drow_d = [_p(x) for x in keep]
+        else:
+            keep = [np.ascontiguousarray(fv[k], dt) for k, dt in
+                    (("dcol_m", np.float64), ("dcol_e", np.int32), ("drow_m", np.float64), ("drow_e", np.int32))]
+            t.dcol_m, t.dcol_e, t.drow_m, t.drow_e = [_p(x) for x in keep]
+        if samples is not None:
+            lst = np.ascontiguousarray(samples, np.int32)
+            keep.append(lst)
+            t.list, t.list_len = _p(lst), lst.size
         t.dx_m, t.dx_e = fv["dx_m"], fv["dx_e"]
         t.skip_sample = -1
         t._keep = keep
@@ -284,6 +309,16 @@ class Plan:
         st = np.zeros(4, np.uint64)
         self.ctx.check(lib().mx_plan_read(self.h, _p(nesc), _p(val), _p(out), _p(st)), "mx_plan_read")
         return nesc, val, out, st
+
+    def unresolved(self):
+        """(sample indices, glitch metrics) the last launch left unresolved (double form)."""
+        cnt = C.c_int32(0)
+        self.ctx.check(lib().mx_plan_unresolved(self.h, None, None, 0, C.byref(cnt)), "mx_plan_unresolved")
+        idx = np.zeros(cnt.value, np.int32)
+        gp = np.zeros(cnt.value)
+        if cnt.value:
+            self.ctx.check(lib().mx_plan_unresolved(self.h, _p(idx), _p(gp), cnt.value, C.byref(cnt)), "mx_plan_unresolved")
+        return idx, gp
 
     def destroy(self):
         if self.h:

@@ -53,6 +53,15 @@ static mxo_fe mpfr_to_fe_trunc(mpfr_srcptr x) {
     return mxo_fe_make(d, (int64_t)ex);
 }
 
+/* The engine's mpf numbers set to `prec` bits (mpf_set_default_prec) carry
+ * ceil((max(53,prec)+127)/64) limbs, and mpf arithmetic keeps one more: frame-level
+ * work here runs MPFR at that many bits, so grid, reference point and orbit track
+ * what the engine holds rather than a coarser rounding of it. */
+static int mpf_bits(int prec) {
+    const int p = prec < 53 ? 53 : prec;
+    return ((p + 127) / 64 + 1) * 64;
+}
+
 static double mpfr_to_ld_trunc(mpfr_srcptr x) {
     if (mpfr_zero_p(x)) return 0.0;
     long ex;
@@ -950,19 +959,20 @@ static void run_pixels(const mxo_ctx* ctx, const mxo_frame* f, const mxo_fe* dco
 
 MXO_EXPORT int mxo_compute_fe(mxo_frame* f) {
     const int W = f->W, H = f->H;
+    const int pb = mpf_bits(f->prec_bits);
     const double esc2 = (double)(f->escape_radius * f->escape_radius);
     const double escR = (double)f->escape_radius;
     mxo_grid g;
-    grid_init(&g, f->cre, f->cim, f->scale, W, H, f->full_h, f->row_base, f->prec_bits);
+    grid_init(&g, f->cre, f->cim, f->scale, W, H, f->full_h, f->row_base, pb);
     mxo_fe dxfe = mpfr_to_fe_trunc(g.dx), dyfe = mpfr_to_fe_trunc(g.dy);
     const int ede = (f->c_method & 2) != 0;
 
     /* reference at the centre pixel (H/2, W/2) */
     mpfr_t rre, rim;
-    mpfr_inits2(f->prec_bits, rre, rim, (mpfr_ptr)0);
+    mpfr_inits2(pb, rre, rim, (mpfr_ptr)0);
     grid_pixel_c(&g, H / 2, W / 2, 0, 0, 1, rre, rim);
     mxo_orbit o;
-    orbit_build(&o, rre, rim, f->prec_bits, f->mxit, escR, ede, 1);
+    orbit_build(&o, rre, rim, pb, f->mxit, escR, ede, 1);
     int ref_virtual = 0;
     double ref_x = W / 2, ref_y = H / 2;
     f->cutoff_sensitive_hint = 0;
@@ -971,7 +981,7 @@ MXO_EXPORT int mxo_compute_fe(mxo_frame* f) {
         if (sensitive) {
             orbit_free(&o);
             grid_view_center(&g, W, H, rre, rim);
-            orbit_build(&o, rre, rim, f->prec_bits, f->mxit, escR, ede, 1);
+            orbit_build(&o, rre, rim, pb, f->mxit, escR, ede, 1);
             ref_virtual = 1;
             ref_x = (W - 1) * 0.5;
             ref_y = (H - 1) * 0.5;
@@ -1263,6 +1273,7 @@ MXO_EXPORT double mxo_point(const char* cre, const char* cim, int wp_bits, int m
 
 /* Grid values for tests: truncated dx/dy as FloatExp and c0 as strings. */
 MXO_EXPORT void mxo_grid_info(const char* cre, const char* cim, const char* scale, int W, int H, int full_h, int row_base, int prec_bits, double* dx_m, int64_t* dx_e, double* dy_m, int64_t* dy_e) {
+    prec_bits = mpf_bits(prec_bits);
     mxo_grid g;
     grid_init(&g, cre, cim, scale, W, H, full_h, row_base, prec_bits);
     mxo_fe a = mpfr_to_fe_trunc(g.dx), b = mpfr_to_fe_trunc(g.dy);
@@ -1535,7 +1546,10 @@ static double pixel_double(const mxo_dctx* c, mxo_c dc, mxo_c dz0, int mxit, int
         /* The engine reads _z_m3[j] with j the pixel's iteration, which can run past
          * the entries this reference filled; the index is clamped to the last one. */
         const int jm = j < mx_ref_it ? j : mx_ref_it;
-        if (zrad < dzr * dzr + dzi * dzi || k == mx_ref_it) {
+        /* The end-of-reference test is k >= mx_ref_it where the engine has k ==: a
+         * series skip that ends at the reference's escape iteration starts the loop at
+         * k == mx_ref_it, and one step later the engine would read past the orbit. */
+        if (zrad < dzr * dzr + dzi * dzi || k >= mx_ref_it) {
             if ((dzr * dzr + dzi * dzi) / zrad > 10000000) {
                 *glitch_p = zrad / c->z_m3[jm];
                 *n_out = -3;
@@ -1545,11 +1559,6 @@ static double pixel_double(const mxo_dctx* c, mxo_c dc, mxo_c dz0, int mxit, int
             dzi = zi;
             k = 0;
             st->rebases++;
-        }
-        if (k == mx_ref_it) {
-            *glitch_p = (double)zrad / c->z_m3[jm];
-            *n_out = -3;
-            return -10.0;
         }
         ++j;
     }
@@ -1594,12 +1603,13 @@ typedef struct {
 
 MXO_EXPORT int mxo_compute_dbl(mxo_dframe* f) {
     const int W = f->W, H = f->H, mxit = f->mxit;
+    const int pb = mpf_bits(f->prec_bits);
     const double esc2 = (double)(f->escape_radius * f->escape_radius);
     const double escR = (double)f->escape_radius;
     mxo_grid g;
-    grid_init(&g, f->cre, f->cim, f->scale, W, H, 0, 0, f->prec_bits);
+    grid_init(&g, f->cre, f->cim, f->scale, W, H, 0, 0, pb);
     mpfr_t t;
-    mpfr_init2(t, f->prec_bits);
+    mpfr_init2(t, pb);
     mxo_c delta;
     mpfr_mul_ui(t, g.dx, W, MPFR_RNDN);
     mpfr_div_ui(t, t, 2, MPFR_RNDN);
@@ -1613,18 +1623,18 @@ MXO_EXPORT int mxo_compute_dbl(mxo_dframe* f) {
     double* drow = (double*)malloc(sizeof(double) * H);
     int ref_i = H / 2, ref_j = W / 2;
     mpfr_t rre, rim;
-    mpfr_inits2(f->prec_bits, rre, rim, (mpfr_ptr)0);
+    mpfr_inits2(pb, rre, rim, (mpfr_ptr)0);
     grid_pixel_c(&g, ref_i, ref_j, 0, 0, 1, rre, rim);
     mxo_orbit o;
     mxo_sa sa;
     memset(&sa, 0, sizeof sa);
-    orbit_build_sa(&o, &sa, rre, rim, f->prec_bits, mxit, escR, delta);
+    orbit_build_sa(&o, &sa, rre, rim, pb, mxit, escR, delta);
     f->sa_it0 = sa.it;
     f->sa_order0 = sa.order;
     f->escalate = 0;
     {
         mpfr_t sc;
-        mpfr_init2(sc, f->prec_bits);
+        mpfr_init2(sc, pb);
         mpfr_set_str(sc, f->scale, 10, MPFR_RNDN);
         if (mpfr_cmp_d(sc, 1e150) > 0) {
             double refmin2 = 1e300;
@@ -1767,7 +1777,7 @@ MXO_EXPORT int mxo_compute_dbl(mxo_dframe* f) {
         ref_j = best % W;
         orbit_free(&o);
         grid_pixel_c(&g, ref_i, ref_j, 0, 0, 1, rre, rim);
-        orbit_build_sa(&o, &sa, rre, rim, f->prec_bits, mxit, escR, delta);
+        orbit_build_sa(&o, &sa, rre, rim, pb, mxit, escR, delta);
         MXO_REF_PIXEL();
         n = 0;
         for (int q = 0; q < nr; ++q)
@@ -1845,4 +1855,119 @@ MXO_EXPORT void mxo_cdiv(int n, const double* a, const double* b, double* out) {
         out[2 * q] = r.r;
         out[2 * q + 1] = r.i;
     }
+}
+
+/* The double loop over caller-supplied orbit/BLA/series arrays (the same arrays the
+ * GPU consumed), so a parity test isolates the kernel.  gp[q] >= 0 marks an
+ * unresolved sample (nesc[q] = INT32_MIN). */
+MXO_EXPORT void mxo_pixels_dbl(const mxo_pixels_in* in, const double* z_m3, const double* A, int sa_it, int sa_order, double delta_r, double delta_i, int n, const double* dcr, const double* dci, double* val, int32_t* nesc, double* gp, long long* stats4) {
+    mxo_bla b;
+    memset(&b, 0, sizeof b);
+    if (in->nlev > 0) {
+        b.nlev = in->nlev;
+        for (int p = 0; p < in->nlev; ++p) {
+            b.off[p] = in->lev_off[p];
+            b.size[p] = in->lev_size[p];
+        }
+        b.r2 = (double*)in->r2;
+        b.l = (int64_t*)in->l;
+        b.coef = (double*)in->coef;
+        b.rmax2 = in->rmax2;
+    }
+    mxo_sa sa;
+    memset(&sa, 0, sizeof sa);
+    for (int x = 0; x < MXO_SA_N; ++x) {
+        sa.A[x].r = A[2 * x];
+        sa.A[x].i = A[2 * x + 1];
+    }
+    sa.order = sa_order;
+    sa.it = sa_it;
+    sa.delta.r = delta_r;
+    sa.delta.i = delta_i;
+    mxo_dctx ctx;
+    memset(&ctx, 0, sizeof ctx);
+    ctx.zf = in->zf;
+    ctx.z_m3 = z_m3;
+    ctx.mx_ref_it = in->mx_ref_it;
+    ctx.bla = in->nlev > 0 ? &b : NULL;
+    ctx.bla_minlevel = in->bla_minlevel;
+    ctx.esc2 = in->esc2;
+    ctx.use_interior = in->use_interior;
+    ctx.interior_eps2 = 3e-9 * 3e-9;
+    ctx.interior_confirm = 4;
+    ctx.int_rep = 1;
+    ctx.sa_it = sa_it;
+    long long s0 = 0, s1 = 0, s2 = 0, s3 = 0;
+#pragma omp parallel for schedule(dynamic, 16) num_threads(in->threads) reduction(+ : s0, s1, s2, s3)
+    for (int q = 0; q < n; ++q) {
+        mxo_c dc = {dcr[q], dci[q]};
+        mxo_c dz = sa_seed(&sa, dc);
+        mxo_stats st = {0, 0, 0, 0, 0, 0};
+        int ne;
+        double g;
+        double v = pixel_double(&ctx, dc, dz, in->mxit, &ne, &g, &st);
+        gp[q] = g;
+        if (ne == -3) {
+            nesc[q] = INT32_MIN;
+            val[q] = -10.0;
+        } else {
+            nesc[q] = ne;
+            val[q] = v;
+        }
+        s0 += st.steps;
+        s1 += st.bla_applies;
+        s2 += st.bla_skipped;
+        s3 += st.rebases;
+    }
+    stats4[0] = s0;
+    stats4[1] = s1;
+    stats4[2] = s2;
+    stats4[3] = s3;
+}
+
+/* First-reference set-up of the double frame (the part of mxo_compute_dbl before the
+ * passes): orbit shadows and |X|^2/1e6 (capacity mxit+1), series coefficients,
+ * delta and the double dc tables.  Returns ref_it. */
+MXO_EXPORT int mxo_dbl_setup(const char* cre, const char* cim, const char* scale, int W, int H, int mxit, float escape_radius, int prec_bits, double* zf, double* z_m3, double* A, double* delta, int* sa_it, int* sa_order, double* dcol, double* drow) {
+    prec_bits = mpf_bits(prec_bits);
+    mxo_grid g;
+    grid_init(&g, cre, cim, scale, W, H, 0, 0, prec_bits);
+    mpfr_t t, rre, rim;
+    mpfr_inits2(prec_bits, t, rre, rim, (mpfr_ptr)0);
+    mxo_c d;
+    mpfr_mul_ui(t, g.dx, W, MPFR_RNDN);
+    mpfr_div_ui(t, t, 2, MPFR_RNDN);
+    d.r = mpfr_to_ld_trunc(t);
+    mpfr_mul_ui(t, g.dy, H, MPFR_RNDN);
+    mpfr_div_ui(t, t, 2, MPFR_RNDN);
+    d.i = mpfr_to_ld_trunc(t);
+    const int ref_i = H / 2, ref_j = W / 2;
+    grid_pixel_c(&g, ref_i, ref_j, 0, 0, 1, rre, rim);
+    mxo_orbit o;
+    mxo_sa sa;
+    memset(&sa, 0, sizeof sa);
+    orbit_build_sa(&o, &sa, rre, rim, prec_bits, mxit, (double)escape_radius, d);
+    memcpy(zf, o.zf, sizeof(double) * 2 * (o.ref_it + 1));
+    memcpy(z_m3, o.z_m3, sizeof(double) * (o.ref_it + 1));
+    for (int x = 0; x < MXO_SA_N; ++x) {
+        A[2 * x] = sa.A[x].r;
+        A[2 * x + 1] = sa.A[x].i;
+    }
+    delta[0] = d.r;
+    delta[1] = d.i;
+    *sa_it = sa.it;
+    *sa_order = sa.order;
+    for (int j = 0; j < W; ++j) {
+        mpfr_mul_si(t, g.dx, j - ref_j, MPFR_RNDN);
+        dcol[j] = mpfr_to_ld_trunc(t);
+    }
+    for (int i = 0; i < H; ++i) {
+        mpfr_mul_si(t, g.dy, i - ref_i, MPFR_RNDN);
+        drow[i] = mpfr_to_ld_trunc(t);
+    }
+    const int ri = o.ref_it;
+    orbit_free(&o);
+    mpfr_clears(t, rre, rim, (mpfr_ptr)0);
+    grid_clear(&g);
+    return ri;
 }

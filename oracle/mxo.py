@@ -65,6 +65,11 @@ def lib():
         L.mxo_sa_coeffs.restype = C.c_int
         L.mxo_sa_seed.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_double, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
         L.mxo_cdiv.argtypes = [C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.mxo_pixels_dbl.argtypes = [C.POINTER(PixelsIn), C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_double, C.c_double,
+                                     C.c_int] + [C.c_void_p] * 6
+        L.mxo_dbl_setup.argtypes = [C.c_char_p, C.c_char_p, C.c_char_p, C.c_int, C.c_int, C.c_int, C.c_float, C.c_int] + \
+            [C.c_void_p] * 4 + [C.POINTER(C.c_int), C.POINTER(C.c_int), C.c_void_p, C.c_void_p]
+        L.mxo_dbl_setup.restype = C.c_int
         L.mxo_fe_op.argtypes = [C.c_int, C.c_int] + [C.c_void_p] * 6
         L.mxo_orbit_shadows.argtypes = [C.c_char_p, C.c_char_p, C.c_int, C.c_int, C.c_double, C.c_int] + [C.c_void_p] * 7
         L.mxo_orbit_shadows.restype = C.c_int
@@ -243,6 +248,55 @@ def pixels_fe(orb, bla, dcr_m, dcr_e, dci_m, dci_e, mxit, esc2, dx_m, dx_e, ede=
     lib().mxo_pixels_fe(C.byref(pi), n, arr(dcr_m, np.float64), arr(dcr_e, np.int64), arr(dci_m, np.float64),
                         arr(dci_e, np.int64), _p(val), _p(nesc), _p(st))
     return val, nesc, dict(steps=int(st[0]), bla_applies=int(st[1]), bla_skipped=int(st[2]), rebases=int(st[3]))
+
+
+def pixels_dbl(zf, z_m3, bla, sa_coef, sa_it, sa_order, sa_delta, dcr, dci, mxit, esc2, use_interior=True,
+               bla_minlevel=3, threads=None):
+    """The double loop (series seed included) over caller-supplied arrays."""
+    keep = []
+
+    def arr(a, dt):
+        a = np.ascontiguousarray(a, dt)
+        keep.append(a)
+        return _p(a)
+
+    pi = PixelsIn()
+    pi.zf = arr(zf, np.float64)
+    pi.mx_ref_it = len(z_m3) - 1
+    if bla is None:
+        pi.nlev = 0
+    else:
+        pi.nlev = bla["nlev"]
+        pi.lev_off, pi.lev_size = arr(bla["off"], np.int64), arr(bla["size"], np.int64)
+        pi.r2, pi.l, pi.coef = arr(bla["r2"], np.float64), arr(bla["l"], np.int64), arr(bla["coef"], np.float64)
+        pi.rmax2 = bla["rmax2"]
+    pi.bla_minlevel, pi.esc2, pi.use_interior = bla_minlevel, esc2, int(use_interior)
+    pi.mxit = mxit
+    pi.threads = threads or os.cpu_count()
+    n = len(dcr)
+    val = np.zeros(n)
+    nesc = np.zeros(n, np.int32)
+    gp = np.zeros(n)
+    st = np.zeros(4, np.int64)
+    lib().mxo_pixels_dbl(C.byref(pi), arr(z_m3, np.float64), arr(sa_coef, np.float64), sa_it, sa_order, sa_delta.real,
+                         sa_delta.imag, n, arr(dcr, np.float64), arr(dci, np.float64), _p(val), _p(nesc), _p(gp), _p(st))
+    return val, nesc, gp, dict(steps=int(st[0]), bla_applies=int(st[1]), bla_skipped=int(st[2]), rebases=int(st[3]))
+
+
+def dbl_setup(cre, cim, scale, W, H, mxit, escape_radius=1e8, prec_bits=None):
+    """First-reference set-up of a double frame: orbit, |X|^2/1e6, series, dc tables."""
+    pb = prec_bits or precision_bits(scale)
+    zf = np.zeros(2 * (mxit + 1))
+    m3 = np.zeros(mxit + 1)
+    A = np.zeros(60)
+    delta = np.zeros(2)
+    it, order = C.c_int(0), C.c_int(0)
+    dcol = np.zeros(W)
+    drow = np.zeros(H)
+    ri = lib().mxo_dbl_setup(cre.encode(), cim.encode(), scale.encode(), W, H, mxit, escape_radius, pb, _p(zf), _p(m3),
+                             _p(A), _p(delta), C.byref(it), C.byref(order), _p(dcol), _p(drow))
+    return dict(ref_it=ri, zf=zf[:2 * (ri + 1)], z_m3=m3[:ri + 1], sa_coef=A, sa_delta=complex(delta[0], delta[1]),
+                sa_it=it.value, sa_order=order.value, dcol_d=dcol, drow_d=drow)
 
 
 def direct(cre, cim, scale, W, H, mxit, c_method=0, escape_radius=1e8, step=1, prec_bits=None, wp_bits=None,

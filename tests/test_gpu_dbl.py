@@ -1,0 +1,103 @@
+"""Parity of the plain-double HIP kernel and its pass driver (through the C ABI) with
+the oracle's restatement of the engine's double path (series seed, double BLA,
+glitch re-referencing), and with the reference's own golden frames.  Escape counts
+bit-exact; smooth values within 1e-9 relative (only log() differs: device math
+library against the host's)."""
+import numpy as np
+import pytest
+
+import mxhip
+import mxo
+from test_host_prep import DBL_FRAMES
+from test_oracle_dbl import _stats
+import cases
+
+REL = 1e-9
+pytestmark = pytest.mark.gpu
+
+FRAMES = dict(DBL_FRAMES)
+FRAMES["ticktock"] = (cases.LOC["ticktock_x"], cases.LOC["ticktock_y"], cases.pow10(141), 120, 80, 200000, 0.0, 0)
+FRAMES["flake"] = (cases.LOC["flake_x"], cases.LOC["flake_y"], cases.pow10(157), 120, 80, 200000, 0.0, 0)
+
+
+def _check(val_g, nesc_g, val_o, nesc_o, label):
+    bad = np.argwhere(nesc_g != nesc_o)
+    assert bad.size == 0, f"{label}: {len(bad)} escape counts differ, first {bad[:5].tolist()} gpu={nesc_g[tuple(bad[0])]} cpu={nesc_o[tuple(bad[0])]}"
+    ext = nesc_o > 0
+    d = np.abs(val_g[ext] - val_o[ext]) / np.maximum(np.abs(val_o[ext]), 1e-300)
+    assert (d <= REL).all(), f"{label}: max rel value diff {d.max()}"
+    assert (val_g[(nesc_o == -1) | (nesc_o == -2)] == -2.0).all()
+
+
+@pytest.mark.parametrize("name", ["c1", "point31", "slowpoint", "ticktock", "flake"])
+def test_double_frame_matches_oracle(ctx, name):
+    cre, cim, scale, W, H, mxit, R, pb = FRAMES[name]
+    print(f"[dbl parity] {name} {W}x{H} mxit={mxit}", flush=True)
+    it, val, nesc, info = ctx.compute(cre, cim, scale, W, H, mxit, escape_radius=R, prec_bits=pb)
+    val_o, nesc_o, out_o, info_o = mxo.compute_dbl(cre, cim, scale, W, H, mxit, escape_radius=R or 1e8, prec_bits=pb or None)
+    assert info["path"] == 1 and info["passes"] == info_o["passes"] and info["sa_it"] == info_o["sa_it0"]
+    _check(val, nesc, val_o, nesc_o, name)
+    assert np.array_equal(it < -1.5, out_o < -1.5) and not (it == -10).any()
+    for k in ("steps", "bla_applies", "bla_skipped", "rebases"):
+        assert info[k] == info_o[k], k
+
+
+@pytest.mark.parametrize("name,golden,tol", [("point31", "point_7e31.f32", (12000.0, 200.0, 4000.0)),
+                                             ("slowpoint", "slowpoint_7e8.f32", (90000.0, 300.0, 3000.0))])
+def test_double_frame_matches_reference_golden(ctx, name, golden, tol):
+    """verify.cpp:11860-11881: class exact, max/mean/p99 smooth difference bounded."""
+    cre, cim, scale, W, H, mxit, R, pb = FRAMES[name]
+    it, val, nesc, info = ctx.compute(cre, cim, scale, W, H, mxit)
+    mis, mx, mean, p99 = _stats(it, cases.golden(golden, H, W))
+    assert mis == 0 and mx <= tol[0] and mean <= tol[1] and p99 <= tol[2], (mis, mx, mean, p99)
+
+
+def test_double_kernel_on_product_arrays(ctx):
+    """Kernel alone: one launch over the frame against the product's own reference,
+    series and dc tables, and the oracle's loop on the same arrays, unresolved
+    samples (glitches) and their metric included."""
+    cre, cim, scale, W, H, mxit, R, pb = FRAMES["point31"]
+    fv = mxhip.frame_prepare(cre, cim, scale, W, H, mxit)
+    ref = ctx.upload(fv)
+    try:
+        t = ctx.tile(fv, form=mxhip.MX_FORM_DOUBLE, interior=True)
+        plan = ctx.plan(ref, t)
+        try:
+            plan.launch()
+            plan.wait()
+            nesc, val, out, st = plan.read()
+            uidx, ugp = plan.unresolved()
+        finally:
+            plan.destroy()
+    finally:
+        ref.release()
+    b = mxo.bla_tables(fv["zf"], fv["ref_it"], fv["dcmax"])
+    bla = dict(nlev=b["nlev"], off=b["off"], size=b["size"], r2=b["r2"], l=b["l"], coef=b["coef"], rmax2=b["rmax2"])
+    jj, ii = np.meshgrid(np.arange(W), np.arange(H))
+    v2, n2, gp2, st2 = mxo.pixels_dbl(fv["zf"], fv["z_m3"], bla, fv["sa_coef"], fv["sa_it"], fv["sa_order"], fv["sa_delta"],
+                                      fv["dcol_d"][jj.ravel()], fv["drow_d"][ii.ravel()], mxit, fv["esc2"],
+                                      bla_minlevel=fv["bla_minlevel"])
+    _check(val, nesc, v2, n2, "point31 tile")
+    un = np.nonzero(gp2 >= 0)[0]
+    assert un.size > 0
+    order = np.argsort(uidx)
+    assert np.array_equal(uidx[order], un) and np.array_equal(ugp[order], gp2[un])
+    assert int(st[0]) == st2["steps"] and int(st[1]) == st2["bla_applies"] and int(st[3]) == st2["rebases"]
+
+
+def test_double_sample_list_equals_grid(ctx):
+    """A launch over a list of samples computes exactly what the grid launch computes
+    for those samples (the later-pass mode)."""
+    cre, cim, scale, W, H, mxit, R, pb = FRAMES["c1"]
+    fv = mxhip.frame_prepare(cre, cim, scale, W, H, mxit, escape_radius=R, prec_bits=pb)
+    ref = ctx.upload(fv)
+    try:
+        n_all, v_all, _, _ = ctx.render(ref, ctx.tile(fv, form=mxhip.MX_FORM_DOUBLE))
+        rng = np.random.default_rng(11)
+        sel = np.sort(rng.choice(W * H, 5000, replace=False)).astype(np.int32)
+        n_l, v_l, _, _ = ctx.render(ref, ctx.tile(fv, form=mxhip.MX_FORM_DOUBLE, samples=sel))
+    finally:
+        ref.release()
+    assert np.array_equal(n_l[sel], n_all[sel]) and np.array_equal(v_l[sel], v_all[sel])
+    rest = np.setdiff1d(np.arange(W * H), sel)
+    assert (n_l[rest] == np.iinfo(np.int32).min).all()

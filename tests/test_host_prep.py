@@ -97,3 +97,36 @@ def test_backend_factory_refuses_unknown_names():
     drv = os.path.join(ROOT, "mandelbrot-explorer_amd", "_build", "backend_driver")
     r = subprocess.run([drv, "names"], capture_output=True, text=True, timeout=60)
     assert r.returncode == 0, r.stderr
+
+
+DBL_FRAMES = {
+    # 640x360 at scale 4e10, bailout 65536, 128-bit precision
+    "c1": ("-0.743643887037151", "0.131825904205330", "40000000000", 640, 360, 8192, 65536.0, 128),
+    # verify.cpp:11876-11881 and 11864-11870
+    "point31": ("-0.749139567333446841955467474699747367338762518832278501811",
+                "0.040823298514634751035521346975478853963578400940553676068", "7354177" + "0" * 25, 48, 32, 500000, 0.0, 0),
+    "slowpoint": ("-1.1758621450236620370", "-0.2447677973532398022", "782679600", 48, 32, 500000, 0.0, 0),
+}
+
+
+@pytest.mark.parametrize("name", sorted(DBL_FRAMES))
+def test_double_prepare_matches_oracle(name):
+    """Plain-double path host preparation: GMP orbit, |X|^2/1e6, series coefficients
+    (calCoefficient), half extent delta and the truncated dc tables equal the
+    oracle's, bit for bit."""
+    cre, cim, scale, W, H, mxit, R, pb = DBL_FRAMES[name]
+    fv = mxhip.frame_prepare(cre, cim, scale, W, H, mxit, escape_radius=R, prec_bits=pb)
+    o = mxo.dbl_setup(cre, cim, scale, W, H, mxit, escape_radius=R or 1e8, prec_bits=pb or None)
+    assert fv["path"] == 1 and fv["ref_virtual"] == 0 and (fv["ref_x"], fv["ref_y"]) == (W // 2, H // 2)
+    assert fv["ref_it"] == o["ref_it"] and (fv["sa_it"], fv["sa_order"]) == (o["sa_it"], o["sa_order"])
+    for k in ("zf", "z_m3", "sa_coef", "dcol_d", "drow_d"):
+        assert np.array_equal(fv[k].view(np.int64), o[k].view(np.int64)), k
+    assert fv["sa_delta"] == o["sa_delta"]
+    b = mxo.bla_tables(fv["zf"], fv["ref_it"], fv["dcmax"])
+    assert b["nlev"] == fv["nlev"] and np.array_equal(b["r2"], fv["bla_r2"]) and np.array_equal(b["coef"], fv["bla_coef"])
+
+
+def test_double_path_refuses_what_it_does_not_build():
+    with pytest.raises(mxhip.MxError) as e:
+        mxhip.frame_prepare("-0.75", "0.1", "1" + "0" * 20, 16, 8, 1000, c_method=mxhip.MX_EXTERIOR_DIST_EST)
+    assert e.value.rc == mxhip.MX_E_UNSUPPORTED
