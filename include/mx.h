@@ -90,6 +90,13 @@ typedef struct {
     int32_t sa_order;
     const double* sa_coef;   /* [60] A_0..A_29 as (re, im) */
     double sa_delta_re, sa_delta_im;
+    /* plain-double path with the exterior distance estimate (NULL otherwise): the
+     * reference derivative D (re, im interleaved, [2*len]), the BLA derivative couplings
+     * (cr, ci, er, ei per entry, [4*total]; mandel_perturbation.h:203-206) and the
+     * derivative-delta series B_0..B_29 ([60]) */
+    const double* df;
+    const double* bla_der;
+    const double* sa_coef_b;
 } mx_ref_desc;
 
 int mx_ref_upload(mx_ctx* ctx, const mx_ref_desc* desc, mx_ref** out);
@@ -240,6 +247,10 @@ typedef struct {
     double sa_delta_re, sa_delta_im;
     const double* dcol_d;    /* [width] */
     const double* drow_d;    /* [height] */
+    const double* dfr;       /* [len] reference derivative (EDE) */
+    const double* dfi;
+    const double* bla_der;   /* [4*bla_total] (EDE) */
+    const double* sa_coef_b; /* [60] (EDE) */
 } mx_frame_view;
 
 int mx_frame_prepare(const mx_request* req, mx_frame** out, mx_frame_view* view);

@@ -8,7 +8,10 @@
 // iteration sa_it+1, the step dz' = 2 X dz + dz^2 + dc against the double orbit,
 // Zhuoran rebase when |z| < |dz| or at the end of the reference, and glitch
 // detection (|dz|^2/|z|^2 > 1e7 at a rebase) that leaves the sample unresolved with
-// its glitch metric |z|^2/_z_m3[j] for the host's next reference choice.
+// its glitch metric |z|^2/_z_m3[j] for the host's next reference choice.  Under the
+// exterior distance estimate the derivative delta dd = dz/dc - D rides along
+// (2444-2456; through a BLA skip dd -> C dz + A dd + E dc, 2742-2749; seeded by its
+// own series) and an escape returns sqrt|z|^2 ln|z|^2 / (dx |dz/dc|) (2487-2488).
 //
 // Execution model as in the floatexp kernels: a persistent grid whose lanes refill
 // from per-XCD atomic work queues (wave-aggregated fetch-add), samples handed out
@@ -22,6 +25,7 @@ namespace {
 
 struct Pd {
     double dzr, dzi, dcr, dci;
+    double ddr, ddi;
     double zsr, zsi, conf_D2, conf_zr, conf_zi;
     double sdzr, sdzi;
     int j, k;
@@ -40,21 +44,34 @@ __device__ __forceinline__ unsigned long long wave_sum(unsigned long long v) {
     return v;
 }
 
-// Series seed (the Horner loop of mandel_perturbation.cpp:2319-2323) with the
+// Series seeds (the Horner loops of mandel_perturbation.cpp:2319-2330) with the
 // explicit complex formulas of mx_dbl.h.
+__device__ __forceinline__ MxC sa_horner(const double2* __restrict__ C, int order, MxC delta, MxC dc) {
+    MxC z = {0.0, 0.0};
+    for (int x = order; x >= 0; --x) {
+        const double2 a = C[x];
+        z = mxc_add(z, MxC{a.x, a.y});
+        z = mxc_mul(z, dc);
+        z = mxc_div(z, delta);
+    }
+    return z;
+}
+
+template <bool EDE>
 __device__ void init_pixel(const MxRefDev& R, Pd& p, double dcr, double dci) {
     p.dcr = dcr;
     p.dci = dci;
     const MxC dc = {dcr, dci}, delta = {R.sa_delta.x, R.sa_delta.y};
-    MxC dz = {0.0, 0.0};
-    for (int x = R.sa_order; x >= 0; --x) {
-        const double2 a = R.sa_A[x];
-        dz = mxc_add(dz, MxC{a.x, a.y});
-        dz = mxc_mul(dz, dc);
-        dz = mxc_div(dz, delta);
-    }
+    const MxC dz = sa_horner(R.sa_A, R.sa_order, delta, dc);
     p.dzr = dz.r;
     p.dzi = dz.i;
+    if (EDE) {
+        const MxC dd = sa_horner(R.sa_B, R.sa_order, delta, dc);
+        p.ddr = dd.r;
+        p.ddi = dd.i;
+    } else {
+        p.ddr = p.ddi = 0.0;
+    }
     p.j = R.sa_it + 1;
     p.k = p.j;
     p.zsr = 1e30;
@@ -75,14 +92,16 @@ __device__ void init_pixel(const MxRefDev& R, Pd& p, double dcr, double dci) {
 // largest aligned level whose radius admits |dz|, landing at least two entries
 // before the end of the reference and not beyond the escape radius.
 struct BlaRes {
-    double nzr, nzi;
+    double nzr, nzi, nddr, nddi;
     int skip;
 };
 
+template <bool EDE>
 __device__ __noinline__ BlaRes try_bla(const double* __restrict__ bla_r2, const double4* __restrict__ bla_coef,
-                                       const double2* __restrict__ zf, const int2* __restrict__ lev, int nlev, int minlevel,
-                                       int mx_ref_it, int s, double dzr, double dzi, double dcr, double dci, double esc2) {
-    BlaRes r = {0.0, 0.0, 0};
+                                       const double4* __restrict__ bla_der, const double2* __restrict__ zf,
+                                       const int2* __restrict__ lev, int nlev, int minlevel, int mx_ref_it, int s, double dzr,
+                                       double dzi, double ddr, double ddi, double dcr, double dci, double esc2) {
+    BlaRes r = {0.0, 0.0, 0.0, 0.0, 0};
     if (s < 1) return r;
     const double zmag2 = dzr * dzr + dzi * dzi;
     const int maxp = nlev - 1;
@@ -105,6 +124,11 @@ __device__ __noinline__ BlaRes try_bla(const double* __restrict__ bla_r2, const 
         const double2 Z = zf[land];
         const double fr = Z.x + nzr, fi = Z.y + nzi;
         if (fr * fr + fi * fi > esc2) return r;
+        if (EDE) {
+            const double4 d = bla_der[q];
+            r.nddr = (d.x * dzr - d.y * dzi) + (c.x * ddr - c.y * ddi) + (d.z * dcr - d.w * dci);
+            r.nddi = (d.x * dzi + d.y * dzr) + (c.x * ddi + c.y * ddr) + (d.z * dci + d.w * dcr);
+        }
         r.nzr = nzr;
         r.nzi = nzi;
         r.skip = l;
@@ -115,7 +139,7 @@ __device__ __noinline__ BlaRes try_bla(const double* __restrict__ bla_r2, const 
 
 // One iteration of the stepParallel loop body for one lane.  Returns 1 when the
 // sample is finished (n, v set; n == INT_MIN: unresolved, *gp set).
-template <bool INTERIOR>
+template <bool INTERIOR, bool EDE>
 __device__ __forceinline__ int trip(const MxRefDev& R, const int2* __restrict__ lev, const MxTileDev& T, Pd& p, double& v, int& n,
                                     double& gp, Counters& c) {
     if (p.j >= T.mxit) {
@@ -138,11 +162,16 @@ __device__ __forceinline__ int trip(const MxRefDev& R, const int2* __restrict__ 
         }
     }
     if ((T.flags & MX_F_BLA) && p.dzr * p.dzr + p.dzi * p.dzi < R.rmax2) {
-        const BlaRes b = try_bla(R.bla_r2, R.bla_coef, R.zf, lev, R.nlev, R.minlevel, R.mx_ref_it, p.k - 1, p.dzr, p.dzi, p.dcr, p.dci, T.esc2);
+        const BlaRes b = try_bla<EDE>(R.bla_r2, R.bla_coef, R.bla_der, R.zf, lev, R.nlev, R.minlevel, R.mx_ref_it, p.k - 1,
+                                      p.dzr, p.dzi, p.ddr, p.ddi, p.dcr, p.dci, T.esc2);
         const int skip = b.skip;
         if (skip > 0) {
             p.dzr = b.nzr;
             p.dzi = b.nzi;
+            if (EDE) {
+                p.ddr = b.nddr;
+                p.ddi = b.nddi;
+            }
             p.k += skip;
             p.j += skip;
             p.xv = 0;
@@ -157,13 +186,30 @@ __device__ __forceinline__ int trip(const MxRefDev& R, const int2* __restrict__ 
             return 0;
         }
     }
+    if (p.k != 0 && !p.xv) p.Xc = R.zf[p.k - 1];
+    double dr = 0.0, di = 0.0;
+    if (EDE) {
+        // dd' = 2 (dd X + dz (D + dd)) from the dz before this step (2444-2456)
+        double t2;
+        if (p.k == 0) {
+            t2 = (p.dzr * (1 + p.ddr) - p.dzi * (1 + p.ddi)) * 2;
+            p.ddi = (p.dzr * (1 + p.ddi) + p.dzi * (1 + p.ddr)) * 2;
+        } else {
+            const double2 D = R.df[p.k - 1];
+            t2 = (p.ddr * p.Xc.x - p.ddi * p.Xc.y + p.dzr * (D.x + p.ddr) - p.dzi * (D.y + p.ddi)) * 2;
+            p.ddi = (p.ddr * p.Xc.y + p.ddi * p.Xc.x + p.dzr * (D.y + p.ddi) + p.dzi * (D.x + p.ddr)) * 2;
+        }
+        p.ddr = t2;
+        const double2 Dk = R.df[p.k];
+        dr = p.ddr + Dk.x;
+        di = p.ddi + Dk.y;
+    }
     const double dzr2 = p.dzr * p.dzr, dzi2 = p.dzi * p.dzi;
     double tmp, dzi;
     if (p.k == 0) {
         tmp = dzr2 - dzi2 + p.dcr;
         dzi = p.dzr * p.dzi * 2 + p.dci;
     } else {
-        if (!p.xv) p.Xc = R.zf[p.k - 1];
         const double xr = p.Xc.x, xi = p.Xc.y;
         tmp = (p.dzr * xr - p.dzi * xi) * 2 + dzr2 - dzi2 + p.dcr;
         dzi = (p.dzr * xi + p.dzi * xr) * 2 + p.dzr * p.dzi * 2 + p.dci;
@@ -179,8 +225,13 @@ __device__ __forceinline__ int trip(const MxRefDev& R, const int2* __restrict__ 
     p.xv = 1;
     c.steps++;
     if (zrad > T.esc2) {
-        const double LG = 0.69314718055994530942; // log(2) rounded to double
-        v = p.j + 1 - log(log(zrad) / 2 / LG) / LG;
+        if (EDE) {
+            const double dxf = ldexp(T.dxfe.m, T.dxfe.e); // mpf_get_ld(_dx)
+            v = sqrt(zrad) / dxf * log(zrad) / sqrt(dr * dr + di * di);
+        } else {
+            const double LG = 0.69314718055994530942; // log(2) rounded to double
+            v = p.j + 1 - log(log(zrad) / 2 / LG) / LG;
+        }
         n = p.j + 1;
         return 1;
     }
@@ -239,6 +290,10 @@ __device__ __forceinline__ int trip(const MxRefDev& R, const int2* __restrict__ 
         }
         p.dzr = zr;
         p.dzi = zi;
+        if (EDE) {
+            p.ddr = dr - 1;
+            p.ddi = di;
+        }
         p.k = 0;
         p.xv = 0;
         c.rebases++;
@@ -261,7 +316,7 @@ __device__ __forceinline__ bool item_to_sample(const MxTileDev& T, unsigned item
     return sidx != T.skip_sample;
 }
 
-template <bool INTERIOR>
+template <bool INTERIOR, bool EDE>
 __global__ __launch_bounds__(256) void mx_dbl_kernel(MxRefDev R, const int2* __restrict__ lev, MxTileDev T) {
     const int lane = __lane_id();
     const unsigned long long lanemask_lt = (1ull << lane) - 1ull;
@@ -288,7 +343,7 @@ __global__ __launch_bounds__(256) void mx_dbl_kernel(MxRefDev R, const int2* __r
             if (need) {
                 const unsigned it = base + (unsigned)__popcll(needmask & lanemask_lt);
                 if (it < nitems && item_to_sample(T, i0 + it, sidx)) {
-                    init_pixel(R, p, T.dcol_d[sidx % T.W], T.drow_d[sidx / T.W]);
+                    init_pixel<EDE>(R, p, T.dcol_d[sidx % T.W], T.drow_d[sidx / T.W]);
                     need = false;
                     active = true;
                 }
@@ -305,7 +360,7 @@ __global__ __launch_bounds__(256) void mx_dbl_kernel(MxRefDev R, const int2* __r
         if (active) {
             double v = 0.0, gp = 0.0;
             int n = 0;
-            if (trip<INTERIOR>(R, lev, T, p, v, n, gp, c)) {
+            if (trip<INTERIOR, EDE>(R, lev, T, p, v, n, gp, c)) {
                 if (n == INT_MIN) {
                     const unsigned slot = atomicAdd(T.unres_count, 1u);
                     T.unres_idx[slot] = sidx;
@@ -331,9 +386,14 @@ __global__ __launch_bounds__(256) void mx_dbl_kernel(MxRefDev R, const int2* __r
 } // namespace
 
 hipError_t mx_launch_dbl(const MxRefDev& R, const int2* lev, const MxTileDev& T, int blocks, int threads, hipStream_t stream) {
-    if (T.flags & MX_F_INTERIOR)
-        hipLaunchKernelGGL((mx_dbl_kernel<true>), dim3(blocks), dim3(threads), 0, stream, R, lev, T);
+    const bool ede = T.flags & MX_F_EDE, interior = T.flags & MX_F_INTERIOR;
+    if (ede && interior)
+        hipLaunchKernelGGL((mx_dbl_kernel<true, true>), dim3(blocks), dim3(threads), 0, stream, R, lev, T);
+    else if (ede)
+        hipLaunchKernelGGL((mx_dbl_kernel<false, true>), dim3(blocks), dim3(threads), 0, stream, R, lev, T);
+    else if (interior)
+        hipLaunchKernelGGL((mx_dbl_kernel<true, false>), dim3(blocks), dim3(threads), 0, stream, R, lev, T);
     else
-        hipLaunchKernelGGL((mx_dbl_kernel<false>), dim3(blocks), dim3(threads), 0, stream, R, lev, T);
+        hipLaunchKernelGGL((mx_dbl_kernel<false, false>), dim3(blocks), dim3(threads), 0, stream, R, lev, T);
     return hipGetLastError();
 }

@@ -44,6 +44,9 @@ struct MxRefDev {
     const double2* sa_A;   // series coefficients A_0..A_29
     double2 sa_delta;
     int sa_it, sa_order;
+    const double2* df;       // reference derivative D (EDE)
+    const double4* bla_der;  // BLA derivative couplings (cr, ci, er, ei) (EDE)
+    const double2* sa_B;     // derivative-delta series (EDE)
 };
 
 struct MxTileDev {

@@ -125,6 +125,7 @@ extern "C" int mx_ref_upload(mx_ctx* ctx, const mx_ref_desc* d, mx_ref** out) {
     if (d->nlev > 0 && (!d->bla_r2 || !d->bla_coef)) return MX_E_INVALID;
     if (total > INT_MAX / 4 || (int64_t)d->len > INT_MAX / 2) return MX_E_INVALID;
     if (d->z_m3 && (!d->sa_coef || d->sa_order < 0 || d->sa_order >= MX_SA_N || d->sa_it < 0)) return MX_E_INVALID;
+    if (d->df && (!d->z_m3 || !d->sa_coef_b || (d->nlev > 0 && !d->bla_der))) return MX_E_INVALID;
     MX_HIP(hipSetDevice(ctx->device));
     const size_t len = (size_t)d->len;
     size_t o_zf = 0;
@@ -138,7 +139,10 @@ extern "C" int mx_ref_upload(mx_ctx* ctx, const mx_ref_desc* d, mx_ref** out) {
     size_t o_lev = align_up(o_ce + 8 * (size_t)total);
     size_t o_m3 = align_up(o_lev + 8 * MX_MAXLEV);
     size_t o_sa = align_up(o_m3 + (d->z_m3 ? 8 * len : 0));
-    size_t bytes = align_up(o_sa + 16 * MX_SA_N);
+    size_t o_df = align_up(o_sa + 16 * MX_SA_N);
+    size_t o_der = align_up(o_df + (d->df ? 16 * len : 0));
+    size_t o_sab = align_up(o_der + (d->df ? 32 * (size_t)total : 0));
+    size_t bytes = align_up(o_sab + 16 * MX_SA_N);
     auto* ref = new (std::nothrow) mx_ref();
     if (!ref) return MX_E_NOMEM;
     ref->ctx = ctx;
@@ -185,6 +189,11 @@ extern "C" int mx_ref_upload(mx_ctx* ctx, const mx_ref_desc* d, mx_ref** out) {
         cp(o_m3, d->z_m3, 8 * len);
         cp(o_sa, d->sa_coef, 16 * MX_SA_N);
     }
+    if (d->df) {
+        cp(o_df, d->df, 16 * len);
+        if (d->nlev > 0) cp(o_der, d->bla_der, 32 * (size_t)total);
+        cp(o_sab, d->sa_coef_b, 16 * MX_SA_N);
+    }
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
     if (e != hipSuccess) {
         ctx->set_err(std::string("reference upload: ") + hipGetErrorString(e));
@@ -212,6 +221,11 @@ extern "C" int mx_ref_upload(mx_ctx* ctx, const mx_ref_desc* d, mx_ref** out) {
         R.sa_it = d->sa_it;
         R.sa_order = d->sa_order;
     }
+    if (d->df) {
+        R.df = (const double2*)(b + o_df);
+        R.bla_der = (const double4*)(b + o_der);
+        R.sa_B = (const double2*)(b + o_sab);
+    }
     ref->lev = (int2*)(b + o_lev);
     *out = ref;
     return MX_OK;
@@ -235,10 +249,10 @@ extern "C" int mx_plan_create(mx_ctx* ctx, const mx_ref* ref, const mx_tile* t, 
         return MX_E_INVALID;
     }
     if (!dbl && (!t->dcol_m || !t->dcol_e || !t->drow_m || !t->drow_e)) return MX_E_INVALID;
-    if (dbl && (!t->dcol_d || !t->drow_d || !ref->dev.z_m3 || t->ede)) {
-        ctx->set_err(t->ede ? "exterior distance estimate on the plain-double path is not built"
-                            : "the plain-double form needs double dc tables and a reference with its series");
-        return t->ede ? MX_E_UNSUPPORTED : MX_E_INVALID;
+    if (dbl && (!t->dcol_d || !t->drow_d || !ref->dev.z_m3 || (t->ede && !ref->dev.df))) {
+        ctx->set_err("the plain-double form needs double dc tables and a reference with its series "
+                     "(and its derivative under EDE)");
+        return MX_E_INVALID;
     }
     if (t->list && (t->list_len < 0 || t->list_len > t->width * t->height)) return MX_E_INVALID;
     if (t->list)
@@ -460,6 +474,17 @@ int upload_frame_ref(mx_ctx* ctx, const mx::Frame& F, mx_ref** ref) {
     d.sa_coef = v.sa_coef;
     d.sa_delta_re = v.sa_delta_re;
     d.sa_delta_im = v.sa_delta_im;
+    std::vector<double> df;
+    if (v.dfr) {
+        df.resize(2 * (size_t)v.len);
+        for (int32_t k = 0; k < v.len; ++k) {
+            df[2 * (size_t)k] = v.dfr[k];
+            df[2 * (size_t)k + 1] = v.dfi[k];
+        }
+        d.df = df.data();
+        d.bla_der = v.bla_der;
+        d.sa_coef_b = v.sa_coef_b;
+    }
     return mx_ref_upload(ctx, &d, ref);
 }
 
@@ -538,6 +563,9 @@ int compute_dbl(mx_ctx* ctx, mx::Frame& F, const mx_request* req, mx_frame_info*
     t.max_iter = F.max_iter;
     t.esc2 = F.esc2;
     t.form = MX_FORM_DOUBLE;
+    t.ede = F.ede;
+    t.dx_m = F.dxfe.m;
+    t.dx_e = F.dxfe.e;
     t.interior = F.interior;
     t.int_rep = F.interior;
     t.bla = F.use_bla && !F.lev_off.empty();

@@ -52,6 +52,10 @@ void Frame::build_orbit(mpf_srcptr cre, mpf_srcptr cim) {
     zfe_m.assign(2 * (size_t)(mxit + 1), 0.0);
     zfe_e.assign(2 * (size_t)(mxit + 1), 0);
     z_m3.assign((size_t)(mxit + 1), 0.0);
+    const bool der = dbl && ede;
+    dfr.assign(der ? (size_t)(mxit + 1) : 0, 0.0);
+    dfi.assign(der ? (size_t)(mxit + 1) : 0, 0.0);
+    if (der) dfr[0] = 1.0; // D_0 = 1
     Gmp zr(prec), zi(prec), t1(prec), t2(prec), nzr(prec), nzi(prec);
     mpf_set(ref_re.v, cre);
     mpf_set(ref_im.v, cim);
@@ -75,7 +79,7 @@ void Frame::build_orbit(mpf_srcptr cre, mpf_srcptr cim) {
     if (dbl) {
         // series approximation restarts with every reference; the order persists
         // (createRef, mandel_perturbation.cpp:4113-4124)
-        for (int x = 0; x < MX_SA_N; ++x) sa_A[x] = MxC{0.0, 0.0};
+        for (int x = 0; x < MX_SA_N; ++x) sa_A[x] = sa_B[x] = MxC{0.0, 0.0};
         sa_A[0] = sa_delta;
         sa_flag = true;
         sa_it = 0;
@@ -92,6 +96,12 @@ void Frame::build_orbit(mpf_srcptr cre, mpf_srcptr cim) {
         mpf_add(zr, nzr, cre);
         mpf_add(zi, nzi, cim);
         store(i);
+        if (der) {
+            // D_i = 2 D_{i-1} X_{i-1} + 1 in double (calCoefficient, 4256-4260)
+            const double a = dfr[i - 1], b = dfi[i - 1], xr = zf[2 * (i - 1)], xi = zf[2 * (i - 1) + 1];
+            dfr[i] = 2.0 * (a * xr - b * xi) + 1.0;
+            dfi[i] = 2.0 * (a * xi + b * xr);
+        }
         if (std::hypot(zf[2 * i], zf[2 * i + 1]) > R) { // std::abs(Comp) > R
             ref_escaped = true;
             ref_it = i;
@@ -105,6 +115,10 @@ void Frame::build_orbit(mpf_srcptr cre, mpf_srcptr cim) {
     zfe_m.resize(2 * (size_t)(ref_it + 1));
     zfe_e.resize(2 * (size_t)(ref_it + 1));
     z_m3.resize((size_t)(ref_it + 1));
+    if (der) {
+        dfr.resize((size_t)(ref_it + 1));
+        dfi.resize((size_t)(ref_it + 1));
+    }
 }
 
 // get_exp (float_math.cpp:23-29): floor(log2|x|), zero -> -2^30
@@ -140,6 +154,16 @@ void Frame::sa_step(int i) {
         for (int k = 0; k < j / 2; ++k) N[j] = mxc_add(N[j], mxc_mul(mxc_mul(two, sa_A[k]), sa_A[j - k - 1]));
         if (j % 2) N[j] = mxc_add(N[j], mxc_mul(sa_A[j / 2], sa_A[j / 2]));
     }
+    MxC NB[MX_SA_N];
+    if (ede) {
+        // B'_j = 2 (B_j X + A_j D + sum_{k<j} A_k B_{j-k-1}), D = _df[i-1] (4279-4285)
+        const MxC X = {zf[2 * (i - 1)], zf[2 * (i - 1) + 1]}, D = {dfr[i - 1], dfi[i - 1]};
+        for (int j = 0; j < MX_SA_N; ++j) {
+            NB[j] = mxc_add(mxc_mul(sa_B[j], X), mxc_mul(sa_A[j], D));
+            for (int k = 0; k < j; ++k) NB[j] = mxc_add(NB[j], mxc_mul(sa_A[k], sa_B[j - k - 1]));
+            NB[j] = mxc_mul(NB[j], two);
+        }
+    }
     N[0] = mxc_add(N[0], sa_delta);
     const int order = sa_check(N);
     if (order < 0) {
@@ -149,6 +173,8 @@ void Frame::sa_step(int i) {
     sa_it = i;
     sa_order = order;
     for (int j = 0; j < MX_SA_N; ++j) sa_A[j] = N[j];
+    if (ede)
+        for (int j = 0; j < MX_SA_N; ++j) sa_B[j] = NB[j];
 }
 
 // Value of a (non-virtual) reference pixel from its own orbit
@@ -291,6 +317,10 @@ void Frame::build_bla(double dcmax_in) {
     }
     bla_r2.assign((size_t)tot, 0.0);
     bla_coef.assign(4 * (size_t)tot, 0.0);
+    // the double path under EDE also carries the derivative delta through a skip:
+    // level 0 C = 2 D_s, E = 0 (mandel_perturbation.cpp:2649-2651)
+    const bool der = dbl && ede;
+    bla_der.assign(der ? 4 * (size_t)tot : 0, 0.0);
     for (int s = 1; s < reflen; ++s) {
         const double zr = zf[2 * s], zi = zf[2 * s + 1];
         const double Zmag = std::sqrt(zr * zr + zi * zi);
@@ -304,6 +334,10 @@ void Frame::build_bla(double dcmax_in) {
         bla_coef[4 * k + 1] = 2.0 * zi;
         bla_coef[4 * k + 2] = 1.0;
         bla_coef[4 * k + 3] = 0.0;
+        if (der) {
+            bla_der[4 * k + 0] = 2.0 * dfr[s];
+            bla_der[4 * k + 1] = 2.0 * dfi[s];
+        }
     }
     for (size_t p = 1; p < lev_off.size(); ++p) {
         const size_t po = (size_t)lev_off[p - 1], no = (size_t)lev_off[p];
@@ -316,6 +350,16 @@ void Frame::build_bla(double dcmax_in) {
             zc[1] = yc[0] * xc[1] + yc[1] * xc[0];
             zc[2] = yc[0] * xc[2] - yc[1] * xc[3] + yc[2];
             zc[3] = yc[0] * xc[3] + yc[1] * xc[2] + yc[3];
+            if (der) {
+                // C_z = C_y A_x + A_y C_x, E_z = C_y B_x + A_y E_x + E_y (2679-2685)
+                const double* xd = &bla_der[4 * x];
+                const double* yd = &bla_der[4 * y];
+                double* zd = &bla_der[4 * z];
+                zd[0] = (yd[0] * xc[0] - yd[1] * xc[1]) + (yc[0] * xd[0] - yc[1] * xd[1]);
+                zd[1] = (yd[0] * xc[1] + yd[1] * xc[0]) + (yc[0] * xd[1] + yc[1] * xd[0]);
+                zd[2] = (yd[0] * xc[2] - yd[1] * xc[3]) + (yc[0] * xd[2] - yc[1] * xd[3]) + yd[2];
+                zd[3] = (yd[0] * xc[3] + yd[1] * xc[2]) + (yc[0] * xd[3] + yc[1] * xd[2]) + yd[3];
+            }
             const double Rx = std::sqrt(bla_r2[x]), Ry = std::sqrt(bla_r2[y]);
             const double Axmag = std::sqrt(xc[0] * xc[0] + xc[1] * xc[1]);
             const double Bxmag = std::sqrt(xc[2] * xc[2] + xc[3] * xc[3]);
@@ -372,10 +416,6 @@ int Frame::prepare(const mx_request& r, std::string& err) {
         return MX_E_UNSUPPORTED;
     }
     dbl = mpf_cmp_d(scale, 1e280) <= 0;
-    if (dbl && ede) {
-        err = "exterior distance estimate on the plain-double path (scale <= 1e280) is not built";
-        return MX_E_UNSUPPORTED;
-    }
     // depth guard for the int32 exponent of the device FloatExp: |log2 scale| < 2^26
     {
         long se;
@@ -638,6 +678,12 @@ void Frame::fill_view(mx_frame_view& v) const {
         v.sa_delta_im = sa_delta.i;
         v.dcol_d = dcol_d.data();
         v.drow_d = drow_d.data();
+        if (ede) {
+            v.dfr = dfr.data();
+            v.dfi = dfi.data();
+            v.bla_der = bla_der.data();
+            v.sa_coef_b = &sa_B[0].r;
+        }
     }
 }
 

@@ -69,6 +69,9 @@ struct Frame {
     int ref_i = 0, ref_j = 0;
     std::vector<double> z_m3;
     MxC sa_A[MX_SA_N] = {};
+    MxC sa_B[MX_SA_N] = {};        // series of the derivative delta (EDE)
+    std::vector<double> dfr, dfi;  // reference derivative D (EDE)
+    std::vector<double> bla_der;   // BLA derivative couplings C, E (EDE)
     MxC sa_delta = {0.0, 0.0};
     int sa_it = 0, sa_order = 0;
     bool sa_flag = false;

@@ -26,7 +26,8 @@ class RefDesc(C.Structure):
                 ("nlev", C.c_int32), ("lev_off", C.c_void_p), ("lev_size", C.c_void_p),
                 ("bla_r2", C.c_void_p), ("bla_coef", C.c_void_p), ("rmax2", C.c_double),
                 ("z_m3", C.c_void_p), ("sa_it", C.c_int32), ("sa_order", C.c_int32), ("sa_coef", C.c_void_p),
-                ("sa_delta_re", C.c_double), ("sa_delta_im", C.c_double)]
+                ("sa_delta_re", C.c_double), ("sa_delta_im", C.c_double), ("df", C.c_void_p), ("bla_der", C.c_void_p),
+                ("sa_coef_b", C.c_void_p)]
 
 
 class Tile(C.Structure):
@@ -73,7 +74,9 @@ class FrameView(C.Structure):
                 ("precision_bits", C.c_int32), ("ref_seconds", C.c_double), ("bla_seconds", C.c_double),
                 ("path", C.c_int32), ("z_m3", C.POINTER(C.c_double)), ("sa_it", C.c_int32), ("sa_order", C.c_int32),
                 ("sa_coef", C.POINTER(C.c_double)), ("sa_delta_re", C.c_double), ("sa_delta_im", C.c_double),
-                ("dcol_d", C.POINTER(C.c_double)), ("drow_d", C.POINTER(C.c_double))]
+                ("dcol_d", C.POINTER(C.c_double)), ("drow_d", C.POINTER(C.c_double)),
+                ("dfr", C.POINTER(C.c_double)), ("dfi", C.POINTER(C.c_double)), ("bla_der", C.POINTER(C.c_double)),
+                ("sa_coef_b", C.POINTER(C.c_double))]
 
 
 # every symbol include/mx.h declares
@@ -162,6 +165,9 @@ def frame_prepare(cre, cim, scale, W, H, mxit, **kw):
             d.update(z_m3=_arr(v.z_m3, n, np.float64), sa_it=v.sa_it, sa_order=v.sa_order,
                      sa_coef=_arr(v.sa_coef, 60, np.float64), sa_delta=complex(v.sa_delta_re, v.sa_delta_im),
                      dcol_d=_arr(v.dcol_d, W, np.float64), drow_d=_arr(v.drow_d, H, np.float64))
+            if v.dfr:
+                d.update(dfr=_arr(v.dfr, n, np.float64), dfi=_arr(v.dfi, n, np.float64),
+                         bla_der=_arr(v.bla_der, 4 * tot, np.float64), sa_coef_b=_arr(v.sa_coef_b, 60, np.float64))
     finally:
         lib().mx_frame_free(fr)
     return d
@@ -232,6 +238,12 @@ class Context:
             d.sa_it, d.sa_order = fv["sa_it"], fv["sa_order"]
             d.sa_coef = a(fv["sa_coef"], np.float64)
             d.sa_delta_re, d.sa_delta_im = fv["sa_delta"].real, fv["sa_delta"].imag
+            if "dfr" in fv:
+                df = np.empty(2 * fv["len"])
+                df[0::2], df[1::2] = fv["dfr"], fv["dfi"]
+                d.df = a(df, np.float64)
+                d.bla_der = a(fv["bla_der"], np.float64) if fv["nlev"] else None
+                d.sa_coef_b = a(fv["sa_coef_b"], np.float64)
         ref = C.c_void_p()
         self.check(lib().mx_ref_upload(self.h, C.byref(d), C.byref(ref)), "mx_ref_upload")
         return Ref(ref)

@@ -1357,20 +1357,30 @@ static int sa_check(const mxo_c* A) {
 
 typedef struct {
     mxo_c A[MXO_SA_N]; /* _Adf_old */
+    mxo_c B[MXO_SA_N]; /* _Bdf_old: series of the derivative delta (EDE) */
     mxo_c delta;       /* _SA_delta */
-    int order, it, flag;
+    int order, it, flag, ede;
 } mxo_sa;
 
-/* One series step (calCoefficient's SA block, MP.cpp:4273-4302): z = _zf[i-1]. */
-static void sa_step(mxo_sa* sa, mxo_c z, int i) {
+/* One series step (calCoefficient's SA block, MP.cpp:4273-4302): z = _zf[i-1],
+ * d = _df[i-1] (EDE only). */
+static void sa_step(mxo_sa* sa, mxo_c z, mxo_c d, int i) {
     if (!sa->flag) return;
-    mxo_c N[MXO_SA_N];
+    mxo_c N[MXO_SA_N], NB[MXO_SA_N];
     mxo_c two = {2.0, 0.0};
     mxo_c z2 = c_mul(two, z);
     for (int j = 0; j < MXO_SA_N; ++j) {
         N[j] = c_mul(z2, sa->A[j]);
         for (int k = 0; k < j / 2; ++k) N[j] = c_add(N[j], c_mul(c_mul(two, sa->A[k]), sa->A[j - k - 1]));
         if (j % 2) N[j] = c_add(N[j], c_mul(sa->A[j / 2], sa->A[j / 2]));
+    }
+    if (sa->ede) {
+        /* B'_j = 2 (B_j Z + A_j D + sum_{k<j} A_k B_{j-k-1}) */
+        for (int j = 0; j < MXO_SA_N; ++j) {
+            NB[j] = c_add(c_mul(sa->B[j], z), c_mul(sa->A[j], d));
+            for (int k = 0; k < j; ++k) NB[j] = c_add(NB[j], c_mul(sa->A[k], sa->B[j - k - 1]));
+            NB[j] = c_mul(NB[j], two);
+        }
     }
     N[0] = c_add(N[0], sa->delta);
     int order = sa_check(N);
@@ -1380,22 +1390,25 @@ static void sa_step(mxo_sa* sa, mxo_c z, int i) {
         sa->it = i;
         sa->order = order;
         memcpy(sa->A, N, sizeof N);
+        if (sa->ede) memcpy(sa->B, NB, sizeof NB);
     }
 }
 
-/* Horner seed of the double loop (MP.cpp:2319-2323) */
-static mxo_c sa_seed(const mxo_sa* sa, mxo_c dc) {
+/* Horner seeds of the double loop (MP.cpp:2319-2330): dz from A, dd from B */
+static mxo_c sa_horner(const mxo_c* C, int order, mxo_c delta, mxo_c dc) {
     mxo_c dz = {0.0, 0.0};
-    for (int x = sa->order; x >= 0; --x) {
-        dz = c_add(dz, sa->A[x]);
+    for (int x = order; x >= 0; --x) {
+        dz = c_add(dz, C[x]);
         dz = c_mul(dz, dc);
-        dz = c_div(dz, sa->delta);
+        dz = c_div(dz, delta);
     }
     return dz;
 }
+static mxo_c sa_seed(const mxo_sa* sa, mxo_c dc) { return sa_horner(sa->A, sa->order, sa->delta, dc); }
 
-/* tryBLA (MP.cpp:2714-2755), no derivative */
-static int try_bla_d(const mxo_bla* b, int minlevel, const double* zf, int s, double* dzr, double* dzi, double dcr, double dci, double esc2, int mx_ref_it) {
+/* tryBLA (MP.cpp:2714-2755); under EDE the derivative delta is carried,
+ * dd -> C dz + A dd + E dc, with the dz from before the skip (2742-2749). */
+static int try_bla_d(const mxo_bla* b, int minlevel, const double* zf, int s, double* dzr, double* dzi, double* ddr, double* ddi, double dcr, double dci, int ede, double esc2, int mx_ref_it) {
     if (s < 1 || !b || b->nlev == 0) return 0;
     double zmag2 = *dzr * *dzr + *dzi * *dzi;
     int maxp = b->nlev - 1;
@@ -1415,6 +1428,13 @@ static int try_bla_d(const mxo_bla* b, int minlevel, const double* zf, int s, do
         double nzi = c[0] * *dzi + c[1] * *dzr + c[2] * dci + c[3] * dcr;
         double fr = zf[2 * land] + nzr, fi = zf[2 * land + 1] + nzi;
         if (fr * fr + fi * fi > esc2) return 0;
+        if (ede) {
+            const double* d = b->der + 4 * k;
+            double nddr = (d[0] * *dzr - d[1] * *dzi) + (c[0] * *ddr - c[1] * *ddi) + (d[2] * dcr - d[3] * dci);
+            double nddi = (d[0] * *dzi + d[1] * *dzr) + (c[0] * *ddi + c[1] * *ddr) + (d[2] * dci + d[3] * dcr);
+            *ddr = nddr;
+            *ddi = nddi;
+        }
         *dzr = nzr;
         *dzi = nzi;
         return (int)b->l[k];
@@ -1425,6 +1445,8 @@ static int try_bla_d(const mxo_bla* b, int minlevel, const double* zf, int s, do
 typedef struct {
     const double* zf;  /* interleaved double orbit, reflen entries */
     const double* z_m3; /* |Z|^2/1e6 per entry */
+    const double* dfr; /* reference derivative D (EDE) */
+    const double* dfi;
     int mx_ref_it;
     const mxo_bla* bla;
     int bla_minlevel;
@@ -1434,16 +1456,21 @@ typedef struct {
     int interior_confirm;
     int int_rep;
     int sa_it;
+    int ede;
+    double dxf;        /* mpf_get_ld(_dx), the DE normalisation */
 } mxo_dctx;
 
-/* The scalar double loop of stepParallel (MP.cpp:2331-2584), smooth output.
+/* The scalar double loop of stepParallel (MP.cpp:2331-2584): smooth output, or the
+ * pixel-normalised exterior distance estimate (2444-2456, 2487-2488) under EDE.
  * Returns the value; *n_out = escape iteration, -1 reached mxit, -2 interior
  * detected, -3 unresolved (glitch or end of reference; *glitch_p set). */
-static double pixel_double(const mxo_dctx* c, mxo_c dc, mxo_c dz0, int mxit, int* n_out, double* glitch_p, mxo_stats* st) {
+static double pixel_double(const mxo_dctx* c, mxo_c dc, mxo_c dz0, mxo_c dd0, int mxit, int* n_out, double* glitch_p, mxo_stats* st) {
     const double* Z = c->zf;
     const int mx_ref_it = c->mx_ref_it;
+    const int ede = c->ede;
     int j = c->sa_it + 1;
     double dzr = dz0.r, dzi = dz0.i, dcr = dc.r, dci = dc.i;
+    double ddr = dd0.r, ddi = dd0.i, dr = 0.0, di = 0.0;
     int k = j;
     double zsr = 1e30, zsi = 1e30;
     int save_j = j, period_win = 1;
@@ -1470,7 +1497,7 @@ static double pixel_double(const mxo_dctx* c, mxo_c dc, mxo_c dz0, int mxit, int
             }
         }
         if (c->bla && dzr * dzr + dzi * dzi < c->bla->rmax2) {
-            int skip = try_bla_d(c->bla, c->bla_minlevel, Z, k - 1, &dzr, &dzi, dcr, dci, c->esc2, mx_ref_it);
+            int skip = try_bla_d(c->bla, c->bla_minlevel, Z, k - 1, &dzr, &dzi, &ddr, &ddi, dcr, dci, ede, c->esc2, mx_ref_it);
             if (skip > 0) {
                 k += skip;
                 j += skip;
@@ -1485,7 +1512,22 @@ static double pixel_double(const mxo_dctx* c, mxo_c dc, mxo_c dz0, int mxit, int
                 continue;
             }
         }
-        double dzr2 = dzr * dzr, dzi2 = dzi * dzi, tmp;
+        double tmp;
+        if (ede) {
+            /* dd' = 2 (dd X + dz (D + dd)), before dz advances */
+            if (k == 0) {
+                tmp = (dzr * (1 + ddr) - dzi * (1 + ddi)) * 2;
+                ddi = (dzr * (1 + ddi) + dzi * (1 + ddr)) * 2;
+            } else {
+                const double xr = Z[2 * (k - 1)], xi = Z[2 * (k - 1) + 1];
+                tmp = (ddr * xr - ddi * xi + dzr * (c->dfr[k - 1] + ddr) - dzi * (c->dfi[k - 1] + ddi)) * 2;
+                ddi = (ddr * xi + ddi * xr + dzr * (c->dfi[k - 1] + ddi) + dzi * (c->dfr[k - 1] + ddr)) * 2;
+            }
+            ddr = tmp;
+            dr = ddr + c->dfr[k];
+            di = ddi + c->dfi[k];
+        }
+        double dzr2 = dzr * dzr, dzi2 = dzi * dzi;
         if (k == 0) {
             tmp = dzr2 - dzi2 + dcr;
             dzi = dzr * dzi * 2 + dci;
@@ -1501,6 +1543,7 @@ static double pixel_double(const mxo_dctx* c, mxo_c dc, mxo_c dz0, int mxit, int
         st->steps++;
         if (zrad > c->esc2) {
             *n_out = j + 1;
+            if (ede) return sqrt(zrad) / c->dxf * log(zrad) / sqrt(dr * dr + di * di);
             return j + 1 - log(log(zrad) / 2 / LG) / LG;
         }
         if (detect_interior) {
@@ -1557,6 +1600,10 @@ static double pixel_double(const mxo_dctx* c, mxo_c dc, mxo_c dz0, int mxit, int
             }
             dzr = zr;
             dzi = zi;
+            if (ede) {
+                ddr = dr - 1;
+                ddi = di;
+            }
             k = 0;
             st->rebases++;
         }
@@ -1569,18 +1616,23 @@ static double pixel_double(const mxo_dctx* c, mxo_c dc, mxo_c dz0, int mxit, int
 /* Reference orbit of the double path with its series coefficients (createRef +
  * calCoefficient, MP.cpp:4014-4187, 4189-4305).  sa keeps `order` from the previous
  * reference, as the engine's _SA_order does. */
-static void orbit_build_sa(mxo_orbit* o, mxo_sa* sa, mpfr_srcptr cre, mpfr_srcptr cim, mpfr_prec_t prec, int mxit, double escape_radius, mxo_c delta) {
-    orbit_build(o, cre, cim, prec, mxit, escape_radius, 0, 0);
-    for (int i = 0; i < MXO_SA_N; ++i) sa->A[i].r = sa->A[i].i = 0.0;
+static void orbit_build_sa(mxo_orbit* o, mxo_sa* sa, mpfr_srcptr cre, mpfr_srcptr cim, mpfr_prec_t prec, int mxit, double escape_radius, mxo_c delta, int ede) {
+    orbit_build(o, cre, cim, prec, mxit, escape_radius, ede, 0);
+    for (int i = 0; i < MXO_SA_N; ++i) {
+        sa->A[i].r = sa->A[i].i = 0.0;
+        sa->B[i].r = sa->B[i].i = 0.0;
+    }
     sa->A[0] = delta;
     sa->delta = delta;
     sa->flag = 1;
     sa->it = 0;
+    sa->ede = ede;
     /* calCoefficient(i) runs the SA step after a non-escaping Z[i] */
     const int last = o->escaped ? o->ref_it - 1 : o->ref_it;
     for (int i = 1; i <= last && sa->flag; ++i) {
         mxo_c z = {o->zf[2 * (i - 1)], o->zf[2 * (i - 1) + 1]};
-        sa_step(sa, z, i);
+        mxo_c d = {ede ? o->dfr[i - 1] : 0.0, ede ? o->dfi[i - 1] : 0.0};
+        sa_step(sa, z, d, i);
     }
 }
 
@@ -1599,6 +1651,7 @@ typedef struct {
     int escalate;        /* 1: the frame belongs to the floatexp path (MP.cpp:1336-1355) */
     int sa_it0, sa_order0;
     long long steps, bla_applies, bla_skipped, rebases;
+    int ede;             /* exterior distance estimate output */
 } mxo_dframe;
 
 MXO_EXPORT int mxo_compute_dbl(mxo_dframe* f) {
@@ -1628,7 +1681,8 @@ MXO_EXPORT int mxo_compute_dbl(mxo_dframe* f) {
     mxo_orbit o;
     mxo_sa sa;
     memset(&sa, 0, sizeof sa);
-    orbit_build_sa(&o, &sa, rre, rim, pb, mxit, escR, delta);
+    const int ede = f->ede;
+    orbit_build_sa(&o, &sa, rre, rim, pb, mxit, escR, delta, ede);
     f->sa_it0 = sa.it;
     f->sa_order0 = sa.order;
     f->escalate = 0;
@@ -1669,6 +1723,8 @@ MXO_EXPORT int mxo_compute_dbl(mxo_dframe* f) {
     ctx.interior_eps2 = 3e-9 * 3e-9;
     ctx.interior_confirm = 4;
     ctx.int_rep = 1;
+    ctx.ede = ede;
+    ctx.dxf = mpfr_to_ld_trunc(g.dx);
     long long steps = 0, ap = 0, sk = 0, rb = 0;
 
     /* set-up for the current reference: BLA, dc tables */
@@ -1679,10 +1735,12 @@ MXO_EXPORT int mxo_compute_dbl(mxo_dframe* f) {
             double mxx = (ref_j > (W - 1) - ref_j ? ref_j : (W - 1) - ref_j) + 0.5;                           \
             double mxy = (ref_i > (H - 1) - ref_i ? ref_i : (H - 1) - ref_i) + 0.5;                           \
             double dcmax = sqrt(dxf * dxf * mxx * mxx + dyf * dyf * mxy * mxy);                               \
-            bla_build(&bla, o.zf, o.dfr, o.dfi, o.ref_it, dcmax, ldexp(1.0, -53), 0);                     \
+            bla_build(&bla, o.zf, o.dfr, o.dfi, o.ref_it, dcmax, ldexp(1.0, -53), ede);                   \
         }                                                                                                     \
         ctx.zf = o.zf;                                                                                        \
         ctx.z_m3 = o.z_m3;                                                                                    \
+        ctx.dfr = o.dfr;                                                                                      \
+        ctx.dfi = o.dfi;                                                                                      \
         ctx.mx_ref_it = o.ref_it;                                                                             \
         ctx.bla = f->use_bla ? &bla : NULL;                                                                   \
         ctx.sa_it = sa.it;                                                                                    \
@@ -1700,7 +1758,7 @@ MXO_EXPORT int mxo_compute_dbl(mxo_dframe* f) {
 #define MXO_REF_PIXEL()                                                                                       \
     do {                                                                                                      \
         int ne_;                                                                                              \
-        f->val[ref_i * W + ref_j] = ref_pixel_value(&o, mxit, 0, esc2, g.dx, &ne_);                           \
+        f->val[ref_i * W + ref_j] = ref_pixel_value(&o, mxit, ede ? 2 : 0, esc2, g.dx, &ne_);                \
         f->n_esc[ref_i * W + ref_j] = ne_;                                                                    \
     } while (0)
 
@@ -1717,10 +1775,12 @@ MXO_EXPORT int mxo_compute_dbl(mxo_dframe* f) {
             int i = p / W, jx = p % W;                                                                        \
             mxo_c dc = {dcol[jx], drow[i]};                                                                   \
             mxo_c dz = sa_seed(&sa, dc);                                                                      \
+            mxo_c dd0 = {0.0, 0.0};                                                                           \
+            if (ede) dd0 = sa_horner(sa.B, sa.order, sa.delta, dc);                                           \
             mxo_stats st = {0, 0, 0, 0, 0, 0};                                                                \
             int ne;                                                                                           \
             double gpp;                                                                                       \
-            double v = pixel_double(&ctx, dc, dz, mxit, &ne, &gpp, &st);                                      \
+            double v = pixel_double(&ctx, dc, dz, dd0, mxit, &ne, &gpp, &st);                                 \
             gp[p] = gpp;                                                                                      \
             if (ne != -3) {                                                                                   \
                 f->val[p] = v;                                                                                \
@@ -1777,7 +1837,7 @@ MXO_EXPORT int mxo_compute_dbl(mxo_dframe* f) {
         ref_j = best % W;
         orbit_free(&o);
         grid_pixel_c(&g, ref_i, ref_j, 0, 0, 1, rre, rim);
-        orbit_build_sa(&o, &sa, rre, rim, pb, mxit, escR, delta);
+        orbit_build_sa(&o, &sa, rre, rim, pb, mxit, escR, delta, ede);
         MXO_REF_PIXEL();
         n = 0;
         for (int q = 0; q < nr; ++q)
@@ -1817,7 +1877,7 @@ MXO_EXPORT int mxo_sa_coeffs(const char* cre, const char* cim, int prec_bits, in
     mxo_sa sa;
     memset(&sa, 0, sizeof sa);
     mxo_c delta = {delta_r, delta_i};
-    orbit_build_sa(&o, &sa, re, im, prec_bits, mxit, escape_radius, delta);
+    orbit_build_sa(&o, &sa, re, im, prec_bits, mxit, escape_radius, delta, 0);
     for (int x = 0; x < MXO_SA_N; ++x) {
         A[2 * x] = sa.A[x].r;
         A[2 * x + 1] = sa.A[x].i;
@@ -1860,7 +1920,7 @@ MXO_EXPORT void mxo_cdiv(int n, const double* a, const double* b, double* out) {
 /* The double loop over caller-supplied orbit/BLA/series arrays (the same arrays the
  * GPU consumed), so a parity test isolates the kernel.  gp[q] >= 0 marks an
  * unresolved sample (nesc[q] = INT32_MIN). */
-MXO_EXPORT void mxo_pixels_dbl(const mxo_pixels_in* in, const double* z_m3, const double* A, int sa_it, int sa_order, double delta_r, double delta_i, int n, const double* dcr, const double* dci, double* val, int32_t* nesc, double* gp, long long* stats4) {
+MXO_EXPORT void mxo_pixels_dbl(const mxo_pixels_in* in, const double* z_m3, const double* dfr, const double* dfi, const double* der, const double* A, const double* B, int sa_it, int sa_order, double delta_r, double delta_i, double dxf, int n, const double* dcr, const double* dci, double* val, int32_t* nesc, double* gp, long long* stats4) {
     mxo_bla b;
     memset(&b, 0, sizeof b);
     if (in->nlev > 0) {
@@ -1872,6 +1932,7 @@ MXO_EXPORT void mxo_pixels_dbl(const mxo_pixels_in* in, const double* z_m3, cons
         b.r2 = (double*)in->r2;
         b.l = (int64_t*)in->l;
         b.coef = (double*)in->coef;
+        b.der = (double*)der;
         b.rmax2 = in->rmax2;
     }
     mxo_sa sa;
@@ -1879,6 +1940,10 @@ MXO_EXPORT void mxo_pixels_dbl(const mxo_pixels_in* in, const double* z_m3, cons
     for (int x = 0; x < MXO_SA_N; ++x) {
         sa.A[x].r = A[2 * x];
         sa.A[x].i = A[2 * x + 1];
+        if (B) {
+            sa.B[x].r = B[2 * x];
+            sa.B[x].i = B[2 * x + 1];
+        }
     }
     sa.order = sa_order;
     sa.it = sa_it;
@@ -1888,6 +1953,8 @@ MXO_EXPORT void mxo_pixels_dbl(const mxo_pixels_in* in, const double* z_m3, cons
     memset(&ctx, 0, sizeof ctx);
     ctx.zf = in->zf;
     ctx.z_m3 = z_m3;
+    ctx.dfr = dfr;
+    ctx.dfi = dfi;
     ctx.mx_ref_it = in->mx_ref_it;
     ctx.bla = in->nlev > 0 ? &b : NULL;
     ctx.bla_minlevel = in->bla_minlevel;
@@ -1897,15 +1964,19 @@ MXO_EXPORT void mxo_pixels_dbl(const mxo_pixels_in* in, const double* z_m3, cons
     ctx.interior_confirm = 4;
     ctx.int_rep = 1;
     ctx.sa_it = sa_it;
+    ctx.ede = in->ede;
+    ctx.dxf = dxf;
     long long s0 = 0, s1 = 0, s2 = 0, s3 = 0;
 #pragma omp parallel for schedule(dynamic, 16) num_threads(in->threads) reduction(+ : s0, s1, s2, s3)
     for (int q = 0; q < n; ++q) {
         mxo_c dc = {dcr[q], dci[q]};
         mxo_c dz = sa_seed(&sa, dc);
+        mxo_c dd0 = {0.0, 0.0};
+        if (in->ede) dd0 = sa_horner(sa.B, sa.order, sa.delta, dc);
         mxo_stats st = {0, 0, 0, 0, 0, 0};
         int ne;
         double g;
-        double v = pixel_double(&ctx, dc, dz, in->mxit, &ne, &g, &st);
+        double v = pixel_double(&ctx, dc, dz, dd0, in->mxit, &ne, &g, &st);
         gp[q] = g;
         if (ne == -3) {
             nesc[q] = INT32_MIN;
@@ -1928,7 +1999,7 @@ MXO_EXPORT void mxo_pixels_dbl(const mxo_pixels_in* in, const double* z_m3, cons
 /* First-reference set-up of the double frame (the part of mxo_compute_dbl before the
  * passes): orbit shadows and |X|^2/1e6 (capacity mxit+1), series coefficients,
  * delta and the double dc tables.  Returns ref_it. */
-MXO_EXPORT int mxo_dbl_setup(const char* cre, const char* cim, const char* scale, int W, int H, int mxit, float escape_radius, int prec_bits, double* zf, double* z_m3, double* A, double* delta, int* sa_it, int* sa_order, double* dcol, double* drow) {
+MXO_EXPORT int mxo_dbl_setup(const char* cre, const char* cim, const char* scale, int W, int H, int mxit, float escape_radius, int prec_bits, double* zf, double* z_m3, double* A, double* delta, int* sa_it, int* sa_order, double* dcol, double* drow, int ede, double* dfr, double* dfi, double* B) {
     prec_bits = mpf_bits(prec_bits);
     mxo_grid g;
     grid_init(&g, cre, cim, scale, W, H, 0, 0, prec_bits);
@@ -1946,12 +2017,18 @@ MXO_EXPORT int mxo_dbl_setup(const char* cre, const char* cim, const char* scale
     mxo_orbit o;
     mxo_sa sa;
     memset(&sa, 0, sizeof sa);
-    orbit_build_sa(&o, &sa, rre, rim, prec_bits, mxit, (double)escape_radius, d);
+    orbit_build_sa(&o, &sa, rre, rim, prec_bits, mxit, (double)escape_radius, d, ede);
+    if (ede) {
+        memcpy(dfr, o.dfr, sizeof(double) * (o.ref_it + 1));
+        memcpy(dfi, o.dfi, sizeof(double) * (o.ref_it + 1));
+    }
     memcpy(zf, o.zf, sizeof(double) * 2 * (o.ref_it + 1));
     memcpy(z_m3, o.z_m3, sizeof(double) * (o.ref_it + 1));
     for (int x = 0; x < MXO_SA_N; ++x) {
         A[2 * x] = sa.A[x].r;
         A[2 * x + 1] = sa.A[x].i;
+        B[2 * x] = sa.B[x].r;
+        B[2 * x + 1] = sa.B[x].i;
     }
     delta[0] = d.r;
     delta[1] = d.i;

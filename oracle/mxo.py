@@ -48,7 +48,7 @@ class DFrame(C.Structure):
         ("val", C.POINTER(C.c_double)), ("n_esc", C.POINTER(C.c_int32)), ("out", C.POINTER(C.c_float)),
         ("passes", C.c_int), ("escalate", C.c_int), ("sa_it0", C.c_int), ("sa_order0", C.c_int),
         ("steps", C.c_longlong), ("bla_applies", C.c_longlong), ("bla_skipped", C.c_longlong),
-        ("rebases", C.c_longlong),
+        ("rebases", C.c_longlong), ("ede", C.c_int),
     ]
 
 
@@ -65,10 +65,10 @@ def lib():
         L.mxo_sa_coeffs.restype = C.c_int
         L.mxo_sa_seed.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_double, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
         L.mxo_cdiv.argtypes = [C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
-        L.mxo_pixels_dbl.argtypes = [C.POINTER(PixelsIn), C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_double, C.c_double,
-                                     C.c_int] + [C.c_void_p] * 6
+        L.mxo_pixels_dbl.argtypes = [C.POINTER(PixelsIn)] + [C.c_void_p] * 6 + [C.c_int, C.c_int, C.c_double, C.c_double,
+                                                                                C.c_double, C.c_int] + [C.c_void_p] * 6
         L.mxo_dbl_setup.argtypes = [C.c_char_p, C.c_char_p, C.c_char_p, C.c_int, C.c_int, C.c_int, C.c_float, C.c_int] + \
-            [C.c_void_p] * 4 + [C.POINTER(C.c_int), C.POINTER(C.c_int), C.c_void_p, C.c_void_p]
+            [C.c_void_p] * 4 + [C.POINTER(C.c_int), C.POINTER(C.c_int), C.c_void_p, C.c_void_p, C.c_int] + [C.c_void_p] * 3
         L.mxo_dbl_setup.restype = C.c_int
         L.mxo_fe_op.argtypes = [C.c_int, C.c_int] + [C.c_void_p] * 6
         L.mxo_orbit_shadows.argtypes = [C.c_char_p, C.c_char_p, C.c_int, C.c_int, C.c_double, C.c_int] + [C.c_void_p] * 7
@@ -120,7 +120,7 @@ def compute_fe(cre, cim, scale, W, H, mxit, c_method=0, escape_radius=1e8, prec_
 
 
 def compute_dbl(cre, cim, scale, W, H, mxit, escape_radius=1e8, prec_bits=None, use_bla=True, bla_minskip=8,
-                interior=True, threads=None):
+                interior=True, threads=None, c_method=0):
     """Whole plain-double frame (Compute's double path: centre-pixel reference with its
     series approximation, BLA, probe gate, passes re-referenced on glitches)."""
     f = DFrame()
@@ -129,6 +129,7 @@ def compute_dbl(cre, cim, scale, W, H, mxit, escape_radius=1e8, prec_bits=None, 
     f.prec_bits = prec_bits or precision_bits(scale)
     f.use_bla, f.bla_minskip, f.interior = int(use_bla), bla_minskip, int(interior)
     f.threads = threads or os.cpu_count()
+    f.ede = int((c_method & 2) != 0)
     val = np.zeros(W * H, np.float64)
     nesc = np.zeros(W * H, np.int32)
     out = np.zeros(W * H, np.float32)
@@ -251,7 +252,7 @@ def pixels_fe(orb, bla, dcr_m, dcr_e, dci_m, dci_e, mxit, esc2, dx_m, dx_e, ede=
 
 
 def pixels_dbl(zf, z_m3, bla, sa_coef, sa_it, sa_order, sa_delta, dcr, dci, mxit, esc2, use_interior=True,
-               bla_minlevel=3, threads=None):
+               bla_minlevel=3, threads=None, ede=False, dfr=None, dfi=None, sa_coef_b=None, dxf=0.0):
     """The double loop (series seed included) over caller-supplied arrays."""
     keep = []
 
@@ -270,7 +271,7 @@ def pixels_dbl(zf, z_m3, bla, sa_coef, sa_it, sa_order, sa_delta, dcr, dci, mxit
         pi.lev_off, pi.lev_size = arr(bla["off"], np.int64), arr(bla["size"], np.int64)
         pi.r2, pi.l, pi.coef = arr(bla["r2"], np.float64), arr(bla["l"], np.int64), arr(bla["coef"], np.float64)
         pi.rmax2 = bla["rmax2"]
-    pi.bla_minlevel, pi.esc2, pi.use_interior = bla_minlevel, esc2, int(use_interior)
+    pi.bla_minlevel, pi.esc2, pi.use_interior, pi.ede = bla_minlevel, esc2, int(use_interior), int(ede)
     pi.mxit = mxit
     pi.threads = threads or os.cpu_count()
     n = len(dcr)
@@ -278,12 +279,15 @@ def pixels_dbl(zf, z_m3, bla, sa_coef, sa_it, sa_order, sa_delta, dcr, dci, mxit
     nesc = np.zeros(n, np.int32)
     gp = np.zeros(n)
     st = np.zeros(4, np.int64)
-    lib().mxo_pixels_dbl(C.byref(pi), arr(z_m3, np.float64), arr(sa_coef, np.float64), sa_it, sa_order, sa_delta.real,
-                         sa_delta.imag, n, arr(dcr, np.float64), arr(dci, np.float64), _p(val), _p(nesc), _p(gp), _p(st))
+    opt = lambda a: arr(a, np.float64) if a is not None else None
+    der = bla.get("der") if (bla is not None and ede) else None
+    lib().mxo_pixels_dbl(C.byref(pi), arr(z_m3, np.float64), opt(dfr), opt(dfi), opt(der), arr(sa_coef, np.float64),
+                         opt(sa_coef_b), sa_it, sa_order, sa_delta.real, sa_delta.imag, dxf, n, arr(dcr, np.float64),
+                         arr(dci, np.float64), _p(val), _p(nesc), _p(gp), _p(st))
     return val, nesc, gp, dict(steps=int(st[0]), bla_applies=int(st[1]), bla_skipped=int(st[2]), rebases=int(st[3]))
 
 
-def dbl_setup(cre, cim, scale, W, H, mxit, escape_radius=1e8, prec_bits=None):
+def dbl_setup(cre, cim, scale, W, H, mxit, escape_radius=1e8, prec_bits=None, ede=False):
     """First-reference set-up of a double frame: orbit, |X|^2/1e6, series, dc tables."""
     pb = prec_bits or precision_bits(scale)
     zf = np.zeros(2 * (mxit + 1))
@@ -293,10 +297,17 @@ def dbl_setup(cre, cim, scale, W, H, mxit, escape_radius=1e8, prec_bits=None):
     it, order = C.c_int(0), C.c_int(0)
     dcol = np.zeros(W)
     drow = np.zeros(H)
+    dfr = np.zeros(mxit + 1)
+    dfi = np.zeros(mxit + 1)
+    B = np.zeros(60)
     ri = lib().mxo_dbl_setup(cre.encode(), cim.encode(), scale.encode(), W, H, mxit, escape_radius, pb, _p(zf), _p(m3),
-                             _p(A), _p(delta), C.byref(it), C.byref(order), _p(dcol), _p(drow))
-    return dict(ref_it=ri, zf=zf[:2 * (ri + 1)], z_m3=m3[:ri + 1], sa_coef=A, sa_delta=complex(delta[0], delta[1]),
-                sa_it=it.value, sa_order=order.value, dcol_d=dcol, drow_d=drow)
+                             _p(A), _p(delta), C.byref(it), C.byref(order), _p(dcol), _p(drow), int(ede), _p(dfr),
+                             _p(dfi), _p(B))
+    d = dict(ref_it=ri, zf=zf[:2 * (ri + 1)], z_m3=m3[:ri + 1], sa_coef=A, sa_delta=complex(delta[0], delta[1]),
+             sa_it=it.value, sa_order=order.value, dcol_d=dcol, drow_d=drow)
+    if ede:
+        d.update(dfr=dfr[:ri + 1], dfi=dfi[:ri + 1], sa_coef_b=B)
+    return d
 
 
 def direct(cre, cim, scale, W, H, mxit, c_method=0, escape_radius=1e8, step=1, prec_bits=None, wp_bits=None,

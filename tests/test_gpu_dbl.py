@@ -29,12 +29,14 @@ def _check(val_g, nesc_g, val_o, nesc_o, label):
     assert (val_g[(nesc_o == -1) | (nesc_o == -2)] == -2.0).all()
 
 
-@pytest.mark.parametrize("name", ["c1", "point31", "slowpoint", "ticktock", "flake"])
-def test_double_frame_matches_oracle(ctx, name):
+@pytest.mark.parametrize("name,c_method", [("c1", 0), ("point31", 0), ("slowpoint", 0), ("ticktock", 0), ("flake", 0),
+                                           ("c1", 2), ("point31", 2), ("ticktock", 2)])
+def test_double_frame_matches_oracle(ctx, name, c_method):
     cre, cim, scale, W, H, mxit, R, pb = FRAMES[name]
-    print(f"[dbl parity] {name} {W}x{H} mxit={mxit}", flush=True)
-    it, val, nesc, info = ctx.compute(cre, cim, scale, W, H, mxit, escape_radius=R, prec_bits=pb)
-    val_o, nesc_o, out_o, info_o = mxo.compute_dbl(cre, cim, scale, W, H, mxit, escape_radius=R or 1e8, prec_bits=pb or None)
+    print(f"[dbl parity] {name} {W}x{H} mxit={mxit} c_method={c_method}", flush=True)
+    it, val, nesc, info = ctx.compute(cre, cim, scale, W, H, mxit, escape_radius=R, prec_bits=pb, c_method=c_method)
+    val_o, nesc_o, out_o, info_o = mxo.compute_dbl(cre, cim, scale, W, H, mxit, escape_radius=R or 1e8, prec_bits=pb or None,
+                                                   c_method=c_method)
     assert info["path"] == 1 and info["passes"] == info_o["passes"] and info["sa_it"] == info_o["sa_it0"]
     _check(val, nesc, val_o, nesc_o, name)
     assert np.array_equal(it < -1.5, out_o < -1.5) and not (it == -10).any()
@@ -52,15 +54,16 @@ def test_double_frame_matches_reference_golden(ctx, name, golden, tol):
     assert mis == 0 and mx <= tol[0] and mean <= tol[1] and p99 <= tol[2], (mis, mx, mean, p99)
 
 
-def test_double_kernel_on_product_arrays(ctx):
+@pytest.mark.parametrize("ede", [False, True])
+def test_double_kernel_on_product_arrays(ctx, ede):
     """Kernel alone: one launch over the frame against the product's own reference,
     series and dc tables, and the oracle's loop on the same arrays, unresolved
     samples (glitches) and their metric included."""
     cre, cim, scale, W, H, mxit, R, pb = FRAMES["point31"]
-    fv = mxhip.frame_prepare(cre, cim, scale, W, H, mxit)
+    fv = mxhip.frame_prepare(cre, cim, scale, W, H, mxit, c_method=2 if ede else 0)
     ref = ctx.upload(fv)
     try:
-        t = ctx.tile(fv, form=mxhip.MX_FORM_DOUBLE, interior=True)
+        t = ctx.tile(fv, form=mxhip.MX_FORM_DOUBLE, interior=True, ede=ede)
         plan = ctx.plan(ref, t)
         try:
             plan.launch()
@@ -71,13 +74,16 @@ def test_double_kernel_on_product_arrays(ctx):
             plan.destroy()
     finally:
         ref.release()
-    b = mxo.bla_tables(fv["zf"], fv["ref_it"], fv["dcmax"])
-    bla = dict(nlev=b["nlev"], off=b["off"], size=b["size"], r2=b["r2"], l=b["l"], coef=b["coef"], rmax2=b["rmax2"])
+    b = mxo.bla_tables(fv["zf"], fv["ref_it"], fv["dcmax"], ede=ede, dfr=fv.get("dfr"), dfi=fv.get("dfi"))
+    bla = dict(nlev=b["nlev"], off=b["off"], size=b["size"], r2=b["r2"], l=b["l"], coef=b["coef"], der=b["der"],
+               rmax2=b["rmax2"])
     jj, ii = np.meshgrid(np.arange(W), np.arange(H))
+    dxf = float(np.ldexp(fv["dx_m"], fv["dx_e"]))
     v2, n2, gp2, st2 = mxo.pixels_dbl(fv["zf"], fv["z_m3"], bla, fv["sa_coef"], fv["sa_it"], fv["sa_order"], fv["sa_delta"],
                                       fv["dcol_d"][jj.ravel()], fv["drow_d"][ii.ravel()], mxit, fv["esc2"],
-                                      bla_minlevel=fv["bla_minlevel"])
-    _check(val, nesc, v2, n2, "point31 tile")
+                                      bla_minlevel=fv["bla_minlevel"], ede=ede, dfr=fv.get("dfr"), dfi=fv.get("dfi"),
+                                      sa_coef_b=fv.get("sa_coef_b"), dxf=dxf)
+    _check(val, nesc, v2, n2, f"point31 tile ede={ede}")
     un = np.nonzero(gp2 >= 0)[0]
     assert un.size > 0
     order = np.argsort(uidx)

@@ -109,24 +109,24 @@ DBL_FRAMES = {
 }
 
 
+@pytest.mark.parametrize("c_method", [0, mxhip.MX_EXTERIOR_DIST_EST])
 @pytest.mark.parametrize("name", sorted(DBL_FRAMES))
-def test_double_prepare_matches_oracle(name):
+def test_double_prepare_matches_oracle(name, c_method):
     """Plain-double path host preparation: GMP orbit, |X|^2/1e6, series coefficients
     (calCoefficient), half extent delta and the truncated dc tables equal the
-    oracle's, bit for bit."""
+    oracle's, bit for bit; under EDE also the reference derivative, the derivative
+    series and the BLA derivative couplings."""
     cre, cim, scale, W, H, mxit, R, pb = DBL_FRAMES[name]
-    fv = mxhip.frame_prepare(cre, cim, scale, W, H, mxit, escape_radius=R, prec_bits=pb)
-    o = mxo.dbl_setup(cre, cim, scale, W, H, mxit, escape_radius=R or 1e8, prec_bits=pb or None)
+    ede = c_method == mxhip.MX_EXTERIOR_DIST_EST
+    fv = mxhip.frame_prepare(cre, cim, scale, W, H, mxit, escape_radius=R, prec_bits=pb, c_method=c_method)
+    o = mxo.dbl_setup(cre, cim, scale, W, H, mxit, escape_radius=R or 1e8, prec_bits=pb or None, ede=ede)
     assert fv["path"] == 1 and fv["ref_virtual"] == 0 and (fv["ref_x"], fv["ref_y"]) == (W // 2, H // 2)
     assert fv["ref_it"] == o["ref_it"] and (fv["sa_it"], fv["sa_order"]) == (o["sa_it"], o["sa_order"])
-    for k in ("zf", "z_m3", "sa_coef", "dcol_d", "drow_d"):
+    keys = ["zf", "z_m3", "sa_coef", "dcol_d", "drow_d"] + (["dfr", "dfi", "sa_coef_b"] if ede else [])
+    for k in keys:
         assert np.array_equal(fv[k].view(np.int64), o[k].view(np.int64)), k
     assert fv["sa_delta"] == o["sa_delta"]
-    b = mxo.bla_tables(fv["zf"], fv["ref_it"], fv["dcmax"])
+    b = mxo.bla_tables(fv["zf"], fv["ref_it"], fv["dcmax"], ede=ede, dfr=fv.get("dfr"), dfi=fv.get("dfi"))
     assert b["nlev"] == fv["nlev"] and np.array_equal(b["r2"], fv["bla_r2"]) and np.array_equal(b["coef"], fv["bla_coef"])
-
-
-def test_double_path_refuses_what_it_does_not_build():
-    with pytest.raises(mxhip.MxError) as e:
-        mxhip.frame_prepare("-0.75", "0.1", "1" + "0" * 20, 16, 8, 1000, c_method=mxhip.MX_EXTERIOR_DIST_EST)
-    assert e.value.rc == mxhip.MX_E_UNSUPPORTED
+    if ede:
+        assert np.array_equal(b["der"].view(np.int64), fv["bla_der"].view(np.int64))

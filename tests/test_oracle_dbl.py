@@ -77,6 +77,24 @@ def test_c1_frame_matches_brute_force():
     assert info["passes"] == 1 and info["sa_it0"] > 500
 
 
+def test_c1_distance_estimate_matches_brute_force():
+    """Exterior distance estimate on the double path (derivative delta seeded by its
+    own series, carried through BLA skips and rebases) against the MPFR brute force's
+    pixel-normalised |z| ln|z| / (dx |dz/dc|)."""
+    cre, cim, scale, W, H, mxit = C1
+    val, nesc, out, info = mxo.compute_dbl(cre, cim, scale, W, H, mxit, escape_radius=65536.0, prec_bits=128, c_method=2)
+    dv, dn = mxo.direct(cre, cim, scale, W, H, mxit, c_method=2, escape_radius=65536.0, step=8, prec_bits=128)
+    m = ~np.isnan(dv)
+    pi, di = nesc[m] < 0, dn[m] < 0
+    assert (pi == di).all()
+    both = ~pi & ~di
+    rel = np.abs(val[m][both] - dv[m][both]) / np.abs(dv[m][both])
+    assert np.median(rel) < 1e-7 and np.mean(rel < 1e-3) > 0.95
+    # same classification and escape counts as the smooth run
+    _, n0, _, _ = mxo.compute_dbl(cre, cim, scale, W, H, mxit, escape_radius=65536.0, prec_bits=128)
+    assert np.array_equal(n0, nesc)
+
+
 def test_bla_on_off_agree():
     """BLA changes the rounding path, not the classification (the reference's
     BLA-on/off comparison method, tests/perf_bench.cpp:569-595)."""
