@@ -125,6 +125,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--mpfr-samples", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-double", action="store_true", help="skip the plain-double path (640x360 @ scale 4e10) leg")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: --gpus {args.gpus} refused: this build runs the path on one GPU only "
@@ -159,6 +160,7 @@ def main():
                 res[f] = run_form(ctx, ref, fv, f, ede, interior, center, 1, 0)
         finally:
             ref.release()
+        double = None if args.no_double else double_leg(ctx, args.no_cpu)
     main_r = res[args.form]
     forms = {f: summary(f, r, mxit, ede) for f, r in res.items()}
     s = forms[args.form]
@@ -203,8 +205,45 @@ def main():
         "forms": forms,
         "accuracy": accuracy,
         "cpu_baseline": cpu,
+        "double_path": double,
     }
     print(json.dumps(line))
+
+
+# The plain-double path's configuration (scale 1e6..1e280): 640x360 at scale 4e10
+# around (-0.743643887037151, 0.131825904205330), mxit 8192, bailout 65536, 128 bits.
+C1 = ("-0.743643887037151", "0.131825904205330", "40000000000", 640, 360, 8192, 65536.0, 128)
+
+
+def double_leg(ctx, no_cpu, reps=5):
+    """Whole frames through mx_compute (host reference + series, probe, passes) on the
+    GPU, and the oracle's restatement of the same frame on the host cores."""
+    cre, cim, scale, W, H, mxit, R, pb = C1
+    ctx.compute(cre, cim, scale, W, H, mxit, escape_radius=R, prec_bits=pb)  # warm-up
+    tot, ker = [], []
+    for _ in range(reps):
+        it, val, nesc, info = ctx.compute(cre, cim, scale, W, H, mxit, escape_radius=R, prec_bits=pb)
+        tot.append(info["total_seconds"])
+        ker.append(info["kernel_ms"] + info["probe_ms"])
+    nominal = int(info["nominal_iterations"])
+    out = {"workload": "640x360 @ scale 4e10 (-0.743643887037151, 0.131825904205330), mxit 8192, bailout 65536, 128-bit",
+           "frame_ms": 1e3 * float(np.median(tot)), "kernel_ms": float(np.median(ker)),
+           "gpix_iter_s_frame": nominal / float(np.median(tot)) / 1e9,
+           "gpix_iter_s_kernel": nominal / (float(np.median(ker)) / 1e3) / 1e9,
+           "nominal_pixel_iterations": nominal, "passes": info["passes"], "sa_it": info["sa_it"],
+           "executed_steps": int(info["steps"]), "bla_applies": int(info["bla_applies"]), "rebases": int(info["rebases"])}
+    log(f"double path: frame {out['frame_ms']:.1f} ms, kernels {out['kernel_ms']:.2f} ms")
+    if not no_cpu:
+        import mxo
+        threads = min(16, os.cpu_count() or 1)
+        t0 = time.perf_counter()
+        _, n_o, _, _ = mxo.compute_dbl(cre, cim, scale, W, H, mxit, escape_radius=R, prec_bits=pb, threads=threads)
+        dt = time.perf_counter() - t0
+        out["cpu_oracle_frame_ms"] = dt * 1e3
+        out["cpu_oracle_threads"] = threads
+        out["n_esc_identical_to_oracle"] = bool(np.array_equal(n_o, nesc))
+        log(f"double path: cpu oracle frame {dt * 1e3:.1f} ms on {threads} threads")
+    return out
 
 
 def mpfr_samples(wl, fv, res, n, center):

@@ -200,6 +200,8 @@ typedef struct {
     uint64_t nominal_iterations; /* sum of n_esc, max_iterations for interior */
     int32_t passes;          /* reference passes (plain-double path re-references on glitches) */
     int32_t sa_it;           /* first reference's series-approximation skip */
+    int32_t escalated;       /* scale <= 1e280 but run on the floatexp path because the
+                                reference passes too close to zero (mandel_perturbation.cpp:1336-1355) */
 } mx_frame_info;
 
 int mx_compute(mx_ctx* ctx, const mx_request* req, mx_frame_info* info);
@@ -251,6 +253,7 @@ typedef struct {
     const double* dfi;
     const double* bla_der;   /* [4*bla_total] (EDE) */
     const double* sa_coef_b; /* [60] (EDE) */
+    int32_t escalated;       /* see mx_frame_info */
 } mx_frame_view;
 
 int mx_frame_prepare(const mx_request* req, mx_frame** out, mx_frame_view* view);

@@ -829,6 +829,7 @@ extern "C" int mx_compute(mx_ctx* ctx, const mx_request* req, mx_frame_info* inf
         info->rebases = st[3];
         info->nominal_iterations = nominal;
         info->passes = 1;
+        info->escalated = F.escalated;
     }
     return MX_OK;
 }

@@ -459,10 +459,16 @@ int Frame::prepare(const mx_request& r, std::string& err) {
         while ((1 << (bla_minlevel + 1)) <= ms) ++bla_minlevel;
     }
     interior = r.interior != 0;
+    escalated = false;
     if (dbl) {
         const int rc = prepare_dbl(scale, err);
-        prepare_seconds = now_s() - t_start;
-        return rc;
+        if (rc != MX_OK || !escalated) {
+            prepare_seconds = now_s() - t_start;
+            return rc;
+        }
+        // the double path's reference passes too close to zero: the frame runs on the
+        // floatexp path from the exact view centre (mandel_perturbation.cpp:1344-1354)
+        dbl = false;
     }
 
     // reference: the centre pixel, then the exact view centre when the pixel
@@ -478,8 +484,22 @@ int Frame::prepare(const mx_request& r, std::string& err) {
         ref_virtual = false;
         ref_x = pj;
         ref_y = pi;
-        build_orbit(pre, pim);
-        if (!ref_escaped || ref_it >= max_iter - 16) {
+        if (escalated) {
+            mpf_mul_ui(t, dx.v, W - 1);
+            mpf_div_ui(t, t, 2);
+            mpf_add(pre, c0_re.v, t);
+            mpf_mul_ui(t, dy.v, H - 1);
+            mpf_div_ui(t, t, 2);
+            mpf_add(pim, c0_im.v, t);
+            ref_virtual = true;
+            ref_x = (W - 1) * 0.5;
+            ref_y = (H - 1) * 0.5;
+            build_orbit(pre, pim);
+            cutoff_sensitive = ref_escaped && ref_it >= max_iter - 16;
+        } else {
+            build_orbit(pre, pim);
+        }
+        if (!escalated && (!ref_escaped || ref_it >= max_iter - 16)) {
             const bool sensitive = ref_escaped || tail_escapes(80);
             if (sensitive) {
                 mpf_mul_ui(t, dx.v, W - 1);
@@ -565,11 +585,7 @@ int Frame::prepare_dbl(mpf_srcptr scale, std::string& err) {
             if (m2 < refmin2) refmin2 = m2;
         }
         const double dcmax_c = 2.0 / mpf_get_d(scale);
-        if (underflowed || std::sqrt(refmin2) * dcmax_c < 1e-300) {
-            err = "the reference passes too close to zero for the plain-double path (the engine escalates such "
-                  "frames to floatexp with an exact-centre reference, which is not built)";
-            return MX_E_UNSUPPORTED;
-        }
+        if (underflowed || std::sqrt(refmin2) * dcmax_c < 1e-300) escalated = true;
     }
     return MX_OK;
 }
@@ -669,6 +685,7 @@ void Frame::fill_view(mx_frame_view& v) const {
     v.ref_seconds = ref_seconds;
     v.bla_seconds = bla_seconds;
     v.path = dbl ? 1 : 2;
+    v.escalated = escalated;
     if (dbl) {
         v.z_m3 = z_m3.data();
         v.sa_it = sa_it;

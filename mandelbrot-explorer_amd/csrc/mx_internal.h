@@ -66,6 +66,7 @@ struct Frame {
     // plain-double path (1e6 < scale <= 1e280): pixel references, the series
     // approximation, the glitch metric's orbit magnitudes and double dc tables
     bool dbl = false;
+    bool escalated = false; // 1e150 < scale <= 1e280 frame moved to the floatexp path
     int ref_i = 0, ref_j = 0;
     std::vector<double> z_m3;
     MxC sa_A[MX_SA_N] = {};

@@ -57,7 +57,7 @@ class FrameInfo(C.Structure):
                 ("probe_ms", C.c_double), ("kernel_ms", C.c_double), ("total_seconds", C.c_double),
                 ("steps", C.c_uint64), ("bla_applies", C.c_uint64), ("bla_skipped", C.c_uint64),
                 ("rebases", C.c_uint64), ("nominal_iterations", C.c_uint64), ("passes", C.c_int32),
-                ("sa_it", C.c_int32)]
+                ("sa_it", C.c_int32), ("escalated", C.c_int32)]
 
 
 class FrameView(C.Structure):
@@ -76,7 +76,7 @@ class FrameView(C.Structure):
                 ("sa_coef", C.POINTER(C.c_double)), ("sa_delta_re", C.c_double), ("sa_delta_im", C.c_double),
                 ("dcol_d", C.POINTER(C.c_double)), ("drow_d", C.POINTER(C.c_double)),
                 ("dfr", C.POINTER(C.c_double)), ("dfi", C.POINTER(C.c_double)), ("bla_der", C.POINTER(C.c_double)),
-                ("sa_coef_b", C.POINTER(C.c_double))]
+                ("sa_coef_b", C.POINTER(C.c_double)), ("escalated", C.c_int32)]
 
 
 # every symbol include/mx.h declares
@@ -160,7 +160,8 @@ def frame_prepare(cre, cim, scale, W, H, mxit, **kw):
                  ref_it=v.ref_it, ref_escaped=v.ref_escaped, ref_virtual=v.ref_virtual, ref_x=v.ref_x,
                  ref_y=v.ref_y, ref_pixel_n=v.ref_pixel_n, ref_pixel_value=v.ref_pixel_value,
                  cutoff_sensitive=v.cutoff_sensitive, precision_bits=v.precision_bits,
-                 ref_seconds=v.ref_seconds, bla_seconds=v.bla_seconds, W=W, H=H, mxit=mxit, path=v.path)
+                 ref_seconds=v.ref_seconds, bla_seconds=v.bla_seconds, W=W, H=H, mxit=mxit, path=v.path,
+                 escalated=v.escalated)
         if v.path == 1:
             d.update(z_m3=_arr(v.z_m3, n, np.float64), sa_it=v.sa_it, sa_order=v.sa_order,
                      sa_coef=_arr(v.sa_coef, 60, np.float64), sa_delta=complex(v.sa_delta_re, v.sa_delta_im),

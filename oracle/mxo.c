@@ -967,16 +967,35 @@ MXO_EXPORT int mxo_compute_fe(mxo_frame* f) {
     mxo_fe dxfe = mpfr_to_fe_trunc(g.dx), dyfe = mpfr_to_fe_trunc(g.dy);
     const int ede = (f->c_method & 2) != 0;
 
-    /* reference at the centre pixel (H/2, W/2) */
+    /* reference at the centre pixel (H/2, W/2); a frame at scale <= 1e280 reaches this
+     * path only by the double path's escalation (MP.cpp:1336-1355), whose reference
+     * is the exact view centre */
     mpfr_t rre, rim;
     mpfr_inits2(pb, rre, rim, (mpfr_ptr)0);
-    grid_pixel_c(&g, H / 2, W / 2, 0, 0, 1, rre, rim);
+    int escalated;
+    {
+        mpfr_t sc;
+        mpfr_init2(sc, pb);
+        mpfr_set_str(sc, f->scale, 10, MPFR_RNDN);
+        escalated = mpfr_cmp_d(sc, 1e280) <= 0;
+        mpfr_clear(sc);
+    }
+    if (escalated)
+        grid_view_center(&g, W, H, rre, rim);
+    else
+        grid_pixel_c(&g, H / 2, W / 2, 0, 0, 1, rre, rim);
     mxo_orbit o;
     orbit_build(&o, rre, rim, pb, f->mxit, escR, ede, 1);
     int ref_virtual = 0;
     double ref_x = W / 2, ref_y = H / 2;
     f->cutoff_sensitive_hint = 0;
-    if (!o.escaped || o.ref_it >= f->mxit - 16) {
+    if (escalated) {
+        ref_virtual = 1;
+        ref_x = (W - 1) * 0.5;
+        ref_y = (H - 1) * 0.5;
+        if (o.escaped && o.ref_it >= f->mxit - 16) f->cutoff_sensitive_hint = 1;
+        if (!o.escaped) f->cutoff_sensitive_hint = 2;
+    } else if (!o.escaped || o.ref_it >= f->mxit - 16) {
         int sensitive = o.escaped || orbit_tail_escapes(&o, 80, esc2);
         if (sensitive) {
             orbit_free(&o);
@@ -1705,6 +1724,16 @@ MXO_EXPORT int mxo_compute_dbl(mxo_dframe* f) {
             if (uf || sqrt(refmin2) * dcmax < 1e-300) f->escalate = 1;
         }
         mpfr_clear(sc);
+    }
+    if (f->escalate) {
+        /* the frame belongs to the floatexp path (mxo_compute_fe) */
+        f->passes = 0;
+        free(dcol);
+        free(drow);
+        orbit_free(&o);
+        mpfr_clears(rre, rim, t, (mpfr_ptr)0);
+        grid_clear(&g);
+        return 1;
     }
     int* idx = (int*)malloc(sizeof(int) * (size_t)W * H);
     int* rest = (int*)malloc(sizeof(int) * (size_t)W * H);

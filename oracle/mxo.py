@@ -171,6 +171,25 @@ def cdiv(a, b):
     return out
 
 
+def compute(cre, cim, scale, W, H, mxit, c_method=0, escape_radius=1e8, prec_bits=None, use_bla=True, bla_minskip=8,
+            interior=True, threads=None):
+    """Mandel::Compute's path selection (mandel_perturbation.cpp:1434, 1569-1576, 1336-1355):
+    the plain-double path at 1e6 < scale <= 1e280 unless its reference escalates the frame
+    to floatexp, the floatexp path above.  Returns (val, n_esc, out, info) with
+    info["path"] 1 (double) or 2 (floatexp) and info["escalated"]."""
+    from decimal import Decimal
+    kw = dict(escape_radius=escape_radius, prec_bits=prec_bits, use_bla=use_bla, bla_minskip=bla_minskip,
+              interior=interior, threads=threads)
+    if Decimal(scale) <= Decimal("1e280"):
+        val, nesc, out, info = compute_dbl(cre, cim, scale, W, H, mxit, c_method=c_method, **kw)
+        if not info["escalate"]:
+            info.update(path=1, escalated=0)
+            return val, nesc, out, info
+    val, nesc, out, info = compute_fe(cre, cim, scale, W, H, mxit, c_method=c_method, **kw)
+    info.update(path=2, escalated=int(Decimal(scale) <= Decimal("1e280")))
+    return val, nesc, out, info
+
+
 def fe_op(op, am, ae, bm=None, be=None):
     am = np.ascontiguousarray(am, np.float64)
     ae = np.ascontiguousarray(ae, np.int64)

@@ -107,3 +107,17 @@ def test_double_sample_list_equals_grid(ctx):
     assert np.array_equal(n_l[sel], n_all[sel]) and np.array_equal(v_l[sel], v_all[sel])
     rest = np.setdiff1d(np.arange(W * H), sel)
     assert (n_l[rest] == np.iinfo(np.int32).min).all()
+
+
+def test_escalated_frame_matches_oracle(ctx):
+    """A 1e200 frame whose double-path reference passes too close to zero: the floatexp
+    path from the exact view centre, bit-exact against the oracle's."""
+    from test_oracle_dbl import ESCALATE
+    cre, cim, scale, W, H, mxit, pb = ESCALATE
+    it, val, nesc, info = ctx.compute(cre, cim, scale, W, H, mxit, prec_bits=pb)
+    val_o, nesc_o, out_o, info_o = mxo.compute(cre, cim, scale, W, H, mxit, prec_bits=pb)
+    assert info["path"] == 2 and info["escalated"] == 1 and info_o["escalated"] == 1
+    assert info["ref_it"] == info_o["ref_it"] and info["ref_virtual"] == 1
+    _check(val, nesc, val_o, nesc_o, "escalated")
+    for k in ("steps", "bla_applies", "bla_skipped", "rebases"):
+        assert info[k] == info_o[k], k

@@ -130,3 +130,15 @@ def test_double_prepare_matches_oracle(name, c_method):
     assert b["nlev"] == fv["nlev"] and np.array_equal(b["r2"], fv["bla_r2"]) and np.array_equal(b["coef"], fv["bla_coef"])
     if ede:
         assert np.array_equal(b["der"].view(np.int64), fv["bla_der"].view(np.int64))
+
+
+def test_escalated_frame_prepares_the_floatexp_reference():
+    """The near-zero escalation: the product hands the frame to the floatexp path with the
+    oracle's reference (exact view centre), orbit and BLA table."""
+    from test_oracle_dbl import ESCALATE
+    cre, cim, scale, W, H, mxit, pb = ESCALATE
+    fv = mxhip.frame_prepare(cre, cim, scale, W, H, mxit, prec_bits=pb)
+    val, nesc, out, info = mxo.compute(cre, cim, scale, W, H, mxit, prec_bits=pb)
+    assert fv["path"] == 2 and fv["escalated"] == 1 and info["escalated"] == 1
+    assert fv["ref_it"] == info["ref_it"] and fv["ref_virtual"] == 1 == info["ref_virtual"]
+    assert (fv["ref_x"], fv["ref_y"]) == (info["ref_x"], info["ref_y"])

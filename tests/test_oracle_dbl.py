@@ -151,3 +151,23 @@ def test_complex_division():
         mag = max(abs(er), abs(ei))
         assert abs(Fr(r.real) - er) <= 3 * mag * Fr(2) ** -52
         assert abs(Fr(r.imag) - ei) <= 3 * mag * Fr(2) ** -52
+
+
+# deep1's period-14164 nucleus at 1e200, odd size so that the centre pixel is the view
+# centre, 3000-bit centre: the reference's |X| underflows double near iteration 14164
+ESCALATE = (cases.LOC["deep1_x"], cases.LOC["deep1_y"], cases.pow10(200), 25, 17, 30000, 3000)
+
+
+def test_near_zero_reference_escalates_to_floatexp():
+    """A double-path frame (1e150 < scale <= 1e280) whose reference passes too close to
+    zero runs on the floatexp path from the exact view centre (MP.cpp:1336-1355); the
+    result matches the MPFR brute force."""
+    cre, cim, scale, W, H, mxit, pb = ESCALATE
+    val, nesc, out, info = mxo.compute(cre, cim, scale, W, H, mxit, prec_bits=pb)
+    assert info["path"] == 2 and info["escalated"] == 1 and info["ref_virtual"] == 1
+    assert (info["ref_x"], info["ref_y"]) == ((W - 1) / 2, (H - 1) / 2)
+    dv, dn = mxo.direct(cre, cim, scale, W, H, mxit, step=4, prec_bits=pb)
+    m = ~np.isnan(dv)
+    assert ((nesc[m] < 0) == (dn[m] < 0)).all()
+    ext = m & (nesc > 0)
+    assert np.abs(val[ext] - dv[ext]).max() < 1e-6
