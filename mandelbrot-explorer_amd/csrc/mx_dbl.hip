@@ -31,8 +31,8 @@ struct Pd {
     int j, k;
     int save_j, period_win, conf_P, conf_next, conf_count, conf_giveup;
     int sk, save_j2, twin2;
-    int xv;        // 1: Xc = X[k-1] is loaded
-    double2 Xc;
+    int xv;        // 1: Xc = X[k-1] and Xn = X[k] are loaded
+    double2 Xc, Xn;
 };
 
 struct Counters {
@@ -186,7 +186,14 @@ __device__ __forceinline__ int trip(const MxRefDev& R, const int2* __restrict__ 
             return 0;
         }
     }
-    if (p.k != 0 && !p.xv) p.Xc = R.zf[p.k - 1];
+    if (!p.xv) {
+        if (p.k != 0) p.Xc = R.zf[p.k - 1];
+        p.Xn = R.zf[p.k];
+    }
+    // the entry after this step's, issued now so its latency overlaps the step; k can
+    // equal mx_ref_it only on the first step after a series skip to the reference's
+    // escape, and then the next step rebases, so the clamped entry goes unused
+    const double2 Xpf = R.zf[min(p.k + 1, R.mx_ref_it)];
     double dr = 0.0, di = 0.0;
     if (EDE) {
         // dd' = 2 (dd X + dz (D + dd)) from the dz before this step (2444-2456)
@@ -215,13 +222,14 @@ __device__ __forceinline__ int trip(const MxRefDev& R, const int2* __restrict__ 
         dzi = (p.dzr * xi + p.dzi * xr) * 2 + p.dzr * p.dzi * 2 + p.dci;
     }
     const double dzr = tmp;
-    const double2 Xn = R.zf[p.k];
+    const double2 Xn = p.Xn;
     const double zr = dzr + Xn.x, zi = dzi + Xn.y;
     const double zrad = zr * zr + zi * zi;
     p.dzr = dzr;
     p.dzi = dzi;
     ++p.k;
-    p.Xc = Xn; // X[k-1] of the next step
+    p.Xc = Xn; // X[k-1] and X[k] of the next step
+    p.Xn = Xpf;
     p.xv = 1;
     c.steps++;
     if (zrad > T.esc2) {

@@ -16,12 +16,20 @@
 // is measured (fraction of samples with identical escape count, max smooth error).
 //
 // Lane scheduling is the plain form's: persistent grid, per-XCD atomic work queues
-// with wave-aggregated refill, 8x8 sample tiles.
+// with wave-aggregated refill, 8x8 sample tiles.  Each trip runs one full step
+// (every rare event handled) and then a run of lean steps that carry only the common
+// case and stop, before committing, at the first step that needs the full path: a
+// wave executes a branch whenever any of its 64 lanes takes it, so per-lane events
+// that are individually rare (a BLA-aligned index every 8 steps, the periodicity
+// check every 16) would otherwise put their code on nearly every wave step.
+//
+// The step arithmetic is written with explicit fused multiply-adds and no other FP
+// contraction, so the full and the lean step produce identical bits.
 #include <climits>
 
 #include "mx_dev.h"
 
-#pragma clang fp contract(fast)
+#pragma clang fp contract(off)
 
 namespace {
 
@@ -37,6 +45,8 @@ MX_HD float fmin_t(float a, float b) { return fminf(a, b); }
 MX_HD double fmin_t(double a, double b) { return fmin(a, b); }
 MX_HD float fabs_t(float a) { return fabsf(a); }
 MX_HD double fabs_t(double a) { return fabs(a); }
+MX_HD float fma_t(float a, float b, float c) { return fmaf(a, b, c); }
+MX_HD double fma_t(double a, double b, double c) { return fma(a, b, c); }
 
 template <typename F>
 struct Vec2;
@@ -74,6 +84,11 @@ struct P32 {
 struct C32 {
     unsigned long long steps, applies, skipped, rebases;
 };
+
+// lean steps per trip (after each full step)
+#ifndef MX_FAST_STEPS
+#define MX_FAST_STEPS 24
+#endif
 
 template <typename F>
 MX_HD F pow2t(int e) {
@@ -278,6 +293,36 @@ __host__ __device__ __noinline__ WK deriv_recover(const MxRefDev& R, int m, int 
     return WK{ldexp(ar, -ea), ldexp(ai, -ea), e + ea, false};
 }
 
+// The common-case arithmetic of one step, shared by the full and the lean step.
+template <typename F>
+MX_HD F wmag2(F wr, F wi) { return fma_t(wr, wr, wi * wi); }
+
+// w' = (2X + s w) w + d
+template <typename F, typename V>
+MX_HD void step_w(const P32<F>& p, V X, F& nwr, F& nwi) {
+    const F fR = fma_t(p.s, p.wr, (F)2 * X.x);
+    const F fI = fma_t(p.s, p.wi, (F)2 * X.y);
+    nwr = fma_t(fR, p.wr, fma_t(-fI, p.wi, p.dr));
+    nwi = fma_t(fR, p.wi, fma_t(fI, p.wr, p.di));
+}
+
+// a step result far below |w| ~ 1: the terms live below the current scale
+template <typename F>
+MX_HD bool step_tiny(const P32<F>& p, F nwr, F nwi) {
+    const F TINY = sizeof(F) == 4 ? (F)0x1p-60f : (F)0x1p-1000;
+    return fabs_t(nwr) < TINY && fabs_t(nwi) < TINY && (nwr != (F)0 || nwi != (F)0 || p.wr != (F)0 || p.wi != (F)0 || p.dcr != (F)0 || p.dci != (F)0);
+}
+
+// J' = 2 z J + 1 in J's scale, z = X + s w (the pre-step delta)
+template <typename F>
+MX_HD void step_j(const P32<F>& p, F zmr, F zmi, F& a, F& b) {
+    a = fma_t((F)2 * zmr, p.jr, fma_t((F)-2 * zmi, p.ji, p.jone));
+    b = (F)2 * fma_t(zmr, p.ji, zmi * p.jr);
+}
+
+template <typename F>
+MX_HD bool renorm_due(F m2) { return m2 > (F)0x1p40f || (m2 < (F)0x1p-40f && m2 > (F)0); }
+
 // BLA skip from table index s = m-1 (tryBLAfe, mandel_perturbation.cpp:2883-2970)
 // with floatexp coefficients (mantissa c, exponents ce).  Updates (w, k) and J.
 template <typename F>
@@ -289,7 +334,7 @@ MX_HD int try_bla32(const MxRefDev& R, const int2* lev, int s, P32<F>& p, int re
     int startp = (s == 1) ? maxp : __builtin_ctz((unsigned)(s - 1));
     if (startp > maxp) startp = maxp;
     if (startp < R.minlevel) return 0;
-    const F w2 = p.wr * p.wr + p.wi * p.wi;
+    const F w2 = wmag2(p.wr, p.wi);
     const int k2 = 2 * p.k;
     if (!(R.rmax2 > 0.0) || !dz2_less(w2, k2, R.rmax2)) return 0;
     for (int q = startp; q >= R.minlevel; --q) {
@@ -415,20 +460,16 @@ MX_HD bool trip32(const MxRefDev& R, const int2* lev, const MxTileDev& T, P32<F>
     const V X = p.Xc;
     F zmr = (F)0, zmi = (F)0;
     if (EDE) {
-        zmr = X.x + p.s * p.wr;
-        zmi = X.y + p.s * p.wi;
+        zmr = fma_t(p.s, p.wr, X.x);
+        zmi = fma_t(p.s, p.wi, X.y);
     }
-    // w' = (2X + s w) w + d
-    const F fR = (F)2 * X.x + p.s * p.wr;
-    const F fI = (F)2 * X.y + p.s * p.wi;
-    const F nwr = fR * p.wr - fI * p.wi + p.dr;
-    const F nwi = fR * p.wi + fI * p.wr + p.di;
+    F nwr, nwi;
+    step_w(p, X, nwr, nwi);
     const F wr0 = p.wr, wi0 = p.wi;
     const int k0 = p.k;
     // |w| stays near 1; a result far below it means the step's terms live below the
     // current scale (a delta that underflowed), recomputed exactly
-    const F TINY = sizeof(F) == 4 ? (F)0x1p-60f : (F)0x1p-1000;
-    if (fabs_t(nwr) < TINY && fabs_t(nwi) < TINY && (nwr != (F)0 || nwi != (F)0 || p.wr != (F)0 || p.wi != (F)0 || p.dcr != (F)0 || p.dci != (F)0)) {
+    if (step_tiny(p, nwr, nwi)) {
         const WK r = step_recover(R, p.m, p.rm, (double)p.wr, (double)p.wi, p.k, (double)p.dcr, p.dcre, (double)p.dci, p.dcie);
         p.wr = (F)r.wr;
         p.wi = (F)r.wi;
@@ -453,20 +494,19 @@ MX_HD bool trip32(const MxRefDev& R, const int2* lev, const MxTileDev& T, P32<F>
             p.kj = r.k;
             p.jone = pow2t<F>(-p.kj);
         } else {
-            const F a = (F)2 * (zmr * p.jr - zmi * p.ji) + p.jone;
-            const F b = (F)2 * (zmr * p.ji + zmi * p.jr);
+            F a, b;
+            step_j(p, zmr, zmi, a, b);
             p.jr = a;
             p.ji = b;
-            const F jm2 = a * a + b * b;
-            if (jm2 > (F)0x1p40f || (jm2 < (F)0x1p-40f && jm2 > (F)0)) renorm_j(p);
+            if (renorm_due(wmag2(a, b))) renorm_j(p);
         }
     }
     const V Xn = p.Xn;
     p.Xc = Xn;
     p.Xn = Xpf;
     p.xv = 1;
-    const F zr = Xn.x + p.s * p.wr, zi = Xn.y + p.s * p.wi;
-    const F zrad = zr * zr + zi * zi;
+    const F zr = fma_t(p.s, p.wr, Xn.x), zi = fma_t(p.s, p.wi, Xn.y);
+    const F zrad = wmag2(zr, zi);
     if (zrad > esc2f) {
         n = p.iter;
         const double zd = (double)zr * (double)zr + (double)zi * (double)zi;
@@ -487,7 +527,7 @@ MX_HD bool trip32(const MxRefDev& R, const int2* lev, const MxTileDev& T, P32<F>
             p.conf_D2 = fmin_t(p.conf_D2 * (F)4 * zrad, (F)1e18f);
             if (p.iter >= p.conf_next) {
                 const F pr = zr - p.conf_zr, pi = zi - p.conf_zi;
-                if (pr * pr + pi * pi < eps2 * zrad && p.conf_D2 < (F)1) {
+                if (wmag2(pr, pi) < eps2 * zrad && p.conf_D2 < (F)1) {
                     if (++p.conf_count >= T.interior_confirm) {
                         n = -2;
                         v = -2.0;
@@ -504,7 +544,7 @@ MX_HD bool trip32(const MxRefDev& R, const int2* lev, const MxTileDev& T, P32<F>
             }
         } else if ((p.iter & 15) == 0 && p.conf_giveup < 3) {
             const F pr = zr - p.zsr, pi = zi - p.zsi;
-            if (pr * pr + pi * pi < eps2 * zrad) {
+            if (wmag2(pr, pi) < eps2 * zrad) {
                 p.conf_P = max(p.iter - p.save_iter, 1);
                 p.conf_zr = zr;
                 p.conf_zi = zi;
@@ -520,12 +560,11 @@ MX_HD bool trip32(const MxRefDev& R, const int2* lev, const MxTileDev& T, P32<F>
             }
         }
     }
-    const F w2 = p.wr * p.wr + p.wi * p.wi;
-    if (w2 > (F)0x1p40f || (w2 < (F)0x1p-40f && w2 > (F)0)) renorm_w(p);
+    if (renorm_due(wmag2(p.wr, p.wi))) renorm_w(p);
     // Rebase candidates: |z|^2 < 1e-8, |z|^2 < |dz|^2 (while 2^(2k) is
     // representable), or the end of the reference; then the exact test and the new
     // delta in FP64 from the FP64 orbit (mandel_perturbation.cpp:3584-3615).
-    const F dz2f = ldexp_t(p.wr * p.wr + p.wi * p.wi, 2 * p.k);
+    const F dz2f = ldexp_t(wmag2(p.wr, p.wi), 2 * p.k);
     if (zrad < (F)1e-8f || zrad < dz2f || p.m >= reflen) {
         // z * 2^-k = X * 2^-k + w in FP64
         double xr = 0.0, xi = 0.0;
@@ -557,6 +596,103 @@ MX_HD bool trip32(const MxRefDev& R, const int2* lev, const MxTileDev& T, P32<F>
         }
     }
     return false;
+}
+
+// Bit pattern of a non-negative float / double: monotone in the value (NaN and +inf
+// above every finite value), so a range test is one subtraction and one compare.
+MX_HD unsigned fbits(float x) { return __builtin_bit_cast(unsigned, x); }
+MX_HD unsigned long long fbits(double x) { return __builtin_bit_cast(unsigned long long, x); }
+
+// Lean steps: exactly the common case of trip32, for up to `budget` steps.  A lane
+// stops, with its state untouched, at the first step whose full path could do more
+// than the plain step: every condition below is the full path's own test or a
+// superset of it (the full path then decides), evaluated without branches and joined
+// into one exit.
+//   before the step: the iteration limit; an active periodicity confirmation; an
+//     exact-repeat save, or the orbit index back at the saved one (a repeat needs it);
+//     a BLA-aligned index whose delta is not above the largest radius by exponent;
+//   after it: |w'|^2 outside [2^-40, 2^40] (renormalisation, or a step underflow, or
+//     zero), the same for the derivative, a derivative step whose terms all vanish,
+//     |z|^2 outside [1e-8, escape radius^2] (escape, rebase candidate), |z| < |dz|
+//     (rebase candidate), the end of the reference, and a near return on a
+//     periodicity check.
+template <typename F, bool EDE, bool INTERIOR>
+MX_HD void fast_steps(const MxRefDev& R, const MxTileDev& T, P32<F>& p, C32& c, F esc2f, int budget) {
+    using V = typename Vec2<F>::T;
+    using U = decltype(fbits((F)0));
+    const int reflen = R.mx_ref_it + 1;
+    const int amask = (1 << R.minlevel) - 1;
+    const bool can_bla = (T.flags & MX_F_BLA) && R.nlev - 1 >= R.minlevel && R.rmax2 > 0.0;
+    int er = 0;
+    if (can_bla) (void)frexp(R.rmax2, &er);
+    const F eps2 = (F)T.interior_eps2;
+    const F R2LO = (F)0x1p-40f, R2HI = (F)0x1p40f;
+    const U r2lo = fbits(R2LO), r2span = fbits(R2HI) - fbits(R2LO);
+    const F ZLO = (F)1e-8f;
+    const U zlo = fbits(ZLO), zspan = fbits(esc2f) - fbits(ZLO);
+    const bool int_rep = INTERIOR && (T.flags & MX_F_INT_REP);
+    unsigned n = 0;
+    for (int u = 0; u < budget; ++u) {
+        int stop = p.iter >= T.mxit;
+        if (INTERIOR) stop |= p.conf_P > 0;
+        if (int_rep) stop |= (int)(p.m == p.sm) | (int)(p.iter - p.save_iter2 >= p.twin2r);
+        if (can_bla) {
+            const F w2 = wmag2(p.wr, p.wi);
+            int ew;
+            (void)frexp_t(w2, &ew);
+            stop |= (int)(p.m >= 2) & (int)(((p.rm - 1) & amask) == 0) & ((int)(ew + 2 * p.k <= er) | (int)(w2 == (F)0));
+        }
+        if (!p.xv) {
+            p.Xc = p.m ? load_X(R, p.rm, (F)0) : V{(F)0, (F)0};
+            p.Xn = load_X(R, p.rm + 1, (F)0);
+            p.xv = 1;
+        }
+        // the next step's entry; when rm + 2 is past the end this step stops anyway
+        const V Xpf = load_X(R, min(p.rm + 2, reflen - 1), (F)0);
+        const V X = p.Xc;
+        F nwr, nwi;
+        step_w(p, X, nwr, nwi);
+        const F w2n = wmag2(nwr, nwi);
+        stop |= fbits(w2n) - r2lo > r2span;
+        F ja = (F)0, jb = (F)0;
+        if (EDE) {
+            const F zmr = fma_t(p.s, p.wr, X.x), zmi = fma_t(p.s, p.wi, X.y);
+            step_j(p, zmr, zmi, ja, jb);
+            stop |= ((int)(zmr == (F)0) & (int)(zmi == (F)0) & (int)(p.jone == (F)0)) | (int)(fbits(wmag2(ja, jb)) - r2lo > r2span);
+        }
+        const V Xn = p.Xn;
+        const F zr = fma_t(p.s, nwr, Xn.x), zi = fma_t(p.s, nwi, Xn.y);
+        const F zrad = wmag2(zr, zi);
+        stop |= (int)(fbits(zrad) - zlo > zspan) | (int)(zrad < ldexp_t(w2n, 2 * p.k)) | (int)(p.m + 1 >= reflen);
+        const int it1 = p.iter + 1;
+        int save = 0;
+        if (INTERIOR) {
+            const int chk = (int)((it1 & 15) == 0) & (int)(p.conf_giveup < 3);
+            const F pr = zr - p.zsr, pi = zi - p.zsi;
+            stop |= chk & (int)(wmag2(pr, pi) < eps2 * zrad);
+            save = chk & (int)(it1 - p.save_iter >= p.period_win);
+        }
+        if (stop) break;
+        p.wr = nwr;
+        p.wi = nwi;
+        ++p.m;
+        ++p.rm;
+        p.iter = it1;
+        if (EDE) {
+            p.jr = ja;
+            p.ji = jb;
+        }
+        p.Xc = Xn;
+        p.Xn = Xpf;
+        if (save) {
+            p.zsr = zr;
+            p.zsi = zi;
+            p.save_iter = it1;
+            p.period_win += p.period_win;
+        }
+        ++n;
+    }
+    c.steps += n;
 }
 
 __device__ __forceinline__ unsigned long long wave_sum(unsigned long long x) {
@@ -627,6 +763,8 @@ __global__ __launch_bounds__(256) void mx_fe32_kernel(MxRefDev R, const int2* __
                 T.nesc[sidx] = n;
                 if (T.out) T.out[sidx] = n < 0 ? -2.0f : (float)v;
                 active = false;
+            } else {
+                fast_steps<F, EDE, INTERIOR>(R, T, p, c, esc2f, MX_FAST_STEPS);
             }
         }
     }
