@@ -40,6 +40,11 @@ struct Counters {
     unsigned long long steps, applies, skipped, rebases;
 };
 
+// lean steps per trip (after each full step)
+#ifndef MX_FAST_STEPS
+#define MX_FAST_STEPS 24
+#endif
+
 MX_HD void init_pixel(Px& p, Fe dcr, Fe dci) {
     p.dcr = dcr;
     p.dci = dci;
@@ -209,6 +214,22 @@ MX_HD int try_bla(const MxRefDev& R, const int2* lev, int s, Fe& S, Fe S2, doubl
     return 0;
 }
 
+// The common-case arithmetic of one step, shared by the full and the lean step
+// (this file is built without FP contraction: identical expressions, identical bits).
+// w' = (2X + s w) w + d, in the engine's order (mandel_perturbation.cpp:3446-3450)
+MX_HD void step_w(const Px& p, double2 X, double& nwr, double& nwi) {
+    const double fR = X.x + X.x + p.s * p.wr;
+    const double fI = X.y + X.y + p.s * p.wi;
+    nwr = fR * p.wr - fI * p.wi + p.dr;
+    nwi = fR * p.wi + fI * p.wr + p.di;
+}
+
+// J' = 2 z J + 1/SJ (mandel_perturbation.cpp:3483-3488)
+MX_HD void step_j(const Px& p, double zmr, double zmi, double& a, double& b) {
+    a = 2.0 * (zmr * p.jr - zmi * p.ji) + p.invSJd;
+    b = 2.0 * (zmr * p.ji + zmi * p.jr);
+}
+
 // One trip of the pixel loop.  Returns true when the sample is finished (value and
 // n_esc written to v/n).  n: escape iteration, -1 = reached mxit, -2 = detected interior.
 template <bool EDE, bool INTERIOR>
@@ -300,10 +321,8 @@ MX_HD bool trip(const MxRefDev& R, const int2* lev, const MxTileDev& T, Px& p, d
             zmife = fe_add(Xife, fe_mul_d(p.S, p.wi));
         }
     }
-    const double fR = X.x + X.x + p.s * p.wr;
-    const double fI = X.y + X.y + p.s * p.wi;
-    const double nwr = fR * p.wr - fI * p.wi + p.dr;
-    const double nwi = fR * p.wi + fI * p.wr + p.di;
+    double nwr, nwi;
+    step_w(p, X, nwr, nwi);
     const bool su = fabs(nwr) < 0x1p-1056 && fabs(nwi) < 0x1p-1056;
     if (su && (nwr != 0.0 || nwi != 0.0 || p.wr != 0.0 || p.wi != 0.0 || p.dcr.m != 0.0 || p.dci.m != 0.0)) {
         Fe Xrfe = fe_make(0.0, 0), Xife = fe_make(0.0, 0);
@@ -337,8 +356,8 @@ MX_HD bool trip(const MxRefDev& R, const int2* lev, const MxTileDev& T, Px& p, d
             p.ji = r.ji;
             p.invSJd = r.invSJd;
         } else {
-            const double a = 2.0 * (zmr * p.jr - zmi * p.ji) + p.invSJd;
-            const double b = 2.0 * (zmr * p.ji + zmi * p.jr);
+            double a, b;
+            step_j(p, zmr, zmi, a, b);
             p.jr = a;
             p.ji = b;
             const double jm2 = p.jr * p.jr + p.ji * p.ji;
@@ -457,6 +476,102 @@ MX_HD bool trip(const MxRefDev& R, const int2* lev, const MxTileDev& T, Px& p, d
     return false;
 }
 
+// Bit pattern of a non-negative double: monotone in the value (NaN and +inf above every
+// finite value), so a range test is one subtraction and one compare.
+MX_HD unsigned long long dbits(double x) { return __builtin_bit_cast(unsigned long long, x); }
+
+// Lean steps: exactly the common case of trip(), for up to `budget` steps.  A lane
+// stops, with its state untouched, at the first step whose full path could do more
+// than the plain step; every condition is the full path's own test or a superset of
+// it (the full path then decides), evaluated without branches and joined into one
+// exit.  Before the step: the iteration limit, an active periodicity confirmation, an
+// exact-repeat save or the orbit index back at the saved one, a BLA-aligned index
+// whose |dz|^2 is not above the largest radius by exponent.  After it: a nonzero
+// |w'|^2 outside [1e-16, 1e16] (renormalisation), a step underflow, the same
+// renormalisation test for the derivative, a derivative step whose terms all vanish,
+// |z|^2 outside [1e-8, escape
+// radius^2], |z| < |dz|, the end of the reference, a near return on a periodicity
+// check.  A wave executes a branch whenever any of its 64 lanes takes it, so these
+// per-lane rare events would otherwise put the full path on nearly every wave step.
+template <bool EDE, bool INTERIOR>
+MX_HD void fast_steps(const MxRefDev& R, const MxTileDev& T, Px& p, Counters& c, int budget) {
+    const double ESC2 = T.esc2;
+    const int reflen = R.mx_ref_it + 1;
+    const int amask = (1 << R.minlevel) - 1;
+    const bool can_bla = (T.flags & MX_F_BLA) && R.nlev - 1 >= R.minlevel && R.rmax2 > 0.0;
+    const int er = can_bla ? fe_from(R.rmax2).e : 0;
+    const unsigned long long mlo = dbits(1e-16), mspan = dbits(1e16) - dbits(1e-16);
+    const unsigned long long zlo = dbits(1e-8), zspan = dbits(ESC2) - dbits(1e-8);
+    const bool int_rep = INTERIOR && (T.flags & MX_F_INT_REP);
+    unsigned n = 0;
+    for (int u = 0; u < budget; ++u) {
+        int stop = p.iter >= T.mxit;
+        if (INTERIOR) stop |= p.conf_P > 0;
+        if (int_rep) stop |= (int)(p.m == p.sm) | (int)(p.iter - p.save_iter2 >= p.twin2r);
+        if (can_bla) {
+            // dz^2 = S^2 |w|^2 has exponent S2.e + e(|w|^2) or one less
+            const double w2 = p.wr * p.wr + p.wi * p.wi;
+            stop |= (int)(p.m >= 2) & (int)(((p.rm - 1) & amask) == 0) & ((int)(p.S2.e + fe_from(w2).e <= er + 1) | (int)(w2 == 0.0));
+        }
+        if (!p.xv) {
+            p.Xc = p.m ? R.zf[p.rm] : make_double2(0.0, 0.0);
+            p.Xn = R.zf[p.rm + 1];
+            p.xv = 1;
+        }
+        // the next step's entry; when rm + 2 is past the end this step stops anyway
+        const double2 Xpf = R.zf[min(p.rm + 2, reflen - 1)];
+        const double2 X = p.Xc;
+        double nwr, nwi;
+        step_w(p, X, nwr, nwi);
+        // the full path's renormalisation test (|w'|^2 that underflows to 0 is left as it
+        // is there) and its step-underflow test
+        const double w2n = nwr * nwr + nwi * nwi;
+        stop |= (int)(dbits(w2n) - mlo > mspan) & (int)(w2n != 0.0);
+        stop |= (int)(fabs(nwr) < 0x1p-1056) & (int)(fabs(nwi) < 0x1p-1056) &
+                ((int)(nwr != 0.0) | (int)(nwi != 0.0) | (int)(p.wr != 0.0) | (int)(p.wi != 0.0) | (int)(p.dcr.m != 0.0) | (int)(p.dci.m != 0.0));
+        double ja = 0.0, jb = 0.0;
+        if (EDE) {
+            const double zmr = X.x + p.s * p.wr, zmi = X.y + p.s * p.wi;
+            step_j(p, zmr, zmi, ja, jb);
+            const double jm2 = ja * ja + jb * jb;
+            stop |= ((int)(zmr == 0.0) & (int)(zmi == 0.0) & (int)(p.invSJd == 0.0)) | ((int)(dbits(jm2) - mlo > mspan) & (int)(jm2 != 0.0));
+        }
+        const double2 Xn = p.Xn;
+        const double zr = Xn.x + p.s * nwr, zi = Xn.y + p.s * nwi;
+        const double zrad = zr * zr + zi * zi;
+        const double dzmag2 = p.s * p.s * w2n;
+        stop |= (int)(dbits(zrad) - zlo > zspan) | (int)(zrad < dzmag2) | (int)(p.m + 1 >= reflen);
+        const int it1 = p.iter + 1;
+        int save = 0;
+        if (INTERIOR) {
+            const int chk = (int)((it1 & 15) == 0) & (int)(p.conf_giveup < 3);
+            const double pr = zr - p.zsr, pi = zi - p.zsi;
+            stop |= chk & (int)(pr * pr + pi * pi < T.interior_eps2 * zrad);
+            save = chk & (int)(it1 - p.save_iter >= p.period_win);
+        }
+        if (stop) break;
+        p.wr = nwr;
+        p.wi = nwi;
+        ++p.m;
+        ++p.rm;
+        p.iter = it1;
+        if (EDE) {
+            p.jr = ja;
+            p.ji = jb;
+        }
+        p.Xc = Xn;
+        p.Xn = Xpf;
+        if (save) {
+            p.zsr = zr;
+            p.zsi = zi;
+            p.save_iter = it1;
+            p.period_win += p.period_win;
+        }
+        ++n;
+    }
+    c.steps += n;
+}
+
 __device__ __forceinline__ unsigned long long wave_sum(unsigned long long x) {
     for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o);
     return x;
@@ -526,6 +641,8 @@ __global__ __launch_bounds__(256) void mx_fe64_kernel(MxRefDev R, const int2* __
                 T.nesc[sidx] = n;
                 if (T.out) T.out[sidx] = n < 0 ? -2.0f : (float)v;
                 active = false;
+            } else {
+                fast_steps<EDE, INTERIOR>(R, T, p, c, MX_FAST_STEPS);
             }
         }
     }
