@@ -201,6 +201,37 @@ MX_HD void load_coef(const MxRefDev& R, int e, double4& c, int2& ce) {
     c = R.bla_coef[e];
     ce = make_int2(0, 0);
 }
+// Orbit entries for the lean steps through a buffer resource: one VALU shift per
+// address instead of a 64-bit add, and indices past the end read as 0.
+template <typename F>
+struct OrbitLoader;
+template <>
+struct OrbitLoader<float> {
+    __amdgpu_buffer_rsrc_t r;
+    int n;
+    MX_HD explicit OrbitLoader(const MxRefDev& R) : n(R.mx_ref_it + 1) {
+#ifdef __HIP_DEVICE_COMPILE__
+        r = __builtin_amdgcn_make_buffer_rsrc((void*)R.zf32, 0, (R.mx_ref_it + 1) * 8, 0x00020000);
+#endif
+        zf = R.zf32;
+    }
+    const float2* zf;
+    MX_HD float2 load(int i) const {
+#ifdef __HIP_DEVICE_COMPILE__
+        return __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(r, i * 8, 0, 0));
+#else
+        return i < n ? zf[i] : float2{0.f, 0.f};
+#endif
+    }
+};
+template <>
+struct OrbitLoader<double> {
+    const double2* zf;
+    int n;
+    MX_HD explicit OrbitLoader(const MxRefDev& R) : zf(R.zf), n(R.mx_ref_it + 1) {}
+    MX_HD double2 load(int i) const { return zf[min(i, n - 1)]; }
+};
+
 template <typename F>
 struct Vec4;
 template <>
@@ -297,11 +328,17 @@ __host__ __device__ __noinline__ WK deriv_recover(const MxRefDev& R, int m, int 
 template <typename F>
 MX_HD F wmag2(F wr, F wi) { return fma_t(wr, wr, wi * wi); }
 
-// w' = (2X + s w) w + d
+// z = X + s w (the pre-step value, which the derivative step also uses), then
+// w' = (X + z) w + d = (2X + s w) w + d
 template <typename F, typename V>
-MX_HD void step_w(const P32<F>& p, V X, F& nwr, F& nwi) {
-    const F fR = fma_t(p.s, p.wr, (F)2 * X.x);
-    const F fI = fma_t(p.s, p.wi, (F)2 * X.y);
+MX_HD void step_z(const P32<F>& p, V X, F& zmr, F& zmi) {
+    zmr = fma_t(p.s, p.wr, X.x);
+    zmi = fma_t(p.s, p.wi, X.y);
+}
+template <typename F, typename V>
+MX_HD void step_w(const P32<F>& p, V X, F zmr, F zmi, F& nwr, F& nwi) {
+    const F fR = zmr + X.x;
+    const F fI = zmi + X.y;
     nwr = fma_t(fR, p.wr, fma_t(-fI, p.wi, p.dr));
     nwi = fma_t(fR, p.wi, fma_t(fI, p.wr, p.di));
 }
@@ -458,13 +495,10 @@ MX_HD bool trip32(const MxRefDev& R, const int2* lev, const MxTileDev& T, P32<F>
     }
     const V Xpf = (p.rm + 2 < reflen) ? load_X(R, p.rm + 2, (F)0) : V{(F)0, (F)0};
     const V X = p.Xc;
-    F zmr = (F)0, zmi = (F)0;
-    if (EDE) {
-        zmr = fma_t(p.s, p.wr, X.x);
-        zmi = fma_t(p.s, p.wi, X.y);
-    }
+    F zmr, zmi;
+    step_z(p, X, zmr, zmi);
     F nwr, nwi;
-    step_w(p, X, nwr, nwi);
+    step_w(p, X, zmr, zmi, nwr, nwi);
     const F wr0 = p.wr, wi0 = p.wi;
     const int k0 = p.k;
     // |w| stays near 1; a result far below it means the step's terms live below the
@@ -631,13 +665,14 @@ MX_HD void fast_steps(const MxRefDev& R, const MxTileDev& T, P32<F>& p, C32& c, 
     const F ZLO = (F)1e-8f;
     const U zlo = fbits(ZLO), zspan = fbits(esc2f) - fbits(ZLO);
     const bool int_rep = INTERIOR && (T.flags & MX_F_INT_REP);
+    const OrbitLoader<F> ld(R);
+    F w2 = wmag2(p.wr, p.wi);
     unsigned n = 0;
     for (int u = 0; u < budget; ++u) {
         int stop = p.iter >= T.mxit;
         if (INTERIOR) stop |= p.conf_P > 0;
         if (int_rep) stop |= (int)(p.m == p.sm) | (int)(p.iter - p.save_iter2 >= p.twin2r);
         if (can_bla) {
-            const F w2 = wmag2(p.wr, p.wi);
             int ew;
             (void)frexp_t(w2, &ew);
             stop |= (int)(p.m >= 2) & (int)(((p.rm - 1) & amask) == 0) & ((int)(ew + 2 * p.k <= er) | (int)(w2 == (F)0));
@@ -647,16 +682,17 @@ MX_HD void fast_steps(const MxRefDev& R, const MxTileDev& T, P32<F>& p, C32& c, 
             p.Xn = load_X(R, p.rm + 1, (F)0);
             p.xv = 1;
         }
-        // the next step's entry; when rm + 2 is past the end this step stops anyway
-        const V Xpf = load_X(R, min(p.rm + 2, reflen - 1), (F)0);
+        // the next step's entry (0 past the end, where this step stops anyway)
+        const V Xpf = ld.load(p.rm + 2);
         const V X = p.Xc;
+        F zmr, zmi;
+        step_z(p, X, zmr, zmi);
         F nwr, nwi;
-        step_w(p, X, nwr, nwi);
+        step_w(p, X, zmr, zmi, nwr, nwi);
         const F w2n = wmag2(nwr, nwi);
         stop |= fbits(w2n) - r2lo > r2span;
         F ja = (F)0, jb = (F)0;
         if (EDE) {
-            const F zmr = fma_t(p.s, p.wr, X.x), zmi = fma_t(p.s, p.wi, X.y);
             step_j(p, zmr, zmi, ja, jb);
             stop |= ((int)(zmr == (F)0) & (int)(zmi == (F)0) & (int)(p.jone == (F)0)) | (int)(fbits(wmag2(ja, jb)) - r2lo > r2span);
         }
@@ -672,25 +708,28 @@ MX_HD void fast_steps(const MxRefDev& R, const MxTileDev& T, P32<F>& p, C32& c, 
             stop |= chk & (int)(wmag2(pr, pi) < eps2 * zrad);
             save = chk & (int)(it1 - p.save_iter >= p.period_win);
         }
-        if (stop) break;
-        p.wr = nwr;
-        p.wi = nwi;
-        ++p.m;
-        ++p.rm;
-        p.iter = it1;
+        // commit by selects, so that the entry loaded above is consumed only by the
+        // next step and its load overlaps this one
+        const bool go = !stop;
+        p.wr = go ? nwr : p.wr;
+        p.wi = go ? nwi : p.wi;
+        p.m += go;
+        p.rm += go;
+        p.iter = go ? it1 : p.iter;
         if (EDE) {
-            p.jr = ja;
-            p.ji = jb;
+            p.jr = go ? ja : p.jr;
+            p.ji = go ? jb : p.ji;
         }
-        p.Xc = Xn;
-        p.Xn = Xpf;
-        if (save) {
-            p.zsr = zr;
-            p.zsi = zi;
-            p.save_iter = it1;
-            p.period_win += p.period_win;
-        }
-        ++n;
+        p.Xc = go ? Xn : p.Xc;
+        p.Xn = go ? Xpf : p.Xn;
+        const bool sv = go && save;
+        p.zsr = sv ? zr : p.zsr;
+        p.zsi = sv ? zi : p.zsi;
+        p.save_iter = sv ? it1 : p.save_iter;
+        p.period_win = sv ? p.period_win + p.period_win : p.period_win;
+        w2 = w2n;
+        n += go;
+        if (stop) break;
     }
     c.steps += n;
 }

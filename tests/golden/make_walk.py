@@ -13,13 +13,18 @@ Definition (the workload spec's walk, with the two choices it leaves open fixed)
     probe mxit (the probe would otherwise follow its own mxit level curve instead of
     the set), double the probe mxit (up to 2^21) and render again;
     pick uniformly among the top ceil(0.1 * n_escaped) candidates (splitmix64 mod k);
-    recentre on that pixel and halve w.
+    recentre on that pixel and halve w;
+    if a probe has no escaping pixel even at the mxit cap (the walk reached a region
+    whose escape counts all exceed 2^21), step back to the previous view and draw
+    again among its candidates (the draw continues the same splitmix64 sequence).
   MPFR precision of each probe = ceil(-log2 w) + 64 bits.
 Centres are recorded at every requested depth; the recorded decimal strings carry
 ceil(-log10 w) + 30 digits.
 
-Usage: python tests/golden/make_walk.py SEED DEPTH10 [DEPTH10 ...]
+Usage: python tests/golden/make_walk.py SEED DEPTH10 [DEPTH10 ...] [--resume DEPTH10]
 e.g.   python tests/golden/make_walk.py 7 300 1000
+--resume continues from the centre recorded at that depth (width 3 / 2^steps, the
+generator advanced by one draw per recorded step).
 """
 import json
 import math
@@ -48,7 +53,7 @@ class SplitMix64:
         return z ^ (z >> 31)
 
 
-def walk(seed, depths, log=print, on_depth=None):
+def walk(seed, depths, log=print, on_depth=None, resume=None):
     depths = sorted(depths)
     rng = SplitMix64(seed)
     getcontext().prec = max(depths) + 80
@@ -57,6 +62,15 @@ def walk(seed, depths, log=print, on_depth=None):
     mxit = 1024
     out = {}
     step = 0
+    if resume is not None:
+        cx, cy = Decimal(resume["re"]), Decimal(resume["im"])
+        step = int(resume["steps"])
+        w = Decimal(3) / (Decimal(2) ** step)
+        mxit = int(resume["probe_mxit"])
+        for _ in range(step):
+            rng.next()
+    prev = None  # (cx, cy, w, cand, k) of the previous view, for a step back
+    redraws = 0
     t0 = time.time()
     while depths:
         target = Decimal(10) ** (-depths[0])
@@ -88,9 +102,21 @@ def walk(seed, depths, log=print, on_depth=None):
                 break
             mxit *= 2
         if not cand:
-            raise RuntimeError("walk stalled: no escaping pixel")
-        cand.sort()
-        k = int(math.ceil(0.1 * len(cand)))
+            if prev is None:
+                raise RuntimeError("walk stalled: no escaping pixel")
+            # step back: the previous view, another draw among its candidates
+            redraws += 1
+            if redraws > 64:
+                raise RuntimeError("walk stalled: 64 redraws without an escaping probe")
+            cx, cy, w, cand, k = prev
+            scale = Decimal(4) / w
+            step -= 1
+            log(f"seed {seed}: step {step + 1} has no escaping pixel; redrawing at step {step}")
+        else:
+            redraws = 0
+            cand.sort()
+            k = int(math.ceil(0.1 * len(cand)))
+        prev = (cx, cy, w, cand, k)
         pick = cand[rng.next() % k][1]
         i, j = divmod(pick, W)
         dw = Decimal(2) / scale
@@ -107,9 +133,16 @@ def walk(seed, depths, log=print, on_depth=None):
 
 
 def main():
-    seed = int(sys.argv[1])
-    depths = [int(x) for x in sys.argv[2:]]
+    args = sys.argv[1:]
+    resume_depth = None
+    if "--resume" in args:
+        i = args.index("--resume")
+        resume_depth = args[i + 1]
+        del args[i:i + 2]
+    seed = int(args[0])
+    depths = [int(x) for x in args[1:]]
     path = os.path.join(HERE, f"walk_seed{seed}.json")
+    resume = json.load(open(path))[resume_depth] if resume_depth else None
 
     def save(res):
         old = json.load(open(path)) if os.path.exists(path) else {}
@@ -118,7 +151,7 @@ def main():
         json.dump(old, open(path, "w"), indent=1)
         print("wrote", path, flush=True)
 
-    save(walk(seed, depths, log=lambda s: print(s, flush=True), on_depth=save))
+    save(walk(seed, depths, log=lambda s: print(s, flush=True), on_depth=save, resume=resume))
 
 
 if __name__ == "__main__":
