@@ -124,6 +124,11 @@ extern "C" int mx_ref_upload(mx_ctx* ctx, const mx_ref_desc* d, mx_ref** out) {
     }
     if (d->nlev > 0 && (!d->bla_r2 || !d->bla_coef)) return MX_E_INVALID;
     if (total > INT_MAX / 4 || (int64_t)d->len > INT_MAX / 2) return MX_E_INVALID;
+    // the kernels address the orbit with 32-bit byte offsets (16 B per entry)
+    if (d->len > (1 << 27)) {
+        ctx->set_err("reference orbit longer than 2^27 entries");
+        return MX_E_INVALID;
+    }
     if (d->z_m3 && (!d->sa_coef || d->sa_order < 0 || d->sa_order >= MX_SA_N || d->sa_it < 0)) return MX_E_INVALID;
     if (d->df && (!d->z_m3 || !d->sa_coef_b || (d->nlev > 0 && !d->bla_der))) return MX_E_INVALID;
     MX_HIP(hipSetDevice(ctx->device));

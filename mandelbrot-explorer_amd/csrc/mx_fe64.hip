@@ -503,6 +503,12 @@ MX_HD void fast_steps(const MxRefDev& R, const MxTileDev& T, Px& p, Counters& c,
     const unsigned long long mlo = dbits(1e-16), mspan = dbits(1e16) - dbits(1e-16);
     const unsigned long long zlo = dbits(1e-8), zspan = dbits(ESC2) - dbits(1e-8);
     const bool int_rep = INTERIOR && (T.flags & MX_F_INT_REP);
+#ifdef __HIP_DEVICE_COMPILE__
+    // orbit entries through a buffer resource: a 32-bit offset per load, and indices
+    // past the end read as 0
+    const __amdgpu_buffer_rsrc_t zrs = __builtin_amdgcn_make_buffer_rsrc((void*)R.zf, 0, reflen * 16, 0x00020000);
+#endif
+    double w2 = p.wr * p.wr + p.wi * p.wi;
     unsigned n = 0;
     for (int u = 0; u < budget; ++u) {
         int stop = p.iter >= T.mxit;
@@ -510,7 +516,6 @@ MX_HD void fast_steps(const MxRefDev& R, const MxTileDev& T, Px& p, Counters& c,
         if (int_rep) stop |= (int)(p.m == p.sm) | (int)(p.iter - p.save_iter2 >= p.twin2r);
         if (can_bla) {
             // dz^2 = S^2 |w|^2 has exponent S2.e + e(|w|^2) or one less
-            const double w2 = p.wr * p.wr + p.wi * p.wi;
             stop |= (int)(p.m >= 2) & (int)(((p.rm - 1) & amask) == 0) & ((int)(p.S2.e + fe_from(w2).e <= er + 1) | (int)(w2 == 0.0));
         }
         if (!p.xv) {
@@ -518,8 +523,12 @@ MX_HD void fast_steps(const MxRefDev& R, const MxTileDev& T, Px& p, Counters& c,
             p.Xn = R.zf[p.rm + 1];
             p.xv = 1;
         }
-        // the next step's entry; when rm + 2 is past the end this step stops anyway
-        const double2 Xpf = R.zf[min(p.rm + 2, reflen - 1)];
+        // the next step's entry (0 past the end, where this step stops anyway)
+#ifdef __HIP_DEVICE_COMPILE__
+        const double2 Xpf = __builtin_bit_cast(double2, __builtin_amdgcn_raw_buffer_load_b128(zrs, (p.rm + 2) * 16, 0, 0));
+#else
+        const double2 Xpf = p.rm + 2 < reflen ? R.zf[p.rm + 2] : make_double2(0.0, 0.0);
+#endif
         const double2 X = p.Xc;
         double nwr, nwi;
         step_w(p, X, nwr, nwi);
@@ -549,25 +558,30 @@ MX_HD void fast_steps(const MxRefDev& R, const MxTileDev& T, Px& p, Counters& c,
             stop |= chk & (int)(pr * pr + pi * pi < T.interior_eps2 * zrad);
             save = chk & (int)(it1 - p.save_iter >= p.period_win);
         }
-        if (stop) break;
-        p.wr = nwr;
-        p.wi = nwi;
-        ++p.m;
-        ++p.rm;
-        p.iter = it1;
+        // commit by selects, so that the entry loaded above is consumed only by the
+        // next step and its load overlaps this one
+        const bool go = !stop;
+        p.wr = go ? nwr : p.wr;
+        p.wi = go ? nwi : p.wi;
+        p.m += go;
+        p.rm += go;
+        p.iter = go ? it1 : p.iter;
         if (EDE) {
-            p.jr = ja;
-            p.ji = jb;
+            p.jr = go ? ja : p.jr;
+            p.ji = go ? jb : p.ji;
         }
-        p.Xc = Xn;
-        p.Xn = Xpf;
-        if (save) {
-            p.zsr = zr;
-            p.zsi = zi;
-            p.save_iter = it1;
-            p.period_win += p.period_win;
-        }
-        ++n;
+        p.Xc.x = go ? Xn.x : p.Xc.x;
+        p.Xc.y = go ? Xn.y : p.Xc.y;
+        p.Xn.x = go ? Xpf.x : p.Xn.x;
+        p.Xn.y = go ? Xpf.y : p.Xn.y;
+        const bool sv = go && save;
+        p.zsr = sv ? zr : p.zsr;
+        p.zsi = sv ? zi : p.zsi;
+        p.save_iter = sv ? it1 : p.save_iter;
+        p.period_win = sv ? p.period_win + p.period_win : p.period_win;
+        w2 = w2n;
+        n += go;
+        if (stop) break;
     }
     c.steps += n;
 }
