@@ -40,6 +40,14 @@ struct Counters {
     unsigned long long steps, applies, skipped, rebases;
 };
 
+// minimum waves per SIMD requested from the register allocator (0: its own choice);
+// the few spills this costs sit on rare paths and the added occupancy hides latency
+#ifndef MX_FE64_WAVES
+#define MX_FE64_WAVES 3
+#endif
+#define MX_WAVES_ATTR(n) MX_WAVES_ATTR_(n)
+#define MX_WAVES_ATTR_(n) __attribute__((amdgpu_waves_per_eu((n) > 0 ? (n) : 1)))
+
 // lean steps per trip (after each full step)
 #ifndef MX_FAST_STEPS
 #define MX_FAST_STEPS 24
@@ -604,7 +612,7 @@ MX_HD bool item_to_sample(const MxTileDev& T, unsigned item, int& si, int& sj) {
 }
 
 template <bool EDE, bool INTERIOR>
-__global__ __launch_bounds__(256) void mx_fe64_kernel(MxRefDev R, const int2* __restrict__ lev, MxTileDev T) {
+__global__ __launch_bounds__(256) MX_WAVES_ATTR(MX_FE64_WAVES) void mx_fe64_kernel(MxRefDev R, const int2* __restrict__ lev, MxTileDev T) {
     const int lane = __lane_id();
     const unsigned long long lanemask_lt = (1ull << lane) - 1ull;
     int q = blockIdx.x % MX_NQUEUE;
