@@ -12,6 +12,7 @@
 #include <vector>
 
 #include "mx_fe.h"
+#include "mx_fixed.h"
 #include "mx_internal.h"
 
 namespace mx {
@@ -45,8 +46,16 @@ struct Gmp {
 };
 
 // Reference orbit: createRef + calCoefficient (mandel_perturbation.cpp:4014-4187,
-// 4189-4249) with the mpf Karatsuba square re = (a+b)(a-b), im = 2ab.
+// 4189-4249) with the Karatsuba square re = (a+b)(a-b), im = 2ab: in mpf on the
+// plain-double path, in fixed point on the floatexp path (the engine's choice there,
+// 4084-4111, 4197-4227), where the shadows come from the fixed-point value.
 void Frame::build_orbit(mpf_srcptr cre, mpf_srcptr cim) {
+    // one integer limb: the fixed-point form needs |z|^2 + |c| well below 2^64
+    // (bigFixedRangeSafe, 4093-4095)
+    if (!dbl && escape_radius <= 0x1p31) {
+        build_orbit_fixed(cre, cim);
+        return;
+    }
     const int mxit = max_iter;
     zf.assign(2 * (size_t)(mxit + 1), 0.0);
     zfe_m.assign(2 * (size_t)(mxit + 1), 0.0);
@@ -119,6 +128,71 @@ void Frame::build_orbit(mpf_srcptr cre, mpf_srcptr cim) {
         dfr.resize((size_t)(ref_it + 1));
         dfi.resize((size_t)(ref_it + 1));
     }
+}
+
+void Frame::build_orbit_fixed(mpf_srcptr cre, mpf_srcptr cim) {
+    const int mxit = max_iter;
+    zf.assign(2 * (size_t)(mxit + 1), 0.0);
+    zfe_m.assign(2 * (size_t)(mxit + 1), 0.0);
+    zfe_e.assign(2 * (size_t)(mxit + 1), 0);
+    z_m3.assign((size_t)(mxit + 1), 0.0);
+    mpf_set(ref_re.v, cre);
+    mpf_set(ref_im.v, cim);
+    // L = precision/64 + 2 limbs: fraction limbs for the precision, the integer limb
+    // and a guard limb (createRef, mandel_perturbation.cpp:4099-4101)
+    const int L = ((int)mpf_get_prec(ref_re.v) + 63) / 64 + 2;
+    FixedArith fx(L, 9);
+    Fixed cr, ci, zr, zi, t1, t2, ab, re, im;
+    for (Fixed* v : {&cr, &ci, &zr, &zi, &t1, &t2, &ab, &re, &im}) fx.make(*v);
+    fx.from_mpf(cr, cre);
+    fx.from_mpf(ci, cim);
+    fx.copy(zr, cr);
+    fx.copy(zi, ci);
+    // entry 0 from the mpf centre itself (4113-4119)
+    {
+        double m;
+        long e;
+        mpf_shadow(cre, m, e);
+        zfe_m[0] = m;
+        zfe_e[0] = (int32_t)e;
+        zf[0] = m == 0.0 ? 0.0 : ldexp(m, (int)e);
+        mpf_shadow(cim, m, e);
+        zfe_m[1] = m;
+        zfe_e[1] = (int32_t)e;
+        zf[1] = m == 0.0 ? 0.0 : ldexp(m, (int)e);
+    }
+    ref_escaped = false;
+    ref_it = mxit;
+    const double R = (double)escape_radius;
+    for (int i = 1; i <= mxit; ++i) {
+        fx.add(t1, zr, zi);
+        fx.sub(t2, zr, zi);
+        fx.mul(ab, zr, zi);
+        fx.mul(re, t1, t2);
+        fx.add(im, ab, ab);
+        fx.add(zr, re, cr);
+        fx.add(zi, im, ci);
+        const Fe fr = fx.to_fe(zr), fi = fx.to_fe(zi);
+        zfe_m[2 * (size_t)i] = fr.m;
+        zfe_e[2 * (size_t)i] = fr.e;
+        zfe_m[2 * (size_t)i + 1] = fi.m;
+        zfe_e[2 * (size_t)i + 1] = fi.e;
+        const double dr = fe_to_double(fr), di = fe_to_double(fi);
+        zf[2 * (size_t)i] = dr;
+        zf[2 * (size_t)i + 1] = di;
+        z_m3[i] = (dr * dr + di * di) / 1000000;
+        if (std::hypot(dr, di) > R) {
+            ref_escaped = true;
+            ref_it = i;
+            break;
+        }
+    }
+    fx.to_mpf(tail_re.v, zr);
+    fx.to_mpf(tail_im.v, zi);
+    zf.resize(2 * (size_t)(ref_it + 1));
+    zfe_m.resize(2 * (size_t)(ref_it + 1));
+    zfe_e.resize(2 * (size_t)(ref_it + 1));
+    z_m3.resize((size_t)(ref_it + 1));
 }
 
 // get_exp (float_math.cpp:23-29): floor(log2|x|), zero -> -2^30

@@ -85,6 +85,7 @@ struct Frame {
   private:
     void init_mpf();
     void build_orbit(mpf_srcptr cre, mpf_srcptr cim);
+    void build_orbit_fixed(mpf_srcptr cre, mpf_srcptr cim);
     void ref_pixel();
     void sa_step(int i);
     int prepare_dbl(mpf_srcptr scale, std::string& err);

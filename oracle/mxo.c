@@ -73,20 +73,77 @@ static double mpfr_to_ld_trunc(mpfr_srcptr x) {
 /* frame grid (MP.cpp:1403-1424)                                              */
 /* ------------------------------------------------------------------------- */
 
+/* The frame-level grid (mpf mode) follows the engine's own GMP mpf arithmetic
+ * operation for operation (MP.cpp:1403-1424 with mpf_set_str, mpf_div, ...), so that
+ * the grid and reference points equal the engine's to the last bit; mpf truncates
+ * where MPFR would round, and long decimal centres convert differently.  Every mpf
+ * result (at most prec+1 limbs) is then held exactly in MPFR at mpf_bits(prec).  The
+ * brute force keeps an MPFR grid at its own working precision. */
 typedef struct {
     mpfr_t c0_re, c0_im, dx, dy;
     mpfr_prec_t prec;
+    int as_mpf;
+    mpf_t fc0_re, fc0_im, fdx, fdy;
 } mxo_grid;
 
-static void grid_init(mxo_grid* g, const char* cre, const char* cim, const char* scale, int W, int H, int full_h, int row_base, mpfr_prec_t prec) {
+/* mpf precision (bits argument of mpf_init2) holding the same limbs as an engine mpf
+ * that mpf_bits() describes */
+static mp_bitcnt_t mpf_prec_of(mpfr_prec_t pb) { return (mp_bitcnt_t)pb - 128; }
+
+static void grid_init(mxo_grid* g, const char* cre, const char* cim, const char* scale, int W, int H, int full_h, int row_base, mpfr_prec_t prec, int as_mpf) {
     g->prec = prec;
+    g->as_mpf = as_mpf;
     mpfr_inits2(prec, g->c0_re, g->c0_im, g->dx, g->dy, (mpfr_ptr)0);
+    const int rows = full_h > 0 ? full_h : H;
+    if (as_mpf) {
+        const mp_bitcnt_t fp = mpf_prec_of(prec);
+        mpf_t c_re, c_im, sc, dw, dh;
+        mpf_init2(c_re, fp);
+        mpf_init2(c_im, fp);
+        mpf_init2(sc, fp);
+        mpf_init2(dw, fp);
+        mpf_init2(dh, fp);
+        mpf_init2(g->fc0_re, fp);
+        mpf_init2(g->fc0_im, fp);
+        mpf_init2(g->fdx, fp);
+        mpf_init2(g->fdy, fp);
+        mpf_set_str(c_re, cre, 10);
+        mpf_set_str(c_im, cim, 10);
+        mpf_set_str(sc, scale, 10);
+        mpf_set_ui(dw, 2);
+        mpf_div(dw, dw, sc);
+        mpf_set(dh, dw);
+        mpf_div_ui(dh, dh, W);
+        mpf_mul_ui(dh, dh, rows);
+        mpf_sub(g->fc0_re, c_re, dw);
+        mpf_sub(g->fc0_im, c_im, dh);
+        mpf_div_ui(g->fdx, dw, W - 1);
+        mpf_mul_ui(g->fdx, g->fdx, 2);
+        mpf_div_ui(g->fdy, dh, rows - 1);
+        mpf_mul_ui(g->fdy, g->fdy, 2);
+        if (row_base > 0) {
+            mpf_t off;
+            mpf_init2(off, fp);
+            mpf_mul_ui(off, g->fdy, row_base);
+            mpf_add(g->fc0_im, g->fc0_im, off);
+            mpf_clear(off);
+        }
+        mpfr_set_f(g->c0_re, g->fc0_re, MPFR_RNDN);
+        mpfr_set_f(g->c0_im, g->fc0_im, MPFR_RNDN);
+        mpfr_set_f(g->dx, g->fdx, MPFR_RNDN);
+        mpfr_set_f(g->dy, g->fdy, MPFR_RNDN);
+        mpf_clear(c_re);
+        mpf_clear(c_im);
+        mpf_clear(sc);
+        mpf_clear(dw);
+        mpf_clear(dh);
+        return;
+    }
     mpfr_t c_re, c_im, sc, dw, dh;
     mpfr_inits2(prec, c_re, c_im, sc, dw, dh, (mpfr_ptr)0);
     mpfr_set_str(c_re, cre, 10, MPFR_RNDN);
     mpfr_set_str(c_im, cim, 10, MPFR_RNDN);
     mpfr_set_str(sc, scale, 10, MPFR_RNDN);
-    int rows = full_h > 0 ? full_h : H;
     mpfr_ui_div(dw, 2, sc, MPFR_RNDN);   /* dw = 2/scale */
     mpfr_div_ui(dh, dw, W, MPFR_RNDN);   /* dh = dw/W*rows */
     mpfr_mul_ui(dh, dh, rows, MPFR_RNDN);
@@ -106,10 +163,35 @@ static void grid_init(mxo_grid* g, const char* cre, const char* cim, const char*
     mpfr_clears(c_re, c_im, sc, dw, dh, (mpfr_ptr)0);
 }
 
-static void grid_clear(mxo_grid* g) { mpfr_clears(g->c0_re, g->c0_im, g->dx, g->dy, (mpfr_ptr)0); }
+static void grid_clear(mxo_grid* g) {
+    mpfr_clears(g->c0_re, g->c0_im, g->dx, g->dy, (mpfr_ptr)0);
+    if (g->as_mpf) {
+        mpf_clear(g->fc0_re);
+        mpf_clear(g->fc0_im);
+        mpf_clear(g->fdx);
+        mpf_clear(g->fdy);
+    }
+}
 
-/* c of pixel (row i, col j) with sub-sample offsets (u,v); MP.cpp:896-915 */
+/* c of pixel (row i, col j) with sub-sample offsets (u,v); MP.cpp:896-915 (mpf mode:
+ * the reference pixel c0 + d*k of setPixelReference / createRef, MP.cpp:4020-4040) */
 static void grid_pixel_c(const mxo_grid* g, int i, int j, int u, int v, int sub, mpfr_ptr cre, mpfr_ptr cim) {
+    if (g->as_mpf && u == 0 && v == 0) {
+        mpf_t t, a, b;
+        mpf_init2(t, mpf_prec_of(g->prec));
+        mpf_init2(a, mpf_prec_of(g->prec));
+        mpf_init2(b, mpf_prec_of(g->prec));
+        mpf_mul_ui(t, g->fdx, j);
+        mpf_add(a, g->fc0_re, t);
+        mpf_mul_ui(t, g->fdy, i);
+        mpf_add(b, g->fc0_im, t);
+        mpfr_set_f(cre, a, MPFR_RNDN);
+        mpfr_set_f(cim, b, MPFR_RNDN);
+        mpf_clear(t);
+        mpf_clear(a);
+        mpf_clear(b);
+        return;
+    }
     mpfr_t t;
     mpfr_init2(t, g->prec);
     mpfr_mul_ui(t, g->dx, j, MPFR_RNDN);
@@ -127,6 +209,24 @@ static void grid_pixel_c(const mxo_grid* g, int i, int j, int u, int v, int sub,
 
 /* exact geometric centre c0 + d*(n-1)/2 (MP.cpp:4024-4036) */
 static void grid_view_center(const mxo_grid* g, int W, int H, mpfr_ptr cre, mpfr_ptr cim) {
+    if (g->as_mpf) {
+        mpf_t t, a, b;
+        mpf_init2(t, mpf_prec_of(g->prec));
+        mpf_init2(a, mpf_prec_of(g->prec));
+        mpf_init2(b, mpf_prec_of(g->prec));
+        mpf_mul_ui(t, g->fdx, W - 1);
+        mpf_div_ui(t, t, 2);
+        mpf_add(a, g->fc0_re, t);
+        mpf_mul_ui(t, g->fdy, H - 1);
+        mpf_div_ui(t, t, 2);
+        mpf_add(b, g->fc0_im, t);
+        mpfr_set_f(cre, a, MPFR_RNDN);
+        mpfr_set_f(cim, b, MPFR_RNDN);
+        mpf_clear(t);
+        mpf_clear(a);
+        mpf_clear(b);
+        return;
+    }
     mpfr_t t;
     mpfr_init2(t, g->prec);
     mpfr_mul_ui(t, g->dx, W - 1, MPFR_RNDN);
@@ -157,11 +257,80 @@ typedef struct {
     mpfr_prec_t prec;
 } mxo_orbit;
 
+/* Fixed-point orbit arithmetic of the floatexp path (src/engine/bigfixed.h; chosen by
+ * createRef at MP.cpp:4084-4111, stepped by calCoefficient at MP.cpp:4197-4227).
+ * Restated on signed integers: a value is the integer V standing for
+ * V * 2^(-64(L-1)).  Sums are exact.  A product is the exact product of the
+ * magnitudes minus the partial products a_i*b_j (limbs i, j) with i + j < L-4 that
+ * the engine's short product leaves out for 16 <= L < 72 (bigfixed.h:75-100), rounded
+ * half up at 2^(64(L-1)) (bigfixed.h:102-125).  The double shadow is the top nonzero
+ * limb times 2^64 plus the next limb, each converted to double, as a FloatExp scaled
+ * by 2^(64(top-L)) (bigfixed.h:305-319). */
+typedef struct {
+    int L;
+    mpz_t a, b, p, u, half;
+} mxo_fx;
+
+static void fx_init(mxo_fx* F, int L) {
+    F->L = L;
+    mpz_inits(F->a, F->b, F->p, F->u, F->half, (mpz_ptr)0);
+    mpz_setbit(F->half, (mp_bitcnt_t)64 * (L - 1) - 1);
+}
+
+static void fx_clear(mxo_fx* F) { mpz_clears(F->a, F->b, F->p, F->u, F->half, (mpz_ptr)0); }
+
+static void fx_mul(mxo_fx* F, mpz_ptr r, mpz_srcptr x, mpz_srcptr y) {
+    const int sg = mpz_sgn(x) * mpz_sgn(y), L = F->L;
+    if (sg == 0) {
+        mpz_set_ui(r, 0);
+        return;
+    }
+    mpz_abs(F->a, x);
+    mpz_abs(F->b, y);
+    mpz_mul(F->p, F->a, F->b);
+    if (L >= 16 && L < 72) {
+        const int lo = L - 4;
+        for (int i = 0; i < lo; ++i) {
+            const mp_limb_t ai = mpz_getlimbn(F->a, i);
+            if (!ai) continue;
+            mpz_tdiv_r_2exp(F->u, F->b, (mp_bitcnt_t)64 * (lo - i)); /* limbs j < lo - i of b */
+            mpz_mul_ui(F->u, F->u, ai);
+            mpz_mul_2exp(F->u, F->u, (mp_bitcnt_t)64 * i);
+            mpz_sub(F->p, F->p, F->u);
+        }
+    }
+    mpz_add(F->p, F->p, F->half);
+    mpz_fdiv_q_2exp(r, F->p, (mp_bitcnt_t)64 * (L - 1));
+    if (sg < 0) mpz_neg(r, r);
+}
+
+static mxo_fe fx_to_fe(mpz_srcptr v, int L) {
+    const size_t n = mpz_size(v);
+    if (n == 0) return mxo_fe_make(0.0, 0);
+    const int top = (int)n - 1;
+    const double hi = (double)mpz_getlimbn(v, top);
+    const double lo = top >= 1 ? (double)mpz_getlimbn(v, top - 1) : 0.0;
+    mxo_fe r = mxo_fe_from(hi * 18446744073709551616.0 + lo);
+    r.e += (int64_t)64 * (top - L);
+    if (mpz_sgn(v) < 0) r.m = -r.m;
+    return r;
+}
+
+/* truncation toward zero of x * 2^(64(L-1)) (bf_from_mpf, bigfixed.h:282-303) */
+static void fx_from_mpfr(mpz_ptr v, mpfr_srcptr x, int L) {
+    mpfr_t t;
+    mpfr_init2(t, mpfr_get_prec(x));
+    mpfr_mul_2ui(t, x, (unsigned long)64 * (L - 1), MPFR_RNDN); /* exact */
+    mpfr_get_z(v, t, MPFR_RNDZ);
+    mpfr_clear(t);
+}
+
 /* Serial full-precision orbit. Escape test on the double shadow with hypot, as
  * std::abs(Comp) > R (MP.cpp:2047-2049, 4272).  The derivative shadow follows the
  * floatexp branch of MP.cpp:4255-4263 when fe_deriv, else the double branch
- * MP.cpp:4265-4267. */
-static void orbit_build(mxo_orbit* o, mpfr_srcptr cre, mpfr_srcptr cim, mpfr_prec_t prec, int mxit, double escape_radius, int deriv, int fe_deriv) {
+ * MP.cpp:4265-4267.  fxL > 0 steps the orbit in the L-limb fixed point above (the
+ * floatexp path's form); fxL == 0 in MPFR at `prec` bits, mpf's Karatsuba square. */
+static void orbit_build(mxo_orbit* o, mpfr_srcptr cre, mpfr_srcptr cim, mpfr_prec_t prec, int mxit, double escape_radius, int deriv, int fe_deriv, int fxL) {
     o->prec = prec;
     o->zf = (double*)malloc(sizeof(double) * 2 * (size_t)(mxit + 1));
     o->zre_fe = (mxo_fe*)malloc(sizeof(mxo_fe) * (size_t)(mxit + 1));
@@ -176,6 +345,16 @@ static void orbit_build(mxo_orbit* o, mpfr_srcptr cre, mpfr_srcptr cim, mpfr_pre
     mpfr_inits2(prec, zr, zi, t1, t2, nzr, nzi, (mpfr_ptr)0);
     mpfr_set(zr, cre, MPFR_RNDN);
     mpfr_set(zi, cim, MPFR_RNDN);
+    mxo_fx F;
+    mpz_t vcr, vci, vzr, vzi, vs, vd, vp, vq;
+    if (fxL > 0) {
+        fx_init(&F, fxL);
+        mpz_inits(vcr, vci, vzr, vzi, vs, vd, vp, vq, (mpz_ptr)0);
+        fx_from_mpfr(vcr, cre, fxL);
+        fx_from_mpfr(vci, cim, fxL);
+        mpz_set(vzr, vcr);
+        mpz_set(vzi, vci);
+    }
     o->zf[0] = mpfr_to_ld_trunc(zr);
     o->zf[1] = mpfr_to_ld_trunc(zi);
     o->zre_fe[0] = mpfr_to_fe_trunc(zr);
@@ -188,18 +367,35 @@ static void orbit_build(mxo_orbit* o, mpfr_srcptr cre, mpfr_srcptr cim, mpfr_pre
     o->ref_it = mxit;
     int i;
     for (i = 1; i <= mxit; ++i) {
-        mpfr_add(t1, zr, zi, MPFR_RNDN); /* a+b */
-        mpfr_sub(t2, zr, zi, MPFR_RNDN); /* a-b */
-        mpfr_mul(nzi, zr, zi, MPFR_RNDN);
-        mpfr_mul(nzr, t1, t2, MPFR_RNDN);
-        mpfr_mul_2ui(nzi, nzi, 1, MPFR_RNDN);
-        mpfr_add(zr, nzr, cre, MPFR_RNDN);
-        mpfr_add(zi, nzi, cim, MPFR_RNDN);
-        double dr = mpfr_to_ld_trunc(zr), di = mpfr_to_ld_trunc(zi);
+        double dr, di;
+        if (fxL > 0) {
+            /* re = (a+b)(a-b) + c_re, im = 2ab + c_im (MP.cpp:4203-4209) */
+            mpz_add(vs, vzr, vzi);
+            mpz_sub(vd, vzr, vzi);
+            fx_mul(&F, vp, vzr, vzi);
+            fx_mul(&F, vq, vs, vd);
+            mpz_add(vzr, vq, vcr);
+            mpz_mul_2exp(vp, vp, 1);
+            mpz_add(vzi, vp, vci);
+            o->zre_fe[i] = fx_to_fe(vzr, fxL);
+            o->zim_fe[i] = fx_to_fe(vzi, fxL);
+            dr = mxo_fe_to_double(o->zre_fe[i]);
+            di = mxo_fe_to_double(o->zim_fe[i]);
+        } else {
+            mpfr_add(t1, zr, zi, MPFR_RNDN); /* a+b */
+            mpfr_sub(t2, zr, zi, MPFR_RNDN); /* a-b */
+            mpfr_mul(nzi, zr, zi, MPFR_RNDN);
+            mpfr_mul(nzr, t1, t2, MPFR_RNDN);
+            mpfr_mul_2ui(nzi, nzi, 1, MPFR_RNDN);
+            mpfr_add(zr, nzr, cre, MPFR_RNDN);
+            mpfr_add(zi, nzi, cim, MPFR_RNDN);
+            dr = mpfr_to_ld_trunc(zr);
+            di = mpfr_to_ld_trunc(zi);
+            o->zre_fe[i] = mpfr_to_fe_trunc(zr);
+            o->zim_fe[i] = mpfr_to_fe_trunc(zi);
+        }
         o->zf[2 * i] = dr;
         o->zf[2 * i + 1] = di;
-        o->zre_fe[i] = mpfr_to_fe_trunc(zr);
-        o->zim_fe[i] = mpfr_to_fe_trunc(zi);
         o->z_m3[i] = (dr * dr + di * di) / 1000000;
         if (deriv) {
             if (fe_deriv) {
@@ -223,6 +419,16 @@ static void orbit_build(mxo_orbit* o, mpfr_srcptr cre, mpfr_srcptr cim, mpfr_pre
         }
     }
     o->len = o->ref_it + 1;
+    if (fxL > 0) {
+        /* the fixed-point tail back in floating point (bf_to_mpf, bigfixed.h:268-279):
+         * exact, the value has at most 64(L-1)+64 significant bits */
+        mpfr_set_z(zr, vzr, MPFR_RNDN);
+        mpfr_div_2ui(zr, zr, (unsigned long)64 * (fxL - 1), MPFR_RNDN);
+        mpfr_set_z(zi, vzi, MPFR_RNDN);
+        mpfr_div_2ui(zi, zi, (unsigned long)64 * (fxL - 1), MPFR_RNDN);
+        mpz_clears(vcr, vci, vzr, vzi, vs, vd, vp, vq, (mpz_ptr)0);
+        fx_clear(&F);
+    }
     mpfr_set(o->last_re, zr, MPFR_RNDN);
     mpfr_set(o->last_im, zi, MPFR_RNDN);
     mpfr_clears(zr, zi, t1, t2, nzr, nzi, (mpfr_ptr)0);
@@ -908,6 +1114,11 @@ typedef struct {
     int cutoff_sensitive_hint;
     double seconds_pixels;
     long long steps, bla_applies, bla_skipped, rebases;
+    /* optional: the reference orbit's shadows, (re, im) interleaved, room for mxit+1
+     * entries each (NULL: not wanted) */
+    double* orbit_zf;
+    double* orbit_m;
+    int64_t* orbit_e;
 } mxo_frame;
 
 static int bla_minlevel_from_skip(int ms) {
@@ -963,7 +1174,7 @@ MXO_EXPORT int mxo_compute_fe(mxo_frame* f) {
     const double esc2 = (double)(f->escape_radius * f->escape_radius);
     const double escR = (double)f->escape_radius;
     mxo_grid g;
-    grid_init(&g, f->cre, f->cim, f->scale, W, H, f->full_h, f->row_base, pb);
+    grid_init(&g, f->cre, f->cim, f->scale, W, H, f->full_h, f->row_base, pb, 1);
     mxo_fe dxfe = mpfr_to_fe_trunc(g.dx), dyfe = mpfr_to_fe_trunc(g.dy);
     const int ede = (f->c_method & 2) != 0;
 
@@ -984,8 +1195,11 @@ MXO_EXPORT int mxo_compute_fe(mxo_frame* f) {
         grid_view_center(&g, W, H, rre, rim);
     else
         grid_pixel_c(&g, H / 2, W / 2, 0, 0, 1, rre, rim);
+    /* one integer limb: the fixed-point orbit needs |z|^2 + |c| well below 2^64
+     * (bigFixedRangeSafe, MP.cpp:4093-4095); L = 64-bit limbs of the mpf centre + 2 */
+    const int fxL = f->escape_radius <= 0x1p31f ? pb / 64 : 0;
     mxo_orbit o;
-    orbit_build(&o, rre, rim, pb, f->mxit, escR, ede, 1);
+    orbit_build(&o, rre, rim, pb, f->mxit, escR, ede, 1, fxL);
     int ref_virtual = 0;
     double ref_x = W / 2, ref_y = H / 2;
     f->cutoff_sensitive_hint = 0;
@@ -1000,7 +1214,7 @@ MXO_EXPORT int mxo_compute_fe(mxo_frame* f) {
         if (sensitive) {
             orbit_free(&o);
             grid_view_center(&g, W, H, rre, rim);
-            orbit_build(&o, rre, rim, pb, f->mxit, escR, ede, 1);
+            orbit_build(&o, rre, rim, pb, f->mxit, escR, ede, 1, fxL);
             ref_virtual = 1;
             ref_x = (W - 1) * 0.5;
             ref_y = (H - 1) * 0.5;
@@ -1078,6 +1292,15 @@ MXO_EXPORT int mxo_compute_fe(mxo_frame* f) {
     f->bla_applies = tot.bla_applies;
     f->bla_skipped = tot.bla_skipped;
     f->rebases = tot.rebases;
+    if (f->orbit_zf)
+        for (int k = 0; k < o.len; ++k) {
+            f->orbit_zf[2 * k] = o.zf[2 * k];
+            f->orbit_zf[2 * k + 1] = o.zf[2 * k + 1];
+            f->orbit_m[2 * k] = o.zre_fe[k].m;
+            f->orbit_e[2 * k] = o.zre_fe[k].e;
+            f->orbit_m[2 * k + 1] = o.zim_fe[k].m;
+            f->orbit_e[2 * k + 1] = o.zim_fe[k].e;
+        }
 
     free(idx);
     free(dcol);
@@ -1114,14 +1337,16 @@ MXO_EXPORT void mxo_fe_op(int op, int n, const double* am, const int64_t* ae, co
     }
 }
 
-/* Reference orbit shadows for a point given as decimal strings. Returns ref_it. */
-MXO_EXPORT int mxo_orbit_shadows(const char* cre, const char* cim, int prec_bits, int mxit, double escape_radius, int deriv, double* zf, double* zre_m, int64_t* zre_e, double* zim_m, int64_t* zim_e, double* dfr, double* dfi) {
+/* Reference orbit shadows for a point given as decimal strings, in MPFR at prec_bits
+ * (fx_limbs == 0) or in the fx_limbs-limb fixed point from the point held at
+ * prec_bits. Returns ref_it. */
+MXO_EXPORT int mxo_orbit_shadows(const char* cre, const char* cim, int prec_bits, int fx_limbs, int mxit, double escape_radius, int deriv, double* zf, double* zre_m, int64_t* zre_e, double* zim_m, int64_t* zim_e, double* dfr, double* dfi) {
     mpfr_t a, b;
     mpfr_inits2(prec_bits, a, b, (mpfr_ptr)0);
     mpfr_set_str(a, cre, 10, MPFR_RNDN);
     mpfr_set_str(b, cim, 10, MPFR_RNDN);
     mxo_orbit o;
-    orbit_build(&o, a, b, prec_bits, mxit, escape_radius, deriv, 1);
+    orbit_build(&o, a, b, prec_bits, mxit, escape_radius, deriv, 1, fx_limbs);
     for (int k = 0; k < o.len; ++k) {
         zf[2 * k] = o.zf[2 * k];
         zf[2 * k + 1] = o.zf[2 * k + 1];
@@ -1140,6 +1365,42 @@ MXO_EXPORT int mxo_orbit_shadows(const char* cre, const char* cim, int prec_bits
     orbit_free(&o);
     mpfr_clears(a, b, (mpfr_ptr)0);
     return r;
+}
+
+/* One fixed-point product and one shadow on L-limb magnitudes (least significant limb
+ * first) with separate signs, for pinning the arithmetic against exact integers. */
+MXO_EXPORT void mxo_fixed_mul(int L, const uint64_t* a, int sa, const uint64_t* b, int sb, uint64_t* r, int* sr) {
+    mxo_fx F;
+    fx_init(&F, L);
+    mpz_t x, y, z;
+    mpz_inits(x, y, z, (mpz_ptr)0);
+    mpz_import(x, (size_t)L, -1, sizeof(uint64_t), 0, 0, a);
+    mpz_import(y, (size_t)L, -1, sizeof(uint64_t), 0, 0, b);
+    if (sa < 0) mpz_neg(x, x);
+    if (sa == 0) mpz_set_ui(x, 0);
+    if (sb < 0) mpz_neg(y, y);
+    if (sb == 0) mpz_set_ui(y, 0);
+    fx_mul(&F, z, x, y);
+    memset(r, 0, sizeof(uint64_t) * (size_t)L);
+    *sr = mpz_sgn(z);
+    /* the engine keeps L limbs; the column above them is zero for |result| < 2^64 */
+    mpz_abs(z, z);
+    mpz_tdiv_r_2exp(z, z, (mp_bitcnt_t)64 * L);
+    mpz_export(r, NULL, -1, sizeof(uint64_t), 0, 0, z);
+    mpz_clears(x, y, z, (mpz_ptr)0);
+    fx_clear(&F);
+}
+
+MXO_EXPORT void mxo_fixed_to_fe(int L, const uint64_t* a, int sa, double* m, int64_t* e) {
+    mpz_t x;
+    mpz_init(x);
+    mpz_import(x, (size_t)L, -1, sizeof(uint64_t), 0, 0, a);
+    if (sa < 0) mpz_neg(x, x);
+    if (sa == 0) mpz_set_ui(x, 0);
+    const mxo_fe f = fx_to_fe(x, L);
+    *m = f.m;
+    *e = f.e;
+    mpz_clear(x);
 }
 
 /* BLA table from a double orbit shadow; writes r2/l/coef (caller sizes them with
@@ -1261,7 +1522,7 @@ MXO_EXPORT void mxo_pixels_fe(const mxo_pixels_in* in, int n, const double* dcr_
  * (step x step) grid; others untouched.  wp_bits = working precision. */
 MXO_EXPORT void mxo_direct(const char* cre, const char* cim, const char* scale, int W, int H, int prec_bits, int wp_bits, int mxit, int c_method, float escape_radius, int step, int threads, int track_d, double* val, int32_t* nesc) {
     mxo_grid g;
-    grid_init(&g, cre, cim, scale, W, H, 0, 0, prec_bits > wp_bits ? prec_bits : wp_bits);
+    grid_init(&g, cre, cim, scale, W, H, 0, 0, prec_bits > wp_bits ? prec_bits : wp_bits, 0);
     const double esc2 = (double)(escape_radius * escape_radius);
     int ny = (H + step - 1) / step, nx = (W + step - 1) / step;
 #pragma omp parallel for schedule(dynamic, 1) num_threads(threads > 0 ? threads : 1)
@@ -1294,7 +1555,7 @@ MXO_EXPORT double mxo_point(const char* cre, const char* cim, int wp_bits, int m
 MXO_EXPORT void mxo_grid_info(const char* cre, const char* cim, const char* scale, int W, int H, int full_h, int row_base, int prec_bits, double* dx_m, int64_t* dx_e, double* dy_m, int64_t* dy_e) {
     prec_bits = mpf_bits(prec_bits);
     mxo_grid g;
-    grid_init(&g, cre, cim, scale, W, H, full_h, row_base, prec_bits);
+    grid_init(&g, cre, cim, scale, W, H, full_h, row_base, prec_bits, 1);
     mxo_fe a = mpfr_to_fe_trunc(g.dx), b = mpfr_to_fe_trunc(g.dy);
     *dx_m = a.m;
     *dx_e = a.e;
@@ -1636,7 +1897,7 @@ static double pixel_double(const mxo_dctx* c, mxo_c dc, mxo_c dz0, mxo_c dd0, in
  * calCoefficient, MP.cpp:4014-4187, 4189-4305).  sa keeps `order` from the previous
  * reference, as the engine's _SA_order does. */
 static void orbit_build_sa(mxo_orbit* o, mxo_sa* sa, mpfr_srcptr cre, mpfr_srcptr cim, mpfr_prec_t prec, int mxit, double escape_radius, mxo_c delta, int ede) {
-    orbit_build(o, cre, cim, prec, mxit, escape_radius, ede, 0);
+    orbit_build(o, cre, cim, prec, mxit, escape_radius, ede, 0, 0);
     for (int i = 0; i < MXO_SA_N; ++i) {
         sa->A[i].r = sa->A[i].i = 0.0;
         sa->B[i].r = sa->B[i].i = 0.0;
@@ -1679,7 +1940,7 @@ MXO_EXPORT int mxo_compute_dbl(mxo_dframe* f) {
     const double esc2 = (double)(f->escape_radius * f->escape_radius);
     const double escR = (double)f->escape_radius;
     mxo_grid g;
-    grid_init(&g, f->cre, f->cim, f->scale, W, H, 0, 0, pb);
+    grid_init(&g, f->cre, f->cim, f->scale, W, H, 0, 0, pb, 1);
     mpfr_t t;
     mpfr_init2(t, pb);
     mxo_c delta;
@@ -2031,7 +2292,7 @@ MXO_EXPORT void mxo_pixels_dbl(const mxo_pixels_in* in, const double* z_m3, cons
 MXO_EXPORT int mxo_dbl_setup(const char* cre, const char* cim, const char* scale, int W, int H, int mxit, float escape_radius, int prec_bits, double* zf, double* z_m3, double* A, double* delta, int* sa_it, int* sa_order, double* dcol, double* drow, int ede, double* dfr, double* dfi, double* B) {
     prec_bits = mpf_bits(prec_bits);
     mxo_grid g;
-    grid_init(&g, cre, cim, scale, W, H, 0, 0, prec_bits);
+    grid_init(&g, cre, cim, scale, W, H, 0, 0, prec_bits, 1);
     mpfr_t t, rre, rim;
     mpfr_inits2(prec_bits, t, rre, rim, (mpfr_ptr)0);
     mxo_c d;

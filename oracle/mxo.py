@@ -24,6 +24,7 @@ class Frame(C.Structure):
         ("cutoff_sensitive_hint", C.c_int), ("seconds_pixels", C.c_double),
         ("steps", C.c_longlong), ("bla_applies", C.c_longlong), ("bla_skipped", C.c_longlong),
         ("rebases", C.c_longlong),
+        ("orbit_zf", C.c_void_p), ("orbit_m", C.c_void_p), ("orbit_e", C.c_void_p),
     ]
 
 
@@ -71,8 +72,10 @@ def lib():
             [C.c_void_p] * 4 + [C.POINTER(C.c_int), C.POINTER(C.c_int), C.c_void_p, C.c_void_p, C.c_int] + [C.c_void_p] * 3
         L.mxo_dbl_setup.restype = C.c_int
         L.mxo_fe_op.argtypes = [C.c_int, C.c_int] + [C.c_void_p] * 6
-        L.mxo_orbit_shadows.argtypes = [C.c_char_p, C.c_char_p, C.c_int, C.c_int, C.c_double, C.c_int] + [C.c_void_p] * 7
+        L.mxo_orbit_shadows.argtypes = [C.c_char_p, C.c_char_p, C.c_int, C.c_int, C.c_int, C.c_double, C.c_int] + [C.c_void_p] * 7
         L.mxo_orbit_shadows.restype = C.c_int
+        L.mxo_fixed_mul.argtypes = [C.c_int, C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_void_p, C.POINTER(C.c_int)]
+        L.mxo_fixed_to_fe.argtypes = [C.c_int, C.c_void_p, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_int64)]
         L.mxo_bla_total.argtypes = [C.c_int]
         L.mxo_bla_total.restype = C.c_int64
         L.mxo_bla_tables.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_double, C.c_double, C.c_int] + [C.c_void_p] * 7
@@ -96,9 +99,17 @@ def precision_bits(scale_str):
 
 
 def compute_fe(cre, cim, scale, W, H, mxit, c_method=0, escape_radius=1e8, prec_bits=None,
-               use_bla=True, bla_minskip=8, interior=True, probe=True, threads=None, full_h=0, row_base=0):
-    """Whole floatexp frame (Compute -> reference -> BLA -> probe -> pass)."""
+               use_bla=True, bla_minskip=8, interior=True, probe=True, threads=None, full_h=0, row_base=0,
+               want_orbit=False):
+    """Whole floatexp frame (Compute -> reference -> BLA -> probe -> pass).  With
+    want_orbit, info["orbit"] holds the reference orbit's shadows (zf, zfe_m, zfe_e,
+    (re, im) interleaved)."""
     f = Frame()
+    if want_orbit:
+        ozf = np.zeros(2 * (mxit + 1))
+        om = np.zeros(2 * (mxit + 1))
+        oe = np.zeros(2 * (mxit + 1), np.int64)
+        f.orbit_zf, f.orbit_m, f.orbit_e = _p(ozf), _p(om), _p(oe)
     f.cre, f.cim, f.scale = cre.encode(), cim.encode(), scale.encode()
     f.W, f.H, f.full_h, f.row_base = W, H, full_h, row_base
     f.mxit, f.c_method, f.escape_radius = mxit, c_method, escape_radius
@@ -116,6 +127,9 @@ def compute_fe(cre, cim, scale, W, H, mxit, c_method=0, escape_radius=1e8, prec_
     info = {k: getattr(f, k) for k in ("ref_it", "ref_escaped", "ref_virtual", "ref_x", "ref_y", "interior_used",
                                         "cutoff_sensitive_hint", "seconds_pixels", "steps", "bla_applies",
                                         "bla_skipped", "rebases")}
+    if want_orbit:
+        n = 2 * (f.ref_it + 1)
+        info["orbit"] = dict(zf=ozf[:n].copy(), zfe_m=om[:n].copy(), zfe_e=oe[:n].copy())
     return val.reshape(H, W), nesc.reshape(H, W), out.reshape(H, W), info
 
 
@@ -202,7 +216,9 @@ def fe_op(op, am, ae, bm=None, be=None):
     return rm, re
 
 
-def orbit(cre, cim, prec_bits, mxit, escape_radius=1e8, deriv=False):
+def orbit(cre, cim, prec_bits, mxit, escape_radius=1e8, deriv=False, fx_limbs=0):
+    """Reference orbit shadows: MPFR at prec_bits, or with fx_limbs > 0 the floatexp
+    path's fixed-point orbit (fx_limbs 64-bit limbs) from the point held at prec_bits."""
     zf = np.zeros(2 * (mxit + 1))
     zrm = np.zeros(mxit + 1)
     zre = np.zeros(mxit + 1, np.int64)
@@ -210,11 +226,36 @@ def orbit(cre, cim, prec_bits, mxit, escape_radius=1e8, deriv=False):
     zie = np.zeros(mxit + 1, np.int64)
     dfr = np.zeros(mxit + 1)
     dfi = np.zeros(mxit + 1)
-    r = lib().mxo_orbit_shadows(cre.encode(), cim.encode(), prec_bits, mxit, float(np.float32(escape_radius)), int(deriv),
+    r = lib().mxo_orbit_shadows(cre.encode(), cim.encode(), prec_bits, int(fx_limbs), mxit, float(np.float32(escape_radius)), int(deriv),
                                 _p(zf), _p(zrm), _p(zre), _p(zim), _p(zie), _p(dfr), _p(dfi))
     n = r + 1
     return dict(ref_it=r, zf=zf[:2 * n].copy(), zre_m=zrm[:n].copy(), zre_e=zre[:n].copy(),
                 zim_m=zim[:n].copy(), zim_e=zie[:n].copy(), dfr=dfr[:n].copy(), dfi=dfi[:n].copy())
+
+
+def _limbs(v, L):
+    a = np.zeros(L, np.uint64)
+    for i in range(L):
+        a[i] = (abs(v) >> (64 * i)) & 0xFFFFFFFFFFFFFFFF
+    return a
+
+
+def fixed_mul(L, x, y):
+    """Fixed-point product of two signed integers standing for x * 2^(-64(L-1)),
+    as the floatexp path's orbit computes it; returns the signed integer result."""
+    a, b, r = _limbs(x, L), _limbs(y, L), np.zeros(L, np.uint64)
+    sr = C.c_int(0)
+    sgn = lambda v: (v > 0) - (v < 0)
+    lib().mxo_fixed_mul(L, _p(a), sgn(x), _p(b), sgn(y), _p(r), C.byref(sr))
+    return sr.value * sum(int(r[i]) << (64 * i) for i in range(L))
+
+
+def fixed_to_fe(L, x):
+    """FloatExp shadow (mantissa, exponent) of the fixed-point value x * 2^(-64(L-1))."""
+    a = _limbs(x, L)
+    m, e = C.c_double(0), C.c_int64(0)
+    lib().mxo_fixed_to_fe(L, _p(a), (x > 0) - (x < 0), C.byref(m), C.byref(e))
+    return m.value, e.value
 
 
 def bla_tables(zf, reflen, dcmax, eps=2.0 ** -53, ede=False, dfr=None, dfi=None):

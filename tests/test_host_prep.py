@@ -25,11 +25,30 @@ def test_library_exports_every_declared_symbol():
     assert lib.mx_version() == 1
 
 
-@pytest.mark.parametrize("name", ["extref875", "minibrot875", "exterior1000", "subpixel737"])
+# fixed-point orbit widths on either side of the short-product range 16 <= L < 72
+# (name: frame, precision bits): 1e1400 holds 4718 bits (76 limbs, full products); a
+# 1e290 frame held at 700 bits has 12 limbs
+FIXED_EDGE = {
+    "exterior1400": (("0", "1", cases.pow10(1400), 24, 16, 4000, 0), 0),
+    "deep290_700bit": ((cases.LOC["deep1_x"], cases.LOC["deep1_y"], cases.pow10(290), 25, 17, 20000, 0), 700),
+}
+
+
+@pytest.mark.parametrize("name", ["extref875", "minibrot875", "exterior1000", "subpixel737", "exterior1400",
+                                  "deep290_700bit"])
 def test_prepare_matches_oracle(name):
-    cre, cim, scale, W, H, mxit, cm = cases.FRAMES[name]
-    fv = mxhip.frame_prepare(cre, cim, scale, W, H, mxit, c_method=cm)
-    val, nesc, out, info = mxo.compute_fe(cre, cim, scale, W, H, mxit, c_method=cm)
+    """Floatexp-path preparation equals the oracle's: the fixed-point reference orbit's
+    shadows bit for bit, reference choice, grid steps, BLA and dc tables, and the
+    oracle's per-pixel loop run on the product's arrays reproduces the oracle frame."""
+    (cre, cim, scale, W, H, mxit, cm), pb = FIXED_EDGE.get(name) or (cases.FRAMES[name], 0)
+    fv = mxhip.frame_prepare(cre, cim, scale, W, H, mxit, c_method=cm, prec_bits=pb)
+    val, nesc, out, info = mxo.compute_fe(cre, cim, scale, W, H, mxit, c_method=cm, prec_bits=pb or None,
+                                          want_orbit=True)
+    limbs = (max(53, pb or mxo.precision_bits(scale)) + 127) // 64 + 1
+    assert (limbs < 16 or limbs >= 72) == (name in FIXED_EDGE)
+    assert fv["path"] == 2 and fv["escalated"] == 0
+    for k in ("zf", "zfe_m", "zfe_e"):
+        assert np.array_equal(info["orbit"][k], fv[k]), k
     assert fv["ref_it"] == info["ref_it"] and fv["ref_virtual"] == info["ref_virtual"]
     assert (fv["ref_x"], fv["ref_y"]) == (info["ref_x"], info["ref_y"])
     # the oracle's orbit at the same point: rebuild it from the grid the oracle uses
@@ -62,8 +81,8 @@ def test_prepare_matches_oracle(name):
 
 
 def test_orbit_shadows_equal_mpfr_truncation():
-    """The product's GMP orbit shadows equal the oracle's MPFR orbit truncated to 53
-    bits, entry by entry.  For an odd frame the centre pixel (H/2, W/2) is the view
+    """The product's fixed-point orbit of the exact point c = i (all values dyadic) has
+    the shadows of the oracle's MPFR orbit truncated to 53 bits, entry by entry.  For an odd frame the centre pixel (H/2, W/2) is the view
     centre, c0 + dx*(W-1)/2, up to the truncation residue of the grid arithmetic
     (|residue| < 2^-(precision-64)); entries that are exactly 0 in the orbit of the
     exact centre c = i carry that residue instead and are compared as zeros."""
