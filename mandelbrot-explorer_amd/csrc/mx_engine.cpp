@@ -798,6 +798,7 @@ extern "C" int mx_compute(mx_ctx* ctx, const mx_request* req, mx_frame_info* inf
     mx_ref_release(ref);
     if (rc != MX_OK) return rc;
     if (!F.ref_virtual) {
+        F.ref_pixel_wait();
         const size_t c = (size_t)(F.H / 2) * F.W + (F.W / 2);
         nesc[c] = F.ref_pixel_n;
         val[c] = F.ref_pixel_value;

@@ -5,10 +5,14 @@
 // values the engine would hold; the BLA table is built in FP64 like buildBLA.
 #include <gmp.h>
 
+#include <algorithm>
+#include <atomic>
 #include <chrono>
+#include <climits>
 #include <cmath>
 #include <cstring>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "mx_fe.h"
@@ -253,13 +257,22 @@ void Frame::sa_step(int i) {
 
 // Value of a (non-virtual) reference pixel from its own orbit
 // (mandel_perturbation.cpp:4145-4160, 4179-4185): getEscapeTime, or the GMP point
-// under EDE, -2 when the reference does not escape.
-void Frame::ref_pixel() {
+// under EDE, -2 when the reference does not escape.  With `async` the GMP point runs
+// on a host thread; ref_pixel_wait() joins it.
+void Frame::ref_pixel(bool async) {
+    ref_pixel_wait();
     if (!ref_escaped) {
         ref_pixel_n = -1;
         ref_pixel_value = -2.0;
     } else if (ede) {
-        ref_pixel_value = point_value(ref_re.v, ref_im.v, max_iter, true, ref_pixel_n);
+        if (async) {
+            ref_pixel_n = -1;
+            ref_pixel_value = -2.0;
+            ref_job = std::thread([this] { job_value = point_value(ref_re.v, ref_im.v, max_iter, true, job_n); });
+            job_pending = true;
+        } else {
+            ref_pixel_value = point_value(ref_re.v, ref_im.v, max_iter, true, ref_pixel_n);
+        }
     } else {
         Gmp t1(prec), t2(prec);
         mpf_mul(t1, tail_re.v, tail_re.v);
@@ -268,6 +281,15 @@ void Frame::ref_pixel() {
         const double rad = mpf_get_d(t1);
         ref_pixel_n = ref_it + 1;
         ref_pixel_value = (ref_it + 1 - std::log(std::log(rad) / 2 / std::log(2)) / std::log(2));
+    }
+}
+
+void Frame::ref_pixel_wait() {
+    if (ref_job.joinable()) ref_job.join();
+    if (job_pending) {
+        ref_pixel_value = job_value;
+        ref_pixel_n = job_n;
+        job_pending = false;
     }
 }
 
@@ -295,7 +317,9 @@ bool Frame::tail_escapes(int extra) {
 
 // accuratePointCompute (mandel_perturbation.cpp:719-894) for the reference pixel's
 // own value under a coloring method; smooth or pixel-normalised EDE, -2 interior.
-double Frame::point_value(mpf_srcptr c_re, mpf_srcptr c_im, int mxit, bool ede, int& n_out) {
+// This is the loop as the engine runs it; point_value below gives the same result
+// faster and falls back to it where it cannot decide.
+double Frame::point_value_serial(mpf_srcptr c_re, mpf_srcptr c_im, int mxit, bool ede, int& n_out) {
     Gmp t1(prec), t2(prec), zr(prec), zi(prec), dr(prec), di(prec), er(prec), ei(prec), e1(prec), e2(prec);
     mpf_set(zr, c_re);
     mpf_set(zi, c_im);
@@ -364,6 +388,195 @@ double Frame::point_value(mpf_srcptr c_re, mpf_srcptr c_im, int mxit, bool ede, 
             n_out = -2;
             break;
         }
+    }
+    return res;
+}
+
+namespace {
+
+// e with 2^(e-1) <= |x| < 2^e for x != 0
+long mpf_exp2(mpf_srcptr x) {
+    long e;
+    (void)mpf_get_d_2exp(&e, x);
+    return e;
+}
+
+// z values from the orbit thread to the EDE derivative thread
+struct ZRing {
+    const int n;                      // slots: 2048, at most ~64 MB
+    std::vector<mpf_t> zr, zi;        // slot k % n holds z_k
+    std::atomic<long> count{0};       // z_0 .. z_{count-1} are in the ring
+    std::atomic<long> used{0};        // z values the consumer is done with
+    std::atomic<long> esc_at{LONG_MAX}, d_at{LONG_MAX};
+    explicit ZRing(mp_bitcnt_t prec)
+        : n((int)std::max<mp_bitcnt_t>(16, std::min<mp_bitcnt_t>(2048, ((mp_bitcnt_t)1 << 29) / (2 * prec + 256)))),
+          zr(n), zi(n) {
+        for (int k = 0; k < n; ++k) {
+            mpf_init2(zr[k], prec);
+            mpf_init2(zi[k], prec);
+        }
+    }
+    ~ZRing() {
+        for (int k = 0; k < n; ++k) {
+            mpf_clear(zr[k]);
+            mpf_clear(zi[k]);
+        }
+    }
+};
+
+} // namespace
+
+// The serial loop's result from fewer multiplies, the orbit on this thread and the
+// EDE derivative dc on a second one reading the z values through a ring:
+//  - |z|^2 is formed only when a component can reach the escape radius
+//    (|z|^2 < 2^(2 emax + 1) otherwise);
+//  - d = 2 d z (the interior early-out |d|^2 < 1e-9) is not iterated in mpf: |d_i|^2 =
+//    4^i prod |z_k|^2 exactly, and the mpf d differs from that by a relative
+//    i * 2^-(precision - 4) at most, so a FloatExp product of the |z_k|^2 shadows
+//    (relative error below 2e-15 per step) decides the test whenever it lies outside
+//    1e-9 * (1 +- (1e-6 + 2e-15 i)); inside that window the serial loop runs instead.
+double Frame::point_value(mpf_srcptr c_re, mpf_srcptr c_im, int mxit, bool ede, int& n_out) {
+    ZRing Q(prec);
+    const int RB = Q.n;
+    int eth;
+    (void)std::frexp(esc2, &eth); // esc2 >= 2^(eth-1)
+    const long last = mxit - 1;   // iterations 1 .. mxit-1
+    double rad_esc = 0.0;
+    bool undecided = false;
+
+    Gmp er(prec), ei(prec);
+    auto eloop = [&] {
+        Gmp t2(prec), e1(prec), e2(prec);
+        mpf_set_ui(er, 1);
+        mpf_set_ui(ei, 0);
+        for (long i = 1; i <= last; ++i) {
+            // needs z_{i-1}; stops past the escape or the interior decision
+            bool go = true;
+            while (Q.count.load(std::memory_order_acquire) < i) {
+                if (Q.esc_at.load(std::memory_order_acquire) < i || Q.d_at.load(std::memory_order_acquire) < i) {
+                    go = false;
+                    break;
+                }
+                std::this_thread::yield();
+            }
+            if (!go) break;
+            mpf_srcptr zr = Q.zr[(i - 1) % RB], zi = Q.zi[(i - 1) % RB];
+            mpf_mul(e1, er, zi);
+            mpf_mul(e2, ei, zr);
+            mpf_add(e2, e1, e2);
+            mpf_mul(e1, er, zr);
+            mpf_mul(t2, ei, zi);
+            Q.used.store(i, std::memory_order_release);
+            mpf_sub(e1, e1, t2);
+            mpf_mul_ui(er, e1, 2);
+            mpf_add_ui(er, er, 1);
+            mpf_mul_ui(ei, e2, 2);
+            if (Q.esc_at.load(std::memory_order_acquire) == i) break;
+        }
+        Q.used.store(LONG_MAX, std::memory_order_release);
+    };
+    std::thread te;
+    if (ede) te = std::thread(eloop);
+    else Q.used.store(LONG_MAX);
+    {
+        Gmp t1(prec), t2(prec), zr(prec), zi(prec);
+        mpf_set(zr, c_re);
+        mpf_set(zi, c_im);
+        if (ede) {
+            mpf_set(Q.zr[0], zr);
+            mpf_set(Q.zi[0], zi);
+            Q.count.store(1, std::memory_order_release);
+        }
+        double dm = 4.0; // |d|^2 = dm * 2^de, d_0 = 2
+        long de = 0;
+        for (long i = 1; i <= last; ++i) {
+            // |d_i|^2 = 4 |d_{i-1}|^2 |z_{i-1}|^2
+            {
+                const int sr = mpf_sgn(zr.v), si = mpf_sgn(zi.v);
+                if (sr == 0 && si == 0) {
+                    dm = 0.0;
+                } else if (dm != 0.0) {
+                    long xr = 0, xi = 0;
+                    const double mr = sr ? mpf_get_d_2exp(&xr, zr) : 0.0, mi = si ? mpf_get_d_2exp(&xi, zi) : 0.0;
+                    const long em = sr == 0 ? xi : si == 0 ? xr : std::max(xr, xi);
+                    const double a = sr ? std::ldexp(mr * mr, (int)std::max(-2000L, 2 * (xr - em))) : 0.0;
+                    const double b = si ? std::ldexp(mi * mi, (int)std::max(-2000L, 2 * (xi - em))) : 0.0;
+                    int k;
+                    dm = std::frexp(4.0 * dm * (a + b), &k);
+                    de += 2 * em + k;
+                }
+            }
+            mpf_mul(t1, zr, zi);
+            mpf_mul(zr, zr, zr);
+            mpf_mul(zi, zi, zi);
+            mpf_sub(zr, zr, zi);
+            mpf_add(zr, zr, c_re);
+            mpf_mul_ui(zi, t1, 2);
+            mpf_add(zi, zi, c_im);
+            // |z|^2 < 2^(2 emax + 1) <= 2^(eth - 1) <= esc2: cannot escape
+            const long ez = std::max(mpf_sgn(zr.v) ? mpf_exp2(zr) : LONG_MIN / 4, mpf_sgn(zi.v) ? mpf_exp2(zi) : LONG_MIN / 4);
+            if (2 * ez + 1 > eth - 1) {
+                mpf_mul(t1, zr, zr);
+                mpf_mul(t2, zi, zi);
+                mpf_add(t1, t1, t2);
+                const double rad = mpf_get_d(t1);
+                if (rad > esc2) {
+                    rad_esc = rad;
+                    Q.esc_at.store(i, std::memory_order_release);
+                    break;
+                }
+            }
+            // the d test of iteration i
+            {
+                const double w = 1e-6 + 2e-15 * (double)i;
+                const double v = dm == 0.0 ? 0.0 : de < -1000 ? 0.0 : de > 1000 ? 1e300 : std::ldexp(dm, (int)de);
+                if (v < 1e-9 * (1.0 - w)) {
+                    Q.d_at.store(i, std::memory_order_release);
+                    break;
+                }
+                if (v <= 1e-9 * (1.0 + w)) {
+                    undecided = true;
+                    Q.d_at.store(i, std::memory_order_release); // stops the derivative thread
+                    break;
+                }
+            }
+            if (ede && i < last) {
+                while (Q.used.load(std::memory_order_acquire) <= i - RB) std::this_thread::yield();
+                mpf_set(Q.zr[i % RB], zr);
+                mpf_set(Q.zi[i % RB], zi);
+                Q.count.store(i + 1, std::memory_order_release);
+            }
+        }
+    }
+    if (ede) te.join();
+    if (undecided) return point_value_serial(c_re, c_im, mxit, ede, n_out);
+
+    const long ie = Q.esc_at.load(), id = Q.d_at.load();
+    double res = -2.0;
+    n_out = -1;
+    if (ie != LONG_MAX) {
+        const double rad = rad_esc;
+        n_out = (int)ie + 1;
+        if (ede) {
+            Gmp t1(prec), e1(prec), e2(prec);
+            mpf_mul(e1, er, er);
+            mpf_mul(e2, ei, ei);
+            mpf_add(e1, e1, e2);
+            mpf_sqrt(e1, e1);
+            mpf_abs(t1, dx.v);
+            mpf_mul(e1, e1, t1);
+            double num = std::isfinite(rad) && rad > 0.0 ? std::sqrt(rad) * std::log(rad) : 0.0;
+            mpf_set_d(e2, num);
+            if (mpf_sgn(e1.v) != 0) {
+                mpf_div(e1, e2, e1);
+                res = mpf_get_d(e1);
+            } else
+                res = 0.0;
+        } else {
+            res = ((double)ie + 1 - std::log(std::log(rad) / 2 / std::log(2)) / std::log(2));
+        }
+    } else if (id != LONG_MAX) {
+        n_out = -2;
     }
     return res;
 }
@@ -590,7 +803,7 @@ int Frame::prepare(const mx_request& r, std::string& err) {
             }
         }
         // value of the reference pixel itself (mandel_perturbation.cpp:4145-4160, 4179-4185)
-        if (!ref_virtual) ref_pixel();
+        if (!ref_virtual) ref_pixel(true);
     }
     ref_seconds = now_s() - t_ref;
 
@@ -677,7 +890,7 @@ void Frame::set_pixel_reference(int pi, int pj) {
     ref_x = pj;
     ref_y = pi;
     build_orbit(pre, pim);
-    ref_pixel();
+    ref_pixel(false);
     ++n_refs;
     ref_seconds += now_s() - t_ref;
     const double t_bla = now_s();
@@ -720,6 +933,7 @@ Frame::Frame() {
 }
 
 Frame::~Frame() {
+    ref_pixel_wait();
     for (mpf_ptr x : {c0_re.v, c0_im.v, dx.v, dy.v, ref_re.v, ref_im.v, tail_re.v, tail_im.v}) mpf_clear(x);
 }
 
@@ -791,6 +1005,7 @@ extern "C" int mx_frame_prepare(const mx_request* req, mx_frame** out, mx_frame_
         delete f;
         return rc;
     }
+    f->impl.ref_pixel_wait();
     if (view) f->impl.fill_view(*view);
     *out = f;
     return MX_OK;

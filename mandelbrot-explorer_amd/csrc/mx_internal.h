@@ -7,6 +7,7 @@
 #include <atomic>
 #include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/mx.h"
@@ -81,16 +82,24 @@ struct Frame {
     // Build the reference at pixel (pi, pj) of the double path with everything a pass
     // against it needs (createRef(random=false) + buildBLA, MP.cpp:1750-1758, 4014-4187).
     void set_pixel_reference(int pi, int pj);
+    // ref_pixel_n / ref_pixel_value are final after this (the floatexp path computes
+    // an EDE reference pixel on a host thread while the frame's kernels run)
+    void ref_pixel_wait();
 
   private:
+    std::thread ref_job;
+    bool job_pending = false;
+    int job_n = -1;
+    double job_value = -2.0;
     void init_mpf();
     void build_orbit(mpf_srcptr cre, mpf_srcptr cim);
     void build_orbit_fixed(mpf_srcptr cre, mpf_srcptr cim);
-    void ref_pixel();
+    void ref_pixel(bool async);
     void sa_step(int i);
     int prepare_dbl(mpf_srcptr scale, std::string& err);
     bool tail_escapes(int extra);
     double point_value(mpf_srcptr c_re, mpf_srcptr c_im, int mxit, bool ede, int& n_out);
+    double point_value_serial(mpf_srcptr c_re, mpf_srcptr c_im, int mxit, bool ede, int& n_out);
     void build_bla(double dcmax);
 };
 

@@ -35,7 +35,7 @@ FIXED_EDGE = {
 
 
 @pytest.mark.parametrize("name", ["extref875", "minibrot875", "exterior1000", "subpixel737", "exterior1400",
-                                  "deep290_700bit"])
+                                  "deep290_700bit", "exterior1000_ede", "subpixel737_ede"])
 def test_prepare_matches_oracle(name):
     """Floatexp-path preparation equals the oracle's: the fixed-point reference orbit's
     shadows bit for bit, reference choice, grid steps, BLA and dc tables, and the
