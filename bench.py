@@ -32,6 +32,7 @@ for sub in ("mandelbrot-explorer_amd", "oracle", "tests"):
 
 import numpy as np  # noqa: E402
 
+import autotune  # noqa: E402
 import mxhip  # noqa: E402
 
 # MI355X published vector peaks.  Neither form issues packed FP32 (v_pk_*) or MFMA:
@@ -80,8 +81,10 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def run_form(ctx, ref, fv, form, ede, interior, center, steps, warmup):
-    tile = ctx.tile(fv, stride=1, ede=ede, interior=interior, form=FORMS[form])
+def run_form(ctx, ref, fv, form, ede, interior, center, steps, warmup, knobs=None):
+    k = knobs or {}
+    tile = ctx.tile(fv, stride=1, ede=ede, interior=interior, form=FORMS[form], bla_minlevel=k.get("bla_minlevel"),
+                    chunk=k.get("chunk", 0), grid_blocks=k.get("blocks_per_cu", 0) * ctx.cu_count)
     tile.skip_sample = -1 if fv["ref_virtual"] else center
     plan = ctx.plan(ref, tile)
     try:
@@ -129,6 +132,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-double", action="store_true", help="skip the plain-double path (640x360 @ scale 4e10) leg")
     ap.add_argument("--no-e2e", action="store_true", help="skip the end-to-end mx_compute frame")
+    ap.add_argument("--c4", action="store_true", help="add the batched uneven-tiles leg (96 tiles, 12 references)")
+    ap.add_argument("--tune", action="store_true", help="autotune chunk / grid size / BLA cutoff before timing")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: --gpus {args.gpus} refused: this build runs the path on one GPU only "
@@ -158,9 +163,17 @@ def main():
             pn = pn.reshape(H, W)[::8, ::8]
             interior = bool(((pn < 0) & (pn != np.iinfo(np.int32).min)).any())
             log(f"probe {probe_ms:.1f} ms, interior detection {'on' if interior else 'off'}")
-            res[args.form] = run_form(ctx, ref, fv, args.form, ede, interior, center, args.steps, args.warmup)
+            tuned = {}
+            if args.tune:
+                for f in [args.form] + compare:
+                    r = autotune.tune(ctx, ref, fv, FORMS[f], ede=ede, interior=interior, log=log)
+                    tuned[f] = {k: r[k] for k in ("chunk", "blocks_per_cu", "bla_minlevel", "stride", "ms")}
+                    tuned[f]["trials"] = len(r["trials"])
+                    log(f"autotune {f}: {tuned[f]}")
+            res[args.form] = run_form(ctx, ref, fv, args.form, ede, interior, center, args.steps, args.warmup,
+                                      tuned.get(args.form))
             for f in compare:
-                res[f] = run_form(ctx, ref, fv, f, ede, interior, center, 1, 0)
+                res[f] = run_form(ctx, ref, fv, f, ede, interior, center, 1, 0, tuned.get(f))
         finally:
             ref.release()
         e2e_ms = None
@@ -173,6 +186,7 @@ def main():
             e2e_ms = (time.time() - t1) * 1e3
             log(f"end-to-end frame (mx_compute): {e2e_ms:.0f} ms")
         double = None if args.no_double else double_leg(ctx, args.no_cpu)
+        batched = c4_leg(ctx) if args.c4 else None
     main_r = res[args.form]
     forms = {f: summary(f, r, mxit, ede) for f, r in res.items()}
     s = forms[args.form]
@@ -222,6 +236,10 @@ def main():
         "cpu_baseline": cpu,
         "double_path": double,
     }
+    if batched is not None:
+        line["batched_tiles"] = batched
+    if tuned:
+        line["autotune"] = tuned
     print(json.dumps(line))
 
 
@@ -269,6 +287,86 @@ def double_leg(ctx, no_cpu, reps=5):
         out["cpu_oracle_threads"] = threads
         out["n_esc_identical_to_oracle"] = bool(np.array_equal(n_o, nesc))
         log(f"double path: cpu oracle frame {dt * 1e3:.1f} ms on {threads} threads")
+    return out
+
+
+def c4_leg(ctx, forms=("fe32", "fe64"), reps=1):
+    """96 uneven tiles over 12 boundary-walk locations (tests/golden/c4_requests.json):
+    each location's reference, BLA table and dc tables are built once and resident;
+    the tiles run as one persistent launch per form (largest work first), and the
+    chosen form also tile by tile, one launch each, for the batching gain."""
+    path = os.path.join(ROOT, "tests", "golden", "c4_requests.json")
+    if not os.path.exists(path):
+        log("c4: tests/golden/c4_requests.json missing (run tests/golden/make_c4.py)")
+        return None
+    spec = json.load(open(path))
+    grid = 1024
+    t0 = time.time()
+    locs = {}
+    for l in spec["locations"]:
+        mx = max(q["max_iter"] for q in spec["requests"] if q["location"] == l["seed"])
+        fv = mxhip.frame_prepare(l["re"], l["im"], "4" + "0" * l["depth"], grid, grid, mx)
+        locs[l["seed"]] = (fv, ctx.upload(fv))
+    prep_s = time.time() - t0
+    log(f"c4: 12 references prepared in {prep_s:.1f} s")
+    out = {"workload": "96 square tiles (64..1024) of 12 seeded walk views (1e-86..1e-276, 1024x1024 grid "
+                       "each), max_iter 1e4..1e7, smooth, BLA, interior detection",
+           "requests": len(spec["requests"]), "samples": int(sum(q["side"] ** 2 for q in spec["requests"])),
+           "host_prepare_s": prep_s}
+    res = {}
+    try:
+        for form in forms:
+            plans, nominal_of = [], []
+            for q in spec["requests"]:
+                fv, ref = locs[q["location"]]
+                x0, y0, s_ = q["x0"], q["y0"], q["side"]
+                sub = dict(fv, W=s_, H=s_, dcol_m=fv["dcol_m"][x0:x0 + s_], dcol_e=fv["dcol_e"][x0:x0 + s_],
+                           drow_m=fv["drow_m"][y0:y0 + s_], drow_e=fv["drow_e"][y0:y0 + s_])
+                t = ctx.tile(sub, form=FORMS[form], interior=True, mxit=q["max_iter"])
+                plans.append(ctx.plan(ref, t))
+            order = sorted(range(len(plans)), key=lambda k: -spec["requests"][k]["side"] ** 2 * spec["requests"][k]["max_iter"])
+            b = ctx.batch([plans[k] for k in order])
+            b.launch()
+            b.wait()  # warm-up
+            t1 = time.perf_counter()
+            kms = []
+            for _ in range(reps):
+                b.launch()
+                kms.append(b.wait())
+            wall = (time.perf_counter() - t1) / reps
+            nominal, steps = 0, 0
+            nescs = []
+            for q, p in zip(spec["requests"], plans):
+                nesc, _, _, st = p.read()
+                nescs.append(nesc)
+                nominal += int(np.where(nesc > 0, nesc, q["max_iter"]).astype(np.int64).sum())
+                steps += int(st[0])
+            r = {"batch_ms": wall * 1e3, "batch_kernel_ms": float(np.mean(kms)), "nominal_pixel_iterations": nominal,
+                 "executed_steps": steps, "gpix_iter_s": nominal / wall / 1e9}
+            log(f"c4 {form}: one launch {wall * 1e3:.0f} ms, {r['gpix_iter_s']:.0f} Gpixel-iter/s")
+            if form == forms[0]:
+                t1 = time.perf_counter()
+                for k in order:
+                    plans[k].launch()
+                    plans[k].wait()
+                seq = time.perf_counter() - t1
+                r["tile_by_tile_ms"] = seq * 1e3
+                r["batch_speedup_vs_tile_by_tile"] = seq / wall
+                log(f"c4 {form}: tile by tile {seq * 1e3:.0f} ms")
+            b.destroy()
+            for p in plans:
+                p.destroy()
+            res[form] = (r, nescs)
+    finally:
+        for fv, ref in locs.values():
+            ref.release()
+    for form, (r, _) in res.items():
+        out[form] = r
+    if len(forms) > 1 and forms[1] in res:
+        a, b_ = res[forms[0]][1], res[forms[1]][1]
+        same = sum(int((x == y).sum()) for x, y in zip(a, b_))
+        out[f"{forms[0]}_vs_{forms[1]}"] = {"n_esc_identical_frac": same / out["samples"],
+                                           "speedup": res[forms[1]][0]["batch_ms"] / res[forms[0]][0]["batch_ms"]}
     return out
 
 

@@ -47,6 +47,7 @@ enum { MX_SUPER_SAMPLING = 1, MX_EXTERIOR_DIST_EST = 2 };
 typedef struct mx_ctx mx_ctx;
 typedef struct mx_ref mx_ref;
 typedef struct mx_plan mx_plan;
+typedef struct mx_batch mx_batch;
 typedef struct mx_frame mx_frame;
 
 /* ---- context ------------------------------------------------------------ */
@@ -60,6 +61,9 @@ int mx_device_count(void);
 int mx_ctx_create(int device, mx_ctx** out);
 void mx_ctx_destroy(mx_ctx* ctx);
 const char* mx_last_error(const mx_ctx* ctx);
+/* Compute units of the context's device, for sizing mx_tile.grid_blocks (the D3D11
+ * backend reads its adapter's properties the same way, compute_backend_d3d11.cpp:213-279). */
+int mx_ctx_cu_count(const mx_ctx* ctx);
 /* IComputeBackend::cancel / resetCancellation (compute_backend.cpp:119-128).
  * cancel() may be called from any thread; running kernels stop taking samples. */
 void mx_cancel(mx_ctx* ctx);
@@ -135,6 +139,12 @@ typedef struct {
      * grid: a later reference pass over the samples the previous pass left unresolved */
     const int32_t* list;
     int32_t list_len;
+    /* launch tuning (0: defaults); results do not depend on either.  chunk: lean
+     * steps per trip between the full step's checks (default 24); grid_blocks:
+     * workgroups of 256 lanes in the persistent grid (default 8 per CU, at most one
+     * per 64 samples). */
+    int32_t chunk;
+    int32_t grid_blocks;
 } mx_tile;
 
 /* A plan holds the device copy of a tile's inputs and outputs so that launches
@@ -153,6 +163,24 @@ void mx_plan_destroy(mx_plan* plan);
  * glitch_p = |z|^2 / _z_m3[iteration], in no particular order.  *count is the number of
  * such samples; at most cap are copied. */
 int mx_plan_unresolved(mx_plan* plan, int32_t* idx, double* glitch, int32_t cap, int32_t* count);
+/* ---- batches: many tiles in one persistent launch ------------------------- */
+
+/* The engine renders a large or multi-view job as a loop of Compute calls, one per
+ * strip or tile (src/gui/gui_export.cpp:119-140 issues one ComputeRequest per strip;
+ * compute_backend.cpp:263-290 serves one request per call).  A batch launches the
+ * tiles of several plans, each with its own reference, size and max_iter, as one
+ * persistent grid: waves walk the plans in the given order (largest work first keeps
+ * the tail short) and move on as each plan's queues drain.  The plans must be of one
+ * floatexp form (FE64, FE32 or SC64) with the same ede and interior flags, each plan
+ * at most once; they stay
+ * owned by the caller, must outlive the batch, and their results are read with
+ * mx_plan_read.  A batch launch equals launching its plans one by one, sample for
+ * sample. */
+int mx_batch_create(mx_ctx* ctx, mx_plan* const* plans, int32_t n, mx_batch** out);
+int mx_batch_launch(mx_batch* batch);                   /* asynchronous */
+int mx_batch_wait(mx_batch* batch, double* kernel_ms);  /* synchronise; time of the last launch */
+void mx_batch_destroy(mx_batch* batch);
+
 /* create + launch + wait + read + destroy */
 int mx_tile_render(mx_ctx* ctx, const mx_ref* ref, const mx_tile* tile, int32_t* n_esc, double* value, float* out, uint64_t* stats4);
 

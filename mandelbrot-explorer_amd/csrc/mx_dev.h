@@ -63,6 +63,7 @@ struct MxTileDev {
     double interior_eps2;
     int interior_confirm;
     int skip_sample; // sample index not computed (the reference pixel), -1 none
+    int chunk;       // lean steps per trip between full-path checks (results do not depend on it)
     Fe dxfe;
     double* val;         // W*H
     int* nesc;           // W*H
@@ -79,3 +80,14 @@ struct MxTileDev {
     double* unres_gp;      // their glitch metric
     unsigned* unres_count;
 };
+
+// One tile of a launch and the reference it is computed against.  A launch runs its
+// jobs in order: each wave takes samples of job j from that job's queues and moves
+// to job j+1 once those queues are drained and its lanes are done, so tiles of
+// different size, depth and iteration limit share one persistent grid.
+struct MxJob {
+    MxRefDev R;
+    const int2* lev;
+    MxTileDev T;
+};
+

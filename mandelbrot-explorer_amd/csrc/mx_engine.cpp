@@ -18,6 +18,13 @@
 
 namespace {
 
+// one persistent launch over a device job table of a floatexp form
+hipError_t launch_jobs(int form, const MxJob* jobs, int njobs, bool ede, bool interior, int blocks, int threads, hipStream_t stream) {
+    if (form == MX_FORM_FE32) return mx_launch_fe32(jobs, njobs, ede, interior, blocks, threads, stream);
+    if (form == MX_FORM_SC64) return mx_launch_sc64(jobs, njobs, ede, interior, blocks, threads, stream);
+    return mx_launch_fe64(jobs, njobs, ede, interior, blocks, threads, stream);
+}
+
 double now_s() { return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count(); }
 
 #define MX_HIP(call)                                                                 \
@@ -101,6 +108,8 @@ extern "C" void mx_ctx_destroy(mx_ctx* ctx) {
     if (ctx->cancel_host) (void)hipHostFree(ctx->cancel_host);
     delete ctx;
 }
+
+extern "C" int mx_ctx_cu_count(const mx_ctx* ctx) { return ctx ? ctx->cu_count : MX_E_INVALID; }
 
 extern "C" const char* mx_last_error(const mx_ctx* ctx) { return ctx ? ctx->err.c_str() : "no context"; }
 
@@ -290,7 +299,8 @@ extern "C" int mx_plan_create(mx_ctx* ctx, const mx_ref* ref, const mx_tile* t, 
     size_t o_uidx = align_up(o_list + 4 * nlist);
     size_t o_ugp = align_up(o_uidx + 4 * nunres);
     size_t o_ucnt = align_up(o_ugp + 8 * nunres);
-    size_t bytes = align_up(o_ucnt + 4);
+    size_t o_job = align_up(o_ucnt + 4);
+    size_t bytes = align_up(o_job + sizeof(MxJob));
     if (hipMalloc(&plan->block, bytes) != hipSuccess) {
         delete plan;
         ctx->set_err("hipMalloc of the tile failed");
@@ -334,6 +344,7 @@ extern "C" int mx_plan_create(mx_ctx* ctx, const mx_ref* ref, const mx_tile* t, 
     T.interior_confirm = t->interior_confirm;
     T.dxfe = fe_make(t->dx_m, t->dx_e);
     T.skip_sample = t->skip_sample;
+    T.chunk = t->chunk > 0 ? std::min(t->chunk, 4096) : 24;
     T.val = (double*)(b + o_val);
     T.nesc = (int*)(b + o_nesc);
     T.out = (float*)(b + o_out);
@@ -353,11 +364,31 @@ extern "C" int mx_plan_create(mx_ctx* ctx, const mx_ref* ref, const mx_tile* t, 
     }
     plan->threads = 256;
     plan->minlevel = t->bla_minlevel;
+    plan->ede = t->ede != 0;
+    plan->grid_request = t->grid_blocks;
+    plan->interior = t->interior != 0;
+    if (!dbl) {
+        // the plan's launch is a one-job table resident beside its tile
+        MxJob J;
+        J.R = ref->dev;
+        J.R.minlevel = t->bla_minlevel;
+        J.lev = ref->lev;
+        J.T = T;
+        plan->job = J;
+        plan->job_dev = (MxJob*)(b + o_job);
+        e = hipMemcpy(plan->job_dev, &J, sizeof J, hipMemcpyHostToDevice);
+        if (e != hipSuccess) {
+            ctx->set_err(std::string("tile set-up: ") + hipGetErrorString(e));
+            mx_plan_destroy(plan);
+            return MX_E_HIP;
+        }
+    }
     // persistent grid: 8 waves per SIMD (32 per CU); never more workgroups than
     // there are 8x8 tiles to hand out
     const int waves = ctx->cu_count * 32;
     const int work_tiles = t->list ? (int)((nlist + 63) / 64) : T.ntiles;
-    plan->blocks = std::max(1, std::min(waves / 4, work_tiles));
+    plan->work_tiles = work_tiles;
+    plan->blocks = std::max(1, std::min(t->grid_blocks > 0 ? t->grid_blocks : waves / 4, work_tiles));
     *out = plan;
     return MX_OK;
 }
@@ -370,16 +401,13 @@ extern "C" int mx_plan_launch(mx_plan* plan) {
     MX_HIP(hipEventRecord(plan->ev0, ctx->stream));
     hipLaunchKernelGGL(mx_reset_kernel, dim3(1024), dim3(256), 0, ctx->stream, T.queue, T.stats, T.unres_count, T.val, T.nesc, T.out, plan->n);
     MX_HIP(hipGetLastError());
-    MxRefDev R = plan->ref->dev;
-    R.minlevel = plan->minlevel;
-    if (plan->form == MX_FORM_DOUBLE)
+    if (plan->form == MX_FORM_DOUBLE) {
+        MxRefDev R = plan->ref->dev;
+        R.minlevel = plan->minlevel;
         MX_HIP(mx_launch_dbl(R, plan->ref->lev, T, plan->blocks, plan->threads, ctx->stream));
-    else if (plan->form == MX_FORM_FE32)
-        MX_HIP(mx_launch_fe32(R, plan->ref->lev, T, plan->blocks, plan->threads, ctx->stream));
-    else if (plan->form == MX_FORM_SC64)
-        MX_HIP(mx_launch_sc64(R, plan->ref->lev, T, plan->blocks, plan->threads, ctx->stream));
-    else
-        MX_HIP(mx_launch_fe64(R, plan->ref->lev, T, plan->blocks, plan->threads, ctx->stream));
+    } else {
+        MX_HIP(launch_jobs(plan->form, plan->job_dev, 1, plan->ede, plan->interior, plan->blocks, plan->threads, ctx->stream));
+    }
     MX_HIP(hipEventRecord(plan->ev1, ctx->stream));
     return MX_OK;
 }
@@ -437,6 +465,97 @@ extern "C" void mx_plan_destroy(mx_plan* plan) {
     if (plan->ev1) (void)hipEventDestroy(plan->ev1);
     if (plan->block) (void)hipFree(plan->block);
     delete plan;
+}
+
+extern "C" int mx_batch_create(mx_ctx* ctx, mx_plan* const* plans, int32_t n, mx_batch** out) {
+    if (!ctx || !plans || n < 1 || !out) return MX_E_INVALID;
+    *out = nullptr;
+    const mx_plan* p0 = plans[0];
+    if (!p0) return MX_E_INVALID;
+    int64_t work = 0;
+    for (int32_t k = 0; k < n; ++k) {
+        const mx_plan* p = plans[k];
+        if (!p || p->ctx != ctx || !p->job_dev) {
+            ctx->set_err("a batch takes floatexp-form plans of this context");
+            return MX_E_INVALID;
+        }
+        if (p->form != p0->form || p->ede != p0->ede || p->interior != p0->interior) {
+            ctx->set_err("the plans of a batch share one form and the same EDE and interior-detection flags");
+            return MX_E_INVALID;
+        }
+        for (int32_t m = 0; m < k; ++m)
+            if (plans[m] == p) {
+                ctx->set_err("a plan appears twice in the batch");
+                return MX_E_INVALID;
+            }
+        work += p->work_tiles;
+    }
+    MX_HIP(hipSetDevice(ctx->device));
+    auto* b = new (std::nothrow) mx_batch();
+    if (!b) return MX_E_NOMEM;
+    b->ctx = ctx;
+    b->plans.assign(plans, plans + n);
+    b->form = p0->form;
+    b->ede = p0->ede;
+    b->interior = p0->interior;
+    std::vector<MxJob> jobs((size_t)n);
+    for (int32_t k = 0; k < n; ++k) jobs[(size_t)k] = plans[k]->job;
+    hipError_t e = hipMalloc(&b->jobs_dev, sizeof(MxJob) * (size_t)n);
+    if (e == hipSuccess) e = hipMemcpy(b->jobs_dev, jobs.data(), sizeof(MxJob) * (size_t)n, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipEventCreate(&b->ev0);
+    if (e == hipSuccess) e = hipEventCreate(&b->ev1);
+    if (e != hipSuccess) {
+        ctx->set_err(std::string("batch set-up: ") + hipGetErrorString(e));
+        mx_batch_destroy(b);
+        return MX_E_HIP;
+    }
+    b->threads = 256;
+    // the first plan's grid size request, if any, sizes the batch's grid
+    const int want = p0->grid_request > 0 ? p0->grid_request : ctx->cu_count * 32 / 4;
+    b->blocks = (int)std::max<int64_t>(1, std::min<int64_t>(want, work));
+    *out = b;
+    return MX_OK;
+}
+
+extern "C" int mx_batch_launch(mx_batch* b) {
+    if (!b) return MX_E_INVALID;
+    mx_ctx* ctx = b->ctx;
+    MX_HIP(hipSetDevice(ctx->device));
+    MX_HIP(hipEventRecord(b->ev0, ctx->stream));
+    for (mx_plan* p : b->plans) {
+        const MxTileDev& T = p->tile;
+        hipLaunchKernelGGL(mx_reset_kernel, dim3(1024), dim3(256), 0, ctx->stream, T.queue, T.stats, T.unres_count, T.val, T.nesc, T.out, p->n);
+        MX_HIP(hipGetLastError());
+    }
+    MX_HIP(launch_jobs(b->form, b->jobs_dev, (int)b->plans.size(), b->ede, b->interior, b->blocks, b->threads, ctx->stream));
+    MX_HIP(hipEventRecord(b->ev1, ctx->stream));
+    return MX_OK;
+}
+
+extern "C" int mx_batch_wait(mx_batch* b, double* kernel_ms) {
+    if (!b) return MX_E_INVALID;
+    mx_ctx* ctx = b->ctx;
+    MX_HIP(hipStreamSynchronize(ctx->stream));
+    if (kernel_ms) {
+        float ms = 0.f;
+        MX_HIP(hipEventElapsedTime(&ms, b->ev0, b->ev1));
+        *kernel_ms = ms;
+    }
+    if (__atomic_load_n(ctx->cancel_host, __ATOMIC_ACQUIRE)) {
+        ctx->set_err("cancelled");
+        return MX_E_CANCELLED;
+    }
+    return MX_OK;
+}
+
+extern "C" void mx_batch_destroy(mx_batch* b) {
+    if (!b) return;
+    (void)hipSetDevice(b->ctx->device);
+    (void)hipStreamSynchronize(b->ctx->stream);
+    if (b->ev0) (void)hipEventDestroy(b->ev0);
+    if (b->ev1) (void)hipEventDestroy(b->ev1);
+    if (b->jobs_dev) (void)hipFree(b->jobs_dev);
+    delete b;
 }
 
 extern "C" int mx_tile_render(mx_ctx* ctx, const mx_ref* ref, const mx_tile* tile, int32_t* n_esc, double* value, float* out, uint64_t* stats4) {

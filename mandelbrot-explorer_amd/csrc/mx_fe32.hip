@@ -93,11 +93,6 @@ struct C32 {
 #define MX_WAVES_ATTR(n) MX_WAVES_ATTR_(n)
 #define MX_WAVES_ATTR_(n) __attribute__((amdgpu_waves_per_eu((n) > 0 ? (n) : 1)))
 
-// lean steps per trip (after each full step)
-#ifndef MX_FAST_STEPS
-#define MX_FAST_STEPS 24
-#endif
-
 template <typename F>
 MX_HD F pow2t(int e) {
     return ldexp_t((F)1, e); // 0 below the subnormal range, inf above the range
@@ -758,7 +753,7 @@ MX_HD bool item_to_sample(const MxTileDev& T, unsigned item, int& si, int& sj) {
 }
 
 template <typename F, bool EDE, bool INTERIOR>
-__global__ __launch_bounds__(256) MX_WAVES_ATTR(MX_FE32_WAVES) void mx_fe32_kernel(MxRefDev R, const int2* __restrict__ lev, MxTileDev T) {
+__device__ __forceinline__ void run_job32(const MxRefDev& R, const int2* __restrict__ lev, const MxTileDev& T) {
     const int lane = __lane_id();
     const unsigned long long lanemask_lt = (1ull << lane) - 1ull;
     const F esc2f = (F)T.esc2;
@@ -811,7 +806,7 @@ __global__ __launch_bounds__(256) MX_WAVES_ATTR(MX_FE32_WAVES) void mx_fe32_kern
                 if (T.out) T.out[sidx] = n < 0 ? -2.0f : (float)v;
                 active = false;
             } else {
-                fast_steps<F, EDE, INTERIOR>(R, T, p, c, esc2f, MX_FAST_STEPS);
+                fast_steps<F, EDE, INTERIOR>(R, T, p, c, esc2f, T.chunk);
             }
         }
     }
@@ -824,29 +819,34 @@ __global__ __launch_bounds__(256) MX_WAVES_ATTR(MX_FE32_WAVES) void mx_fe32_kern
     }
 }
 
+template <typename F, bool EDE, bool INTERIOR>
+__global__ __launch_bounds__(256) MX_WAVES_ATTR(MX_FE32_WAVES) void mx_fe32_kernel(const MxJob* __restrict__ jobs, int njobs) {
+    for (int j = 0; j < njobs; ++j) run_job32<F, EDE, INTERIOR>(jobs[j].R, jobs[j].lev, jobs[j].T);
+}
+
 } // namespace
 
 template <typename F>
-static void launch_scaled(const MxRefDev& R, const int2* lev, const MxTileDev& T, int blocks, int threads, hipStream_t stream) {
-    const bool ede = T.flags & MX_F_EDE, interior = T.flags & MX_F_INTERIOR;
+static void launch_scaled(const MxJob* jobs, int njobs, bool ede, bool interior, int blocks, int threads, hipStream_t stream) {
     if (ede && interior)
-        hipLaunchKernelGGL((mx_fe32_kernel<F, true, true>), dim3(blocks), dim3(threads), 0, stream, R, lev, T);
+        hipLaunchKernelGGL((mx_fe32_kernel<F, true, true>), dim3(blocks), dim3(threads), 0, stream, jobs, njobs);
     else if (ede)
-        hipLaunchKernelGGL((mx_fe32_kernel<F, true, false>), dim3(blocks), dim3(threads), 0, stream, R, lev, T);
+        hipLaunchKernelGGL((mx_fe32_kernel<F, true, false>), dim3(blocks), dim3(threads), 0, stream, jobs, njobs);
     else if (interior)
-        hipLaunchKernelGGL((mx_fe32_kernel<F, false, true>), dim3(blocks), dim3(threads), 0, stream, R, lev, T);
+        hipLaunchKernelGGL((mx_fe32_kernel<F, false, true>), dim3(blocks), dim3(threads), 0, stream, jobs, njobs);
     else
-        hipLaunchKernelGGL((mx_fe32_kernel<F, false, false>), dim3(blocks), dim3(threads), 0, stream, R, lev, T);
+        hipLaunchKernelGGL((mx_fe32_kernel<F, false, false>), dim3(blocks), dim3(threads), 0, stream, jobs, njobs);
 }
 
-// FP32 delta (the mixed form)
-hipError_t mx_launch_fe32(const MxRefDev& R, const int2* lev, const MxTileDev& T, int blocks, int threads, hipStream_t stream) {
-    launch_scaled<float>(R, lev, T, blocks, threads, stream);
+// FP32 delta (the mixed form); `jobs` is a device array, every job with the same EDE
+// and interior-detection flags
+hipError_t mx_launch_fe32(const MxJob* jobs, int njobs, bool ede, bool interior, int blocks, int threads, hipStream_t stream) {
+    launch_scaled<float>(jobs, njobs, ede, interior, blocks, threads, stream);
     return hipGetLastError();
 }
 
 // The same power-of-two-scaled algorithm with an FP64 delta
-hipError_t mx_launch_sc64(const MxRefDev& R, const int2* lev, const MxTileDev& T, int blocks, int threads, hipStream_t stream) {
-    launch_scaled<double>(R, lev, T, blocks, threads, stream);
+hipError_t mx_launch_sc64(const MxJob* jobs, int njobs, bool ede, bool interior, int blocks, int threads, hipStream_t stream) {
+    launch_scaled<double>(jobs, njobs, ede, interior, blocks, threads, stream);
     return hipGetLastError();
 }

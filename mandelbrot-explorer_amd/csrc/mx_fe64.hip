@@ -48,11 +48,6 @@ struct Counters {
 #define MX_WAVES_ATTR(n) MX_WAVES_ATTR_(n)
 #define MX_WAVES_ATTR_(n) __attribute__((amdgpu_waves_per_eu((n) > 0 ? (n) : 1)))
 
-// lean steps per trip (after each full step)
-#ifndef MX_FAST_STEPS
-#define MX_FAST_STEPS 24
-#endif
-
 MX_HD void init_pixel(Px& p, Fe dcr, Fe dci) {
     p.dcr = dcr;
     p.dci = dci;
@@ -612,7 +607,7 @@ MX_HD bool item_to_sample(const MxTileDev& T, unsigned item, int& si, int& sj) {
 }
 
 template <bool EDE, bool INTERIOR>
-__global__ __launch_bounds__(256) MX_WAVES_ATTR(MX_FE64_WAVES) void mx_fe64_kernel(MxRefDev R, const int2* __restrict__ lev, MxTileDev T) {
+__device__ __forceinline__ void run_job64(const MxRefDev& R, const int2* __restrict__ lev, const MxTileDev& T) {
     const int lane = __lane_id();
     const unsigned long long lanemask_lt = (1ull << lane) - 1ull;
     int q = blockIdx.x % MX_NQUEUE;
@@ -664,7 +659,7 @@ __global__ __launch_bounds__(256) MX_WAVES_ATTR(MX_FE64_WAVES) void mx_fe64_kern
                 if (T.out) T.out[sidx] = n < 0 ? -2.0f : (float)v;
                 active = false;
             } else {
-                fast_steps<EDE, INTERIOR>(R, T, p, c, MX_FAST_STEPS);
+                fast_steps<EDE, INTERIOR>(R, T, p, c, T.chunk);
             }
         }
     }
@@ -677,18 +672,23 @@ __global__ __launch_bounds__(256) MX_WAVES_ATTR(MX_FE64_WAVES) void mx_fe64_kern
     }
 }
 
+template <bool EDE, bool INTERIOR>
+__global__ __launch_bounds__(256) MX_WAVES_ATTR(MX_FE64_WAVES) void mx_fe64_kernel(const MxJob* __restrict__ jobs, int njobs) {
+    for (int j = 0; j < njobs; ++j) run_job64<EDE, INTERIOR>(jobs[j].R, jobs[j].lev, jobs[j].T);
+}
+
 } // namespace
 
-// Host-side launcher (called from the engine's C++).
-hipError_t mx_launch_fe64(const MxRefDev& R, const int2* lev, const MxTileDev& T, int blocks, int threads, hipStream_t stream) {
-    const bool ede = T.flags & MX_F_EDE, interior = T.flags & MX_F_INTERIOR;
+// Host-side launcher (called from the engine's C++); `jobs` is a device array, every
+// job with the same EDE and interior-detection flags.
+hipError_t mx_launch_fe64(const MxJob* jobs, int njobs, bool ede, bool interior, int blocks, int threads, hipStream_t stream) {
     if (ede && interior)
-        hipLaunchKernelGGL((mx_fe64_kernel<true, true>), dim3(blocks), dim3(threads), 0, stream, R, lev, T);
+        hipLaunchKernelGGL((mx_fe64_kernel<true, true>), dim3(blocks), dim3(threads), 0, stream, jobs, njobs);
     else if (ede)
-        hipLaunchKernelGGL((mx_fe64_kernel<true, false>), dim3(blocks), dim3(threads), 0, stream, R, lev, T);
+        hipLaunchKernelGGL((mx_fe64_kernel<true, false>), dim3(blocks), dim3(threads), 0, stream, jobs, njobs);
     else if (interior)
-        hipLaunchKernelGGL((mx_fe64_kernel<false, true>), dim3(blocks), dim3(threads), 0, stream, R, lev, T);
+        hipLaunchKernelGGL((mx_fe64_kernel<false, true>), dim3(blocks), dim3(threads), 0, stream, jobs, njobs);
     else
-        hipLaunchKernelGGL((mx_fe64_kernel<false, false>), dim3(blocks), dim3(threads), 0, stream, R, lev, T);
+        hipLaunchKernelGGL((mx_fe64_kernel<false, false>), dim3(blocks), dim3(threads), 0, stream, jobs, njobs);
     return hipGetLastError();
 }

@@ -143,9 +143,24 @@ struct mx_plan {
     size_t bytes = 0;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     size_t n = 0;
+    bool ede = false, interior = false;
+    MxJob job{};                // floatexp forms: the launch's job (host copy)
+    MxJob* job_dev = nullptr;   // and its device copy, inside `block`
+    int work_tiles = 0;         // 64-sample work items of one launch
+    int grid_request = 0;       // caller's grid size (0: default)
 };
 
-hipError_t mx_launch_fe64(const MxRefDev& R, const int2* lev, const MxTileDev& T, int blocks, int threads, hipStream_t stream);
-hipError_t mx_launch_fe32(const MxRefDev& R, const int2* lev, const MxTileDev& T, int blocks, int threads, hipStream_t stream);
-hipError_t mx_launch_sc64(const MxRefDev& R, const int2* lev, const MxTileDev& T, int blocks, int threads, hipStream_t stream);
+struct mx_batch {
+    mx_ctx* ctx = nullptr;
+    std::vector<mx_plan*> plans;
+    MxJob* jobs_dev = nullptr;
+    int form = MX_FORM_FE64;
+    bool ede = false, interior = false;
+    int blocks = 0, threads = 256;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+};
+
+hipError_t mx_launch_fe64(const MxJob* jobs, int njobs, bool ede, bool interior, int blocks, int threads, hipStream_t stream);
+hipError_t mx_launch_fe32(const MxJob* jobs, int njobs, bool ede, bool interior, int blocks, int threads, hipStream_t stream);
+hipError_t mx_launch_sc64(const MxJob* jobs, int njobs, bool ede, bool interior, int blocks, int threads, hipStream_t stream);
 hipError_t mx_launch_dbl(const MxRefDev& R, const int2* lev, const MxTileDev& T, int blocks, int threads, hipStream_t stream);

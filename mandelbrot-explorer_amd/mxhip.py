@@ -37,7 +37,8 @@ class Tile(C.Structure):
                 ("interior_eps2", C.c_double), ("interior_confirm", C.c_int32),
                 ("dcol_m", C.c_void_p), ("dcol_e", C.c_void_p), ("drow_m", C.c_void_p), ("drow_e", C.c_void_p),
                 ("dx_m", C.c_double), ("dx_e", C.c_int32), ("skip_sample", C.c_int32),
-                ("dcol_d", C.c_void_p), ("drow_d", C.c_void_p), ("list", C.c_void_p), ("list_len", C.c_int32)]
+                ("dcol_d", C.c_void_p), ("drow_d", C.c_void_p), ("list", C.c_void_p), ("list_len", C.c_int32),
+                ("chunk", C.c_int32), ("grid_blocks", C.c_int32)]
 
 
 class Request(C.Structure):
@@ -83,7 +84,8 @@ class FrameView(C.Structure):
 EXPORTS = ["mx_version", "mx_device_count", "mx_ctx_create", "mx_ctx_destroy", "mx_last_error", "mx_cancel",
            "mx_reset_cancellation", "mx_ref_upload", "mx_ref_release", "mx_plan_create", "mx_plan_launch",
            "mx_plan_wait", "mx_plan_read", "mx_plan_unresolved", "mx_plan_destroy", "mx_tile_render", "mx_compute",
-           "mx_frame_prepare", "mx_frame_free"]
+           "mx_frame_prepare", "mx_frame_free", "mx_batch_create", "mx_batch_launch", "mx_batch_wait",
+           "mx_batch_destroy", "mx_ctx_cu_count"]
 
 _LIB = None
 
@@ -112,6 +114,11 @@ def lib():
         L.mx_plan_destroy.argtypes = [vp]
         L.mx_plan_unresolved.argtypes = [vp, vp, vp, C.c_int32, C.POINTER(C.c_int32)]
         L.mx_tile_render.argtypes = [vp, vp, C.POINTER(Tile), vp, vp, vp, vp]
+        L.mx_ctx_cu_count.argtypes = [vp]
+        L.mx_batch_create.argtypes = [vp, C.POINTER(vp), C.c_int32, C.POINTER(vp)]
+        L.mx_batch_launch.argtypes = [vp]
+        L.mx_batch_wait.argtypes = [vp, C.POINTER(C.c_double)]
+        L.mx_batch_destroy.argtypes = [vp]
         L.mx_compute.argtypes = [vp, C.POINTER(Request), C.POINTER(FrameInfo)]
         L.mx_frame_prepare.argtypes = [C.POINTER(Request), C.POINTER(vp), C.POINTER(FrameView)]
         L.mx_frame_free.argtypes = [vp]
@@ -180,6 +187,7 @@ class Context:
         rc = lib().mx_ctx_create(device, C.byref(self.h))
         if rc != MX_OK:
             raise MxError(rc, "mx_ctx_create")
+        self.cu_count = lib().mx_ctx_cu_count(self.h)
 
     def close(self):
         if self.h:
@@ -250,8 +258,9 @@ class Context:
         return Ref(ref)
 
     def tile(self, fv, stride=1, form=MX_FORM_FE64, ede=False, interior=True, int_rep=None, bla=True,
-             bla_minlevel=None, mxit=None, samples=None):
+             bla_minlevel=None, mxit=None, samples=None, chunk=0, grid_blocks=0):
         t = Tile()
+        t.chunk, t.grid_blocks = chunk, grid_blocks
         t.width, t.height, t.stride = fv["W"], fv["H"], stride
         t.max_iter = mxit or fv["mxit"]
         t.esc2, t.form, t.ede = fv["esc2"], form, int(ede)
@@ -291,6 +300,13 @@ class Context:
         self.check(lib().mx_plan_create(self.h, ref.h, C.byref(tile), C.byref(p)), "mx_plan_create")
         return Plan(self, p, tile)
 
+    def batch(self, plans):
+        """One persistent launch over several plans (run in the given order)."""
+        arr = (C.c_void_p * len(plans))(*[p.h for p in plans])
+        b = C.c_void_p()
+        self.check(lib().mx_batch_create(self.h, arr, len(plans), C.byref(b)), "mx_batch_create")
+        return Batch(self, b, plans)
+
 
 class Ref:
     def __init__(self, h):
@@ -299,6 +315,24 @@ class Ref:
     def release(self):
         if self.h:
             lib().mx_ref_release(self.h)
+            self.h = C.c_void_p()
+
+
+class Batch:
+    def __init__(self, ctx, h, plans):
+        self.ctx, self.h, self.plans = ctx, h, list(plans)
+
+    def launch(self):
+        self.ctx.check(lib().mx_batch_launch(self.h), "mx_batch_launch")
+
+    def wait(self):
+        ms = C.c_double()
+        self.ctx.check(lib().mx_batch_wait(self.h, C.byref(ms)), "mx_batch_wait")
+        return ms.value
+
+    def destroy(self):
+        if self.h:
+            lib().mx_batch_destroy(self.h)
             self.h = C.c_void_p()
 
 
