@@ -133,7 +133,9 @@ def main():
     ap.add_argument("--no-double", action="store_true", help="skip the plain-double path (640x360 @ scale 4e10) leg")
     ap.add_argument("--no-e2e", action="store_true", help="skip the end-to-end mx_compute frame")
     ap.add_argument("--c4", action="store_true", help="add the batched uneven-tiles leg (96 tiles, 12 references)")
-    ap.add_argument("--tune", action="store_true", help="autotune chunk / grid size / BLA cutoff before timing")
+    ap.add_argument("--no-tune", action="store_true", help="skip the autotuner (chunk / grid size / BLA cutoff)")
+    ap.add_argument("--c5", action="store_true", help="add the zoom-sequence leg (hipGraph of 1280x720 frames)")
+    ap.add_argument("--c5-frames", type=int, default=240, help="first N frames of the zoom sequence")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: --gpus {args.gpus} refused: this build runs the path on one GPU only "
@@ -164,10 +166,10 @@ def main():
             interior = bool(((pn < 0) & (pn != np.iinfo(np.int32).min)).any())
             log(f"probe {probe_ms:.1f} ms, interior detection {'on' if interior else 'off'}")
             tuned = {}
-            if args.tune:
+            if not args.no_tune:
                 for f in [args.form] + compare:
                     r = autotune.tune(ctx, ref, fv, FORMS[f], ede=ede, interior=interior, log=log)
-                    tuned[f] = {k: r[k] for k in ("chunk", "blocks_per_cu", "bla_minlevel", "stride", "ms")}
+                    tuned[f] = {k: r[k] for k in ("chunk", "blocks_per_cu", "bla_minlevel", "window", "ms")}
                     tuned[f]["trials"] = len(r["trials"])
                     log(f"autotune {f}: {tuned[f]}")
             res[args.form] = run_form(ctx, ref, fv, args.form, ede, interior, center, args.steps, args.warmup,
@@ -187,6 +189,7 @@ def main():
             log(f"end-to-end frame (mx_compute): {e2e_ms:.0f} ms")
         double = None if args.no_double else double_leg(ctx, args.no_cpu)
         batched = c4_leg(ctx) if args.c4 else None
+        zoom = c5_leg(ctx, args.c5_frames) if args.c5 else None
     main_r = res[args.form]
     forms = {f: summary(f, r, mxit, ede) for f, r in res.items()}
     s = forms[args.form]
@@ -240,6 +243,8 @@ def main():
         line["batched_tiles"] = batched
     if tuned:
         line["autotune"] = tuned
+    if zoom is not None:
+        line["zoom_sequence"] = zoom
     print(json.dumps(line))
 
 
@@ -319,10 +324,9 @@ def c4_leg(ctx, forms=("fe32", "fe64"), reps=1):
             plans, nominal_of = [], []
             for q in spec["requests"]:
                 fv, ref = locs[q["location"]]
-                x0, y0, s_ = q["x0"], q["y0"], q["side"]
-                sub = dict(fv, W=s_, H=s_, dcol_m=fv["dcol_m"][x0:x0 + s_], dcol_e=fv["dcol_e"][x0:x0 + s_],
-                           drow_m=fv["drow_m"][y0:y0 + s_], drow_e=fv["drow_e"][y0:y0 + s_])
+                sub = autotune.subframe(fv, q["x0"], q["y0"], q["side"], q["side"])
                 t = ctx.tile(sub, form=FORMS[form], interior=True, mxit=q["max_iter"])
+                t.skip_sample = sub["skip"]
                 plans.append(ctx.plan(ref, t))
             order = sorted(range(len(plans)), key=lambda k: -spec["requests"][k]["side"] ** 2 * spec["requests"][k]["max_iter"])
             b = ctx.batch([plans[k] for k in order])
@@ -368,6 +372,104 @@ def c4_leg(ctx, forms=("fe32", "fe64"), reps=1):
         out[f"{forms[0]}_vs_{forms[1]}"] = {"n_esc_identical_frac": same / out["samples"],
                                            "speedup": res[forms[1]][0]["batch_ms"] / res[forms[0]][0]["batch_ms"]}
     return out
+
+
+def c5_frames(n=240, width_end_exp=500, mxit0=1000, mxit1=5_000_000):
+    """Frame k of the zoom: view width 4 (1e-500/4)^(k/(n-1)) (scale = 4/width, as a
+    decimal integer) and max_iter 1e3 + (5e6 - 1e3) k/(n-1), linear in k."""
+    from decimal import Decimal, getcontext
+    getcontext().prec = width_end_exp + 40
+    end = Decimal(10) ** (-width_end_exp) / 4
+    out = []
+    for k in range(n):
+        t = Decimal(k) / Decimal(n - 1)
+        scale = (Decimal(1) / end) ** t  # 4 / (4 end^t)
+        out.append(dict(k=k, scale=format(scale.to_integral_value(), "f"), scale_d=scale,
+                        mxit=int(round(mxit0 + (mxit1 - mxit0) * k / (n - 1)))))
+    return out
+
+
+def c5_leg(ctx, nframes=240, forms=("fe32", "fe64")):
+    """Zoom animation: 1280x720 frames from width 4 to 1e-500 around the seed-42 walk
+    centre, captured as one HIP graph per form.  Frames at scale <= 1e6 (the engine's
+    shallow, non-perturbation path) are left out; every other frame runs in the
+    floatexp forms against a reference at the exact view centre, recomputed only when
+    the precision (in 64-bit limbs) grows and built to the largest max_iter of its
+    frames, with one BLA table per reference (the shallowest frame's, valid for the
+    deeper ones)."""
+    seed = 42
+    path = os.path.join(ROOT, "tests", "golden", "walk_seed42.json")
+    walk = json.load(open(path)) if os.path.exists(path) else {}
+    if "500" not in walk:
+        # until the seed-42 walk reaches 1e-500: the seed-7 centre, said in the output
+        seed = 7
+        walk = json.load(open(os.path.join(ROOT, "tests", "golden", "walk_seed7.json")))
+        log("c5: no seed-42 centre at 1e-500 yet (tests/golden/make_walk.py 42 500); using the seed-7 centre")
+    cre, cim = walk["500"]["re"], walk["500"]["im"]
+    W, H = 1280, 720
+    opts = mxhip.MX_OPT_VIEW_CENTER_REF | mxhip.MX_OPT_FLOATEXP
+    frames = [f for f in c5_frames()[:nframes] if f["scale_d"] > 1000000]
+    for f in frames:
+        f["prec"] = int(len(f["scale"]) * 3.321928094887362) + 64
+        f["limbs"] = (f["prec"] + 127) // 64
+    groups = {}
+    for f in frames:
+        groups.setdefault(f["limbs"], []).append(f)
+    t0 = time.time()
+    refs, plans_by_form = [], {fm: [] for fm in forms}
+    nominal_mxit = []
+    try:
+        for limbs, fs in sorted(groups.items()):
+            prec = max(f["prec"] for f in fs)
+            mx = max(f["mxit"] for f in fs)
+            gv = mxhip.frame_prepare(cre, cim, fs[0]["scale"], W, H, mx, prec_bits=prec, options=opts)
+            ref = ctx.upload(gv)
+            refs.append(ref)
+            for f in fs:
+                fv = mxhip.frame_prepare(cre, cim, f["scale"], W, H, 1, prec_bits=prec, options=opts, use_bla=0)
+                fv = dict(fv, nlev=gv["nlev"], bla_minlevel=gv["bla_minlevel"])
+                for fm in forms:
+                    t = ctx.tile(fv, form=FORMS[fm], interior=True, mxit=f["mxit"])
+                    plans_by_form[fm].append(ctx.plan(ref, t))
+                nominal_mxit.append(f["mxit"])
+        prep_s = time.time() - t0
+        log(f"c5: {len(frames)} frames, {len(groups)} references, host {prep_s:.1f} s")
+        out = {"workload": f"{len(frames)} frames 1280x720, width 4 -> 1e-500 (frames at scale <= 1e6 left out), "
+                           f"seed-{seed} walk centre, max_iter 1e3 -> 5e6 linear, smooth, BLA, interior detection",
+               "frames": len(frames), "references": len(groups), "host_prepare_s": prep_s}
+        for fm in forms:
+            plans = plans_by_form[fm]
+            g = ctx.graph(plans)
+            g.launch()
+            g.wait()  # warm-up
+            t1 = time.perf_counter()
+            g.launch()
+            gms = g.wait()
+            wall = time.perf_counter() - t1
+            nominal = 0
+            for p, mx in zip(plans, nominal_mxit):
+                nesc = p.read()[0]
+                nominal += int(np.where(nesc > 0, nesc, mx).astype(np.int64).sum())
+            r = {"graph_ms": wall * 1e3, "graph_gpu_ms": gms, "ms_per_frame": wall * 1e3 / len(plans),
+                 "nominal_pixel_iterations": nominal, "gpix_iter_s": nominal / wall / 1e9}
+            if fm == forms[0]:
+                t1 = time.perf_counter()
+                for p in plans:
+                    p.launch()
+                    p.wait()
+                r["frame_by_frame_ms"] = (time.perf_counter() - t1) * 1e3
+            g.destroy()
+            log(f"c5 {fm}: graph {wall * 1e3:.0f} ms ({r['ms_per_frame']:.1f} ms/frame), {r['gpix_iter_s']:.0f} Gpixel-iter/s")
+            out[fm] = r
+        if len(forms) > 1:
+            out[f"speedup_{forms[0]}_vs_{forms[1]}"] = out[forms[1]]["graph_ms"] / out[forms[0]]["graph_ms"]
+        return out
+    finally:
+        for fm in forms:
+            for p in plans_by_form[fm]:
+                p.destroy()
+        for r_ in refs:
+            r_.release()
 
 
 def mpfr_samples(wl, fv, res, n, center):

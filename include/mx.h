@@ -48,6 +48,7 @@ typedef struct mx_ctx mx_ctx;
 typedef struct mx_ref mx_ref;
 typedef struct mx_plan mx_plan;
 typedef struct mx_batch mx_batch;
+typedef struct mx_graph mx_graph;
 typedef struct mx_frame mx_frame;
 
 /* ---- context ------------------------------------------------------------ */
@@ -181,6 +182,19 @@ int mx_batch_launch(mx_batch* batch);                   /* asynchronous */
 int mx_batch_wait(mx_batch* batch, double* kernel_ms);  /* synchronise; time of the last launch */
 void mx_batch_destroy(mx_batch* batch);
 
+/* ---- captured sequences: a zoom animation's frames as one HIP graph --------- */
+
+/* The engine renders an animation as one Compute call per frame (the export loop of
+ * src/gui/gui_export.cpp:119-140 per image).  A graph records, once, every plan's
+ * reset and launch in the given order on the context's stream; each mx_graph_launch
+ * replays the whole sequence with no per-frame host launch work.  Plans: floatexp
+ * forms, owned by the caller, outliving the graph; results are read per plan with
+ * mx_plan_read after mx_graph_wait.  A replay equals launching the plans one by one. */
+int mx_graph_create(mx_ctx* ctx, mx_plan* const* plans, int32_t n, mx_graph** out);
+int mx_graph_launch(mx_graph* graph);                    /* asynchronous */
+int mx_graph_wait(mx_graph* graph, double* ms);          /* synchronise; time of the last replay */
+void mx_graph_destroy(mx_graph* graph);
+
 /* create + launch + wait + read + destroy */
 int mx_tile_render(mx_ctx* ctx, const mx_ref* ref, const mx_tile* tile, int32_t* n_esc, double* value, float* out, uint64_t* stats4);
 
@@ -206,7 +220,18 @@ typedef struct {
     float* iterations;       /* [width*height] engine float buffer, caller-owned */
     double* values;          /* optional [width*height] */
     int32_t* n_esc;          /* optional [width*height] */
+    int32_t options;         /* MX_OPT_* bits; 0 = the engine's behaviour */
 } mx_request;
+
+/* Request options outside the engine's behaviour, for frame sequences that share one
+ * reference (a zoom animation around a fixed centre):
+ *   MX_OPT_VIEW_CENTER_REF  reference at the exact view centre c0 + d*(n-1)/2 (the
+ *                           engine's fallback reference, mandel_perturbation.cpp:
+ *                           1326-1373) instead of the centre pixel, so that frames of
+ *                           one centre share it;
+ *   MX_OPT_FLOATEXP         the floatexp path at any scale above 1e6 (the engine takes
+ *                           the plain-double path up to 1e280). */
+enum { MX_OPT_VIEW_CENTER_REF = 1, MX_OPT_FLOATEXP = 2 };
 
 typedef struct {
     int32_t path;            /* 1 = plain-double path, 2 = floatexp rescaled path */

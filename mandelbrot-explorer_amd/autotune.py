@@ -10,9 +10,12 @@ chunk and grid_blocks do not change any result.
 
 The search is coordinate descent from the defaults (chunk 24, 8 blocks per CU, the
 frame's own cutoff): each knob in turn is swept with the others fixed and the fastest
-value kept.  A sample is the frame's sample grid taken every `stride` rows and columns
-(about 2 M samples: enough work to fill the grid, a fraction of the frame's cost).
-Results can be kept in a JSON file keyed by form, flags and CU count.
+value kept.  The sample is the central window of the frame with about 2 M samples
+(the frame's aspect ratio): enough work to fill the grid, a fraction of the frame's
+cost, and neighbouring samples side by side as in the frame, so a wave's lanes
+diverge as they do there (a strided subsample puts distant samples in one wave and
+runs several times slower per sample).  Results can be kept in a JSON file keyed by
+form, flags and CU count.
 """
 import json
 import os
@@ -21,13 +24,38 @@ import mxhip
 
 CHUNKS = (8, 16, 24, 32, 48, 64)
 BLOCKS_PER_CU = (2, 3, 4, 5, 6, 8)
-MINLEVELS = (1, 2, 3, 4, 5, 6)
+MINLEVELS = (1, 2, 3, 4, 5, 6, 7, 8, 9, 10)
 
 
-def _time(ctx, ref, fv, form, ede, interior, stride, mxit, knobs, reps):
-    t = ctx.tile(fv, stride=stride, form=form, ede=ede, interior=interior, mxit=mxit,
+def window(fv, target_samples):
+    """The central window of frame `fv` holding about target_samples samples."""
+    W, H = fv["W"], fv["H"]
+    f = min(1.0, (target_samples / float(W * H)) ** 0.5)
+    w, h = max(8, int(W * f)) // 8 * 8 or W, max(8, int(H * f)) // 8 * 8 or H
+    w, h = min(w, W), min(h, H)
+    x0, y0 = (W - w) // 2, (H - h) // 2
+    return subframe(fv, x0, y0, w, h)
+
+
+def subframe(fv, x0, y0, w, h):
+    """The w x h block of frame `fv` at column x0, row y0, with the index of the frame's
+    reference pixel inside it ("skip", -1 when outside or virtual): that sample's value
+    comes from the reference orbit, and its delta iteration (dc = 0) would run the
+    whole orbit one step at a time."""
+    sub = dict(fv, W=w, H=h, dcol_m=fv["dcol_m"][x0:x0 + w], dcol_e=fv["dcol_e"][x0:x0 + w],
+               drow_m=fv["drow_m"][y0:y0 + h], drow_e=fv["drow_e"][y0:y0 + h], skip=-1)
+    if not fv.get("ref_virtual", 1):
+        ri, rj = int(fv["ref_y"]) - y0, int(fv["ref_x"]) - x0
+        if 0 <= ri < h and 0 <= rj < w:
+            sub["skip"] = ri * w + rj
+    return sub
+
+
+def _time(ctx, ref, fv, form, ede, interior, mxit, knobs, reps):
+    t = ctx.tile(fv, form=form, ede=ede, interior=interior, mxit=mxit,
                  bla_minlevel=knobs["bla_minlevel"], chunk=knobs["chunk"],
                  grid_blocks=knobs["blocks_per_cu"] * ctx.cu_count)
+    t.skip_sample = fv["skip"]
     p = ctx.plan(ref, t)
     try:
         best = None
@@ -43,16 +71,13 @@ def _time(ctx, ref, fv, form, ede, interior, stride, mxit, knobs, reps):
 def tune(ctx, ref, fv, form, ede=False, interior=True, mxit=None, target_samples=2_000_000, reps=1, log=None):
     """Best knobs for rendering frame `fv` (a frame_prepare dict whose reference is
     uploaded as `ref`) in `form`.  Returns dict(chunk, blocks_per_cu, bla_minlevel,
-    stride, ms, trials)."""
-    n = fv["W"] * fv["H"]
-    stride = 1
-    while n // ((stride + 1) ** 2) >= target_samples:
-        stride += 1
+    window, ms, trials)."""
+    sub = window(fv, target_samples)
     knobs = dict(chunk=24, blocks_per_cu=8, bla_minlevel=fv["bla_minlevel"])
     trials = []
 
     def run(k):
-        ms = _time(ctx, ref, fv, form, ede, interior, stride, mxit, k, reps)
+        ms = _time(ctx, ref, sub, form, ede, interior, mxit, k, reps)
         trials.append(dict(k, ms=ms))
         if log:
             log(f"autotune {k} -> {ms:.1f} ms")
@@ -70,7 +95,7 @@ def tune(ctx, ref, fv, form, ede=False, interior=True, mxit=None, target_samples
             ms = run(k)
             if ms < best:
                 best, knobs = ms, k
-    return dict(knobs, stride=stride, ms=best, trials=trials)
+    return dict(knobs, window=[sub["W"], sub["H"]], ms=best, trials=trials)
 
 
 def key(form, ede, interior, cu_count):

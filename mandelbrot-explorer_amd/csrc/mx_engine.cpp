@@ -558,6 +558,84 @@ extern "C" void mx_batch_destroy(mx_batch* b) {
     delete b;
 }
 
+extern "C" int mx_graph_create(mx_ctx* ctx, mx_plan* const* plans, int32_t n, mx_graph** out) {
+    if (!ctx || !plans || n < 1 || !out) return MX_E_INVALID;
+    *out = nullptr;
+    for (int32_t k = 0; k < n; ++k)
+        if (!plans[k] || plans[k]->ctx != ctx || !plans[k]->job_dev) {
+            ctx->set_err("a graph takes floatexp-form plans of this context");
+            return MX_E_INVALID;
+        }
+    MX_HIP(hipSetDevice(ctx->device));
+    auto* g = new (std::nothrow) mx_graph();
+    if (!g) return MX_E_NOMEM;
+    g->ctx = ctx;
+    g->plans.assign(plans, plans + n);
+    // every plan's reset (a kernel: no memset node) and launch, in order, on the
+    // context's stream
+    hipError_t e = hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal);
+    hipError_t el = hipSuccess;
+    if (e == hipSuccess) {
+        for (mx_plan* p : g->plans) {
+            const MxTileDev& T = p->tile;
+            hipLaunchKernelGGL(mx_reset_kernel, dim3(1024), dim3(256), 0, ctx->stream, T.queue, T.stats, T.unres_count, T.val, T.nesc, T.out, p->n);
+            el = hipGetLastError();
+            if (el != hipSuccess) break;
+            el = launch_jobs(p->form, p->job_dev, 1, p->ede, p->interior, p->blocks, p->threads, ctx->stream);
+            if (el != hipSuccess) break;
+        }
+        e = hipStreamEndCapture(ctx->stream, &g->graph);
+        if (e == hipSuccess) e = el;
+    }
+    if (e == hipSuccess) e = hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0);
+    if (e == hipSuccess) e = hipEventCreate(&g->ev0);
+    if (e == hipSuccess) e = hipEventCreate(&g->ev1);
+    if (e != hipSuccess) {
+        ctx->set_err(std::string("graph capture: ") + hipGetErrorString(e));
+        mx_graph_destroy(g);
+        return MX_E_HIP;
+    }
+    *out = g;
+    return MX_OK;
+}
+
+extern "C" int mx_graph_launch(mx_graph* g) {
+    if (!g) return MX_E_INVALID;
+    mx_ctx* ctx = g->ctx;
+    MX_HIP(hipSetDevice(ctx->device));
+    MX_HIP(hipEventRecord(g->ev0, ctx->stream));
+    MX_HIP(hipGraphLaunch(g->exec, ctx->stream));
+    MX_HIP(hipEventRecord(g->ev1, ctx->stream));
+    return MX_OK;
+}
+
+extern "C" int mx_graph_wait(mx_graph* g, double* ms) {
+    if (!g) return MX_E_INVALID;
+    mx_ctx* ctx = g->ctx;
+    MX_HIP(hipStreamSynchronize(ctx->stream));
+    if (ms) {
+        float f = 0.f;
+        MX_HIP(hipEventElapsedTime(&f, g->ev0, g->ev1));
+        *ms = f;
+    }
+    if (__atomic_load_n(ctx->cancel_host, __ATOMIC_ACQUIRE)) {
+        ctx->set_err("cancelled");
+        return MX_E_CANCELLED;
+    }
+    return MX_OK;
+}
+
+extern "C" void mx_graph_destroy(mx_graph* g) {
+    if (!g) return;
+    (void)hipSetDevice(g->ctx->device);
+    (void)hipStreamSynchronize(g->ctx->stream);
+    if (g->exec) (void)hipGraphExecDestroy(g->exec);
+    if (g->graph) (void)hipGraphDestroy(g->graph);
+    if (g->ev0) (void)hipEventDestroy(g->ev0);
+    if (g->ev1) (void)hipEventDestroy(g->ev1);
+    delete g;
+}
+
 extern "C" int mx_tile_render(mx_ctx* ctx, const mx_ref* ref, const mx_tile* tile, int32_t* n_esc, double* value, float* out, uint64_t* stats4) {
     if (!ctx) return MX_E_INVALID;
     BusyGuard busy(ctx);

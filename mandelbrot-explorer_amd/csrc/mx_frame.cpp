@@ -702,7 +702,8 @@ int Frame::prepare(const mx_request& r, std::string& err) {
         err = "scale <= 1e6: the engine's shallow (non-perturbation) path is not on this path";
         return MX_E_UNSUPPORTED;
     }
-    dbl = mpf_cmp_d(scale, 1e280) <= 0;
+    dbl = mpf_cmp_d(scale, 1e280) <= 0 && !(r.options & MX_OPT_FLOATEXP);
+    const bool view_ref = (r.options & MX_OPT_VIEW_CENTER_REF) != 0;
     // depth guard for the int32 exponent of the device FloatExp: |log2 scale| < 2^26
     {
         long se;
@@ -771,7 +772,7 @@ int Frame::prepare(const mx_request& r, std::string& err) {
         ref_virtual = false;
         ref_x = pj;
         ref_y = pi;
-        if (escalated) {
+        if (escalated || view_ref) {
             mpf_mul_ui(t, dx.v, W - 1);
             mpf_div_ui(t, t, 2);
             mpf_add(pre, c0_re.v, t);
@@ -786,7 +787,7 @@ int Frame::prepare(const mx_request& r, std::string& err) {
         } else {
             build_orbit(pre, pim);
         }
-        if (!escalated && (!ref_escaped || ref_it >= max_iter - 16)) {
+        if (!escalated && !view_ref && (!ref_escaped || ref_it >= max_iter - 16)) {
             const bool sensitive = ref_escaped || tail_escapes(80);
             if (sensitive) {
                 mpf_mul_ui(t, dx.v, W - 1);

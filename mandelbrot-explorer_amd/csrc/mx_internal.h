@@ -160,6 +160,14 @@ struct mx_batch {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
 };
 
+struct mx_graph {
+    mx_ctx* ctx = nullptr;
+    std::vector<mx_plan*> plans;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+};
+
 hipError_t mx_launch_fe64(const MxJob* jobs, int njobs, bool ede, bool interior, int blocks, int threads, hipStream_t stream);
 hipError_t mx_launch_fe32(const MxJob* jobs, int njobs, bool ede, bool interior, int blocks, int threads, hipStream_t stream);
 hipError_t mx_launch_sc64(const MxJob* jobs, int njobs, bool ede, bool interior, int blocks, int threads, hipStream_t stream);

@@ -12,6 +12,7 @@ LIB_PATH = os.path.join(_HERE, "libmxhip.so")
 MX_OK, MX_E_INVALID, MX_E_NODEVICE, MX_E_HIP, MX_E_CANCELLED, MX_E_BUSY, MX_E_NOMEM, MX_E_UNSUPPORTED = 0, -1, -2, -3, -4, -5, -6, -7
 MX_FORM_FE64, MX_FORM_FE32, MX_FORM_DOUBLE, MX_FORM_SC64 = 0, 1, 2, 3
 MX_EXTERIOR_DIST_EST = 2
+MX_OPT_VIEW_CENTER_REF, MX_OPT_FLOATEXP = 1, 2
 ERRORS = {-1: "invalid", -2: "no device", -3: "hip error", -4: "cancelled", -5: "busy", -6: "out of memory", -7: "unsupported"}
 
 
@@ -47,7 +48,7 @@ class Request(C.Structure):
                 ("sub", C.c_int32), ("max_iterations", C.c_int32), ("coloring_method", C.c_int32),
                 ("escape_radius", C.c_float), ("precision_bits", C.c_int32), ("form", C.c_int32),
                 ("use_bla", C.c_int32), ("bla_minskip", C.c_int32), ("interior", C.c_int32),
-                ("iterations", C.c_void_p), ("values", C.c_void_p), ("n_esc", C.c_void_p)]
+                ("iterations", C.c_void_p), ("values", C.c_void_p), ("n_esc", C.c_void_p), ("options", C.c_int32)]
 
 
 class FrameInfo(C.Structure):
@@ -85,7 +86,8 @@ EXPORTS = ["mx_version", "mx_device_count", "mx_ctx_create", "mx_ctx_destroy", "
            "mx_reset_cancellation", "mx_ref_upload", "mx_ref_release", "mx_plan_create", "mx_plan_launch",
            "mx_plan_wait", "mx_plan_read", "mx_plan_unresolved", "mx_plan_destroy", "mx_tile_render", "mx_compute",
            "mx_frame_prepare", "mx_frame_free", "mx_batch_create", "mx_batch_launch", "mx_batch_wait",
-           "mx_batch_destroy", "mx_ctx_cu_count"]
+           "mx_batch_destroy", "mx_ctx_cu_count", "mx_graph_create", "mx_graph_launch", "mx_graph_wait",
+           "mx_graph_destroy"]
 
 _LIB = None
 
@@ -115,6 +117,10 @@ def lib():
         L.mx_plan_unresolved.argtypes = [vp, vp, vp, C.c_int32, C.POINTER(C.c_int32)]
         L.mx_tile_render.argtypes = [vp, vp, C.POINTER(Tile), vp, vp, vp, vp]
         L.mx_ctx_cu_count.argtypes = [vp]
+        L.mx_graph_create.argtypes = [vp, C.POINTER(vp), C.c_int32, C.POINTER(vp)]
+        L.mx_graph_launch.argtypes = [vp]
+        L.mx_graph_wait.argtypes = [vp, C.POINTER(C.c_double)]
+        L.mx_graph_destroy.argtypes = [vp]
         L.mx_batch_create.argtypes = [vp, C.POINTER(vp), C.c_int32, C.POINTER(vp)]
         L.mx_batch_launch.argtypes = [vp]
         L.mx_batch_wait.argtypes = [vp, C.POINTER(C.c_double)]
@@ -131,12 +137,13 @@ def _p(a):
 
 
 def make_request(cre, cim, scale, W, H, mxit, c_method=0, escape_radius=0.0, prec_bits=0, form=MX_FORM_FE64,
-                 use_bla=-1, bla_minskip=0, interior=-1, full_h=0, row_base=0):
+                 use_bla=-1, bla_minskip=0, interior=-1, full_h=0, row_base=0, options=0):
     r = Request()
     r.center_re, r.center_im, r.scale = cre.encode(), cim.encode(), scale.encode()
     r.width, r.height, r.full_height, r.row_base, r.sub = W, H, full_h, row_base, 1
     r.max_iterations, r.coloring_method, r.escape_radius = mxit, c_method, escape_radius
     r.precision_bits, r.form, r.use_bla, r.bla_minskip, r.interior = prec_bits, form, use_bla, bla_minskip, interior
+    r.options = options
     return r
 
 
@@ -300,6 +307,13 @@ class Context:
         self.check(lib().mx_plan_create(self.h, ref.h, C.byref(tile), C.byref(p)), "mx_plan_create")
         return Plan(self, p, tile)
 
+    def graph(self, plans):
+        """The plans' launches, in order, captured as one HIP graph."""
+        arr = (C.c_void_p * len(plans))(*[p.h for p in plans])
+        g = C.c_void_p()
+        self.check(lib().mx_graph_create(self.h, arr, len(plans), C.byref(g)), "mx_graph_create")
+        return Graph(self, g, plans)
+
     def batch(self, plans):
         """One persistent launch over several plans (run in the given order)."""
         arr = (C.c_void_p * len(plans))(*[p.h for p in plans])
@@ -315,6 +329,24 @@ class Ref:
     def release(self):
         if self.h:
             lib().mx_ref_release(self.h)
+            self.h = C.c_void_p()
+
+
+class Graph:
+    def __init__(self, ctx, h, plans):
+        self.ctx, self.h, self.plans = ctx, h, list(plans)
+
+    def launch(self):
+        self.ctx.check(lib().mx_graph_launch(self.h), "mx_graph_launch")
+
+    def wait(self):
+        ms = C.c_double()
+        self.ctx.check(lib().mx_graph_wait(self.h, C.byref(ms)), "mx_graph_wait")
+        return ms.value
+
+    def destroy(self):
+        if self.h:
+            lib().mx_graph_destroy(self.h)
             self.h = C.c_void_p()
 
 

@@ -13,9 +13,11 @@ Definition (the workload spec's batch, with the choices it leaves open fixed):
   share one reference orbit (built to the largest max_iter among them), one BLA
   table and one pair of dc tables.
 E_HI is 300 here (the spec's range reaches 1e-1000; walks to that depth take hours
-per seed on the CPU).
+per seed on the CPU).  The location walks are make_walk.py's, probe max_iter capped
+at 2^21 (at 2^16 they stall near 1e-41, every probe of the region exceeding it).
 
-Writes tests/golden/c4_requests.json and the walk fixtures walk_seed<k>.json.
+Writes tests/golden/c4_seed<k>.json (one location centre each) and, once all twelve
+exist, tests/golden/c4_requests.json.
 Usage: python tests/golden/make_c4.py [--plan] [--seeds 1,2,...]
   --plan: print the draws only; --seeds: walk only these locations (the request file
   is written once every location's centre exists)
@@ -33,6 +35,7 @@ E_LO, E_HI = 50, 300
 SIDES = (64, 128, 256, 512, 1024)
 NLOC, NREQ = 12, 96
 GRID = 1024
+WALK_CAP = 1 << 21
 
 
 def unit(rng):
@@ -67,18 +70,15 @@ def main():
         seeds = {int(x) for x in sys.argv[sys.argv.index("--seeds") + 1].split(",")}
     out = {"_definition": "tests/golden/make_c4.py (seeded batch of uneven tiles)", "locations": [], "requests": reqs}
     for l in locs:
-        path = os.path.join(HERE, f"walk_seed{l['seed']}.json")
-        have = json.load(open(path)) if os.path.exists(path) else {}
-        key = str(l["depth"])
-        if key not in have and (seeds is None or l["seed"] in seeds):
-            res = walk(l["seed"], [l["depth"]], log=lambda s: print(s, flush=True))
-            have = json.load(open(path)) if os.path.exists(path) else {}
-            have.update(res)
-            have["_definition"] = "tests/golden/make_walk.py (seeded boundary walk)"
-            json.dump(have, open(path, "w"), indent=1)
+        path = os.path.join(HERE, f"c4_seed{l['seed']}.json")
+        if not os.path.exists(path) and (seeds is None or l["seed"] in seeds):
+            res = walk(l["seed"], [l["depth"]], log=lambda s: print(s, flush=True), mxit_cap=WALK_CAP)
+            c = res[str(l["depth"])]
+            json.dump(dict(seed=l["seed"], depth=l["depth"], re=c["re"], im=c["im"], width=c["width"],
+                           steps=c["steps"], probe_mxit=c["probe_mxit"], walk_mxit_cap=WALK_CAP), open(path, "w"), indent=1)
             print("location", l, "done", flush=True)
-        if key in have:
-            c = have[key]
+        if os.path.exists(path):
+            c = json.load(open(path))
             out["locations"].append(dict(seed=l["seed"], depth=l["depth"], re=c["re"], im=c["im"]))
     if len(out["locations"]) == NLOC:
         json.dump(out, open(os.path.join(HERE, "c4_requests.json"), "w"), indent=1)

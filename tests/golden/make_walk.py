@@ -53,7 +53,7 @@ class SplitMix64:
         return z ^ (z >> 31)
 
 
-def walk(seed, depths, log=print, on_depth=None, resume=None):
+def walk(seed, depths, log=print, on_depth=None, resume=None, mxit_cap=MXIT_CAP):
     depths = sorted(depths)
     rng = SplitMix64(seed)
     getcontext().prec = max(depths) + 80
@@ -98,7 +98,7 @@ def walk(seed, depths, log=print, on_depth=None, resume=None):
             flat = nesc.ravel()
             cand = [(-int(flat[k]), k) for k in range(W * H) if flat[k] > 0]
             top = -min(cand)[0] if cand else 0
-            if (len(cand) >= 41 and top <= mxit // 2) or mxit >= MXIT_CAP:
+            if (len(cand) >= 41 and top <= mxit // 2) or mxit >= mxit_cap:
                 break
             mxit *= 2
         if not cand:
