@@ -425,6 +425,7 @@ def c5_leg(ctx, nframes=240, forms=("fe32", "fe64")):
             gv = mxhip.frame_prepare(cre, cim, fs[0]["scale"], W, H, mx, prec_bits=prec, options=opts)
             ref = ctx.upload(gv)
             refs.append(ref)
+            log(f"c5: reference {len(refs)}/{len(groups)} ({len(fs)} frames, {prec} bits, ref_it {gv['ref_it']})")
             for f in fs:
                 fv = mxhip.frame_prepare(cre, cim, f["scale"], W, H, 1, prec_bits=prec, options=opts, use_bla=0)
                 fv = dict(fv, nlev=gv["nlev"], bla_minlevel=gv["bla_minlevel"])
@@ -441,7 +442,7 @@ def c5_leg(ctx, nframes=240, forms=("fe32", "fe64")):
             plans = plans_by_form[fm]
             g = ctx.graph(plans)
             g.launch()
-            g.wait()  # warm-up
+            log(f"c5 {fm}: warm-up replay {g.wait():.0f} ms")
             t1 = time.perf_counter()
             g.launch()
             gms = g.wait()
@@ -453,6 +454,7 @@ def c5_leg(ctx, nframes=240, forms=("fe32", "fe64")):
             r = {"graph_ms": wall * 1e3, "graph_gpu_ms": gms, "ms_per_frame": wall * 1e3 / len(plans),
                  "nominal_pixel_iterations": nominal, "gpix_iter_s": nominal / wall / 1e9}
             if fm == forms[0]:
+                log(f"c5 {fm}: graph replay {wall * 1e3:.0f} ms; frame by frame next")
                 t1 = time.perf_counter()
                 for p in plans:
                     p.launch()

@@ -146,6 +146,13 @@ typedef struct {
      * per 64 samples). */
     int32_t chunk;
     int32_t grid_blocks;
+    /* floatexp forms: explicit samples instead of the grid (adaptive supersampling's
+     * sub-samples, mandel_perturbation.cpp:1886-1981): n_samples entries whose dc is
+     * (sample_dc_m[2q] 2^sample_dc_e[2q], sample_dc_m[2q+1] 2^sample_dc_e[2q+1]); outputs
+     * are indexed by entry, width/height/stride and the dc tables are not used */
+    int32_t n_samples;
+    const double* sample_dc_m;
+    const int32_t* sample_dc_e;
 } mx_tile;
 
 /* A plan holds the device copy of a tile's inputs and outputs so that launches
@@ -207,7 +214,13 @@ typedef struct {
     int32_t width, height;   /* ComputeRequest::width/height */
     int32_t full_height;     /* strip export: 0 = the whole image (mandel_perturbation.cpp:1399-1424) */
     int32_t row_base;
-    int32_t sub;             /* must be 1 on this path */
+    int32_t sub;             /* output cells per pixel side, odd; the iteration (and
+                                values, n_esc) buffers hold (height sub) x (width sub)
+                                cells, pixel (i, j) at its centre (getIndex,
+                                mandel_perturbation.cpp:4334-4340).  With
+                                MX_SUPER_SAMPLING the other cells of pixels that differ
+                                from a neighbour are computed (adaptive supersampling,
+                                floatexp path); cells not computed are left as they are */
     int32_t max_iterations;
     int32_t coloring_method; /* 0 or MX_EXTERIOR_DIST_EST */
     float escape_radius;     /* 0 => 1e8f (MANDEL_BAILOUT) */
@@ -255,6 +268,8 @@ typedef struct {
     int32_t sa_it;           /* first reference's series-approximation skip */
     int32_t escalated;       /* scale <= 1e280 but run on the floatexp path because the
                                 reference passes too close to zero (mandel_perturbation.cpp:1336-1355) */
+    int32_t ss_pixels;       /* pixels refined by adaptive supersampling */
+    double ss_ms;            /* their sub-sample pass (kernel) */
 } mx_frame_info;
 
 int mx_compute(mx_ctx* ctx, const mx_request* req, mx_frame_info* info);

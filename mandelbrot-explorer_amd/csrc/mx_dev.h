@@ -75,7 +75,9 @@ struct MxTileDev {
     const double* dcol_d;  // per column dc (double)
     const double* drow_d;  // per row dc (double)
     const int* list;       // sample indices to compute (null: the grid)
-    int list_len;
+    int list_len;          // entries of `list`, or of `sdc`
+    const Fe* sdc;         // floatexp forms: explicit (re, im) dc per sample (null: the grid);
+                           // outputs are then indexed by list entry
     int* unres_idx;        // unresolved samples of this launch
     double* unres_gp;      // their glitch metric
     unsigned* unres_count;

@@ -256,13 +256,20 @@ extern "C" void mx_ref_release(mx_ref* ref) {
 }
 
 extern "C" int mx_plan_create(mx_ctx* ctx, const mx_ref* ref, const mx_tile* t, mx_plan** out) {
-    if (!ctx || !ref || !t || !out || t->width < 1 || t->height < 1 || t->stride < 1 || t->max_iter < 1) return MX_E_INVALID;
+    if (!ctx || !ref || !t || !out || t->max_iter < 1) return MX_E_INVALID;
+    if (!t->sample_dc_m && (t->width < 1 || t->height < 1 || t->stride < 1)) return MX_E_INVALID;
+    if (t->sample_dc_m && t->n_samples > INT_MAX / 2) return MX_E_INVALID;
     const bool dbl = t->form == MX_FORM_DOUBLE;
     if (t->form != MX_FORM_FE64 && t->form != MX_FORM_FE32 && t->form != MX_FORM_SC64 && !dbl) {
         ctx->set_err("unknown kernel form");
         return MX_E_INVALID;
     }
-    if (!dbl && (!t->dcol_m || !t->dcol_e || !t->drow_m || !t->drow_e)) return MX_E_INVALID;
+    const bool sdc = t->sample_dc_m != nullptr;
+    if (sdc && (dbl || !t->sample_dc_e || t->n_samples < 1 || t->list)) {
+        ctx->set_err("explicit samples need a floatexp form, both dc arrays and n_samples >= 1");
+        return MX_E_INVALID;
+    }
+    if (!dbl && !sdc && (!t->dcol_m || !t->dcol_e || !t->drow_m || !t->drow_e)) return MX_E_INVALID;
     if (dbl && (!t->dcol_d || !t->drow_d || !ref->dev.z_m3 || (t->ede && !ref->dev.df))) {
         ctx->set_err("the plain-double form needs double dc tables and a reference with its series "
                      "(and its derivative under EDE)");
@@ -273,7 +280,7 @@ extern "C" int mx_plan_create(mx_ctx* ctx, const mx_ref* ref, const mx_tile* t, 
         for (int32_t q = 0; q < t->list_len; ++q)
             if (t->list[q] < 0 || (int64_t)t->list[q] >= (int64_t)t->width * t->height) return MX_E_INVALID;
     if (t->bla && (ref->dev.nlev == 0 || t->bla_minlevel < 0)) return MX_E_INVALID;
-    if ((int64_t)t->width * t->height > (int64_t)INT_MAX / 2) return MX_E_INVALID;
+    if (!t->sample_dc_m && (int64_t)t->width * t->height > (int64_t)INT_MAX / 2) return MX_E_INVALID;
     *out = nullptr;
     MX_HIP(hipSetDevice(ctx->device));
     auto* plan = new (std::nothrow) mx_plan();
@@ -281,7 +288,7 @@ extern "C" int mx_plan_create(mx_ctx* ctx, const mx_ref* ref, const mx_tile* t, 
     plan->ctx = ctx;
     plan->ref = ref;
     plan->form = t->form;
-    const size_t W = (size_t)t->width, H = (size_t)t->height, n = W * H;
+    const size_t W = sdc ? (size_t)t->n_samples : (size_t)t->width, H = sdc ? 1 : (size_t)t->height, n = W * H;
     plan->n = n;
     const size_t nlist = t->list ? (size_t)t->list_len : 0;
     // unresolved samples: at most one per sample of the launch
@@ -299,7 +306,8 @@ extern "C" int mx_plan_create(mx_ctx* ctx, const mx_ref* ref, const mx_tile* t, 
     size_t o_uidx = align_up(o_list + 4 * nlist);
     size_t o_ugp = align_up(o_uidx + 4 * nunres);
     size_t o_ucnt = align_up(o_ugp + 8 * nunres);
-    size_t o_job = align_up(o_ucnt + 4);
+    size_t o_sdc = align_up(o_ucnt + 4);
+    size_t o_job = align_up(o_sdc + (sdc ? sizeof(Fe) * 2 * n : 0));
     size_t bytes = align_up(o_job + sizeof(MxJob));
     if (hipMalloc(&plan->block, bytes) != hipSuccess) {
         delete plan;
@@ -309,7 +317,9 @@ extern "C" int mx_plan_create(mx_ctx* ctx, const mx_ref* ref, const mx_tile* t, 
     plan->bytes = bytes;
     char* b = (char*)plan->block;
     std::vector<Fe> col(W, Fe{0.0, 0}), row(H, Fe{0.0, 0});
-    if (!dbl) {
+    std::vector<Fe> sv(sdc ? 2 * n : 0);
+    for (size_t q = 0; q < sv.size(); ++q) sv[q] = fe_make(t->sample_dc_m[q], t->sample_dc_e[q]);
+    if (!dbl && !sdc) {
         for (size_t j = 0; j < W; ++j) col[j] = fe_make(t->dcol_m[j], t->dcol_e[j]);
         for (size_t i = 0; i < H; ++i) row[i] = fe_make(t->drow_m[i], t->drow_e[i]);
     }
@@ -318,6 +328,7 @@ extern "C" int mx_plan_create(mx_ctx* ctx, const mx_ref* ref, const mx_tile* t, 
     if (dbl && e == hipSuccess) e = hipMemcpyAsync(b + o_dcd, t->dcol_d, 8 * W, hipMemcpyHostToDevice, ctx->stream);
     if (dbl && e == hipSuccess) e = hipMemcpyAsync(b + o_drd, t->drow_d, 8 * H, hipMemcpyHostToDevice, ctx->stream);
     if (nlist && e == hipSuccess) e = hipMemcpyAsync(b + o_list, t->list, 4 * nlist, hipMemcpyHostToDevice, ctx->stream);
+    if (sdc && e == hipSuccess) e = hipMemcpyAsync(b + o_sdc, sv.data(), sizeof(Fe) * sv.size(), hipMemcpyHostToDevice, ctx->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
     if (e == hipSuccess) e = hipEventCreate(&plan->ev0);
     if (e == hipSuccess) e = hipEventCreate(&plan->ev1);
@@ -330,13 +341,13 @@ extern "C" int mx_plan_create(mx_ctx* ctx, const mx_ref* ref, const mx_tile* t, 
     std::memset(&T, 0, sizeof T);
     T.dcol = (const Fe*)(b + o_col);
     T.drow = (const Fe*)(b + o_row);
-    T.W = t->width;
-    T.H = t->height;
-    T.stride = t->stride;
-    T.nx = (t->width + t->stride - 1) / t->stride;
-    T.ny = (t->height + t->stride - 1) / t->stride;
+    T.W = (int)W;
+    T.H = (int)H;
+    T.stride = sdc ? 1 : t->stride;
+    T.nx = (T.W + T.stride - 1) / T.stride;
+    T.ny = (T.H + T.stride - 1) / T.stride;
     T.tiles_x = (T.nx + 7) / 8;
-    T.ntiles = T.tiles_x * ((T.ny + 7) / 8);
+    T.ntiles = sdc ? (int)((n + 63) / 64) : T.tiles_x * ((T.ny + 7) / 8);
     T.mxit = t->max_iter;
     T.flags = (t->ede ? MX_F_EDE : 0) | (t->interior ? MX_F_INTERIOR : 0) | (t->int_rep ? MX_F_INT_REP : 0) | (t->bla ? MX_F_BLA : 0);
     T.esc2 = t->esc2;
@@ -361,6 +372,11 @@ extern "C" int mx_plan_create(mx_ctx* ctx, const mx_ref* ref, const mx_tile* t, 
     if (nlist) {
         T.list = (const int*)(b + o_list);
         T.list_len = (int)nlist;
+    }
+    if (sdc) {
+        T.sdc = (const Fe*)(b + o_sdc);
+        T.list_len = (int)n;
+        T.skip_sample = -1;
     }
     plan->threads = 256;
     plan->minlevel = t->bla_minlevel;
@@ -655,6 +671,52 @@ extern "C" int mx_tile_render(mx_ctx* ctx, const mx_ref* ref, const mx_tile* til
 
 namespace {
 
+// cell of sample (i, j, u, v) in the (H sub) x (W sub) output (getIndex,
+// mandel_perturbation.cpp:4334-4340)
+struct CellMap {
+    int W, S;
+    size_t operator()(int i, int j, int u, int v) const {
+        return ((size_t)i * S + S / 2 + u) * (size_t)W * S + ((size_t)j * S + S / 2 + v);
+    }
+};
+
+// Pixels (row-major) that adaptive supersampling refines, from the base values in
+// the output buffer (mandel_perturbation.cpp:1796-1883): under EDE a sign change
+// against a neighbour (the interior/exterior edge); otherwise log of the largest
+// difference to a neighbour above log(max_iter) / 8 (MANDEL_SS_K = 8), with the
+// float logarithm of the float difference.
+std::vector<int32_t> ss_flags(const mx::Frame& F, const float* it, const CellMap& cm) {
+    std::vector<int32_t> out;
+    const double thresh = std::log((double)F.max_iter) / 8.0;
+    for (int i = 0; i < F.H; ++i)
+        for (int j = 0; j < F.W; ++j) {
+            const float cv = it[cm(i, j, 0, 0)];
+            bool need = false;
+            if (F.ede) {
+                for (int xi = -1; xi <= 1 && !need; ++xi)
+                    for (int yi = -1; yi <= 1; ++yi) {
+                        if (xi == 0 && yi == 0) continue;
+                        const int ny = i + yi, nx = j + xi;
+                        if (nx >= 0 && nx < F.W && ny >= 0 && ny < F.H && cv * it[cm(ny, nx, 0, 0)] < 0) {
+                            need = true;
+                            break;
+                        }
+                    }
+            } else {
+                float diff = 0.0f;
+                for (int xi = -1; xi <= 1; ++xi)
+                    for (int yi = -1; yi <= 1; ++yi) {
+                        if (xi == 0 && yi == 0) continue;
+                        const int ny = i + yi, nx = j + xi;
+                        if (nx >= 0 && nx < F.W && ny >= 0 && ny < F.H) diff = std::max(diff, std::fabs(cv - it[cm(ny, nx, 0, 0)]));
+                    }
+                need = std::log(diff) > thresh;
+            }
+            if (need) out.push_back(i * F.W + j);
+        }
+    return out;
+}
+
 int upload_frame_ref(mx_ctx* ctx, const mx::Frame& F, mx_ref** ref) {
     mx_frame_view v;
     F.fill_view(v);
@@ -853,14 +915,17 @@ int compute_dbl(mx_ctx* ctx, mx::Frame& F, const mx_request* req, mx_frame_info*
     }
     if (ref) mx_ref_release(ref);
     if (rc != MX_OK) return rc;
+    const CellMap cm{F.W, F.sub};
     uint64_t nominal = 0;
-    for (size_t k = 0; k < n; ++k) {
-        const int32_t ne = nesc[k];
-        req->iterations[k] = ne == INT_MIN ? -10.0f : ne < 0 ? -2.0f : (float)val[k];
-        nominal += ne > 0 ? (uint64_t)ne : (uint64_t)F.max_iter;
-    }
-    if (req->values) std::memcpy(req->values, val.data(), 8 * n);
-    if (req->n_esc) std::memcpy(req->n_esc, nesc.data(), 4 * n);
+    for (int i = 0; i < F.H; ++i)
+        for (int j = 0; j < F.W; ++j) {
+            const size_t k = (size_t)i * F.W + j, c = cm(i, j, 0, 0);
+            const int32_t ne = nesc[k];
+            req->iterations[c] = ne == INT_MIN ? -10.0f : ne < 0 ? -2.0f : (float)val[k];
+            if (req->values) req->values[c] = val[k];
+            if (req->n_esc) req->n_esc[c] = ne;
+            nominal += ne > 0 ? (uint64_t)ne : (uint64_t)F.max_iter;
+        }
     if (info) {
         std::memset(info, 0, sizeof *info);
         info->path = 1;
@@ -1000,16 +1065,81 @@ extern "C" int mx_compute(mx_ctx* ctx, const mx_request* req, mx_frame_info* inf
         nesc[c] = F.ref_pixel_n;
         val[c] = F.ref_pixel_value;
     }
+    // results into the (H sub) x (W sub) buffer, pixel (i, j) at its centre cell
+    const CellMap cm{F.W, F.sub};
     uint64_t nominal = 0;
-    for (size_t k = 0; k < n; ++k) {
-        const int32_t ne = nesc[k];
-        req->iterations[k] = ne < 0 ? -2.0f : (float)val[k];
-        nominal += ne > 0 ? (uint64_t)ne : (uint64_t)F.max_iter;
+    for (int i = 0; i < F.H; ++i)
+        for (int j = 0; j < F.W; ++j) {
+            const size_t k = (size_t)i * F.W + j, c = cm(i, j, 0, 0);
+            const int32_t ne = nesc[k];
+            req->iterations[c] = ne < 0 ? -2.0f : (float)val[k];
+            if (req->values) req->values[c] = val[k];
+            if (req->n_esc) req->n_esc[c] = ne;
+            nominal += ne > 0 ? (uint64_t)ne : (uint64_t)F.max_iter;
+        }
+    int ss_pixels = 0;
+    double ss_ms = 0.0;
+    if (F.ss) {
+        // adaptive supersampling (mandel_perturbation.cpp:1789-1981): flag the pixels
+        // that differ from a neighbour, then every other cell of each flagged pixel in
+        // one pass against the same reference and BLA table
+        const std::vector<int32_t> flagged = ss_flags(F, req->iterations, cm);
+        ss_pixels = (int)flagged.size();
+        const int S = F.sub, h = S / 2;
+        const size_t m = flagged.size() * (size_t)(S * S - 1);
+        if (m > 0) {
+            std::vector<double> dm(2 * m);
+            std::vector<int32_t> de(2 * m);
+            std::vector<size_t> cells(m);
+            size_t q = 0;
+            for (int32_t pix : flagged) {
+                const int i = pix / F.W, j = pix % F.W;
+                for (int yi = -h; yi <= h; ++yi)
+                    for (int xi = -h; xi <= h; ++xi) {
+                        if (xi == 0 && yi == 0) continue;
+                        // stepParallel's dc of a sub-sample (mandel_perturbation.cpp:2160-2163)
+                        const double px = j + (double)xi / S, py = i + (double)yi / S;
+                        const Fe a = fe_mul_d(F.dxfe, px - F.ref_x), b = fe_mul_d(F.dyfe, py - F.ref_y);
+                        dm[2 * q] = a.m;
+                        de[2 * q] = a.e;
+                        dm[2 * q + 1] = b.m;
+                        de[2 * q + 1] = b.e;
+                        cells[q] = cm(i, j, yi, xi);
+                        ++q;
+                    }
+            }
+            mx_tile ts = t;
+            ts.skip_sample = -1;
+            ts.n_samples = (int32_t)m;
+            ts.sample_dc_m = dm.data();
+            ts.sample_dc_e = de.data();
+            std::vector<int32_t> sn(m);
+            std::vector<double> svl(m);
+            uint64_t st2[4] = {0, 0, 0, 0};
+            rc = mx_ref_upload(ctx, &d, &ref);
+            if (rc != MX_OK) return rc;
+            mx_plan* plan = nullptr;
+            rc = mx_plan_create(ctx, ref, &ts, &plan);
+            if (rc == MX_OK) rc = mx_plan_launch(plan);
+            if (rc == MX_OK) rc = mx_plan_wait(plan, &ss_ms);
+            if (rc == MX_OK) rc = mx_plan_read(plan, sn.data(), svl.data(), nullptr, st2);
+            mx_plan_destroy(plan);
+            mx_ref_release(ref);
+            if (rc != MX_OK) return rc;
+            for (size_t k = 0; k < m; ++k) {
+                const size_t c = cells[k];
+                req->iterations[c] = sn[k] < 0 ? -2.0f : (float)svl[k];
+                if (req->values) req->values[c] = svl[k];
+                if (req->n_esc) req->n_esc[c] = sn[k];
+                nominal += sn[k] > 0 ? (uint64_t)sn[k] : (uint64_t)F.max_iter;
+            }
+            for (int k = 0; k < 4; ++k) st[k] += st2[k];
+        }
     }
-    if (req->values) std::memcpy(req->values, val.data(), 8 * n);
-    if (req->n_esc) std::memcpy(req->n_esc, nesc.data(), 4 * n);
     if (info) {
         std::memset(info, 0, sizeof *info);
+        info->ss_pixels = ss_pixels;
+        info->ss_ms = ss_ms;
         info->path = 2;
         info->ref_it = F.ref_it;
         info->ref_escaped = F.ref_escaped;

@@ -752,6 +752,24 @@ MX_HD bool item_to_sample(const MxTileDev& T, unsigned item, int& si, int& sj) {
     return si * T.W + sj != T.skip_sample;
 }
 
+// The next sample of work item `item`: a grid sample (separable dc tables, output at
+// row * W + column) or an entry of the explicit-dc list (output at the entry).
+MX_HD bool take_sample(const MxTileDev& T, unsigned item, int& sidx, Fe& dcr, Fe& dci) {
+    if (T.sdc) {
+        if (item >= (unsigned)T.list_len) return false;
+        sidx = (int)item;
+        dcr = T.sdc[2 * item];
+        dci = T.sdc[2 * item + 1];
+        return true;
+    }
+    int si, sj;
+    if (!item_to_sample(T, item, si, sj)) return false;
+    sidx = si * T.W + sj;
+    dcr = T.dcol[sj];
+    dci = T.drow[si];
+    return true;
+}
+
 template <typename F, bool EDE, bool INTERIOR>
 __device__ __forceinline__ void run_job32(const MxRefDev& R, const int2* __restrict__ lev, const MxTileDev& T) {
     const int lane = __lane_id();
@@ -779,10 +797,9 @@ __device__ __forceinline__ void run_job32(const MxRefDev& R, const int2* __restr
             if (need) {
                 const unsigned it = base + (unsigned)__popcll(needmask & lanemask_lt);
                 if (it < nitems) {
-                    int si, sj;
-                    if (item_to_sample(T, t0 * 64u + it, si, sj)) {
-                        sidx = si * T.W + sj;
-                        init32(p, T.dcol[sj], T.drow[si]);
+                    Fe dcr, dci;
+                    if (take_sample(T, t0 * 64u + it, sidx, dcr, dci)) {
+                        init32(p, dcr, dci);
                         need = false;
                         active = true;
                     }

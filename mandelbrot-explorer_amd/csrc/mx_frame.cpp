@@ -406,11 +406,13 @@ struct ZRing {
     const int n;                      // slots: 2048, at most ~64 MB
     std::vector<mpf_t> zr, zi;        // slot k % n holds z_k
     std::atomic<long> count{0};       // z_0 .. z_{count-1} are in the ring
-    std::atomic<long> used{0};        // z values the consumer is done with
+    std::atomic<long> used[2];        // z values each consumer is done with
     std::atomic<long> esc_at{LONG_MAX}, d_at{LONG_MAX};
     explicit ZRing(mp_bitcnt_t prec)
         : n((int)std::max<mp_bitcnt_t>(16, std::min<mp_bitcnt_t>(2048, ((mp_bitcnt_t)1 << 29) / (2 * prec + 256)))),
           zr(n), zi(n) {
+        used[0].store(0);
+        used[1].store(0);
         for (int k = 0; k < n; ++k) {
             mpf_init2(zr[k], prec);
             mpf_init2(zi[k], prec);
@@ -427,7 +429,8 @@ struct ZRing {
 } // namespace
 
 // The serial loop's result from fewer multiplies, the orbit on this thread and the
-// EDE derivative dc on a second one reading the z values through a ring:
+// EDE derivative dc on two more (one per component) reading the z values through a
+// ring:
 //  - |z|^2 is formed only when a component can reach the escape radius
 //    (|z|^2 < 2^(2 emax + 1) otherwise);
 //  - d = 2 d z (the interior early-out |d|^2 < 1e-9) is not iterated in mpf: |d_i|^2 =
@@ -444,40 +447,67 @@ double Frame::point_value(mpf_srcptr c_re, mpf_srcptr c_im, int mxit, bool ede, 
     double rad_esc = 0.0;
     bool undecided = false;
 
-    Gmp er(prec), ei(prec);
-    auto eloop = [&] {
-        Gmp t2(prec), e1(prec), e2(prec);
-        mpf_set_ui(er, 1);
-        mpf_set_ui(ei, 0);
-        for (long i = 1; i <= last; ++i) {
-            // needs z_{i-1}; stops past the escape or the interior decision
-            bool go = true;
-            while (Q.count.load(std::memory_order_acquire) < i) {
-                if (Q.esc_at.load(std::memory_order_acquire) < i || Q.d_at.load(std::memory_order_acquire) < i) {
-                    go = false;
-                    break;
-                }
+    // the EDE derivative dc, one thread per component: iteration i of either reads
+    // both components of iteration i-1 (buffers by parity) and starts once the other
+    // component has finished i-1, so neither overwrites a value still being read
+    Gmp er0(prec), er1(prec), ei0(prec), ei1(prec);
+    mpf_ptr ER[2] = {er0.v, er1.v}, EI[2] = {ei0.v, ei1.v};
+    std::atomic<long> done_c[2];
+    done_c[0].store(0);
+    done_c[1].store(0);
+    mpf_set_ui(ER[0], 1);
+    mpf_set_ui(EI[0], 0);
+    auto stopped = [&](long i) {
+        return Q.esc_at.load(std::memory_order_acquire) < i || Q.d_at.load(std::memory_order_acquire) < i;
+    };
+    auto wait_for = [&](const std::atomic<long>& a, long v, long i) {
+        for (int spin = 0; a.load(std::memory_order_acquire) < v; ++spin) {
+            if (stopped(i)) return false;
+            if (spin < 4096)
+                __builtin_ia32_pause();
+            else
                 std::this_thread::yield();
-            }
-            if (!go) break;
+        }
+        return true;
+    };
+    auto eloop = [&](int which) {
+        Gmp a(prec), b(prec);
+        std::atomic<long>& mine = done_c[which];
+        const std::atomic<long>& other = done_c[1 - which];
+        for (long i = 1; i <= last; ++i) {
+            // needs z_{i-1} and the other component of iteration i-1
+            if (!wait_for(Q.count, i, i) || !wait_for(other, i - 1, i)) break;
             mpf_srcptr zr = Q.zr[(i - 1) % RB], zi = Q.zi[(i - 1) % RB];
-            mpf_mul(e1, er, zi);
-            mpf_mul(e2, ei, zr);
-            mpf_add(e2, e1, e2);
-            mpf_mul(e1, er, zr);
-            mpf_mul(t2, ei, zi);
-            Q.used.store(i, std::memory_order_release);
-            mpf_sub(e1, e1, t2);
-            mpf_mul_ui(er, e1, 2);
-            mpf_add_ui(er, er, 1);
-            mpf_mul_ui(ei, e2, 2);
+            mpf_srcptr pr = ER[(i - 1) & 1], pi = EI[(i - 1) & 1];
+            if (which == 0) {
+                // re: 2 (er zr - ei zi) + 1
+                mpf_mul(a, pr, zr);
+                mpf_mul(b, pi, zi);
+                Q.used[0].store(i, std::memory_order_release);
+                mpf_sub(a, a, b);
+                mpf_mul_ui(ER[i & 1], a, 2);
+                mpf_add_ui(ER[i & 1], ER[i & 1], 1);
+            } else {
+                // im: 2 (er zi + ei zr)
+                mpf_mul(a, pr, zi);
+                mpf_mul(b, pi, zr);
+                Q.used[1].store(i, std::memory_order_release);
+                mpf_add(a, a, b);
+                mpf_mul_ui(EI[i & 1], a, 2);
+            }
+            mine.store(i, std::memory_order_release);
             if (Q.esc_at.load(std::memory_order_acquire) == i) break;
         }
-        Q.used.store(LONG_MAX, std::memory_order_release);
+        Q.used[which].store(LONG_MAX, std::memory_order_release);
     };
-    std::thread te;
-    if (ede) te = std::thread(eloop);
-    else Q.used.store(LONG_MAX);
+    std::thread te0, te1;
+    if (ede) {
+        te0 = std::thread(eloop, 0);
+        te1 = std::thread(eloop, 1);
+    } else {
+        Q.used[0].store(LONG_MAX);
+        Q.used[1].store(LONG_MAX);
+    }
     {
         Gmp t1(prec), t2(prec), zr(prec), zi(prec);
         mpf_set(zr, c_re);
@@ -541,14 +571,18 @@ double Frame::point_value(mpf_srcptr c_re, mpf_srcptr c_im, int mxit, bool ede, 
                 }
             }
             if (ede && i < last) {
-                while (Q.used.load(std::memory_order_acquire) <= i - RB) std::this_thread::yield();
+                while (std::min(Q.used[0].load(std::memory_order_acquire), Q.used[1].load(std::memory_order_acquire)) <= i - RB)
+                    std::this_thread::yield();
                 mpf_set(Q.zr[i % RB], zr);
                 mpf_set(Q.zi[i % RB], zi);
                 Q.count.store(i + 1, std::memory_order_release);
             }
         }
     }
-    if (ede) te.join();
+    if (ede) {
+        te0.join();
+        te1.join();
+    }
     if (undecided) return point_value_serial(c_re, c_im, mxit, ede, n_out);
 
     const long ie = Q.esc_at.load(), id = Q.d_at.load();
@@ -559,6 +593,7 @@ double Frame::point_value(mpf_srcptr c_re, mpf_srcptr c_im, int mxit, bool ede, 
         n_out = (int)ie + 1;
         if (ede) {
             Gmp t1(prec), e1(prec), e2(prec);
+            mpf_srcptr er = ER[ie & 1], ei = EI[ie & 1];
             mpf_mul(e1, er, er);
             mpf_mul(e2, ei, ei);
             mpf_add(e1, e1, e2);
@@ -670,10 +705,16 @@ int Frame::prepare(const mx_request& r, std::string& err) {
         err = "invalid request";
         return MX_E_INVALID;
     }
-    if (r.sub != 1) {
-        err = "sub > 1 (adaptive supersampling) is not on this path";
-        return MX_E_UNSUPPORTED;
+    // sub samples per pixel side: the output is the (H sub) x (W sub) buffer of
+    // getIndex (mandel_perturbation.cpp:4334-4340), pixel (i, j) at its centre cell; an
+    // even sub would place the adaptive sub-samples of the last row and column past
+    // the buffer
+    if (r.sub > 1 && (r.sub % 2 == 0 || r.sub > 15)) {
+        err = "sub must be odd and at most 15";
+        return MX_E_INVALID;
     }
+    sub = r.sub;
+    ss = sub > 1 && (r.coloring_method & MX_SUPER_SAMPLING) != 0;
     if ((r.coloring_method & ~(MX_EXTERIOR_DIST_EST | MX_SUPER_SAMPLING)) != 0) {
         err = "coloring method not on this path (smooth and exterior DE only)";
         return MX_E_UNSUPPORTED;
@@ -703,6 +744,11 @@ int Frame::prepare(const mx_request& r, std::string& err) {
         return MX_E_UNSUPPORTED;
     }
     dbl = mpf_cmp_d(scale, 1e280) <= 0 && !(r.options & MX_OPT_FLOATEXP);
+    if (dbl && ss) {
+        err = "adaptive supersampling on the plain-double path (scale <= 1e280) is not on this path "
+              "(MX_OPT_FLOATEXP runs such frames on the floatexp path)";
+        return MX_E_UNSUPPORTED;
+    }
     const bool view_ref = (r.options & MX_OPT_VIEW_CENTER_REF) != 0;
     // depth guard for the int32 exponent of the device FloatExp: |log2 scale| < 2^26
     {

@@ -38,6 +38,8 @@ struct Frame {
     double esc2 = 0.0;
     mp_bitcnt_t prec = 64;
     int bla_minlevel = 3;
+    int sub = 1;      // output cells per pixel side
+    bool ss = false;  // adaptive supersampling (sub > 1 and MX_SUPER_SAMPLING)
 
     // grid
     Mpf c0_re, c0_im, dx, dy;

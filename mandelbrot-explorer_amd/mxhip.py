@@ -39,7 +39,8 @@ class Tile(C.Structure):
                 ("dcol_m", C.c_void_p), ("dcol_e", C.c_void_p), ("drow_m", C.c_void_p), ("drow_e", C.c_void_p),
                 ("dx_m", C.c_double), ("dx_e", C.c_int32), ("skip_sample", C.c_int32),
                 ("dcol_d", C.c_void_p), ("drow_d", C.c_void_p), ("list", C.c_void_p), ("list_len", C.c_int32),
-                ("chunk", C.c_int32), ("grid_blocks", C.c_int32)]
+                ("chunk", C.c_int32), ("grid_blocks", C.c_int32), ("n_samples", C.c_int32),
+                ("sample_dc_m", C.c_void_p), ("sample_dc_e", C.c_void_p)]
 
 
 class Request(C.Structure):
@@ -59,7 +60,7 @@ class FrameInfo(C.Structure):
                 ("probe_ms", C.c_double), ("kernel_ms", C.c_double), ("total_seconds", C.c_double),
                 ("steps", C.c_uint64), ("bla_applies", C.c_uint64), ("bla_skipped", C.c_uint64),
                 ("rebases", C.c_uint64), ("nominal_iterations", C.c_uint64), ("passes", C.c_int32),
-                ("sa_it", C.c_int32), ("escalated", C.c_int32)]
+                ("sa_it", C.c_int32), ("escalated", C.c_int32), ("ss_pixels", C.c_int32), ("ss_ms", C.c_double)]
 
 
 class FrameView(C.Structure):
@@ -137,10 +138,10 @@ def _p(a):
 
 
 def make_request(cre, cim, scale, W, H, mxit, c_method=0, escape_radius=0.0, prec_bits=0, form=MX_FORM_FE64,
-                 use_bla=-1, bla_minskip=0, interior=-1, full_h=0, row_base=0, options=0):
+                 use_bla=-1, bla_minskip=0, interior=-1, full_h=0, row_base=0, options=0, sub=1):
     r = Request()
     r.center_re, r.center_im, r.scale = cre.encode(), cim.encode(), scale.encode()
-    r.width, r.height, r.full_height, r.row_base, r.sub = W, H, full_h, row_base, 1
+    r.width, r.height, r.full_height, r.row_base, r.sub = W, H, full_h, row_base, sub
     r.max_iterations, r.coloring_method, r.escape_radius = mxit, c_method, escape_radius
     r.precision_bits, r.form, r.use_bla, r.bla_minskip, r.interior = prec_bits, form, use_bla, bla_minskip, interior
     r.options = options
@@ -220,17 +221,20 @@ class Context:
     def reset_cancellation(self):
         lib().mx_reset_cancellation(self.h)
 
-    def compute(self, cre, cim, scale, W, H, mxit, **kw):
-        req = make_request(cre, cim, scale, W, H, mxit, **kw)
-        it = np.full(W * H, -10.0, np.float32)
-        val = np.zeros(W * H)
-        nesc = np.zeros(W * H, np.int32)
+    def compute(self, cre, cim, scale, W, H, mxit, sub=1, **kw):
+        """One frame; with sub > 1 the buffers are (H sub) x (W sub) cells (-10, 0 and
+        INT_MIN where no sample was computed)."""
+        req = make_request(cre, cim, scale, W, H, mxit, sub=sub, **kw)
+        n = W * H * sub * sub
+        it = np.full(n, -10.0, np.float32)
+        val = np.zeros(n)
+        nesc = np.full(n, np.iinfo(np.int32).min, np.int32)
         req.iterations, req.values, req.n_esc = _p(it), _p(val), _p(nesc)
         info = FrameInfo()
         rc = lib().mx_compute(self.h, C.byref(req), C.byref(info))
         self.check(rc, "mx_compute")
         inf = {k: getattr(info, k) for k, _ in FrameInfo._fields_}
-        return it.reshape(H, W), val.reshape(H, W), nesc.reshape(H, W), inf
+        return it.reshape(H * sub, W * sub), val.reshape(H * sub, W * sub), nesc.reshape(H * sub, W * sub), inf
 
     def upload(self, fv):
         """Upload a reference from a frame_prepare() dict (or the same keys)."""

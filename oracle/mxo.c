@@ -1119,7 +1119,59 @@ typedef struct {
     double* orbit_zf;
     double* orbit_m;
     int64_t* orbit_e;
+    /* output cells per pixel side (0 or 1: none); with c_method's super-sampling bit
+     * (1) the adaptive sub-samples.  cells/cell_val/cell_nesc: (H sub) x (W sub),
+     * written where a sample is computed; ss_pixels: pixels refined */
+    int sub;
+    float* cells;
+    double* cell_val;
+    int32_t* cell_nesc;
+    int ss_pixels;
 } mxo_frame;
+
+/* output cell of sample (i, j, u, v) (getIndex, MP.cpp:4334-4340) */
+static size_t ss_cell(int W, int sub, int i, int j, int u, int v) {
+    return ((size_t)i * sub + sub / 2 + u) * (size_t)W * sub + ((size_t)j * sub + sub / 2 + v);
+}
+
+/* Adaptive supersampling's pixel test (MP.cpp:1796-1883) on the base float values
+ * `out` (W x H): under EDE, a neighbour of the opposite sign; otherwise the float log
+ * of the largest float difference to a neighbour above log(mxit) / 8.  flags[p] = 1
+ * for the pixels refined; returns their count. */
+static int ss_flag_pixels(const float* out, int W, int H, int mxit, int ede, unsigned char* flags) {
+    const double thresh = log((double)mxit) / 8.0;
+    int cnt = 0;
+    for (int i = 0; i < H; ++i)
+        for (int j = 0; j < W; ++j) {
+            const float cv = out[i * W + j];
+            int need = 0;
+            if (ede) {
+                for (int dy = -1; dy <= 1 && !need; ++dy)
+                    for (int dx = -1; dx <= 1 && !need; ++dx) {
+                        const int y = i + dy, x = j + dx;
+                        if ((dx || dy) && x >= 0 && x < W && y >= 0 && y < H && cv * out[y * W + x] < 0.0f) need = 1;
+                    }
+            } else {
+                float diff = 0.0f;
+                for (int dy = -1; dy <= 1; ++dy)
+                    for (int dx = -1; dx <= 1; ++dx) {
+                        const int y = i + dy, x = j + dx;
+                        if ((dx || dy) && x >= 0 && x < W && y >= 0 && y < H) {
+                            const float d = fabsf(cv - out[y * W + x]);
+                            if (d > diff) diff = d;
+                        }
+                    }
+                need = logf(diff) > thresh;
+            }
+            flags[i * W + j] = (unsigned char)need;
+            cnt += need;
+        }
+    return cnt;
+}
+
+MXO_EXPORT int mxo_ss_flags(const float* out, int W, int H, int mxit, int ede, unsigned char* flags) {
+    return ss_flag_pixels(out, W, H, mxit, ede, flags);
+}
 
 static int bla_minlevel_from_skip(int ms) {
     int l = 0;
@@ -1288,6 +1340,54 @@ MXO_EXPORT int mxo_compute_fe(mxo_frame* f) {
         f->n_esc[cidx] = ne;
     }
     for (int p = 0; p < W * H; ++p) f->out[p] = (f->n_esc[p] < 0) ? -2.f : (float)f->val[p];
+    f->ss_pixels = 0;
+    if (f->sub > 1) {
+        const int S = f->sub, h = S / 2;
+        for (int i = 0; i < H; ++i)
+            for (int j = 0; j < W; ++j) {
+                const size_t c = ss_cell(W, S, i, j, 0, 0);
+                f->cells[c] = f->out[i * W + j];
+                f->cell_val[c] = f->val[i * W + j];
+                f->cell_nesc[c] = f->n_esc[i * W + j];
+            }
+        if (f->c_method & 1) {
+            unsigned char* flags = (unsigned char*)malloc((size_t)W * H);
+            f->ss_pixels = ss_flag_pixels(f->out, W, H, f->mxit, ede, flags);
+            /* every other cell of a flagged pixel against the same reference and BLA
+             * table; dc from the fractional pixel position (MP.cpp:2160-2163) */
+            const int per = S * S - 1;
+            int np = 0;
+            int* pix = (int*)malloc(sizeof(int) * (size_t)(f->ss_pixels + 1));
+            for (int p = 0; p < W * H; ++p)
+                if (flags[p]) pix[np++] = p;
+            long long steps = 0, ap = 0, sk = 0, rb = 0;
+#pragma omp parallel for schedule(dynamic, 8) num_threads(f->threads) reduction(+ : steps, ap, sk, rb)
+            for (int q = 0; q < np * per; ++q) {
+                const int p = pix[q / per], r = q % per;
+                const int k = r < (per / 2) ? r : r + 1; /* skip the centre */
+                const int yi = k / S - h, xi = k % S - h;
+                const int i = p / W, j = p % W;
+                const double px = j + (double)xi / S, py = i + (double)yi / S;
+                mxo_stats st = {0, 0, 0, 0, 0, 0};
+                int ne;
+                const double v = pixel_rescaled(&ctx, mxo_fe_mul_d(dxfe, px - ref_x), mxo_fe_mul_d(dyfe, py - ref_y), f->mxit, &ne, &st);
+                const size_t c = ss_cell(W, S, i, j, yi, xi);
+                f->cell_val[c] = v;
+                f->cell_nesc[c] = ne;
+                f->cells[c] = ne < 0 ? -2.f : (float)v;
+                steps += st.steps;
+                ap += st.bla_applies;
+                sk += st.bla_skipped;
+                rb += st.rebases;
+            }
+            tot.steps += steps;
+            tot.bla_applies += ap;
+            tot.bla_skipped += sk;
+            tot.rebases += rb;
+            free(pix);
+            free(flags);
+        }
+    }
     f->steps = tot.steps;
     f->bla_applies = tot.bla_applies;
     f->bla_skipped = tot.bla_skipped;

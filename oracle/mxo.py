@@ -25,6 +25,8 @@ class Frame(C.Structure):
         ("steps", C.c_longlong), ("bla_applies", C.c_longlong), ("bla_skipped", C.c_longlong),
         ("rebases", C.c_longlong),
         ("orbit_zf", C.c_void_p), ("orbit_m", C.c_void_p), ("orbit_e", C.c_void_p),
+        ("sub", C.c_int), ("cells", C.c_void_p), ("cell_val", C.c_void_p), ("cell_nesc", C.c_void_p),
+        ("ss_pixels", C.c_int),
     ]
 
 
@@ -74,6 +76,8 @@ def lib():
         L.mxo_fe_op.argtypes = [C.c_int, C.c_int] + [C.c_void_p] * 6
         L.mxo_orbit_shadows.argtypes = [C.c_char_p, C.c_char_p, C.c_int, C.c_int, C.c_int, C.c_double, C.c_int] + [C.c_void_p] * 7
         L.mxo_orbit_shadows.restype = C.c_int
+        L.mxo_ss_flags.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_void_p]
+        L.mxo_ss_flags.restype = C.c_int
         L.mxo_fixed_mul.argtypes = [C.c_int, C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_void_p, C.POINTER(C.c_int)]
         L.mxo_fixed_to_fe.argtypes = [C.c_int, C.c_void_p, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_int64)]
         L.mxo_bla_total.argtypes = [C.c_int]
@@ -100,11 +104,19 @@ def precision_bits(scale_str):
 
 def compute_fe(cre, cim, scale, W, H, mxit, c_method=0, escape_radius=1e8, prec_bits=None,
                use_bla=True, bla_minskip=8, interior=True, probe=True, threads=None, full_h=0, row_base=0,
-               want_orbit=False):
+               want_orbit=False, sub=1):
     """Whole floatexp frame (Compute -> reference -> BLA -> probe -> pass).  With
     want_orbit, info["orbit"] holds the reference orbit's shadows (zf, zfe_m, zfe_e,
-    (re, im) interleaved)."""
+    (re, im) interleaved).  With sub > 1, info["cells"] holds the (H sub) x (W sub)
+    float, value and escape-count cells (-10, 0, INT_MIN where nothing was computed),
+    with the adaptive sub-samples when c_method has the super-sampling bit (1)."""
     f = Frame()
+    if sub > 1:
+        nc = W * H * sub * sub
+        cf = np.full(nc, -10.0, np.float32)
+        cv = np.zeros(nc)
+        cn = np.full(nc, np.iinfo(np.int32).min, np.int32)
+        f.sub, f.cells, f.cell_val, f.cell_nesc = sub, _p(cf), _p(cv), _p(cn)
     if want_orbit:
         ozf = np.zeros(2 * (mxit + 1))
         om = np.zeros(2 * (mxit + 1))
@@ -130,6 +142,9 @@ def compute_fe(cre, cim, scale, W, H, mxit, c_method=0, escape_radius=1e8, prec_
     if want_orbit:
         n = 2 * (f.ref_it + 1)
         info["orbit"] = dict(zf=ozf[:n].copy(), zfe_m=om[:n].copy(), zfe_e=oe[:n].copy())
+    if sub > 1:
+        info["cells"] = (cf.reshape(H * sub, W * sub), cv.reshape(H * sub, W * sub), cn.reshape(H * sub, W * sub))
+        info["ss_pixels"] = f.ss_pixels
     return val.reshape(H, W), nesc.reshape(H, W), out.reshape(H, W), info
 
 
@@ -256,6 +271,15 @@ def fixed_to_fe(L, x):
     m, e = C.c_double(0), C.c_int64(0)
     lib().mxo_fixed_to_fe(L, _p(a), (x > 0) - (x < 0), C.byref(m), C.byref(e))
     return m.value, e.value
+
+
+def ss_flags(out, mxit, ede):
+    """Adaptive supersampling's refined pixels (bool array) from base float values."""
+    out = np.ascontiguousarray(out, np.float32)
+    H, W = out.shape
+    fl = np.zeros(W * H, np.uint8)
+    lib().mxo_ss_flags(_p(out), W, H, mxit, int(ede), _p(fl))
+    return fl.reshape(H, W).astype(bool)
 
 
 def bla_tables(zf, reflen, dcmax, eps=2.0 ** -53, ede=False, dfr=None, dfi=None):

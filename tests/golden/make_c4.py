@@ -18,9 +18,9 @@ at 2^21 (at 2^16 they stall near 1e-41, every probe of the region exceeding it).
 
 Writes tests/golden/c4_seed<k>.json (one location centre each) and, once all twelve
 exist, tests/golden/c4_requests.json.
-Usage: python tests/golden/make_c4.py [--plan] [--seeds 1,2,...]
+Usage: python tests/golden/make_c4.py [--plan] [--seeds 1,2,...] [--gpu] [--out DIR]
   --plan: print the draws only; --seeds: walk only these locations (the request file
-  is written once every location's centre exists)
+  is written once every location's centre exists); --gpu, --out: as make_walk.py
 """
 import json
 import math
@@ -29,7 +29,7 @@ import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, HERE)
-from make_walk import SplitMix64, walk  # noqa: E402
+from make_walk import GpuProbe, SplitMix64, option, walk  # noqa: E402
 
 E_LO, E_HI = 50, 300
 SIDES = (64, 128, 256, 512, 1024)
@@ -65,14 +65,29 @@ def main():
             print(l)
         print(sum(q["side"] ** 2 for q in reqs), "samples")
         return
-    seeds = None
-    if "--seeds" in sys.argv:
-        seeds = {int(x) for x in sys.argv[sys.argv.index("--seeds") + 1].split(",")}
+    args = sys.argv[1:]
+    seeds = option(args, "--seeds")
+    seeds = None if seeds is None else {int(x) for x in seeds.split(",")}
+    out_dir = option(args, "--out", HERE)
+    gpu = GpuProbe() if "--gpu" in args else None
     out = {"_definition": "tests/golden/make_c4.py (seeded batch of uneven tiles)", "locations": [], "requests": reqs}
     for l in locs:
-        path = os.path.join(HERE, f"c4_seed{l['seed']}.json")
+        path = os.path.join(out_dir, f"c4_seed{l['seed']}.json")
         if not os.path.exists(path) and (seeds is None or l["seed"] in seeds):
-            res = walk(l["seed"], [l["depth"]], log=lambda s: print(s, flush=True), mxit_cap=WALK_CAP)
+            # checkpoints every 50 decades (resumable: a later run continues from the
+            # deepest one recorded)
+            ck_path = os.path.join(out_dir, f"c4_seed{l['seed']}_checkpoints.json")
+            cks = json.load(open(ck_path)) if os.path.exists(ck_path) else {}
+            done = [int(d) for d in cks]
+            resume = cks[str(max(done))] if done else None
+            targets = [d for d in range(50, l["depth"], 50) if not done or d > max(done)] + [l["depth"]]
+
+            def save_ck(res, ck_path=ck_path, cks=cks):
+                cks.update(res)
+                json.dump(cks, open(ck_path, "w"), indent=1)
+
+            res = walk(l["seed"], targets, log=lambda s: print(s, flush=True), on_depth=save_ck, resume=resume,
+                       mxit_cap=WALK_CAP, gpu=gpu)
             c = res[str(l["depth"])]
             json.dump(dict(seed=l["seed"], depth=l["depth"], re=c["re"], im=c["im"], width=c["width"],
                            steps=c["steps"], probe_mxit=c["probe_mxit"], walk_mxit_cap=WALK_CAP), open(path, "w"), indent=1)
@@ -81,7 +96,7 @@ def main():
             c = json.load(open(path))
             out["locations"].append(dict(seed=l["seed"], depth=l["depth"], re=c["re"], im=c["im"]))
     if len(out["locations"]) == NLOC:
-        json.dump(out, open(os.path.join(HERE, "c4_requests.json"), "w"), indent=1)
+        json.dump(out, open(os.path.join(out_dir, "c4_requests.json"), "w"), indent=1)
         print("wrote c4_requests.json", flush=True)
 
 

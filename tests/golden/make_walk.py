@@ -21,10 +21,13 @@ Definition (the workload spec's walk, with the two choices it leaves open fixed)
 Centres are recorded at every requested depth; the recorded decimal strings carry
 ceil(-log10 w) + 30 digits.
 
-Usage: python tests/golden/make_walk.py SEED DEPTH10 [DEPTH10 ...] [--resume DEPTH10]
+Usage: python tests/golden/make_walk.py SEED DEPTH10 [DEPTH10 ...] [--resume DEPTH10] [--gpu] [--out DIR]
 e.g.   python tests/golden/make_walk.py 7 300 1000
 --resume continues from the centre recorded at that depth (width 3 / 2^steps, the
-generator advanced by one draw per recorded step).
+generator advanced by the recorded number of draws; one per step in records without
+a draw count).  --gpu renders the probes above
+scale 1e6 with the product's plain form on the GPU (same escape counts as the oracle,
+so the same walk); --out writes the JSON there instead of beside this script.
 """
 import json
 import math
@@ -53,7 +56,27 @@ class SplitMix64:
         return z ^ (z >> 31)
 
 
-def walk(seed, depths, log=print, on_depth=None, resume=None, mxit_cap=MXIT_CAP):
+class GpuProbe:
+    """Probes above scale 1e6 rendered by the product's plain FP64 form on the GPU with
+    the oracle's frame semantics (floatexp path at every depth, the exact view centre
+    as reference at scale <= 1e280, the centre pixel with its fallback above): the
+    plain form's escape counts equal the oracle's bit for bit, so the walk is the
+    oracle-driven one, at GPU speed."""
+
+    def __init__(self):
+        sys.path.insert(0, os.path.join(HERE, "..", "..", "mandelbrot-explorer_amd"))
+        import mxhip
+        self.mx = mxhip
+        self.ctx = mxhip.Context(0)
+
+    def __call__(self, cre, cim, scale, W, H, mxit, prec_bits, scale_small):
+        opts = self.mx.MX_OPT_FLOATEXP | (self.mx.MX_OPT_VIEW_CENTER_REF if scale_small else 0)
+        _, val, nesc, _ = self.ctx.compute(cre, cim, scale, W, H, mxit, prec_bits=prec_bits, form=self.mx.MX_FORM_FE64,
+                                           options=opts)
+        return val, nesc
+
+
+def walk(seed, depths, log=print, on_depth=None, resume=None, mxit_cap=MXIT_CAP, gpu=None):
     depths = sorted(depths)
     rng = SplitMix64(seed)
     getcontext().prec = max(depths) + 80
@@ -62,12 +85,15 @@ def walk(seed, depths, log=print, on_depth=None, resume=None, mxit_cap=MXIT_CAP)
     mxit = 1024
     out = {}
     step = 0
+    draws = 0
     if resume is not None:
         cx, cy = Decimal(resume["re"]), Decimal(resume["im"])
         step = int(resume["steps"])
         w = Decimal(3) / (Decimal(2) ** step)
         mxit = int(resume["probe_mxit"])
-        for _ in range(step):
+        # the generator's position: the recorded draw count (older records: one per step)
+        draws = int(resume.get("draws", step))
+        for _ in range(draws):
             rng.next()
     prev = None  # (cx, cy, w, cand, k) of the previous view, for a step back
     redraws = 0
@@ -78,7 +104,7 @@ def walk(seed, depths, log=print, on_depth=None, resume=None, mxit_cap=MXIT_CAP)
             digits = depths[0] + 30
             out[str(depths[0])] = dict(re=format(cx.quantize(Decimal(10) ** -digits), "f"),
                                        im=format(cy.quantize(Decimal(10) ** -digits), "f"),
-                                       width=format(w, ".6e"), steps=step, probe_mxit=mxit)
+                                       width=format(w, ".6e"), steps=step, probe_mxit=mxit, draws=draws)
             log(f"seed {seed}: reached 1e-{depths[0]} after {step} steps ({time.time() - t0:.0f}s)")
             if on_depth:
                 on_depth(out)
@@ -92,7 +118,11 @@ def walk(seed, depths, log=print, on_depth=None, resume=None, mxit_cap=MXIT_CAP)
         W = H = 64
         while True:
             if sc_int is not None and scale > 1e6:
-                val, nesc, _, _ = mxo.compute_fe(format(cx, "f"), format(cy, "f"), sc_int, W, H, mxit, prec_bits=prec, threads=os.cpu_count())
+                if gpu is not None:
+                    val, nesc = gpu(format(cx, "f"), format(cy, "f"), sc_int, W, H, mxit, prec, scale <= Decimal("1e280"))
+                else:
+                    val, nesc, _, _ = mxo.compute_fe(format(cx, "f"), format(cy, "f"), sc_int, W, H, mxit, prec_bits=prec,
+                                                     threads=os.cpu_count())
             else:
                 val, nesc = mxo.direct(format(cx, "f"), format(cy, "f"), format(scale, "f"), W, H, mxit, prec_bits=prec, wp_bits=prec, threads=os.cpu_count())
             flat = nesc.ravel()
@@ -118,6 +148,7 @@ def walk(seed, depths, log=print, on_depth=None, resume=None, mxit_cap=MXIT_CAP)
             k = int(math.ceil(0.1 * len(cand)))
         prev = (cx, cy, w, cand, k)
         pick = cand[rng.next() % k][1]
+        draws += 1
         i, j = divmod(pick, W)
         dw = Decimal(2) / scale
         dx = dw / (W - 1) * 2
@@ -132,16 +163,26 @@ def walk(seed, depths, log=print, on_depth=None, resume=None, mxit_cap=MXIT_CAP)
     return out
 
 
+def option(args, name, default=None):
+    """value of `name VALUE` in args (removed from args)"""
+    if name not in args:
+        return default
+    i = args.index(name)
+    v = args[i + 1]
+    del args[i:i + 2]
+    return v
+
+
 def main():
     args = sys.argv[1:]
-    resume_depth = None
-    if "--resume" in args:
-        i = args.index("--resume")
-        resume_depth = args[i + 1]
-        del args[i:i + 2]
+    gpu = GpuProbe() if "--gpu" in args else None
+    if gpu is not None:
+        args.remove("--gpu")
+    resume_depth = option(args, "--resume")
+    out_dir = option(args, "--out", HERE)
     seed = int(args[0])
     depths = [int(x) for x in args[1:]]
-    path = os.path.join(HERE, f"walk_seed{seed}.json")
+    path = os.path.join(out_dir, f"walk_seed{seed}.json")
     resume = json.load(open(path))[resume_depth] if resume_depth else None
 
     def save(res):
@@ -151,7 +192,7 @@ def main():
         json.dump(old, open(path, "w"), indent=1)
         print("wrote", path, flush=True)
 
-    save(walk(seed, depths, log=lambda s: print(s, flush=True), on_depth=save, resume=resume))
+    save(walk(seed, depths, log=lambda s: print(s, flush=True), on_depth=save, resume=resume, gpu=gpu))
 
 
 if __name__ == "__main__":

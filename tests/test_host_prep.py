@@ -161,3 +161,21 @@ def test_escalated_frame_prepares_the_floatexp_reference():
     assert fv["path"] == 2 and fv["escalated"] == 1 and info["escalated"] == 1
     assert fv["ref_it"] == info["ref_it"] and fv["ref_virtual"] == 1 == info["ref_virtual"]
     assert (fv["ref_x"], fv["ref_y"]) == (info["ref_x"], info["ref_y"])
+
+
+@pytest.mark.parametrize("sub,c_method,scale,rc", [
+    (2, 1, cases.pow10(300), mxhip.MX_E_INVALID),          # even sub: past the engine's buffer
+    (17, 1, cases.pow10(300), mxhip.MX_E_INVALID),
+    (3, 1, "40000000000", mxhip.MX_E_UNSUPPORTED),         # adaptive SS on the plain-double path
+])
+def test_supersampling_requests_refused(sub, c_method, scale, rc):
+    with pytest.raises(mxhip.MxError) as e:
+        mxhip.frame_prepare("0", "1", scale, 16, 8, 1000, c_method=c_method, sub=sub)
+    assert e.value.rc == rc
+
+
+def test_supersampling_prepare_accepted():
+    fv = mxhip.frame_prepare("0", "1", cases.pow10(300), 16, 8, 1000, c_method=1, sub=3)
+    assert fv["path"] == 2
+    fv = mxhip.frame_prepare("0", "1", "40000000000", 16, 8, 1000, c_method=1, sub=3, options=mxhip.MX_OPT_FLOATEXP)
+    assert fv["path"] == 2
