@@ -77,7 +77,7 @@ void mx_reset_cancellation(mx_ctx* ctx);
  * (2613-2709).  zf[2k], zf[2k+1] = X_{k+1}.  BLA level p has lev_size[p] entries
  * starting at lev_off[p], each skipping 2^p iterations. */
 typedef struct {
-    int32_t len;             /* entries = mx_ref_it + 1 */
+    int32_t len;             /* entries = mx_ref_it + 1 (a periodic reference: the period) */
     const double* zf;        /* [2*len] */
     const double* zfe_m;     /* [2*len] FloatExp mantissas (re, im) */
     const int32_t* zfe_e;    /* [2*len] FloatExp exponents (re, im) */
@@ -102,6 +102,10 @@ typedef struct {
     const double* df;
     const double* bla_der;
     const double* sa_coef_b;
+    /* > 0: a periodic-nucleus reference (createPeriodicRef, mandel_perturbation.cpp:
+     * 1108-1172): the floatexp kernels read X_m at (m-1) mod period, never rebase at
+     * its end, and land BLA skips inside one period; len must equal it */
+    int32_t period;
 } mx_ref_desc;
 
 int mx_ref_upload(mx_ctx* ctx, const mx_ref_desc* desc, mx_ref** out);
@@ -243,8 +247,11 @@ typedef struct {
  *                           1326-1373) instead of the centre pixel, so that frames of
  *                           one centre share it;
  *   MX_OPT_FLOATEXP         the floatexp path at any scale above 1e6 (the engine takes
- *                           the plain-double path up to 1e280). */
-enum { MX_OPT_VIEW_CENTER_REF = 1, MX_OPT_FLOATEXP = 2 };
+ *                           the plain-double path up to 1e280);
+ *   MX_OPT_NO_PERIODIC      no search for a nucleus in view: always the full reference
+ *                           from the centre pixel (the engine's MANDEL_PERIODIC=0,
+ *                           mandel_perturbation.cpp:1240-1250). */
+enum { MX_OPT_VIEW_CENTER_REF = 1, MX_OPT_FLOATEXP = 2, MX_OPT_NO_PERIODIC = 4 };
 
 typedef struct {
     int32_t path;            /* 1 = plain-double path, 2 = floatexp rescaled path */
@@ -270,6 +277,8 @@ typedef struct {
                                 reference passes too close to zero (mandel_perturbation.cpp:1336-1355) */
     int32_t ss_pixels;       /* pixels refined by adaptive supersampling */
     double ss_ms;            /* their sub-sample pass (kernel) */
+    int32_t period;          /* > 0: the reference was the period-`period` nucleus in view */
+    int32_t subpixel;        /* a nucleus was found but spans < 4 pixels: BLA was turned off */
 } mx_frame_info;
 
 int mx_compute(mx_ctx* ctx, const mx_request* req, mx_frame_info* info);
@@ -322,6 +331,8 @@ typedef struct {
     const double* bla_der;   /* [4*bla_total] (EDE) */
     const double* sa_coef_b; /* [60] (EDE) */
     int32_t escalated;       /* see mx_frame_info */
+    int32_t period;          /* > 0: periodic-nucleus reference, len == period */
+    int32_t subpixel;        /* a nucleus in view smaller than 4 pixels: full reference, BLA off */
 } mx_frame_view;
 
 int mx_frame_prepare(const mx_request* req, mx_frame** out, mx_frame_view* view);

@@ -37,7 +37,8 @@ struct MxRefDev {
     int mx_ref_it;
     int nlev;
     int minlevel;
-    int pad0;
+    int period; // > 0: periodic reference, orbit read at (m-1) mod period
+    int nent;   // entries in zf/zfe/zf32: mx_ref_it+1, or period+2 (two wrap copies)
     double rmax2;
     // plain-double path
     const double* z_m3;    // |X|^2 / 1e6 per entry (glitch metric)

@@ -140,13 +140,20 @@ extern "C" int mx_ref_upload(mx_ctx* ctx, const mx_ref_desc* d, mx_ref** out) {
     }
     if (d->z_m3 && (!d->sa_coef || d->sa_order < 0 || d->sa_order >= MX_SA_N || d->sa_it < 0)) return MX_E_INVALID;
     if (d->df && (!d->z_m3 || !d->sa_coef_b || (d->nlev > 0 && !d->bla_der))) return MX_E_INVALID;
+    if (d->period < 0 || (d->period > 0 && (d->period != d->len || d->len < 3 || d->z_m3))) {
+        ctx->set_err("a periodic reference holds exactly one period (len == period >= 3) on a floatexp path");
+        return MX_E_INVALID;
+    }
     MX_HIP(hipSetDevice(ctx->device));
     const size_t len = (size_t)d->len;
+    // a periodic orbit is stored with copies of its first two entries after the last,
+    // so the kernels read X at rm + 1 and rm + 2 across the wrap without a modulo
+    const size_t nent = d->period > 0 ? len + 2 : len;
     size_t o_zf = 0;
-    size_t o_zfm = align_up(o_zf + 16 * len);
-    size_t o_zfe = align_up(o_zfm + 16 * len);
-    size_t o_zf32 = align_up(o_zfe + 8 * len);
-    size_t o_r2 = align_up(o_zf32 + 8 * len);
+    size_t o_zfm = align_up(o_zf + 16 * nent);
+    size_t o_zfe = align_up(o_zfm + 16 * nent);
+    size_t o_zf32 = align_up(o_zfe + 8 * nent);
+    size_t o_r2 = align_up(o_zf32 + 8 * nent);
     size_t o_coef = align_up(o_r2 + 8 * (size_t)total);
     size_t o_cf = align_up(o_coef + 32 * (size_t)total);
     size_t o_ce = align_up(o_cf + 16 * (size_t)total);
@@ -167,8 +174,8 @@ extern "C" int mx_ref_upload(mx_ctx* ctx, const mx_ref_desc* d, mx_ref** out) {
         return MX_E_NOMEM;
     }
     char* b = (char*)ref->block;
-    std::vector<float> zf32(2 * len);
-    for (size_t k = 0; k < 2 * len; ++k) zf32[k] = (float)d->zf[k];
+    std::vector<float> zf32(2 * nent);
+    for (size_t k = 0; k < 2 * nent; ++k) zf32[k] = (float)d->zf[k % (2 * len)];
     // FP32-floatexp copy of the coefficients for the mixed form: A and B each get one
     // exponent (that of their larger component) and FP32 mantissas in [0.5, 1)
     std::vector<float4> cf((size_t)total);
@@ -191,7 +198,12 @@ extern "C" int mx_ref_upload(mx_ctx* ctx, const mx_ref_desc* d, mx_ref** out) {
     cp(o_zf, d->zf, 16 * len);
     cp(o_zfm, d->zfe_m, 16 * len);
     cp(o_zfe, d->zfe_e, 8 * len);
-    cp(o_zf32, zf32.data(), 8 * len);
+    if (nent > len) {
+        cp(o_zf + 16 * len, d->zf, 16 * (nent - len));
+        cp(o_zfm + 16 * len, d->zfe_m, 16 * (nent - len));
+        cp(o_zfe + 8 * len, d->zfe_e, 8 * (nent - len));
+    }
+    cp(o_zf32, zf32.data(), 8 * nent);
     if (d->nlev > 0) {
         cp(o_r2, d->bla_r2, 8 * (size_t)total);
         cp(o_coef, d->bla_coef, 32 * (size_t)total);
@@ -225,7 +237,9 @@ extern "C" int mx_ref_upload(mx_ctx* ctx, const mx_ref_desc* d, mx_ref** out) {
     R.bla_coef = (const double4*)(b + o_coef);
     R.bla_coeff = (const float4*)(b + o_cf);
     R.bla_coefe = (const int2*)(b + o_ce);
-    R.mx_ref_it = d->len - 1;
+    R.mx_ref_it = d->period > 0 ? d->period : d->len - 1;
+    R.period = d->period > 0 ? d->period : 0;
+    R.nent = (int)nent;
     R.nlev = d->nlev;
     R.rmax2 = d->rmax2;
     if (d->z_m3) {
@@ -723,6 +737,7 @@ int upload_frame_ref(mx_ctx* ctx, const mx::Frame& F, mx_ref** ref) {
     mx_ref_desc d;
     std::memset(&d, 0, sizeof d);
     d.len = v.len;
+    d.period = v.period;
     d.zf = v.zf;
     d.zfe_m = v.zfe_m;
     d.zfe_e = v.zfe_e;
@@ -981,6 +996,7 @@ extern "C" int mx_compute(mx_ctx* ctx, const mx_request* req, mx_frame_info* inf
     mx_ref_desc d;
     std::memset(&d, 0, sizeof d);
     d.len = v.len;
+    d.period = v.period;
     d.zf = v.zf;
     d.zfe_m = v.zfe_m;
     d.zfe_e = v.zfe_e;
@@ -1163,6 +1179,8 @@ extern "C" int mx_compute(mx_ctx* ctx, const mx_request* req, mx_frame_info* inf
         info->nominal_iterations = nominal;
         info->passes = 1;
         info->escalated = F.escalated;
+        info->period = F.period;
+        info->subpixel = F.subpixel;
     }
     return MX_OK;
 }

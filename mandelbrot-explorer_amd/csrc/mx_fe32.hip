@@ -212,9 +212,9 @@ template <>
 struct OrbitLoader<float> {
     __amdgpu_buffer_rsrc_t r;
     int n;
-    MX_HD explicit OrbitLoader(const MxRefDev& R) : n(R.mx_ref_it + 1) {
+    MX_HD explicit OrbitLoader(const MxRefDev& R) : n(R.nent) {
 #ifdef __HIP_DEVICE_COMPILE__
-        r = __builtin_amdgcn_make_buffer_rsrc((void*)R.zf32, 0, (R.mx_ref_it + 1) * 8, 0x00020000);
+        r = __builtin_amdgcn_make_buffer_rsrc((void*)R.zf32, 0, R.nent * 8, 0x00020000);
 #endif
         zf = R.zf32;
     }
@@ -231,7 +231,7 @@ template <>
 struct OrbitLoader<double> {
     const double2* zf;
     int n;
-    MX_HD explicit OrbitLoader(const MxRefDev& R) : zf(R.zf), n(R.mx_ref_it + 1) {}
+    MX_HD explicit OrbitLoader(const MxRefDev& R) : zf(R.zf), n(R.nent) {}
     MX_HD double2 load(int i) const { return zf[min(i, n - 1)]; }
 };
 
@@ -416,7 +416,10 @@ MX_HD int try_bla32(const MxRefDev& R, const int2* lev, int s, P32<F>& p, int re
         const F sn = pow2t<F>(nk);
         const F fr = X.x + sn * nwr, fi = X.y + sn * nwi;
         if (fr * fr + fi * fi > esc2f) return 0;
-        if (nwr == (F)0 && nwi == (F)0) return 0; // unrepresentable landing, as the plain form
+        // unrepresentable landing: rejected on a full reference, kept as a zero delta at
+        // unit scale on a periodic one (as the plain form)
+        const bool wzero = nwr == (F)0 && nwi == (F)0;
+        if (wzero && !R.period) return 0;
         if (ede) {
             // J -> A J + B (mandel_perturbation.cpp:3399-3414): A J at 2^(kj + ce.x), B at 2^ce.y
             const F tr = c.x * p.jr - c.y * p.ji, ti = c.x * p.ji + c.y * p.jr;
@@ -444,7 +447,7 @@ MX_HD int try_bla32(const MxRefDev& R, const int2* lev, int s, P32<F>& p, int re
         }
         p.wr = nwr;
         p.wi = nwi;
-        p.k = nk;
+        p.k = wzero ? 0 : nk;
         set_scale(p);
         return 1 << q;
     }
@@ -455,6 +458,9 @@ template <typename F, bool EDE, bool INTERIOR>
 MX_HD bool trip32(const MxRefDev& R, const int2* lev, const MxTileDev& T, P32<F>& p, double& v, int& n, C32& c, F esc2f) {
     using V = typename Vec2<F>::T;
     const int reflen = R.mx_ref_it + 1;
+    // a periodic reference wraps instead of ending (mandel_perturbation.cpp:3308-3313)
+    const int per = R.period;
+    const int mend = per ? INT_MAX : reflen;
     if (p.iter >= T.mxit) {
         n = -1;
         v = -2.0;
@@ -476,11 +482,12 @@ MX_HD bool trip32(const MxRefDev& R, const int2* lev, const MxTileDev& T, P32<F>
         }
     }
     if ((T.flags & MX_F_BLA) && p.m >= 2) {
-        const int skip = try_bla32(R, lev, p.rm, p, reflen, esc2f, EDE);
+        const int skip = try_bla32(R, lev, p.rm, p, per ? per : reflen, esc2f, EDE);
         if (skip > 0) {
             p.m += skip;
             p.iter += skip;
             p.rm += skip;
+            if (per && p.rm >= per) p.rm -= per;
             c.applies++;
             c.skipped += skip;
             p.zsr = p.zsi = (F)1e30f;
@@ -496,7 +503,8 @@ MX_HD bool trip32(const MxRefDev& R, const int2* lev, const MxTileDev& T, P32<F>
         p.Xc = p.m ? load_X(R, p.rm, (F)0) : V{(F)0, (F)0};
         p.Xn = load_X(R, p.rm + 1, (F)0);
     }
-    const V Xpf = (p.rm + 2 < reflen) ? load_X(R, p.rm + 2, (F)0) : V{(F)0, (F)0};
+    const V Xpf = (p.rm + 2 < R.nent) ? load_X(R, p.rm + 2, (F)0) : V{(F)0, (F)0};
+    const int rm_prev = p.rm;
     const V X = p.Xc;
     F zmr, zmi;
     step_z(p, X, zmr, zmi);
@@ -521,11 +529,12 @@ MX_HD bool trip32(const MxRefDev& R, const int2* lev, const MxTileDev& T, P32<F>
     ++p.m;
     ++p.iter;
     ++p.rm;
+    if (per && p.rm == per) p.rm = 0;
     c.steps++;
     if (EDE) {
         // J' = 2 z J + 1 in J's own scale
         if (zmr == (F)0 && zmi == (F)0 && p.jone == (F)0) {
-            const WK r = deriv_recover(R, p.m - 1, p.rm - 1, (double)wr0, (double)wi0, k0, (double)p.jr, (double)p.ji, p.kj);
+            const WK r = deriv_recover(R, p.m - 1, rm_prev, (double)wr0, (double)wi0, k0, (double)p.jr, (double)p.ji, p.kj);
             p.jr = (F)r.wr;
             p.ji = (F)r.wi;
             p.kj = r.k;
@@ -602,7 +611,7 @@ MX_HD bool trip32(const MxRefDev& R, const int2* lev, const MxTileDev& T, P32<F>
     // representable), or the end of the reference; then the exact test and the new
     // delta in FP64 from the FP64 orbit (mandel_perturbation.cpp:3584-3615).
     const F dz2f = ldexp_t(wmag2(p.wr, p.wi), 2 * p.k);
-    if (zrad < (F)1e-8f || zrad < dz2f || p.m >= reflen) {
+    if (zrad < (F)1e-8f || zrad < dz2f || p.m >= mend) {
         // z * 2^-k = X * 2^-k + w in FP64
         double xr = 0.0, xi = 0.0;
         if (p.m) {
@@ -614,7 +623,7 @@ MX_HD bool trip32(const MxRefDev& R, const int2* lev, const MxTileDev& T, P32<F>
         const double zsr = xr + (double)p.wr, zsi = xi + (double)p.wi;
         const double zs2 = zsr * zsr + zsi * zsi;
         const double ws2 = (double)p.wr * p.wr + (double)p.wi * p.wi;
-        if (p.m >= reflen || zs2 < ws2) {
+        if (p.m >= mend || zs2 < ws2) {
             const double a = fmax(fabs(zsr), fabs(zsi));
             if (a == 0.0) {
                 p.wr = p.wi = (F)0;
@@ -670,23 +679,33 @@ MX_HD void fast_steps(const MxRefDev& R, const MxTileDev& T, P32<F>& p, C32& c, 
     const bool int_rep = INTERIOR && (T.flags & MX_F_INT_REP);
     const OrbitLoader<F> ld(R);
     F w2 = wmag2(p.wr, p.wi);
-    unsigned n = 0;
+    // one counter for the steps taken: orbit index, reference index and iteration
+    // advance together, and every test on them becomes a bound on the counter
+    const int m0 = p.m, rm0 = p.rm, it0 = p.iter;
+    const int n_iter = T.mxit - it0;                 // iter >= mxit
+    // m + 1 >= reflen after the step; periodic: rm + 1 == period (the wrap, taken by
+    // the full step)
+    const int n_end = R.period ? R.period - 1 - rm0 : reflen - 1 - m0;
+    const int n_rep = p.sm - m0;                      // m == sm
+    const int n_twin = p.save_iter2 + p.twin2r - it0; // iter - save_iter2 >= twin2r
+    int n = 0;
     for (int u = 0; u < budget; ++u) {
-        int stop = p.iter >= T.mxit;
+        const int rm = rm0 + n;
+        int stop = n >= n_iter;
         if (INTERIOR) stop |= p.conf_P > 0;
-        if (int_rep) stop |= (int)(p.m == p.sm) | (int)(p.iter - p.save_iter2 >= p.twin2r);
+        if (int_rep) stop |= (int)(n == n_rep) | (int)(n >= n_twin);
         if (can_bla) {
             int ew;
             (void)frexp_t(w2, &ew);
-            stop |= (int)(p.m >= 2) & (int)(((p.rm - 1) & amask) == 0) & ((int)(ew + 2 * p.k <= er) | (int)(w2 == (F)0));
+            stop |= (int)(m0 + n >= 2) & (int)(((rm - 1) & amask) == 0) & ((int)(ew + 2 * p.k <= er) | (int)(w2 == (F)0));
         }
         if (!p.xv) {
-            p.Xc = p.m ? load_X(R, p.rm, (F)0) : V{(F)0, (F)0};
-            p.Xn = load_X(R, p.rm + 1, (F)0);
+            p.Xc = m0 + n ? load_X(R, rm, (F)0) : V{(F)0, (F)0};
+            p.Xn = load_X(R, rm + 1, (F)0);
             p.xv = 1;
         }
         // the next step's entry (0 past the end, where this step stops anyway)
-        const V Xpf = ld.load(p.rm + 2);
+        const V Xpf = ld.load(rm + 2);
         const V X = p.Xc;
         F zmr, zmi;
         step_z(p, X, zmr, zmi);
@@ -702,8 +721,8 @@ MX_HD void fast_steps(const MxRefDev& R, const MxTileDev& T, P32<F>& p, C32& c, 
         const V Xn = p.Xn;
         const F zr = fma_t(p.s, nwr, Xn.x), zi = fma_t(p.s, nwi, Xn.y);
         const F zrad = wmag2(zr, zi);
-        stop |= (int)(fbits(zrad) - zlo > zspan) | (int)(zrad < ldexp_t(w2n, 2 * p.k)) | (int)(p.m + 1 >= reflen);
-        const int it1 = p.iter + 1;
+        stop |= (int)(fbits(zrad) - zlo > zspan) | (int)(zrad < ldexp_t(w2n, 2 * p.k)) | (int)(n >= n_end);
+        const int it1 = it0 + n + 1;
         int save = 0;
         if (INTERIOR) {
             const int chk = (int)((it1 & 15) == 0) & (int)(p.conf_giveup < 3);
@@ -716,25 +735,26 @@ MX_HD void fast_steps(const MxRefDev& R, const MxTileDev& T, P32<F>& p, C32& c, 
         const bool go = !stop;
         p.wr = go ? nwr : p.wr;
         p.wi = go ? nwi : p.wi;
-        p.m += go;
-        p.rm += go;
-        p.iter = go ? it1 : p.iter;
         if (EDE) {
             p.jr = go ? ja : p.jr;
             p.ji = go ? jb : p.ji;
         }
         p.Xc = go ? Xn : p.Xc;
         p.Xn = go ? Xpf : p.Xn;
-        const bool sv = go && save;
-        p.zsr = sv ? zr : p.zsr;
-        p.zsi = sv ? zi : p.zsi;
-        p.save_iter = sv ? it1 : p.save_iter;
-        p.period_win = sv ? p.period_win + p.period_win : p.period_win;
+        if (INTERIOR && go && save) {
+            p.zsr = zr;
+            p.zsi = zi;
+            p.save_iter = it1;
+            p.period_win += p.period_win;
+        }
         w2 = w2n;
         n += go;
         if (stop) break;
     }
-    c.steps += n;
+    p.m = m0 + n;
+    p.rm = rm0 + n;
+    p.iter = it0 + n;
+    c.steps += (unsigned)n;
 }
 
 __device__ __forceinline__ unsigned long long wave_sum(unsigned long long x) {

@@ -170,7 +170,8 @@ __host__ __device__ __noinline__ SW bla_land_fe(const double2* zf, int land, dou
     return SW{Snew, fe_to_double(fe_div(nzr, Snew)), fe_to_double(fe_div(nzi, Snew)), true};
 }
 
-// tryBLAfe (mandel_perturbation.cpp:2883-2970).  s = m-1, reflen = mx_ref_it+1.
+// tryBLAfe (mandel_perturbation.cpp:2883-2970).  s = m-1, reflen = mx_ref_it+1, or
+// the period for a periodic reference (landings stay inside one period).
 MX_HD int try_bla(const MxRefDev& R, const int2* lev, int s, Fe& S, Fe S2, double& wr, double& wi, double dr, double di, int reflen, double esc2, double4& ab) {
     if (s < 1) return 0;
     const int maxp = R.nlev - 1;
@@ -198,12 +199,27 @@ MX_HD int try_bla(const MxRefDev& R, const int2* lev, int s, Fe& S, Fe S2, doubl
             const double fi = X.y + fe_to_double(fe_mul_d(S, nwi));
             if (fr * fr + fi * fi > esc2) return 0;
             const double wm2 = nwr * nwr + nwi * nwi;
-            if (wm2 == 0.0) return 0; // full reference: unrepresentable landing rejected
-            const double wmag = sqrt(wm2);
-            S = fe_mul(S, fe_from(wmag));
-            const double inv = 1.0 / wmag;
-            wr = nwr * inv;
-            wi = nwi * inv;
+            if (wm2 == 0.0) {
+                // a full reference rejects the unrepresentable landing; a periodic one
+                // keeps the nonzero unit delta (blaReRescaleUnderflow, 2770-2781)
+                if (!R.period) return 0;
+                if (nwr == 0.0 && nwi == 0.0) {
+                    S = fe_make(1.0, 0);
+                    wr = wi = 0.0;
+                } else {
+                    const Fe wmag = fe_sqrt(fe_add(fe_mul(fe_from(nwr), fe_from(nwr)), fe_mul(fe_from(nwi), fe_from(nwi))));
+                    S = fe_mul(S, wmag);
+                    const double inv = 1.0 / fe_to_double(wmag);
+                    wr = nwr * inv;
+                    wi = nwi * inv;
+                }
+            } else {
+                const double wmag = sqrt(wm2);
+                S = fe_mul(S, fe_from(wmag));
+                const double inv = 1.0 / wmag;
+                wr = nwr * inv;
+                wi = nwi * inv;
+            }
         } else {
             const SW r = bla_land_fe(R.zf, land, esc2, bc, S, wr, wi, dr, di);
             if (!r.ok) return 0;
@@ -239,6 +255,9 @@ template <bool EDE, bool INTERIOR>
 MX_HD bool trip(const MxRefDev& R, const int2* lev, const MxTileDev& T, Px& p, double& v, int& n, Counters& c) {
     const double ESC2 = T.esc2;
     const int reflen = R.mx_ref_it + 1;
+    // a periodic reference wraps instead of ending (mandel_perturbation.cpp:3308-3313)
+    const int per = R.period;
+    const int mend = per ? INT_MAX : reflen;
     if (p.iter >= T.mxit) {
         n = -1;
         v = -2.0;
@@ -261,7 +280,7 @@ MX_HD bool trip(const MxRefDev& R, const int2* lev, const MxTileDev& T, Px& p, d
     }
     if ((T.flags & MX_F_BLA) && p.m >= 2) {
         double4 ab;
-        const int skip = try_bla(R, lev, p.rm, p.S, p.S2, p.wr, p.wi, p.dr, p.di, reflen, ESC2, ab);
+        const int skip = try_bla(R, lev, p.rm, p.S, p.S2, p.wr, p.wi, p.dr, p.di, per ? per : reflen, ESC2, ab);
         if (skip > 0) {
             if constexpr (EDE) {
                 Fe Jr = fe_mul_d(p.SJ, p.jr), Ji = fe_mul_d(p.SJ, p.ji);
@@ -281,6 +300,7 @@ MX_HD bool trip(const MxRefDev& R, const int2* lev, const MxTileDev& T, Px& p, d
             p.m += skip;
             p.iter += skip;
             p.rm += skip;
+            if (per && p.rm >= per) p.rm -= per;
             c.applies++;
             c.skipped += skip;
             p.s = fe_to_double(p.S);
@@ -298,12 +318,14 @@ MX_HD bool trip(const MxRefDev& R, const int2* lev, const MxTileDev& T, Px& p, d
         }
     }
     // X_m and X_{m+1} come from the previous trip; X_{m+2} is fetched one trip ahead
-    // (m <= mx_ref_it here, so rm+1 is in range; rm+2 only when m+1 <= mx_ref_it).
+    // (m <= mx_ref_it here, so rm+1 is in range; rm+2 only when m+1 <= mx_ref_it; a
+    // periodic orbit carries copies of its first two entries after the last, so
+    // rm+1 and rm+2 read the wrapped entries without a modulo).
     if (!p.xv) {
         p.Xc = p.m ? R.zf[p.rm] : make_double2(0.0, 0.0);
         p.Xn = R.zf[p.rm + 1];
     }
-    const double2 Xpf = (p.rm + 2 < reflen) ? R.zf[p.rm + 2] : make_double2(0.0, 0.0);
+    const double2 Xpf = (p.rm + 2 < R.nent) ? R.zf[p.rm + 2] : make_double2(0.0, 0.0);
     double2 X = p.Xc;
     double zmr = 0.0, zmi = 0.0;
     bool d_underflow = false;
@@ -350,6 +372,7 @@ MX_HD bool trip(const MxRefDev& R, const int2* lev, const MxTileDev& T, Px& p, d
     ++p.m;
     ++p.iter;
     ++p.rm;
+    if (per && p.rm == per) p.rm = 0;
     c.steps++;
     if constexpr (EDE) {
         if (d_underflow) {
@@ -445,7 +468,7 @@ MX_HD bool trip(const MxRefDev& R, const int2* lev, const MxTileDev& T, Px& p, d
         p.di = fe_to_double(fe_div(p.dci, p.S));
     }
     const double dzmag2 = p.s * p.s * (p.wr * p.wr + p.wi * p.wi);
-    if (zrad < 1e-8 || zrad < dzmag2 || p.m >= reflen) {
+    if (zrad < 1e-8 || zrad < dzmag2 || p.m >= mend) {
         Fe Xmr = fe_make(0.0, 0), Xmi = fe_make(0.0, 0);
         if (p.m) {
             const double2 mm = R.zfe_m[p.rm];
@@ -457,7 +480,7 @@ MX_HD bool trip(const MxRefDev& R, const int2* lev, const MxTileDev& T, Px& p, d
         const Fe zrfe = fe_add(Xmr, Swr), zife = fe_add(Xmi, Swi);
         const Fe zradfe = fe_add(fe_mul(zrfe, zrfe), fe_mul(zife, zife));
         const Fe dzfe = fe_add(fe_mul(Swr, Swr), fe_mul(Swi, Swi));
-        if (p.m >= reflen || fe_abs_less(zradfe, dzfe)) {
+        if (p.m >= mend || fe_abs_less(zradfe, dzfe)) {
             const Fe Snew = fe_sqrt(zradfe);
             if (Snew.m == 0.0) {
                 p.wr = p.wi = 0.0;
@@ -509,28 +532,38 @@ MX_HD void fast_steps(const MxRefDev& R, const MxTileDev& T, Px& p, Counters& c,
 #ifdef __HIP_DEVICE_COMPILE__
     // orbit entries through a buffer resource: a 32-bit offset per load, and indices
     // past the end read as 0
-    const __amdgpu_buffer_rsrc_t zrs = __builtin_amdgcn_make_buffer_rsrc((void*)R.zf, 0, reflen * 16, 0x00020000);
+    const __amdgpu_buffer_rsrc_t zrs = __builtin_amdgcn_make_buffer_rsrc((void*)R.zf, 0, R.nent * 16, 0x00020000);
 #endif
     double w2 = p.wr * p.wr + p.wi * p.wi;
-    unsigned n = 0;
+    // one counter for the steps taken: orbit index, reference index and iteration
+    // advance together, and every test on them becomes a bound on the counter
+    const int m0 = p.m, rm0 = p.rm, it0 = p.iter;
+    const int n_iter = T.mxit - it0;                 // iter >= mxit
+    // m + 1 >= reflen after the step; periodic: rm + 1 == period (the wrap, taken by
+    // the full step)
+    const int n_end = R.period ? R.period - 1 - rm0 : reflen - 1 - m0;
+    const int n_rep = p.sm - m0;                      // m == sm
+    const int n_twin = p.save_iter2 + p.twin2r - it0; // iter - save_iter2 >= twin2r
+    int n = 0;
     for (int u = 0; u < budget; ++u) {
-        int stop = p.iter >= T.mxit;
+        const int rm = rm0 + n;
+        int stop = n >= n_iter;
         if (INTERIOR) stop |= p.conf_P > 0;
-        if (int_rep) stop |= (int)(p.m == p.sm) | (int)(p.iter - p.save_iter2 >= p.twin2r);
+        if (int_rep) stop |= (int)(n == n_rep) | (int)(n >= n_twin);
         if (can_bla) {
             // dz^2 = S^2 |w|^2 has exponent S2.e + e(|w|^2) or one less
-            stop |= (int)(p.m >= 2) & (int)(((p.rm - 1) & amask) == 0) & ((int)(p.S2.e + fe_from(w2).e <= er + 1) | (int)(w2 == 0.0));
+            stop |= (int)(m0 + n >= 2) & (int)(((rm - 1) & amask) == 0) & ((int)(p.S2.e + fe_from(w2).e <= er + 1) | (int)(w2 == 0.0));
         }
         if (!p.xv) {
-            p.Xc = p.m ? R.zf[p.rm] : make_double2(0.0, 0.0);
-            p.Xn = R.zf[p.rm + 1];
+            p.Xc = m0 + n ? R.zf[rm] : make_double2(0.0, 0.0);
+            p.Xn = R.zf[rm + 1];
             p.xv = 1;
         }
         // the next step's entry (0 past the end, where this step stops anyway)
 #ifdef __HIP_DEVICE_COMPILE__
-        const double2 Xpf = __builtin_bit_cast(double2, __builtin_amdgcn_raw_buffer_load_b128(zrs, (p.rm + 2) * 16, 0, 0));
+        const double2 Xpf = __builtin_bit_cast(double2, __builtin_amdgcn_raw_buffer_load_b128(zrs, (rm + 2) * 16, 0, 0));
 #else
-        const double2 Xpf = p.rm + 2 < reflen ? R.zf[p.rm + 2] : make_double2(0.0, 0.0);
+        const double2 Xpf = rm + 2 < R.nent ? R.zf[rm + 2] : make_double2(0.0, 0.0);
 #endif
         const double2 X = p.Xc;
         double nwr, nwi;
@@ -552,8 +585,8 @@ MX_HD void fast_steps(const MxRefDev& R, const MxTileDev& T, Px& p, Counters& c,
         const double zr = Xn.x + p.s * nwr, zi = Xn.y + p.s * nwi;
         const double zrad = zr * zr + zi * zi;
         const double dzmag2 = p.s * p.s * w2n;
-        stop |= (int)(dbits(zrad) - zlo > zspan) | (int)(zrad < dzmag2) | (int)(p.m + 1 >= reflen);
-        const int it1 = p.iter + 1;
+        stop |= (int)(dbits(zrad) - zlo > zspan) | (int)(zrad < dzmag2) | (int)(n >= n_end);
+        const int it1 = it0 + n + 1;
         int save = 0;
         if (INTERIOR) {
             const int chk = (int)((it1 & 15) == 0) & (int)(p.conf_giveup < 3);
@@ -566,9 +599,6 @@ MX_HD void fast_steps(const MxRefDev& R, const MxTileDev& T, Px& p, Counters& c,
         const bool go = !stop;
         p.wr = go ? nwr : p.wr;
         p.wi = go ? nwi : p.wi;
-        p.m += go;
-        p.rm += go;
-        p.iter = go ? it1 : p.iter;
         if (EDE) {
             p.jr = go ? ja : p.jr;
             p.ji = go ? jb : p.ji;
@@ -577,16 +607,20 @@ MX_HD void fast_steps(const MxRefDev& R, const MxTileDev& T, Px& p, Counters& c,
         p.Xc.y = go ? Xn.y : p.Xc.y;
         p.Xn.x = go ? Xpf.x : p.Xn.x;
         p.Xn.y = go ? Xpf.y : p.Xn.y;
-        const bool sv = go && save;
-        p.zsr = sv ? zr : p.zsr;
-        p.zsi = sv ? zi : p.zsi;
-        p.save_iter = sv ? it1 : p.save_iter;
-        p.period_win = sv ? p.period_win + p.period_win : p.period_win;
+        if (INTERIOR && go && save) {
+            p.zsr = zr;
+            p.zsi = zi;
+            p.save_iter = it1;
+            p.period_win += p.period_win;
+        }
         w2 = w2n;
         n += go;
         if (stop) break;
     }
-    c.steps += n;
+    p.m = m0 + n;
+    p.rm = rm0 + n;
+    p.iter = it0 + n;
+    c.steps += (unsigned)n;
 }
 
 __device__ __forceinline__ unsigned long long wave_sum(unsigned long long x) {

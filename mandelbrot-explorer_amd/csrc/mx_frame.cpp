@@ -53,14 +53,17 @@ struct Gmp {
 // 4189-4249) with the Karatsuba square re = (a+b)(a-b), im = 2ab: in mpf on the
 // plain-double path, in fixed point on the floatexp path (the engine's choice there,
 // 4084-4111, 4197-4227), where the shadows come from the fixed-point value.
-void Frame::build_orbit(mpf_srcptr cre, mpf_srcptr cim) {
+// With period_len > 0 the orbit of a nucleus: entries 0 .. period_len-1 in mpf, no
+// escape test (createPeriodicRef, mandel_perturbation.cpp:1108-1172, which builds it
+// with calCoefficient's mpf branch).
+void Frame::build_orbit(mpf_srcptr cre, mpf_srcptr cim, int period_len) {
     // one integer limb: the fixed-point form needs |z|^2 + |c| well below 2^64
     // (bigFixedRangeSafe, 4093-4095)
-    if (!dbl && escape_radius <= 0x1p31) {
+    if (!period_len && !dbl && escape_radius <= 0x1p31) {
         build_orbit_fixed(cre, cim);
         return;
     }
-    const int mxit = max_iter;
+    const int mxit = period_len ? period_len - 1 : max_iter;
     zf.assign(2 * (size_t)(mxit + 1), 0.0);
     zfe_m.assign(2 * (size_t)(mxit + 1), 0.0);
     zfe_e.assign(2 * (size_t)(mxit + 1), 0);
@@ -115,7 +118,7 @@ void Frame::build_orbit(mpf_srcptr cre, mpf_srcptr cim) {
             dfr[i] = 2.0 * (a * xr - b * xi) + 1.0;
             dfi[i] = 2.0 * (a * xi + b * xr);
         }
-        if (std::hypot(zf[2 * i], zf[2 * i + 1]) > R) { // std::abs(Comp) > R
+        if (!period_len && std::hypot(zf[2 * i], zf[2 * i + 1]) > R) { // std::abs(Comp) > R
             ref_escaped = true;
             ref_it = i;
             break;
@@ -124,6 +127,20 @@ void Frame::build_orbit(mpf_srcptr cre, mpf_srcptr cim) {
     }
     mpf_set(tail_re.v, zr);
     mpf_set(tail_im.v, zi);
+    if (period_len) {
+        // one period: buildBLA(period) reads indices up to period-1
+        ref_escaped = false;
+        ref_it = period_len;
+        zf.resize(2 * (size_t)period_len);
+        zfe_m.resize(2 * (size_t)period_len);
+        zfe_e.resize(2 * (size_t)period_len);
+        z_m3.resize((size_t)period_len);
+        if (der) {
+            dfr.resize((size_t)period_len);
+            dfi.resize((size_t)period_len);
+        }
+        return;
+    }
     zf.resize(2 * (size_t)(ref_it + 1));
     zfe_m.resize(2 * (size_t)(ref_it + 1));
     zfe_e.resize(2 * (size_t)(ref_it + 1));
@@ -197,6 +214,184 @@ void Frame::build_orbit_fixed(mpf_srcptr cre, mpf_srcptr cim) {
     zfe_m.resize(2 * (size_t)(ref_it + 1));
     zfe_e.resize(2 * (size_t)(ref_it + 1));
     z_m3.resize((size_t)(ref_it + 1));
+}
+
+// ---- periodic-nucleus reference (mandel_perturbation.cpp:76-224, 1072-1219) ----
+
+namespace {
+
+// z = z^2 + c with the Karatsuba square (g_sqadd)
+void nuc_sqadd(mpf_ptr zr, mpf_ptr zi, mpf_srcptr cr, mpf_srcptr ci, mpf_ptr t1, mpf_ptr t2, mpf_ptr t3) {
+    mpf_add(t1, zr, zi);
+    mpf_sub(t2, zr, zi);
+    mpf_mul(t3, zr, zi);
+    mpf_mul(zr, t1, t2);
+    mpf_add(zr, zr, cr);
+    mpf_mul_ui(zi, t3, 2);
+    mpf_add(zi, zi, ci);
+}
+
+// Period of the disk |c - (cr, ci)| <= r by ball arithmetic (g_ballPeriod): the disk
+// of z_n has radius R' = (2|z| + R) R + r plus a roundoff allowance; the first n with
+// |z_n| <= R_n; 0 when the disk leaves the bailout region or n passes maxp.
+int ball_period(mpf_srcptr cr, mpf_srcptr ci, mpf_srcptr r, int maxp, mp_bitcnt_t prec) {
+    Gmp zr(prec), zi(prec), R(prec), R2m(prec), zabs2(prec), nb(prec), ru(prec), t(prec), t1(prec), t2(prec), t3(prec);
+    mpf_set_ui(zr, 0);
+    mpf_set_ui(zi, 0);
+    mpf_set_ui(R, 0);
+    const double cab = std::hypot(mpf_get_d(cr), mpf_get_d(ci));
+    mpf_set_ui(ru, 1);
+    mpf_div_2exp(ru, ru, prec > 64 ? prec - 32 : prec / 2);
+    for (int p = 1; p <= maxp; ++p) {
+        const double za = std::hypot(mpf_get_d(zr), mpf_get_d(zi));
+        mpf_set_d(t, 2.0 * za);
+        mpf_add(nb, t, R);
+        mpf_mul(nb, nb, R);
+        mpf_add(nb, nb, r);
+        mpf_set_d(t, za * za + cab + 1.0);
+        mpf_mul(t, t, ru);
+        mpf_add(nb, nb, t);
+        mpf_set(R, nb);
+        nuc_sqadd(zr, zi, cr, ci, t1, t2, t3);
+        mpf_mul(t1, zr, zr);
+        mpf_mul(t2, zi, zi);
+        mpf_add(zabs2, t1, t2);
+        mpf_mul(R2m, R, R);
+        if (mpf_cmp(zabs2, R2m) <= 0) return p;
+        const double zb = std::hypot(mpf_get_d(zr), mpf_get_d(zi));
+        if (zb > cab + 2.0 + mpf_get_d(R)) return 0;
+    }
+    return 0;
+}
+
+// Newton's method for the period-p nucleus, c <- c - z_p / (dz_p/dc) from z_0 = 0
+// (g_newton); true once a step is below the tolerance 2^-(prec-48) in modulus
+bool newton_nucleus(mpf_ptr cr, mpf_ptr ci, int period, mp_bitcnt_t prec, int maxit) {
+    Gmp zr(prec), zi(prec), dr(prec), di(prec), den(prec), nr(prec), ni(prec), tol(prec), step2(prec), t1(prec), t2(prec), t3(prec);
+    mpf_set_ui(tol, 1);
+    mpf_div_2exp(tol, tol, prec > 96 ? prec - 48 : prec / 2);
+    mpf_mul(tol, tol, tol);
+    for (int it = 0; it < maxit; ++it) {
+        mpf_set_ui(zr, 0);
+        mpf_set_ui(zi, 0);
+        mpf_set_ui(dr, 0);
+        mpf_set_ui(di, 0);
+        for (int i = 0; i < period; ++i) {
+            mpf_mul(t1, zr, dr);
+            mpf_mul(t2, zi, di);
+            mpf_sub(t3, t1, t2);
+            mpf_mul_ui(t3, t3, 2);
+            mpf_add_ui(t3, t3, 1);
+            mpf_mul(t1, zr, di);
+            mpf_mul(t2, zi, dr);
+            mpf_add(di, t1, t2);
+            mpf_mul_ui(di, di, 2);
+            mpf_set(dr, t3);
+            nuc_sqadd(zr, zi, cr, ci, t1, t2, t3);
+        }
+        mpf_mul(t1, dr, dr);
+        mpf_mul(t2, di, di);
+        mpf_add(den, t1, t2);
+        if (mpf_sgn(den.v) == 0) return false;
+        mpf_mul(t1, zr, dr);
+        mpf_mul(t2, zi, di);
+        mpf_add(t3, t1, t2);
+        mpf_div(nr, t3, den);
+        mpf_mul(t1, zi, dr);
+        mpf_mul(t2, zr, di);
+        mpf_sub(t3, t1, t2);
+        mpf_div(ni, t3, den);
+        mpf_sub(cr, cr, nr);
+        mpf_sub(ci, ci, ni);
+        mpf_mul(t1, nr, nr);
+        mpf_mul(t2, ni, ni);
+        mpf_add(step2, t1, t2);
+        if (mpf_cmp(step2, tol) <= 0) return true;
+    }
+    return false;
+}
+
+} // namespace
+
+// findNucleus (mandel_perturbation.cpp:1072-1106): the period of the disk around the
+// view centre of radius 2/scale, refined by Newton; the period when the nucleus lies
+// in that disk, else 0
+int Frame::find_nucleus(mpf_ptr nre, mpf_ptr nim, mpf_srcptr scale, int maxp) {
+    const mp_bitcnt_t p = mpf_get_prec(c0_re.v);
+    Gmp cre(p), cim(p), r(p), dh(p), dist(p), dr(p), di(p);
+    mpf_set_ui(r, 2);
+    mpf_div(r, r, scale);
+    mpf_add(cre, c0_re.v, r);
+    mpf_mul_ui(dh, r, H);
+    mpf_div_ui(dh, dh, W);
+    mpf_add(cim, c0_im.v, dh);
+    const int period = ball_period(cre, cim, r, maxp, p);
+    if (period <= 0) return 0;
+    mpf_set(nre, cre);
+    mpf_set(nim, cim);
+    if (!newton_nucleus(nre, nim, period, p, 200)) return 0;
+    mpf_sub(dr, nre, cre);
+    mpf_sub(di, nim, cim);
+    mpf_mul(dr, dr, dr);
+    mpf_mul(di, di, di);
+    mpf_add(dist, dr, di);
+    mpf_sqrt(dist, dist);
+    return mpf_cmp(dist, r) <= 0 ? period : 0;
+}
+
+// nucleusSizeMag (mandel_perturbation.cpp:1196-1219): the island's linear size
+// 1 / (|b| |l|^2), l = prod 2 z_k, b = 1 + sum 1/l_k over the one-period orbit
+Fe Frame::nucleus_size() const {
+    const int p = period;
+    if (p < 2) return fe_make(1.0, 1000000);
+    Fe lr = fe_make(1.0, 0), li = fe_make(0.0, 0), br = fe_make(1.0, 0), bi = fe_make(0.0, 0);
+    for (int k = 1; k < p; ++k) {
+        const Fe zr = fe_make(zfe_m[2 * (size_t)(k - 1)], zfe_e[2 * (size_t)(k - 1)]);
+        const Fe zi = fe_make(zfe_m[2 * (size_t)(k - 1) + 1], zfe_e[2 * (size_t)(k - 1) + 1]);
+        const Fe nr = fe_mul_d(fe_sub(fe_mul(zr, lr), fe_mul(zi, li)), 2.0);
+        const Fe ni = fe_mul_d(fe_add(fe_mul(zr, li), fe_mul(zi, lr)), 2.0);
+        lr = nr;
+        li = ni;
+        const Fe l2 = fe_add(fe_mul(lr, lr), fe_mul(li, li));
+        if (l2.m == 0.0) continue;
+        br = fe_add(br, fe_div(lr, l2));
+        bi = fe_sub(bi, fe_div(li, l2));
+    }
+    const Fe lmag = fe_sqrt(fe_add(fe_mul(lr, lr), fe_mul(li, li)));
+    const Fe bmag = fe_sqrt(fe_add(fe_mul(br, br), fe_mul(bi, bi)));
+    const Fe denom = fe_mul(bmag, fe_mul(lmag, lmag));
+    if (denom.m == 0.0) return fe_make(1.0, 1000000);
+    return fe_div(fe_make(1.0, 0), denom);
+}
+
+// The periodic branch of buildReferenceOrbit (mandel_perturbation.cpp:1238-1324): a
+// nucleus in view whose island spans at least 4 pixels becomes the reference, its
+// orbit one period long and read modulo the period; a sub-pixel island or none
+// leaves the full reference to the caller (false).
+bool Frame::try_periodic(mpf_srcptr scale) {
+    const mp_bitcnt_t np = mpf_get_prec(c0_re.v);
+    Gmp nre(np), nim(np), t(np);
+    const int p = find_nucleus(nre, nim, scale, max_iter);
+    if (!(p > 2 && p <= max_iter)) return false;
+    // the nucleus's fractional pixel position (createPeriodicRef, 1121-1127)
+    mpf_sub(t, nre, c0_re.v);
+    mpf_div(t, t, dx.v);
+    const double rx = mpf_get_d(t);
+    mpf_sub(t, nim, c0_im.v);
+    mpf_div(t, t, dy.v);
+    const double ry = mpf_get_d(t);
+    build_orbit(nre, nim, p);
+    period = p;
+    if (fe_abs_less(nucleus_size(), fe_mul_d(dxfe, 4.0))) {
+        period = 0;
+        subpixel = true;
+        return false;
+    }
+    ref_virtual = true;
+    ref_x = rx;
+    ref_y = ry;
+    cutoff_sensitive = false;
+    return true;
 }
 
 // get_exp (float_math.cpp:23-29): floor(log2|x|), zero -> -2^30
@@ -808,7 +1003,13 @@ int Frame::prepare(const mx_request& r, std::string& err) {
     // reference: the centre pixel, then the exact view centre when the pixel
     // reference is bounded-but-sensitive (mandel_perturbation.cpp:1326-1373)
     const double t_ref = now_s();
-    {
+    period = 0;
+    subpixel = false;
+    // a nucleus in view on the deep floatexp path with BLA (buildReferenceOrbit,
+    // mandel_perturbation.cpp:1238-1250): a resolved one becomes the reference
+    if (!escalated && !view_ref && use_bla && !(r.options & MX_OPT_NO_PERIODIC) && mpf_cmp_d(scale, 1e280) > 0 && try_periodic(scale)) {
+        ref_seconds = now_s() - t_ref;
+    } else {
         Gmp pre(prec), pim(prec), t(prec);
         const int pi = H / 2, pj = W / 2;
         mpf_mul_ui(t, dx.v, pj);
@@ -851,8 +1052,10 @@ int Frame::prepare(const mx_request& r, std::string& err) {
         }
         // value of the reference pixel itself (mandel_perturbation.cpp:4145-4160, 4179-4185)
         if (!ref_virtual) ref_pixel(true);
+        ref_seconds = now_s() - t_ref;
     }
-    ref_seconds = now_s() - t_ref;
+    // a sub-pixel nucleus: BLA off for the frame (Compute, mandel_perturbation.cpp:1624-1629)
+    if (subpixel) use_bla = false;
 
     const double t_bla = now_s();
     if (use_bla) {
@@ -986,7 +1189,9 @@ Frame::~Frame() {
 
 void Frame::fill_view(mx_frame_view& v) const {
     std::memset(&v, 0, sizeof v);
-    v.len = ref_it + 1;
+    v.len = period ? period : ref_it + 1;
+    v.period = period;
+    v.subpixel = subpixel;
     v.zf = zf.data();
     v.zfe_m = zfe_m.data();
     v.zfe_e = zfe_e.data();

@@ -48,6 +48,8 @@ struct Frame {
     // reference
     Mpf ref_re, ref_im, tail_re, tail_im;
     int ref_it = 0;
+    int period = 0;        // > 0: the reference is a period-`period` nucleus, its orbit one period
+    bool subpixel = false; // a nucleus in view spans < 4 pixels: full reference with BLA off
     bool ref_escaped = false, ref_virtual = false, cutoff_sensitive = false;
     double ref_x = 0.0, ref_y = 0.0;
     int ref_pixel_n = -1;
@@ -94,7 +96,10 @@ struct Frame {
     int job_n = -1;
     double job_value = -2.0;
     void init_mpf();
-    void build_orbit(mpf_srcptr cre, mpf_srcptr cim);
+    void build_orbit(mpf_srcptr cre, mpf_srcptr cim, int period_len = 0);
+    int find_nucleus(mpf_ptr nre, mpf_ptr nim, mpf_srcptr scale, int maxp);
+    bool try_periodic(mpf_srcptr scale);
+    Fe nucleus_size() const;
     void build_orbit_fixed(mpf_srcptr cre, mpf_srcptr cim);
     void ref_pixel(bool async);
     void sa_step(int i);

@@ -12,7 +12,7 @@ LIB_PATH = os.path.join(_HERE, "libmxhip.so")
 MX_OK, MX_E_INVALID, MX_E_NODEVICE, MX_E_HIP, MX_E_CANCELLED, MX_E_BUSY, MX_E_NOMEM, MX_E_UNSUPPORTED = 0, -1, -2, -3, -4, -5, -6, -7
 MX_FORM_FE64, MX_FORM_FE32, MX_FORM_DOUBLE, MX_FORM_SC64 = 0, 1, 2, 3
 MX_EXTERIOR_DIST_EST = 2
-MX_OPT_VIEW_CENTER_REF, MX_OPT_FLOATEXP = 1, 2
+MX_OPT_VIEW_CENTER_REF, MX_OPT_FLOATEXP, MX_OPT_NO_PERIODIC = 1, 2, 4
 ERRORS = {-1: "invalid", -2: "no device", -3: "hip error", -4: "cancelled", -5: "busy", -6: "out of memory", -7: "unsupported"}
 
 
@@ -28,7 +28,7 @@ class RefDesc(C.Structure):
                 ("bla_r2", C.c_void_p), ("bla_coef", C.c_void_p), ("rmax2", C.c_double),
                 ("z_m3", C.c_void_p), ("sa_it", C.c_int32), ("sa_order", C.c_int32), ("sa_coef", C.c_void_p),
                 ("sa_delta_re", C.c_double), ("sa_delta_im", C.c_double), ("df", C.c_void_p), ("bla_der", C.c_void_p),
-                ("sa_coef_b", C.c_void_p)]
+                ("sa_coef_b", C.c_void_p), ("period", C.c_int32)]
 
 
 class Tile(C.Structure):
@@ -60,7 +60,8 @@ class FrameInfo(C.Structure):
                 ("probe_ms", C.c_double), ("kernel_ms", C.c_double), ("total_seconds", C.c_double),
                 ("steps", C.c_uint64), ("bla_applies", C.c_uint64), ("bla_skipped", C.c_uint64),
                 ("rebases", C.c_uint64), ("nominal_iterations", C.c_uint64), ("passes", C.c_int32),
-                ("sa_it", C.c_int32), ("escalated", C.c_int32), ("ss_pixels", C.c_int32), ("ss_ms", C.c_double)]
+                ("sa_it", C.c_int32), ("escalated", C.c_int32), ("ss_pixels", C.c_int32), ("ss_ms", C.c_double),
+                ("period", C.c_int32), ("subpixel", C.c_int32)]
 
 
 class FrameView(C.Structure):
@@ -79,7 +80,8 @@ class FrameView(C.Structure):
                 ("sa_coef", C.POINTER(C.c_double)), ("sa_delta_re", C.c_double), ("sa_delta_im", C.c_double),
                 ("dcol_d", C.POINTER(C.c_double)), ("drow_d", C.POINTER(C.c_double)),
                 ("dfr", C.POINTER(C.c_double)), ("dfi", C.POINTER(C.c_double)), ("bla_der", C.POINTER(C.c_double)),
-                ("sa_coef_b", C.POINTER(C.c_double)), ("escalated", C.c_int32)]
+                ("sa_coef_b", C.POINTER(C.c_double)), ("escalated", C.c_int32), ("period", C.c_int32),
+                ("subpixel", C.c_int32)]
 
 
 # every symbol include/mx.h declares
@@ -176,7 +178,7 @@ def frame_prepare(cre, cim, scale, W, H, mxit, **kw):
                  ref_y=v.ref_y, ref_pixel_n=v.ref_pixel_n, ref_pixel_value=v.ref_pixel_value,
                  cutoff_sensitive=v.cutoff_sensitive, precision_bits=v.precision_bits,
                  ref_seconds=v.ref_seconds, bla_seconds=v.bla_seconds, W=W, H=H, mxit=mxit, path=v.path,
-                 escalated=v.escalated)
+                 escalated=v.escalated, period=v.period, subpixel=v.subpixel)
         if v.path == 1:
             d.update(z_m3=_arr(v.z_m3, n, np.float64), sa_it=v.sa_it, sa_order=v.sa_order,
                      sa_coef=_arr(v.sa_coef, 60, np.float64), sa_delta=complex(v.sa_delta_re, v.sa_delta_im),
@@ -247,6 +249,7 @@ class Context:
 
         d = RefDesc()
         d.len = fv["len"]
+        d.period = fv.get("period", 0)
         d.zf, d.zfe_m, d.zfe_e = a(fv["zf"], np.float64), a(fv["zfe_m"], np.float64), a(fv["zfe_e"], np.int32)
         d.nlev = fv["nlev"]
         if d.nlev:

@@ -679,6 +679,7 @@ typedef struct {
     int int_rep;
     int ede;
     mxo_fe dxfe;
+    int per; /* > 0: periodic reference, X_m read at (m-1) mod per (MP.cpp:3308-3313) */
 } mxo_ctx;
 
 typedef struct {
@@ -804,8 +805,12 @@ static int try_bla_fe(const mxo_ctx* c, int s, mxo_fe* S, mxo_fe S2, double* wr,
             if (fr * fr + fi * fi > c->esc2) return 0;
             double wm2 = nwr * nwr + nwi * nwi;
             if (wm2 == 0.0) {
-                /* non-periodic reference: reject the unrepresentable landing */
-                return 0;
+                /* a periodic reference keeps the nonzero unit delta; a full one rejects
+                 * the unrepresentable landing (MP.cpp:2919-2930) */
+                if (c->per > 0)
+                    bla_rerescale_underflow(S, wr, wi, nwr, nwi);
+                else
+                    return 0;
             } else {
                 double wmag = sqrt(wm2);
                 *S = mxo_fe_mul(*S, mxo_fe_from(wmag));
@@ -895,7 +900,7 @@ static double pixel_rescaled(const mxo_ctx* c, mxo_fe dcr, mxo_fe dci, int mxit,
         }
         if (c->bla && m >= 2) {
             double ab[4];
-            int skip = try_bla_fe(c, rm, &S, S2, &wr, &wi, dr, di, reflen, deriv ? ab : NULL);
+            int skip = try_bla_fe(c, rm, &S, S2, &wr, &wi, dr, di, c->per ? c->per : reflen, deriv ? ab : NULL);
             if (skip > 0) {
                 if (deriv) {
                     mxo_fe Jr = mxo_fe_mul_d(SJ, jr), Ji = mxo_fe_mul_d(SJ, ji);
@@ -915,6 +920,7 @@ static double pixel_rescaled(const mxo_ctx* c, mxo_fe dcr, mxo_fe dci, int mxit,
                 m += skip;
                 iter += skip;
                 rm += skip;
+                if (c->per && rm >= c->per) rm -= c->per;
                 st->bla_applies++;
                 st->bla_skipped += skip;
                 s = mxo_fe_to_double(S);
@@ -964,6 +970,7 @@ static double pixel_rescaled(const mxo_ctx* c, mxo_fe dcr, mxo_fe dci, int mxit,
         ++m;
         ++iter;
         ++rm;
+        if (c->per && rm == c->per) rm = 0;
         st->steps++;
         if (deriv) {
             if (deriv_underflow) {
@@ -1050,14 +1057,14 @@ static double pixel_rescaled(const mxo_ctx* c, mxo_fe dcr, mxo_fe dci, int mxit,
             st->rescales++;
         }
         double dzmag2 = s * s * (wr * wr + wi * wi);
-        if (zrad < 1e-8 || zrad < dzmag2 || m >= reflen) {
+        if (zrad < 1e-8 || zrad < dzmag2 || (!c->per && m >= reflen)) {
             mxo_fe Xmr = m ? c->zre_fe[rm] : mxo_fe_make(0.0, 0);
             mxo_fe Xmi = m ? c->zim_fe[rm] : mxo_fe_make(0.0, 0);
             mxo_fe Swr = mxo_fe_mul_d(S, wr), Swi = mxo_fe_mul_d(S, wi);
             mxo_fe zrfe = mxo_fe_add(Xmr, Swr), zife = mxo_fe_add(Xmi, Swi);
             mxo_fe zradfe = mxo_fe_add(mxo_fe_mul(zrfe, zrfe), mxo_fe_mul(zife, zife));
             mxo_fe dzfe = mxo_fe_add(mxo_fe_mul(Swr, Swr), mxo_fe_mul(Swi, Swi));
-            if (m >= reflen || mxo_fe_abs_less(zradfe, dzfe)) {
+            if ((!c->per && m >= reflen) || mxo_fe_abs_less(zradfe, dzfe)) {
                 mxo_fe Snew = mxo_fe_sqrt(zradfe);
                 if (Snew.m == 0.0) {
                     wr = wi = 0.0;
@@ -1127,6 +1134,12 @@ typedef struct {
     double* cell_val;
     int32_t* cell_nesc;
     int ss_pixels;
+    /* periodic-nucleus reference (MP.cpp:1238-1324): input no_periodic skips the
+     * search (MANDEL_PERIODIC=0); outputs the nucleus period used (0: full reference)
+     * and whether a nucleus was found sub-pixel (then BLA is off for the frame) */
+    int no_periodic;
+    int period;
+    int subpixel;
 } mxo_frame;
 
 /* output cell of sample (i, j, u, v) (getIndex, MP.cpp:4334-4340) */
@@ -1220,6 +1233,230 @@ static void run_pixels(const mxo_ctx* ctx, const mxo_frame* f, const mxo_fe* dco
     tot->rebases += rb;
 }
 
+/* ------------------------------------------------------------------------- */
+/* periodic-nucleus reference (MP.cpp:76-224 g_ballPeriod / g_newton,          */
+/* 1072-1219 findNucleus / createPeriodicRef / nucleusSizeMag)                  */
+/* ------------------------------------------------------------------------- */
+
+/* z = z^2 + c, Karatsuba square in mpf (g_sqadd, MP.cpp:97-105) */
+static void nuc_sqadd(mpf_ptr zr, mpf_ptr zi, mpf_srcptr cr, mpf_srcptr ci, mpf_ptr t1, mpf_ptr t2, mpf_ptr t3) {
+    mpf_add(t1, zr, zi);
+    mpf_sub(t2, zr, zi);
+    mpf_mul(t3, zr, zi);
+    mpf_mul(zr, t1, t2);
+    mpf_add(zr, zr, cr);
+    mpf_mul_ui(zi, t3, 2);
+    mpf_add(zi, zi, ci);
+}
+
+/* g_ballPeriod (MP.cpp:116-166): the disk |c - c0| <= r through z -> z^2 + c as a
+ * disk of radius R_{n+1} = (2|z_n| + R_n) R_n + r + (|z_n|^2 + |c| + 1) 2^-(prec-32);
+ * the first n with |z_n|^2 <= R_n^2, or 0 once |z_n| > |c| + 2 + R_n or n > maxp */
+static int nuc_ball_period(mpf_srcptr cr, mpf_srcptr ci, mpf_srcptr r, int maxp, mp_bitcnt_t prec) {
+    mpf_t zr, zi, R, R2m, zabs2, nb, ru, t, t1, t2, t3;
+    mpf_t* all[] = {&zr, &zi, &R, &R2m, &zabs2, &nb, &ru, &t, &t1, &t2, &t3};
+    for (int k = 0; k < 11; ++k) mpf_init2(*all[k], prec);
+    mpf_set_ui(zr, 0);
+    mpf_set_ui(zi, 0);
+    mpf_set_ui(R, 0);
+    const double cab = hypot(mpf_get_d(cr), mpf_get_d(ci));
+    mpf_set_ui(ru, 1);
+    mpf_div_2exp(ru, ru, prec > 64 ? prec - 32 : prec / 2);
+    int period = 0;
+    for (int n = 1; n <= maxp; ++n) {
+        const double za = hypot(mpf_get_d(zr), mpf_get_d(zi));
+        mpf_set_d(t, 2.0 * za);
+        mpf_add(nb, t, R);
+        mpf_mul(nb, nb, R);
+        mpf_add(nb, nb, r);
+        mpf_set_d(t, za * za + cab + 1.0);
+        mpf_mul(t, t, ru);
+        mpf_add(nb, nb, t);
+        mpf_set(R, nb);
+        nuc_sqadd(zr, zi, cr, ci, t1, t2, t3);
+        mpf_mul(t1, zr, zr);
+        mpf_mul(t2, zi, zi);
+        mpf_add(zabs2, t1, t2);
+        mpf_mul(R2m, R, R);
+        if (mpf_cmp(zabs2, R2m) <= 0) {
+            period = n;
+            break;
+        }
+        if (hypot(mpf_get_d(zr), mpf_get_d(zi)) > cab + 2.0 + mpf_get_d(R)) break;
+    }
+    for (int k = 0; k < 11; ++k) mpf_clear(*all[k]);
+    return period;
+}
+
+/* g_newton (MP.cpp:168-222): c <- c - z_p / z_p' with (z, z') from (0, 0) by
+ * z' <- 2 z z' + 1, z <- z^2 + c; converged once |step|^2 <= 2^-2(prec-48) */
+static int nuc_newton(mpf_ptr cr, mpf_ptr ci, int period, mp_bitcnt_t prec, int maxit) {
+    mpf_t zr, zi, dr, di, den, nr, ni, tol, step2, t1, t2, t3;
+    mpf_t* all[] = {&zr, &zi, &dr, &di, &den, &nr, &ni, &tol, &step2, &t1, &t2, &t3};
+    for (int k = 0; k < 12; ++k) mpf_init2(*all[k], prec);
+    mpf_set_ui(tol, 1);
+    mpf_div_2exp(tol, tol, prec > 96 ? prec - 48 : prec / 2);
+    mpf_mul(tol, tol, tol);
+    int conv = 0;
+    for (int it = 0; it < maxit; ++it) {
+        mpf_set_ui(zr, 0);
+        mpf_set_ui(zi, 0);
+        mpf_set_ui(dr, 0);
+        mpf_set_ui(di, 0);
+        for (int i = 0; i < period; ++i) {
+            mpf_mul(t1, zr, dr);
+            mpf_mul(t2, zi, di);
+            mpf_sub(t3, t1, t2);
+            mpf_mul_ui(t3, t3, 2);
+            mpf_add_ui(t3, t3, 1);
+            mpf_mul(t1, zr, di);
+            mpf_mul(t2, zi, dr);
+            mpf_add(di, t1, t2);
+            mpf_mul_ui(di, di, 2);
+            mpf_set(dr, t3);
+            nuc_sqadd(zr, zi, cr, ci, t1, t2, t3);
+        }
+        mpf_mul(t1, dr, dr);
+        mpf_mul(t2, di, di);
+        mpf_add(den, t1, t2);
+        if (mpf_sgn(den) == 0) break;
+        mpf_mul(t1, zr, dr);
+        mpf_mul(t2, zi, di);
+        mpf_add(t3, t1, t2);
+        mpf_div(nr, t3, den);
+        mpf_mul(t1, zi, dr);
+        mpf_mul(t2, zr, di);
+        mpf_sub(t3, t1, t2);
+        mpf_div(ni, t3, den);
+        mpf_sub(cr, cr, nr);
+        mpf_sub(ci, ci, ni);
+        mpf_mul(t1, nr, nr);
+        mpf_mul(t2, ni, ni);
+        mpf_add(step2, t1, t2);
+        if (mpf_cmp(step2, tol) <= 0) {
+            conv = 1;
+            break;
+        }
+    }
+    for (int k = 0; k < 12; ++k) mpf_clear(*all[k]);
+    return conv;
+}
+
+/* findNucleus (MP.cpp:1072-1106): search disk centred on the view centre
+ * c0 + (r, r H / W), radius r = 2 / scale; the period when Newton converges to a
+ * point inside the disk, else 0 */
+static int nuc_find(const mxo_grid* g, const char* scale, int W, int H, int maxp, mpf_ptr nre, mpf_ptr nim) {
+    const mp_bitcnt_t prec = mpf_get_prec(g->fc0_re);
+    mpf_t sc, cre, cim, r, dh, dist, dr, di;
+    mpf_t* all[] = {&sc, &cre, &cim, &r, &dh, &dist, &dr, &di};
+    for (int k = 0; k < 8; ++k) mpf_init2(*all[k], prec);
+    mpf_set_str(sc, scale, 10);
+    mpf_set_ui(r, 2);
+    mpf_div(r, r, sc);
+    mpf_add(cre, g->fc0_re, r);
+    mpf_mul_ui(dh, r, H);
+    mpf_div_ui(dh, dh, W);
+    mpf_add(cim, g->fc0_im, dh);
+    int result = 0;
+    const int period = nuc_ball_period(cre, cim, r, maxp, prec);
+    if (period > 0) {
+        mpf_set(nre, cre);
+        mpf_set(nim, cim);
+        if (nuc_newton(nre, nim, period, prec, 200)) {
+            mpf_sub(dr, nre, cre);
+            mpf_sub(di, nim, cim);
+            mpf_mul(dr, dr, dr);
+            mpf_mul(di, di, di);
+            mpf_add(dist, dr, di);
+            mpf_sqrt(dist, dist);
+            if (mpf_cmp(dist, r) <= 0) result = period;
+        }
+    }
+    for (int k = 0; k < 8; ++k) mpf_clear(*all[k]);
+    return result;
+}
+
+static double mpf_get_ld_c(mpf_srcptr x) {
+    long e;
+    const double d = mpf_get_d_2exp(&e, x);
+    return ldexp(d, (int)e);
+}
+
+static mxo_fe mpf_to_fe_c(mpf_srcptr x) {
+    if (mpf_sgn(x) == 0) return mxo_fe_make(0.0, 0);
+    long e;
+    const double d = mpf_get_d_2exp(&e, x);
+    return mxo_fe_make(d, (int64_t)e);
+}
+
+/* createPeriodicRef (MP.cpp:1108-1172): the nucleus's orbit for one period, X_1 = c
+ * .. X_p, in mpf with the Karatsuba square (calCoefficient's mpf branch, 4230-4245);
+ * shadows from mpf_get_d_2exp */
+static void orbit_build_periodic(mxo_orbit* o, mpf_srcptr cre, mpf_srcptr cim, int period) {
+    const mp_bitcnt_t prec = mpf_get_prec(cre);
+    o->prec = (mpfr_prec_t)prec;
+    o->zf = (double*)malloc(sizeof(double) * 2 * (size_t)period);
+    o->zre_fe = (mxo_fe*)malloc(sizeof(mxo_fe) * (size_t)period);
+    o->zim_fe = (mxo_fe*)malloc(sizeof(mxo_fe) * (size_t)period);
+    o->dfr = (double*)calloc((size_t)period, sizeof(double));
+    o->dfi = (double*)calloc((size_t)period, sizeof(double));
+    o->z_m3 = (double*)malloc(sizeof(double) * (size_t)period);
+    mpfr_inits2((mpfr_prec_t)prec, o->last_re, o->last_im, o->c_re, o->c_im, (mpfr_ptr)0);
+    mpfr_set_f(o->c_re, cre, MPFR_RNDN);
+    mpfr_set_f(o->c_im, cim, MPFR_RNDN);
+    mpf_t zr, zi, t1, t2, nzr, nzi;
+    mpf_t* all[] = {&zr, &zi, &t1, &t2, &nzr, &nzi};
+    for (int k = 0; k < 6; ++k) mpf_init2(*all[k], prec);
+    mpf_set(zr, cre);
+    mpf_set(zi, cim);
+    for (int i = 0; i < period; ++i) {
+        if (i > 0) {
+            mpf_add(t1, zr, zi);
+            mpf_sub(t2, zr, zi);
+            mpf_mul(nzi, zr, zi);
+            mpf_mul(nzr, t1, t2);
+            mpf_mul_ui(nzi, nzi, 2);
+            mpf_add(zr, nzr, cre);
+            mpf_add(zi, nzi, cim);
+        }
+        o->zf[2 * i] = mpf_get_ld_c(zr);
+        o->zf[2 * i + 1] = mpf_get_ld_c(zi);
+        o->zre_fe[i] = mpf_to_fe_c(zr);
+        o->zim_fe[i] = mpf_to_fe_c(zi);
+        o->z_m3[i] = (o->zf[2 * i] * o->zf[2 * i] + o->zf[2 * i + 1] * o->zf[2 * i + 1]) / 1000000;
+    }
+    o->dfr[0] = 1.0;
+    o->escaped = 0;
+    o->ref_it = period;
+    o->len = period;
+    mpfr_set_f(o->last_re, zr, MPFR_RNDN);
+    mpfr_set_f(o->last_im, zi, MPFR_RNDN);
+    for (int k = 0; k < 6; ++k) mpf_clear(*all[k]);
+}
+
+/* nucleusSizeMag (MP.cpp:1196-1219): 1 / (|b| |l|^2) with l = prod_{k<p} 2 z_k and
+ * b = 1 + sum 1 / l_k, z_k = X_k, in FloatExp */
+static mxo_fe nuc_size(const mxo_orbit* o, int p) {
+    if (p < 2) return mxo_fe_make(1.0, 1000000);
+    mxo_fe lr = mxo_fe_make(1.0, 0), li = mxo_fe_make(0.0, 0), br = mxo_fe_make(1.0, 0), bi = mxo_fe_make(0.0, 0);
+    for (int k = 1; k < p; ++k) {
+        const mxo_fe zr = o->zre_fe[k - 1], zi = o->zim_fe[k - 1];
+        const mxo_fe nr = mxo_fe_scale2(mxo_fe_sub(mxo_fe_mul(zr, lr), mxo_fe_mul(zi, li)), 1);
+        const mxo_fe ni = mxo_fe_scale2(mxo_fe_add(mxo_fe_mul(zr, li), mxo_fe_mul(zi, lr)), 1);
+        lr = nr;
+        li = ni;
+        const mxo_fe l2 = mxo_fe_add(mxo_fe_mul(lr, lr), mxo_fe_mul(li, li));
+        if (l2.m == 0.0) continue;
+        br = mxo_fe_add(br, mxo_fe_div(lr, l2));
+        bi = mxo_fe_sub(bi, mxo_fe_div(li, l2));
+    }
+    const mxo_fe lmag = mxo_fe_sqrt(mxo_fe_add(mxo_fe_mul(lr, lr), mxo_fe_mul(li, li)));
+    const mxo_fe bmag = mxo_fe_sqrt(mxo_fe_add(mxo_fe_mul(br, br), mxo_fe_mul(bi, bi)));
+    const mxo_fe den = mxo_fe_mul(bmag, mxo_fe_mul(lmag, lmag));
+    if (den.m == 0.0) return mxo_fe_make(1.0, 1000000);
+    return mxo_fe_div(mxo_fe_make(1.0, 0), den);
+}
+
 MXO_EXPORT int mxo_compute_fe(mxo_frame* f) {
     const int W = f->W, H = f->H;
     const int pb = mpf_bits(f->prec_bits);
@@ -1243,18 +1480,56 @@ MXO_EXPORT int mxo_compute_fe(mxo_frame* f) {
         escalated = mpfr_cmp_d(sc, 1e280) <= 0;
         mpfr_clear(sc);
     }
-    if (escalated)
-        grid_view_center(&g, W, H, rre, rim);
-    else
-        grid_pixel_c(&g, H / 2, W / 2, 0, 0, 1, rre, rim);
     /* one integer limb: the fixed-point orbit needs |z|^2 + |c| well below 2^64
      * (bigFixedRangeSafe, MP.cpp:4093-4095); L = 64-bit limbs of the mpf centre + 2 */
     const int fxL = f->escape_radius <= 0x1p31f ? pb / 64 : 0;
     mxo_orbit o;
-    orbit_build(&o, rre, rim, pb, f->mxit, escR, ede, 1, fxL);
     int ref_virtual = 0;
     double ref_x = W / 2, ref_y = H / 2;
     f->cutoff_sensitive_hint = 0;
+    f->period = 0;
+    f->subpixel = 0;
+    int have_ref = 0;
+    /* a nucleus in view on the deep path with BLA (buildReferenceOrbit, MP.cpp:
+     * 1238-1324): resolved (at least 4 pixels) it is the reference, its orbit one
+     * period long; sub-pixel, the centre pixel's full reference with BLA off */
+    if (!escalated && f->use_bla && !f->no_periodic) {
+        const mp_bitcnt_t np = mpf_get_prec(g.fc0_re);
+        mpf_t nre, nim, t;
+        mpf_init2(nre, np);
+        mpf_init2(nim, np);
+        mpf_init2(t, np);
+        const int p = nuc_find(&g, f->scale, W, H, f->mxit, nre, nim);
+        if (p > 2 && p <= f->mxit) {
+            /* the nucleus's fractional pixel position (MP.cpp:1121-1127) */
+            mpf_sub(t, nre, g.fc0_re);
+            mpf_div(t, t, g.fdx);
+            const double rx = mpf_get_d(t);
+            mpf_sub(t, nim, g.fc0_im);
+            mpf_div(t, t, g.fdy);
+            const double ry = mpf_get_d(t);
+            orbit_build_periodic(&o, nre, nim, p);
+            if (mxo_fe_abs_less(nuc_size(&o, p), mxo_fe_mul_d(dxfe, 4.0))) {
+                orbit_free(&o);
+                f->subpixel = 1;
+            } else {
+                have_ref = 1;
+                f->period = p;
+                ref_virtual = 1;
+                ref_x = rx;
+                ref_y = ry;
+            }
+        }
+        mpf_clear(nre);
+        mpf_clear(nim);
+        mpf_clear(t);
+    }
+    if (!have_ref) {
+    if (escalated)
+        grid_view_center(&g, W, H, rre, rim);
+    else
+        grid_pixel_c(&g, H / 2, W / 2, 0, 0, 1, rre, rim);
+    orbit_build(&o, rre, rim, pb, f->mxit, escR, ede, 1, fxL);
     if (escalated) {
         ref_virtual = 1;
         ref_x = (W - 1) * 0.5;
@@ -1274,6 +1549,8 @@ MXO_EXPORT int mxo_compute_fe(mxo_frame* f) {
             if (!o.escaped) f->cutoff_sensitive_hint = 2; /* needs a brute-force probe at mxit+64 */
         }
     }
+    }
+    const int use_bla = f->use_bla && !f->subpixel;
     f->ref_it = o.ref_it;
     f->ref_escaped = o.escaped;
     f->ref_virtual = ref_virtual;
@@ -1282,7 +1559,7 @@ MXO_EXPORT int mxo_compute_fe(mxo_frame* f) {
 
     mxo_bla bla;
     memset(&bla, 0, sizeof bla);
-    if (f->use_bla) {
+    if (use_bla) {
         double dxf = fabs(mpfr_to_ld_trunc(g.dx)), dyf = fabs(mpfr_to_ld_trunc(g.dy));
         double a = ref_x, bx = (double)(W - 1) - ref_x;
         double mxx = (a > bx ? a : bx) + 0.5;
@@ -1297,7 +1574,8 @@ MXO_EXPORT int mxo_compute_fe(mxo_frame* f) {
     ctx.zre_fe = o.zre_fe;
     ctx.zim_fe = o.zim_fe;
     ctx.mx_ref_it = o.ref_it;
-    ctx.bla = f->use_bla ? &bla : NULL;
+    ctx.bla = use_bla ? &bla : NULL;
+    ctx.per = f->period;
     ctx.bla_minlevel = bla_minlevel_from_skip(f->bla_minskip > 0 ? f->bla_minskip : 8);
     ctx.esc2 = esc2;
     ctx.use_interior = f->interior;
@@ -1405,7 +1683,7 @@ MXO_EXPORT int mxo_compute_fe(mxo_frame* f) {
     free(idx);
     free(dcol);
     free(drow);
-    if (f->use_bla) bla_free(&bla);
+    if (use_bla) bla_free(&bla);
     orbit_free(&o);
     mpfr_clears(rre, rim, (mpfr_ptr)0);
     grid_clear(&g);
@@ -1559,10 +1837,11 @@ typedef struct {
     int64_t dx_e;
     int mxit;
     int threads;
+    int period; /* > 0: a periodic reference of `period` entries (mx_ref_it == period) */
 } mxo_pixels_in;
 
 MXO_EXPORT void mxo_pixels_fe(const mxo_pixels_in* in, int n, const double* dcr_m, const int64_t* dcr_e, const double* dci_m, const int64_t* dci_e, double* val, int32_t* nesc, long long* stats4) {
-    int64_t len = in->mx_ref_it + 1;
+    int64_t len = in->period > 0 ? in->period : in->mx_ref_it + 1;
     mxo_fe* zre = (mxo_fe*)malloc(sizeof(mxo_fe) * len);
     mxo_fe* zim = (mxo_fe*)malloc(sizeof(mxo_fe) * len);
     for (int64_t k = 0; k < len; ++k) {
@@ -1596,6 +1875,7 @@ MXO_EXPORT void mxo_pixels_fe(const mxo_pixels_in* in, int n, const double* dcr_
     ctx.int_rep = 1;
     ctx.ede = in->ede;
     ctx.dxfe = mxo_fe_make(in->dx_m, in->dx_e);
+    ctx.per = in->period > 0 ? in->period : 0;
     long long steps = 0, ap = 0, sk = 0, rb = 0;
 #pragma omp parallel for schedule(dynamic, 16) num_threads(in->threads > 0 ? in->threads : 1) reduction(+ : steps, ap, sk, rb)
     for (int q = 0; q < n; ++q) {
@@ -1662,6 +1942,36 @@ MXO_EXPORT void mxo_grid_info(const char* cre, const char* cim, const char* scal
     *dy_m = b.m;
     *dy_e = b.e;
     grid_clear(&g);
+}
+
+/* findNucleus and nucleusSizeMag for one view: the period (0: none in the search
+ * disk), the nucleus as decimal strings (digits with an implied point before them,
+ * and the exponent: value = 0.<digits> * 10^exp) and the island size in FloatExp */
+MXO_EXPORT int mxo_nucleus(const char* cre, const char* cim, const char* scale, int W, int H, int maxp, int prec_bits, char* re_digits, long* re_exp, char* im_digits, long* im_exp, int ndig, double* size_m, int64_t* size_e) {
+    prec_bits = mpf_bits(prec_bits);
+    mxo_grid g;
+    grid_init(&g, cre, cim, scale, W, H, 0, 0, prec_bits, 1);
+    const mp_bitcnt_t np = mpf_get_prec(g.fc0_re);
+    mpf_t nre, nim;
+    mpf_init2(nre, np);
+    mpf_init2(nim, np);
+    const int p = nuc_find(&g, scale, W, H, maxp, nre, nim);
+    *size_m = 0.0;
+    *size_e = 0;
+    if (p > 0) {
+        mpf_get_str(re_digits, re_exp, 10, (size_t)ndig - 2, nre);
+        mpf_get_str(im_digits, im_exp, 10, (size_t)ndig - 2, nim);
+        mxo_orbit o;
+        orbit_build_periodic(&o, nre, nim, p);
+        const mxo_fe sz = nuc_size(&o, p);
+        *size_m = sz.m;
+        *size_e = sz.e;
+        orbit_free(&o);
+    }
+    mpf_clear(nre);
+    mpf_clear(nim);
+    grid_clear(&g);
+    return p;
 }
 
 /* ========================================================================= */

@@ -26,7 +26,7 @@ class Frame(C.Structure):
         ("rebases", C.c_longlong),
         ("orbit_zf", C.c_void_p), ("orbit_m", C.c_void_p), ("orbit_e", C.c_void_p),
         ("sub", C.c_int), ("cells", C.c_void_p), ("cell_val", C.c_void_p), ("cell_nesc", C.c_void_p),
-        ("ss_pixels", C.c_int),
+        ("ss_pixels", C.c_int), ("no_periodic", C.c_int), ("period", C.c_int), ("subpixel", C.c_int),
     ]
 
 
@@ -38,7 +38,7 @@ class PixelsIn(C.Structure):
         ("r2", C.c_void_p), ("l", C.c_void_p), ("coef", C.c_void_p), ("rmax2", C.c_double),
         ("bla_minlevel", C.c_int), ("esc2", C.c_double), ("use_interior", C.c_int),
         ("ede", C.c_int), ("dx_m", C.c_double), ("dx_e", C.c_int64), ("mxit", C.c_int),
-        ("threads", C.c_int),
+        ("threads", C.c_int), ("period", C.c_int),
     ]
 
 
@@ -104,12 +104,15 @@ def precision_bits(scale_str):
 
 def compute_fe(cre, cim, scale, W, H, mxit, c_method=0, escape_radius=1e8, prec_bits=None,
                use_bla=True, bla_minskip=8, interior=True, probe=True, threads=None, full_h=0, row_base=0,
-               want_orbit=False, sub=1):
+               want_orbit=False, sub=1, periodic=True):
     """Whole floatexp frame (Compute -> reference -> BLA -> probe -> pass).  With
     want_orbit, info["orbit"] holds the reference orbit's shadows (zf, zfe_m, zfe_e,
     (re, im) interleaved).  With sub > 1, info["cells"] holds the (H sub) x (W sub)
     float, value and escape-count cells (-10, 0, INT_MIN where nothing was computed),
-    with the adaptive sub-samples when c_method has the super-sampling bit (1)."""
+    with the adaptive sub-samples when c_method has the super-sampling bit (1).
+    periodic=False skips the search for a nucleus in view (full reference always);
+    info["period"] is the nucleus period used as the reference (0: none) and
+    info["subpixel"] whether a nucleus was found below 4 pixels (BLA then off)."""
     f = Frame()
     if sub > 1:
         nc = W * H * sub * sub
@@ -128,6 +131,7 @@ def compute_fe(cre, cim, scale, W, H, mxit, c_method=0, escape_radius=1e8, prec_
     f.prec_bits = prec_bits or precision_bits(scale)
     f.use_bla, f.bla_minskip, f.bla_eps = int(use_bla), bla_minskip, 0.0
     f.interior, f.probe = int(interior), int(probe)
+    f.no_periodic = int(not periodic)
     f.threads = threads or os.cpu_count()
     val = np.zeros(W * H, np.float64)
     nesc = np.zeros(W * H, np.int32)
@@ -138,9 +142,9 @@ def compute_fe(cre, cim, scale, W, H, mxit, c_method=0, escape_radius=1e8, prec_
     lib().mxo_compute_fe(C.byref(f))
     info = {k: getattr(f, k) for k in ("ref_it", "ref_escaped", "ref_virtual", "ref_x", "ref_y", "interior_used",
                                         "cutoff_sensitive_hint", "seconds_pixels", "steps", "bla_applies",
-                                        "bla_skipped", "rebases")}
+                                        "bla_skipped", "rebases", "period", "subpixel")}
     if want_orbit:
-        n = 2 * (f.ref_it + 1)
+        n = 2 * (f.period if f.period else f.ref_it + 1)
         info["orbit"] = dict(zf=ozf[:n].copy(), zfe_m=om[:n].copy(), zfe_e=oe[:n].copy())
     if sub > 1:
         info["cells"] = (cf.reshape(H * sub, W * sub), cv.reshape(H * sub, W * sub), cn.reshape(H * sub, W * sub))
@@ -302,8 +306,9 @@ def bla_tables(zf, reflen, dcmax, eps=2.0 ** -53, ede=False, dfr=None, dfi=None)
 
 
 def pixels_fe(orb, bla, dcr_m, dcr_e, dci_m, dci_e, mxit, esc2, dx_m, dx_e, ede=False, use_interior=True,
-              bla_minlevel=3, threads=None):
-    """Per-pixel rescaled loop over caller-supplied orbit/BLA arrays."""
+              bla_minlevel=3, threads=None, period=0):
+    """Per-pixel rescaled loop over caller-supplied orbit/BLA arrays (period > 0: the
+    orbit is one period of a nucleus, read modulo the period)."""
     keep = []
 
     def arr(a, dt):
@@ -315,7 +320,8 @@ def pixels_fe(orb, bla, dcr_m, dcr_e, dci_m, dci_e, mxit, esc2, dx_m, dx_e, ede=
     pi.zf = arr(orb["zf"], np.float64)
     pi.zre_m, pi.zre_e = arr(orb["zre_m"], np.float64), arr(orb["zre_e"], np.int64)
     pi.zim_m, pi.zim_e = arr(orb["zim_m"], np.float64), arr(orb["zim_e"], np.int64)
-    pi.mx_ref_it = len(orb["zre_m"]) - 1
+    pi.mx_ref_it = period if period else len(orb["zre_m"]) - 1
+    pi.period = period
     if bla is None:
         pi.nlev = 0
     else:
@@ -410,6 +416,28 @@ def point(cre, cim, wp_bits, mxit, esc2=float(np.float32(1e8) * np.float32(1e8))
     v = lib().mxo_point(cre.encode(), cim.encode(), wp_bits, mxit, int(ede), int(track_d), esc2, dx.encode(),
                         C.cast(C.byref(n), C.c_void_p))
     return v, n.value
+
+
+def nucleus(cre, cim, scale, W, H, maxp, prec_bits=None):
+    """The oracle's nucleus search for a view: (period, nucleus as a decimal string pair,
+    island size as a float log10), period 0 when there is none."""
+    nd = 4000
+    rb, ib = C.create_string_buffer(nd), C.create_string_buffer(nd)
+    re_, ie = C.c_long(), C.c_long()
+    sm, se = C.c_double(), C.c_int64()
+    p = lib().mxo_nucleus(cre.encode(), cim.encode(), scale.encode(), W, H, maxp, prec_bits or precision_bits(scale),
+                          rb, C.byref(re_), ib, C.byref(ie), nd, C.byref(sm), C.byref(se))
+    if p <= 0:
+        return 0, None, None
+
+    def dec(buf, e):
+        d = buf.value.decode()
+        neg = d.startswith("-")
+        d = d.lstrip("-")
+        return ("-" if neg else "") + "0." + d + f"e{e.value}"
+
+    import math
+    return p, (dec(rb, re_), dec(ib, ie)), math.log10(abs(sm.value)) + se.value * math.log10(2.0)
 
 
 def grid_info(cre, cim, scale, W, H, prec_bits=None, full_h=0, row_base=0):

@@ -41,6 +41,8 @@ FRAMES = {
     # verify.cpp:11827: sub-pixel nucleus at 1e737, all exterior
     "subpixel737": (LOC["deep1_x"], LOC["deep1_y"], pow10(737), 48, 36, 60000, 0),
     "subpixel737_ede": (LOC["deep1_x"], LOC["deep1_y"], pow10(737), 48, 36, 60000, 2),
+    # the resolved nucleus frame with the exterior distance estimate
+    "minibrot875_ede": (LOC["deep1_x"], LOC["deep1_y"], pow10(875), 24, 18, 300000, 2),
     # odd sizes and a strip of a taller frame (strip API, mandel_perturbation.cpp:1399-1424)
     "minibrot875_odd": (LOC["deep1_x"], LOC["deep1_y"], pow10(875), 37, 23, 300000, 0),
 }

@@ -23,13 +23,15 @@ def _check(val_g, nesc_g, val_o, nesc_o, label):
     assert (val_g[~ext] == -2.0).all()
 
 
-@pytest.mark.parametrize("name", ["extref875", "minibrot875", "minibrot875_odd", "exterior1000", "exterior1000_ede", "subpixel737_ede"])
+@pytest.mark.parametrize("name", ["extref875", "minibrot875", "minibrot875_odd", "minibrot875_ede", "exterior1000",
+                                  "exterior1000_ede", "subpixel737_ede"])
 def test_frame_matches_oracle(ctx, name):
     cre, cim, scale, W, H, mxit, cm = cases.FRAMES[name]
     print(f"[parity] {name} {W}x{H} mxit={mxit} c_method={cm}", flush=True)
     it, val, nesc, info = ctx.compute(cre, cim, scale, W, H, mxit, c_method=cm)
     val_o, nesc_o, out_o, info_o = mxo.compute_fe(cre, cim, scale, W, H, mxit, c_method=cm)
     assert info["ref_it"] == info_o["ref_it"] and info["ref_virtual"] == info_o["ref_virtual"]
+    assert (info["period"], info["subpixel"]) == (info_o["period"], info_o["subpixel"])
     assert info["interior_used"] == info_o["interior_used"]
     _check(val, nesc, val_o, nesc_o, name)
     # the engine's float buffer
@@ -68,10 +70,28 @@ def test_tile_on_product_arrays(ctx):
     finally:
         ref.release()
     orb = dict(zf=fv["zf"], zre_m=fv["zfe_m"][0::2], zre_e=fv["zfe_e"][0::2], zim_m=fv["zfe_m"][1::2], zim_e=fv["zfe_e"][1::2])
-    b = mxo.bla_tables(fv["zf"], fv["len"] - 1, fv["dcmax"])
+    assert fv["period"] == 14164
+    b = mxo.bla_tables(fv["zf"], fv["ref_it"], fv["dcmax"])
     jj, ii = np.meshgrid(np.arange(W), np.arange(H))
     v2, n2, st2 = mxo.pixels_fe(orb, dict(nlev=b["nlev"], off=b["off"], size=b["size"], r2=b["r2"], l=b["l"], coef=b["coef"], rmax2=b["rmax2"]),
                                 fv["dcol_m"][jj.ravel()], fv["dcol_e"][jj.ravel()], fv["drow_m"][ii.ravel()], fv["drow_e"][ii.ravel()],
-                                mxit, fv["esc2"], fv["dx_m"], fv["dx_e"], use_interior=True, bla_minlevel=fv["bla_minlevel"])
+                                mxit, fv["esc2"], fv["dx_m"], fv["dx_e"], use_interior=True, bla_minlevel=fv["bla_minlevel"],
+                                period=fv["period"])
     _check(val, nesc, v2, n2, "minibrot875_odd tile")
     assert int(st[0]) == st2["steps"] and int(st[2]) == st2["bla_skipped"]
+
+
+def test_periodic_reference_and_full_reference(ctx):
+    """The resolved nucleus frame with its periodic reference (the engine's default)
+    and with the nucleus search switched off (a full reference from the centre pixel):
+    each equals the oracle run the same way; both give the reference's golden classes."""
+    cre, cim, scale, W, H, mxit, cm = cases.FRAMES["minibrot875"]
+    g = cases.golden("minibrot875.f32", H, W)
+    for periodic in (True, False):
+        print(f"[parity] minibrot875 periodic {periodic}", flush=True)
+        opts = 0 if periodic else mxhip.MX_OPT_NO_PERIODIC
+        it, val, nesc, info = ctx.compute(cre, cim, scale, W, H, mxit, options=opts)
+        val_o, nesc_o, out_o, info_o = mxo.compute_fe(cre, cim, scale, W, H, mxit, periodic=periodic)
+        assert info["period"] == info_o["period"] == (14164 if periodic else 0)
+        _check(val, nesc, val_o, nesc_o, f"minibrot875 periodic={periodic}")
+        assert np.array_equal(it < -1.5, g < -1.5)

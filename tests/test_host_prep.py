@@ -50,27 +50,34 @@ def test_prepare_matches_oracle(name):
     for k in ("zf", "zfe_m", "zfe_e"):
         assert np.array_equal(info["orbit"][k], fv[k]), k
     assert fv["ref_it"] == info["ref_it"] and fv["ref_virtual"] == info["ref_virtual"]
+    assert (fv["period"], fv["subpixel"]) == (info["period"], info["subpixel"])
     assert (fv["ref_x"], fv["ref_y"]) == (info["ref_x"], info["ref_y"])
     # the oracle's orbit at the same point: rebuild it from the grid the oracle uses
     (dxm, dxe), (dym, dye) = mxo.grid_info(cre, cim, scale, W, H)
     assert (fv["dx_m"], fv["dx_e"], fv["dy_m"], fv["dy_e"]) == (dxm, dxe, dym, dye)
-    # BLA tables built by the product equal the oracle's build on the same orbit, bit for bit
-    b = mxo.bla_tables(fv["zf"], fv["len"] - 1, fv["dcmax"])
-    assert b["nlev"] == fv["nlev"]
-    assert np.array_equal(b["off"], fv["lev_off"]) and np.array_equal(b["size"], fv["lev_size"])
-    assert np.array_equal(b["r2"].view(np.int64), fv["bla_r2"].view(np.int64))
-    assert np.array_equal(b["coef"].view(np.int64), fv["bla_coef"].view(np.int64))
-    assert b["rmax2"] == fv["rmax2"]
+    # BLA tables built by the product equal the oracle's build on the same orbit, bit for bit;
+    # a sub-pixel nucleus in view turns BLA off (mandel_perturbation.cpp:1624-1629)
+    b = mxo.bla_tables(fv["zf"], fv["ref_it"], fv["dcmax"])
+    if fv["subpixel"]:
+        assert fv["nlev"] == 0 and fv["rmax2"] == 0.0
+    else:
+        assert b["nlev"] == fv["nlev"]
+        assert np.array_equal(b["off"], fv["lev_off"]) and np.array_equal(b["size"], fv["lev_size"])
+        assert np.array_equal(b["r2"].view(np.int64), fv["bla_r2"].view(np.int64))
+        assert np.array_equal(b["coef"].view(np.int64), fv["bla_coef"].view(np.int64))
+        assert b["rmax2"] == fv["rmax2"]
     # dc tables: fe_mul_d(dxfe, j - ref_x) (mandel_perturbation.cpp:2160-2163)
     am, ae = mxo.fe_op(1, np.full(W, dxm), np.full(W, dxe, np.int64), np.arange(W) - fv["ref_x"], None)
     assert np.array_equal(am, fv["dcol_m"]) and np.array_equal(ae, fv["dcol_e"])
     # the oracle's per-pixel loop on the product's orbit reproduces the oracle's own frame
     orb = dict(zf=fv["zf"], zre_m=fv["zfe_m"][0::2], zre_e=fv["zfe_e"][0::2], zim_m=fv["zfe_m"][1::2], zim_e=fv["zfe_e"][1::2])
-    bla = dict(nlev=b["nlev"], off=b["off"], size=b["size"], r2=b["r2"], l=b["l"], coef=b["coef"], rmax2=b["rmax2"])
+    bla = None if fv["subpixel"] else dict(nlev=b["nlev"], off=b["off"], size=b["size"], r2=b["r2"], l=b["l"],
+                                           coef=b["coef"], rmax2=b["rmax2"])
     jj, ii = np.meshgrid(np.arange(W), np.arange(H))
     v2, n2, _ = mxo.pixels_fe(orb, bla, fv["dcol_m"][jj.ravel()], fv["dcol_e"][jj.ravel()], fv["drow_m"][ii.ravel()],
                               fv["drow_e"][ii.ravel()], mxit, fv["esc2"], fv["dx_m"], fv["dx_e"], ede=bool(cm & 2),
-                              use_interior=bool(info["interior_used"]), bla_minlevel=fv["bla_minlevel"])
+                              use_interior=bool(info["interior_used"]), bla_minlevel=fv["bla_minlevel"],
+                              period=fv["period"])
     mask = np.ones((H, W), bool)
     if not fv["ref_virtual"]:
         mask[H // 2, W // 2] = False
