@@ -264,8 +264,90 @@ int ball_period(mpf_srcptr cr, mpf_srcptr ci, mpf_srcptr r, int maxp, mp_bitcnt_
     return 0;
 }
 
+// z_p and z_p' = dz_p/dc from (0, 0) for one Newton iteration, with the serial loop's
+// operations on the same values: the orbit on this thread, the two components of the
+// derivative on two more, reading z_i through a ring.  Step i of either component
+// reads both components of step i-1 (buffers by parity) and starts once the other
+// has finished step i-1, so neither overwrites a value still being read; a ring slot
+// is refilled once both have read it.
+void nucleus_orbit_deriv(mpf_srcptr cr, mpf_srcptr ci, int period, mp_bitcnt_t prec, mpf_ptr zr, mpf_ptr zi, mpf_ptr dr_out,
+                         mpf_ptr di_out) {
+    const int RB = 1024;
+    std::vector<mpf_t> qr(RB), qi(RB);
+    for (int k = 0; k < RB; ++k) {
+        mpf_init2(qr[k], prec);
+        mpf_init2(qi[k], prec);
+    }
+    std::atomic<long> count{0}, used[2], done[2];
+    for (int k = 0; k < 2; ++k) {
+        used[k].store(0);
+        done[k].store(0);
+    }
+    Gmp dr0(prec), dr1(prec), di0(prec), di1(prec);
+    mpf_ptr DR[2] = {dr0.v, dr1.v}, DI[2] = {di0.v, di1.v};
+    mpf_set_ui(DR[0], 0);
+    mpf_set_ui(DI[0], 0);
+    auto wait_for = [](const std::atomic<long>& a, long v) {
+        for (int spin = 0; a.load(std::memory_order_acquire) < v; ++spin) {
+            if (spin < 4096)
+                __builtin_ia32_pause();
+            else
+                std::this_thread::yield();
+        }
+    };
+    auto dloop = [&](int which) {
+        Gmp t1(prec), t2(prec), t3(prec);
+        for (long i = 0; i < period; ++i) {
+            wait_for(count, i + 1);
+            wait_for(done[1 - which], i);
+            mpf_srcptr z_r = qr[i % RB], z_i = qi[i % RB];
+            mpf_srcptr pr = DR[i & 1], pi = DI[i & 1];
+            if (which == 0) {
+                // dr' = 2 (zr dr - zi di) + 1
+                mpf_mul(t1, z_r, pr);
+                mpf_mul(t2, z_i, pi);
+                used[0].store(i + 1, std::memory_order_release);
+                mpf_sub(t3, t1, t2);
+                mpf_mul_ui(t3, t3, 2);
+                mpf_add_ui(DR[(i + 1) & 1], t3, 1);
+            } else {
+                // di' = 2 (zr di + zi dr)
+                mpf_mul(t1, z_r, pi);
+                mpf_mul(t2, z_i, pr);
+                used[1].store(i + 1, std::memory_order_release);
+                mpf_add(t3, t1, t2);
+                mpf_mul_ui(DI[(i + 1) & 1], t3, 2);
+            }
+            done[which].store(i + 1, std::memory_order_release);
+        }
+    };
+    std::thread th0(dloop, 0), th1(dloop, 1);
+    {
+        Gmp t1(prec), t2(prec), t3(prec);
+        mpf_set_ui(zr, 0);
+        mpf_set_ui(zi, 0);
+        for (long i = 0; i < period; ++i) {
+            while (std::min(used[0].load(std::memory_order_acquire), used[1].load(std::memory_order_acquire)) <= i - RB)
+                std::this_thread::yield();
+            mpf_set(qr[i % RB], zr);
+            mpf_set(qi[i % RB], zi);
+            count.store(i + 1, std::memory_order_release);
+            nuc_sqadd(zr, zi, cr, ci, t1, t2, t3);
+        }
+    }
+    th0.join();
+    th1.join();
+    mpf_set(dr_out, DR[period & 1]);
+    mpf_set(di_out, DI[period & 1]);
+    for (int k = 0; k < RB; ++k) {
+        mpf_clear(qr[k]);
+        mpf_clear(qi[k]);
+    }
+}
+
 // Newton's method for the period-p nucleus, c <- c - z_p / (dz_p/dc) from z_0 = 0
-// (g_newton); true once a step is below the tolerance 2^-(prec-48) in modulus
+// (g_newton); true once a step is below the tolerance 2^-(prec-48) in modulus.  Long
+// periods run the orbit and the derivative on three threads (nucleus_orbit_deriv).
 bool newton_nucleus(mpf_ptr cr, mpf_ptr ci, int period, mp_bitcnt_t prec, int maxit) {
     Gmp zr(prec), zi(prec), dr(prec), di(prec), den(prec), nr(prec), ni(prec), tol(prec), step2(prec), t1(prec), t2(prec), t3(prec);
     mpf_set_ui(tol, 1);
@@ -276,19 +358,22 @@ bool newton_nucleus(mpf_ptr cr, mpf_ptr ci, int period, mp_bitcnt_t prec, int ma
         mpf_set_ui(zi, 0);
         mpf_set_ui(dr, 0);
         mpf_set_ui(di, 0);
-        for (int i = 0; i < period; ++i) {
-            mpf_mul(t1, zr, dr);
-            mpf_mul(t2, zi, di);
-            mpf_sub(t3, t1, t2);
-            mpf_mul_ui(t3, t3, 2);
-            mpf_add_ui(t3, t3, 1);
-            mpf_mul(t1, zr, di);
-            mpf_mul(t2, zi, dr);
-            mpf_add(di, t1, t2);
-            mpf_mul_ui(di, di, 2);
-            mpf_set(dr, t3);
-            nuc_sqadd(zr, zi, cr, ci, t1, t2, t3);
-        }
+        if (period >= 256)
+            nucleus_orbit_deriv(cr, ci, period, prec, zr, zi, dr, di);
+        else
+            for (int i = 0; i < period; ++i) {
+                mpf_mul(t1, zr, dr);
+                mpf_mul(t2, zi, di);
+                mpf_sub(t3, t1, t2);
+                mpf_mul_ui(t3, t3, 2);
+                mpf_add_ui(t3, t3, 1);
+                mpf_mul(t1, zr, di);
+                mpf_mul(t2, zi, dr);
+                mpf_add(di, t1, t2);
+                mpf_mul_ui(di, di, 2);
+                mpf_set(dr, t3);
+                nuc_sqadd(zr, zi, cr, ci, t1, t2, t3);
+            }
         mpf_mul(t1, dr, dr);
         mpf_mul(t2, di, di);
         mpf_add(den, t1, t2);
