@@ -134,6 +134,8 @@ def main():
     ap.add_argument("--no-e2e", action="store_true", help="skip the end-to-end mx_compute frame")
     ap.add_argument("--c4", action="store_true", help="add the batched uneven-tiles leg (96 tiles, 12 references)")
     ap.add_argument("--no-tune", action="store_true", help="skip the autotuner (chunk / grid size / BLA cutoff)")
+    ap.add_argument("--knobs", default="", help="JSON file of launch knobs per form: read instead of tuning when "
+                    "it exists, written after tuning otherwise (profiled reruns use the timed run's knobs)")
     ap.add_argument("--c5", action="store_true", help="add the zoom-sequence leg (hipGraph of 1280x720 frames)")
     ap.add_argument("--c5-frames", type=int, default=240, help="first N frames of the zoom sequence")
     args = ap.parse_args()
@@ -166,12 +168,17 @@ def main():
             interior = bool(((pn < 0) & (pn != np.iinfo(np.int32).min)).any())
             log(f"probe {probe_ms:.1f} ms, interior detection {'on' if interior else 'off'}")
             tuned = {}
-            if not args.no_tune:
+            if args.knobs and os.path.exists(args.knobs):
+                tuned = json.load(open(args.knobs))
+                log(f"knobs from {args.knobs}: {tuned}")
+            elif not args.no_tune:
                 for f in [args.form] + compare:
                     r = autotune.tune(ctx, ref, fv, FORMS[f], ede=ede, interior=interior, log=log)
                     tuned[f] = {k: r[k] for k in ("chunk", "blocks_per_cu", "bla_minlevel", "window", "ms")}
                     tuned[f]["trials"] = len(r["trials"])
                     log(f"autotune {f}: {tuned[f]}")
+                if args.knobs:
+                    json.dump(tuned, open(args.knobs, "w"), indent=1)
             res[args.form] = run_form(ctx, ref, fv, args.form, ede, interior, center, args.steps, args.warmup,
                                       tuned.get(args.form))
             for f in compare:

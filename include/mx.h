@@ -145,7 +145,7 @@ typedef struct {
     const int32_t* list;
     int32_t list_len;
     /* launch tuning (0: defaults); results do not depend on either.  chunk: lean
-     * steps per trip between the full step's checks (default 24); grid_blocks:
+     * steps per trip between the full step's checks (default 32); grid_blocks:
      * workgroups of 256 lanes in the persistent grid (default 8 per CU, at most one
      * per 64 samples). */
     int32_t chunk;

@@ -8,7 +8,7 @@ Three knobs, timed on the device on a strided sample of the frame to be rendered
                the engine's cutoff (minimum skip 8, mandel_perturbation.cpp:1520-1530).
 chunk and grid_blocks do not change any result.
 
-The search is coordinate descent from the defaults (chunk 24, 8 blocks per CU, the
+The search is coordinate descent from the defaults (chunk 32, 8 blocks per CU, the
 frame's own cutoff): each knob in turn is swept with the others fixed and the fastest
 value kept.  The sample is the central window of the frame with about 2 M samples
 (the frame's aspect ratio): enough work to fill the grid, a fraction of the frame's
@@ -73,7 +73,7 @@ def tune(ctx, ref, fv, form, ede=False, interior=True, mxit=None, target_samples
     uploaded as `ref`) in `form`.  Returns dict(chunk, blocks_per_cu, bla_minlevel,
     window, ms, trials)."""
     sub = window(fv, target_samples)
-    knobs = dict(chunk=24, blocks_per_cu=8, bla_minlevel=fv["bla_minlevel"])
+    knobs = dict(chunk=32, blocks_per_cu=8, bla_minlevel=fv["bla_minlevel"])
     trials = []
 
     def run(k):

@@ -104,6 +104,10 @@ class HipComputeBackend final : public IComputeBackend {
         if (const char* e = std::getenv("MANDEL_BLA")) r.use_bla = std::atoi(e) != 0;
         if (const char* e = std::getenv("MANDEL_INTERIOR")) r.interior = std::atoi(e) != 0;
         if (const char* e = std::getenv("MANDEL_BLA_MINSKIP")) r.bla_minskip = std::atoi(e);
+        // MANDEL_PERIODIC=0: no nucleus search (buildReferenceOrbit, mandel_perturbation.cpp:
+        // 1240-1250); the search runs by default above 1e280 with BLA on
+        if (const char* e = std::getenv("MANDEL_PERIODIC"))
+            if (std::atoi(e) == 0) r.options |= MX_OPT_NO_PERIODIC;
         r.iterations = request.iterations;
         if (request.progress) request.progress->store(0.0f, std::memory_order_relaxed);
         mx_frame_info info;

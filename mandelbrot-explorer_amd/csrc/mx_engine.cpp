@@ -369,7 +369,7 @@ extern "C" int mx_plan_create(mx_ctx* ctx, const mx_ref* ref, const mx_tile* t, 
     T.interior_confirm = t->interior_confirm;
     T.dxfe = fe_make(t->dx_m, t->dx_e);
     T.skip_sample = t->skip_sample;
-    T.chunk = t->chunk > 0 ? std::min(t->chunk, 4096) : 24;
+    T.chunk = t->chunk > 0 ? std::min(t->chunk, 4096) : 32;
     T.val = (double*)(b + o_val);
     T.nesc = (int*)(b + o_nesc);
     T.out = (float*)(b + o_out);

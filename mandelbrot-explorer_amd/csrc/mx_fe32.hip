@@ -680,25 +680,30 @@ MX_HD void fast_steps(const MxRefDev& R, const MxTileDev& T, P32<F>& p, C32& c, 
     const OrbitLoader<F> ld(R);
     F w2 = wmag2(p.wr, p.wi);
     // one counter for the steps taken: orbit index, reference index and iteration
-    // advance together, and every test on them becomes a bound on the counter
+    // advance together, and the tests on them alone bound the trip count instead: the
+    // loop never takes the step at which one would stop it
     const int m0 = p.m, rm0 = p.rm, it0 = p.iter;
-    const int n_iter = T.mxit - it0;                 // iter >= mxit
+    int lim = min(budget, T.mxit - it0);              // iter >= mxit
     // m + 1 >= reflen after the step; periodic: rm + 1 == period (the wrap, taken by
     // the full step)
-    const int n_end = R.period ? R.period - 1 - rm0 : reflen - 1 - m0;
-    const int n_rep = p.sm - m0;                      // m == sm
-    const int n_twin = p.save_iter2 + p.twin2r - it0; // iter - save_iter2 >= twin2r
+    lim = min(lim, R.period ? R.period - 1 - rm0 : reflen - 1 - m0);
+    if (int_rep) {
+        lim = min(lim, p.save_iter2 + p.twin2r - it0); // iter - save_iter2 >= twin2r
+        if (p.sm - m0 >= 0) lim = min(lim, p.sm - m0);   // m == sm
+    }
+    if (INTERIOR && p.conf_P > 0) lim = 0;            // an active periodicity confirmation
+    // a BLA-aligned index (rm - 1 a multiple of 2^minlevel, m >= 2) whose |dz|^2 is not
+    // above the largest radius by exponent: 2^(2k) |w|^2 with e(|w|^2) <= er - 2k, i.e.
+    // |w|^2 <= 2^(er - 2k) (equality included: a superset)
+    const int emax = sizeof(F) == 4 ? 128 : 1024, emin = sizeof(F) == 4 ? -150 : -1075; // inf, 0
+    const F wthr = can_bla ? ldexp_t((F)1, max(min(er - 2 * p.k, emax), emin)) : (F)0;
+    const int nal = 1 - rm0, nm2 = 2 - m0;
+    const bool chk_ok = INTERIOR && p.conf_giveup < 3;
     int n = 0;
-    for (int u = 0; u < budget; ++u) {
+    for (int u = 0; u < lim; ++u) {
         const int rm = rm0 + n;
-        int stop = n >= n_iter;
-        if (INTERIOR) stop |= p.conf_P > 0;
-        if (int_rep) stop |= (int)(n == n_rep) | (int)(n >= n_twin);
-        if (can_bla) {
-            int ew;
-            (void)frexp_t(w2, &ew);
-            stop |= (int)(m0 + n >= 2) & (int)(((rm - 1) & amask) == 0) & ((int)(ew + 2 * p.k <= er) | (int)(w2 == (F)0));
-        }
+        int stop = 0;
+        if (can_bla) stop = (int)(((n - nal) & amask) == 0) & (int)(n >= nm2) & (int)(w2 <= wthr);
         if (!p.xv) {
             p.Xc = m0 + n ? load_X(R, rm, (F)0) : V{(F)0, (F)0};
             p.Xn = load_X(R, rm + 1, (F)0);
@@ -721,11 +726,11 @@ MX_HD void fast_steps(const MxRefDev& R, const MxTileDev& T, P32<F>& p, C32& c, 
         const V Xn = p.Xn;
         const F zr = fma_t(p.s, nwr, Xn.x), zi = fma_t(p.s, nwi, Xn.y);
         const F zrad = wmag2(zr, zi);
-        stop |= (int)(fbits(zrad) - zlo > zspan) | (int)(zrad < ldexp_t(w2n, 2 * p.k)) | (int)(n >= n_end);
+        stop |= (int)(fbits(zrad) - zlo > zspan) | (int)(zrad < ldexp_t(w2n, 2 * p.k));
         const int it1 = it0 + n + 1;
         int save = 0;
         if (INTERIOR) {
-            const int chk = (int)((it1 & 15) == 0) & (int)(p.conf_giveup < 3);
+            const int chk = (int)((it1 & 15) == 0) & (int)chk_ok;
             const F pr = zr - p.zsr, pi = zi - p.zsi;
             stop |= chk & (int)(wmag2(pr, pi) < eps2 * zrad);
             save = chk & (int)(it1 - p.save_iter >= p.period_win);

@@ -536,24 +536,30 @@ MX_HD void fast_steps(const MxRefDev& R, const MxTileDev& T, Px& p, Counters& c,
 #endif
     double w2 = p.wr * p.wr + p.wi * p.wi;
     // one counter for the steps taken: orbit index, reference index and iteration
-    // advance together, and every test on them becomes a bound on the counter
+    // advance together, and the tests on them alone bound the trip count instead: the
+    // loop never takes the step at which one would stop it
     const int m0 = p.m, rm0 = p.rm, it0 = p.iter;
-    const int n_iter = T.mxit - it0;                 // iter >= mxit
+    int lim = min(budget, T.mxit - it0); // iter >= mxit
     // m + 1 >= reflen after the step; periodic: rm + 1 == period (the wrap, taken by
     // the full step)
-    const int n_end = R.period ? R.period - 1 - rm0 : reflen - 1 - m0;
-    const int n_rep = p.sm - m0;                      // m == sm
-    const int n_twin = p.save_iter2 + p.twin2r - it0; // iter - save_iter2 >= twin2r
+    lim = min(lim, R.period ? R.period - 1 - rm0 : reflen - 1 - m0);
+    if (int_rep) {
+        lim = min(lim, p.save_iter2 + p.twin2r - it0);  // iter - save_iter2 >= twin2r
+        if (p.sm - m0 >= 0) lim = min(lim, p.sm - m0); // m == sm
+    }
+    if (INTERIOR && p.conf_P > 0) lim = 0; // an active periodicity confirmation
+    // a BLA-aligned index (rm - 1 a multiple of 2^minlevel, m >= 2) whose |dz|^2 is not
+    // above the largest radius by exponent: dz^2 = S^2 |w|^2 has exponent S2.e + e(|w|^2)
+    // or one less, tested as e(|w|^2) <= er + 1 - S2.e, i.e. |w|^2 <= 2^(er + 1 - S2.e)
+    // (equality included: a superset)
+    const double wthr = can_bla ? ldexp(1.0, max(min(er + 1 - p.S2.e, 1024), -1075)) : 0.0;
+    const int nal = 1 - rm0, nm2 = 2 - m0;
+    const bool chk_ok = INTERIOR && p.conf_giveup < 3;
     int n = 0;
-    for (int u = 0; u < budget; ++u) {
+    for (int u = 0; u < lim; ++u) {
         const int rm = rm0 + n;
-        int stop = n >= n_iter;
-        if (INTERIOR) stop |= p.conf_P > 0;
-        if (int_rep) stop |= (int)(n == n_rep) | (int)(n >= n_twin);
-        if (can_bla) {
-            // dz^2 = S^2 |w|^2 has exponent S2.e + e(|w|^2) or one less
-            stop |= (int)(m0 + n >= 2) & (int)(((rm - 1) & amask) == 0) & ((int)(p.S2.e + fe_from(w2).e <= er + 1) | (int)(w2 == 0.0));
-        }
+        int stop = 0;
+        if (can_bla) stop = (int)(((n - nal) & amask) == 0) & (int)(n >= nm2) & (int)(w2 <= wthr);
         if (!p.xv) {
             p.Xc = m0 + n ? R.zf[rm] : make_double2(0.0, 0.0);
             p.Xn = R.zf[rm + 1];
@@ -585,11 +591,11 @@ MX_HD void fast_steps(const MxRefDev& R, const MxTileDev& T, Px& p, Counters& c,
         const double zr = Xn.x + p.s * nwr, zi = Xn.y + p.s * nwi;
         const double zrad = zr * zr + zi * zi;
         const double dzmag2 = p.s * p.s * w2n;
-        stop |= (int)(dbits(zrad) - zlo > zspan) | (int)(zrad < dzmag2) | (int)(n >= n_end);
+        stop |= (int)(dbits(zrad) - zlo > zspan) | (int)(zrad < dzmag2);
         const int it1 = it0 + n + 1;
         int save = 0;
         if (INTERIOR) {
-            const int chk = (int)((it1 & 15) == 0) & (int)(p.conf_giveup < 3);
+            const int chk = (int)((it1 & 15) == 0) & (int)chk_ok;
             const double pr = zr - p.zsr, pi = zi - p.zsi;
             stop |= chk & (int)(pr * pr + pi * pi < T.interior_eps2 * zrad);
             save = chk & (int)(it1 - p.save_iter >= p.period_win);

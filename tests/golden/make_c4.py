@@ -16,6 +16,8 @@ E_HI is 300 here (the spec's range reaches 1e-1000; walks to that depth take hou
 per seed on the CPU).  The location walks are make_walk.py's, probe max_iter capped
 at 2^21 (at 2^16 they stall near 1e-41, every probe of the region exceeding it).
 
+A location whose walk stalls (64 redraws without an escaping probe) takes the walk of
+seed k + 100 j instead, j = 1, 2, 3 (recorded as walk_seed).
 Writes tests/golden/c4_seed<k>.json (one location centre each) and, once all twelve
 exist, tests/golden/c4_requests.json.
 Usage: python tests/golden/make_c4.py [--plan] [--seeds 1,2,...] [--gpu] [--out DIR]
@@ -74,22 +76,34 @@ def main():
     for l in locs:
         path = os.path.join(out_dir, f"c4_seed{l['seed']}.json")
         if not os.path.exists(path) and (seeds is None or l["seed"] in seeds):
-            # checkpoints every 50 decades (resumable: a later run continues from the
-            # deepest one recorded)
-            ck_path = os.path.join(out_dir, f"c4_seed{l['seed']}_checkpoints.json")
-            cks = json.load(open(ck_path)) if os.path.exists(ck_path) else {}
-            done = [int(d) for d in cks]
-            resume = cks[str(max(done))] if done else None
-            targets = [d for d in range(50, l["depth"], 50) if not done or d > max(done)] + [l["depth"]]
+            # a walk that stalls (64 redraws without an escaping probe) is replaced by
+            # the walk of seed k + 100 j, j = 1, 2, 3, for the same location and depth
+            for j in range(4):
+                ws = l["seed"] + 100 * j
+                # checkpoints every 50 decades (resumable: a later run continues from the
+                # deepest one recorded)
+                ck_name = f"c4_seed{l['seed']}_checkpoints.json" if j == 0 else f"c4_seed{l['seed']}_walk{ws}_checkpoints.json"
+                ck_path = os.path.join(out_dir, ck_name)
+                cks = json.load(open(ck_path)) if os.path.exists(ck_path) else {}
+                done = [int(d) for d in cks]
+                resume = cks[str(max(done))] if done else None
+                targets = [d for d in range(50, l["depth"], 50) if not done or d > max(done)] + [l["depth"]]
 
-            def save_ck(res, ck_path=ck_path, cks=cks):
-                cks.update(res)
-                json.dump(cks, open(ck_path, "w"), indent=1)
+                def save_ck(res, ck_path=ck_path, cks=cks):
+                    cks.update(res)
+                    json.dump(cks, open(ck_path, "w"), indent=1)
 
-            res = walk(l["seed"], targets, log=lambda s: print(s, flush=True), on_depth=save_ck, resume=resume,
-                       mxit_cap=WALK_CAP, gpu=gpu)
+                try:
+                    res = walk(ws, targets, log=lambda s: print(s, flush=True), on_depth=save_ck, resume=resume,
+                               mxit_cap=WALK_CAP, gpu=gpu)
+                except RuntimeError as e:
+                    if "stalled" not in str(e) or j == 3:
+                        raise
+                    print(f"location {l['seed']}: walk of seed {ws} stalled, next seed", flush=True)
+                    continue
+                break
             c = res[str(l["depth"])]
-            json.dump(dict(seed=l["seed"], depth=l["depth"], re=c["re"], im=c["im"], width=c["width"],
+            json.dump(dict(seed=l["seed"], walk_seed=ws, depth=l["depth"], re=c["re"], im=c["im"], width=c["width"],
                            steps=c["steps"], probe_mxit=c["probe_mxit"], walk_mxit_cap=WALK_CAP), open(path, "w"), indent=1)
             print("location", l, "done", flush=True)
         if os.path.exists(path):
