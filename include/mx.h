@@ -283,6 +283,27 @@ typedef struct {
 
 int mx_compute(mx_ctx* ctx, const mx_request* req, mx_frame_info* info);
 
+/* ---- colouring and downsampling (tools/render.cpp:252-302) ------------------ */
+
+/* The engine's palette (render.cpp:21-30: colorMapInit through mono_cubic_interpolate,
+ * engine/interpolate.cpp:7-65): out[3*2048], channels r, g, b of 2048 floats in 0..255.
+ * Host only. */
+int mx_palette(float* out);
+
+/* render.cpp's colour pass on the device: the (height*ss) x (width*ss) float buffer
+ * `iterations` (the engine's layout: smooth value or distance estimate, negative for
+ * interior / empty) mapped through the palette (getColor, render.cpp:69-85: density
+ * 60 is the engine's default, MANDEL_DENS), averaged over each ss x ss block in linear
+ * light (exponent 2.2) and written as width*height RGB bytes.  coloring_method selects
+ * the distance-estimate mapping (MX_EXTERIOR_DIST_EST). */
+int mx_colorize(mx_ctx* ctx, const float* iterations, int32_t width, int32_t height, int32_t ss, int32_t coloring_method,
+                float density, uint8_t* rgb);
+
+/* The same from a launched plan's device output (a full grid tile, stride 1, no sample
+ * list, sides multiples of ss): the tile's W/ss x H/ss pixels, without copying its
+ * float samples to the host.  No counterpart in the engine, which colours on the host. */
+int mx_plan_colorize(mx_plan* plan, int32_t ss, int32_t coloring_method, float density, uint8_t* rgb);
+
 /* ---- host-only frame preparation (no GPU needed) --------------------------- */
 
 /* Everything Mandel::Compute prepares on the host before the delta loop: grid,

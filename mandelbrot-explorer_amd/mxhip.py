@@ -90,7 +90,7 @@ EXPORTS = ["mx_version", "mx_device_count", "mx_ctx_create", "mx_ctx_destroy", "
            "mx_plan_wait", "mx_plan_read", "mx_plan_unresolved", "mx_plan_destroy", "mx_tile_render", "mx_compute",
            "mx_frame_prepare", "mx_frame_free", "mx_batch_create", "mx_batch_launch", "mx_batch_wait",
            "mx_batch_destroy", "mx_ctx_cu_count", "mx_graph_create", "mx_graph_launch", "mx_graph_wait",
-           "mx_graph_destroy"]
+           "mx_graph_destroy", "mx_palette", "mx_colorize", "mx_plan_colorize"]
 
 _LIB = None
 
@@ -154,6 +154,15 @@ def _arr(ptr, n, dt):
     if n == 0 or not ptr:
         return np.zeros(0, dt)
     return np.ctypeslib.as_array(ptr, shape=(n,)).astype(dt, copy=True)
+
+
+def palette():
+    """The engine's palette, (3, 2048) float32 (host only)."""
+    pal = np.zeros(3 * 2048, np.float32)
+    rc = lib().mx_palette(_p(pal))
+    if rc != MX_OK:
+        raise MxError(rc, "mx_palette")
+    return pal.reshape(3, 2048)
 
 
 def frame_prepare(cre, cim, scale, W, H, mxit, **kw):
@@ -222,6 +231,17 @@ class Context:
 
     def reset_cancellation(self):
         lib().mx_reset_cancellation(self.h)
+
+    def colorize(self, it, ss=1, c_method=0, density=60.0):
+        """RGB bytes (H, W, 3) of the (H ss) x (W ss) float buffer `it` (the engine's
+        layout), coloured and downsampled on the device."""
+        it = np.ascontiguousarray(it, np.float32)
+        Hs, Ws = it.shape
+        W, H = Ws // ss, Hs // ss
+        assert W * ss == Ws and H * ss == Hs
+        rgb = np.zeros((H, W, 3), np.uint8)
+        self.check(lib().mx_colorize(self.h, _p(it), W, H, ss, c_method, C.c_float(density), _p(rgb)), "mx_colorize")
+        return rgb
 
     def compute(self, cre, cim, scale, W, H, mxit, sub=1, **kw):
         """One frame; with sub > 1 the buffers are (H sub) x (W sub) cells (-10, 0 and
@@ -405,6 +425,13 @@ class Plan:
         if cnt.value:
             self.ctx.check(lib().mx_plan_unresolved(self.h, _p(idx), _p(gp), cnt.value, C.byref(cnt)), "mx_plan_unresolved")
         return idx, gp
+
+    def colorize(self, ss=1, c_method=0, density=60.0):
+        """RGB bytes (H/ss, W/ss, 3) of the last launch's float output, coloured on the device."""
+        W, H = self.tile.width // ss, self.tile.height // ss
+        rgb = np.zeros((H, W, 3), np.uint8)
+        self.ctx.check(lib().mx_plan_colorize(self.h, ss, c_method, C.c_float(density), _p(rgb)), "mx_plan_colorize")
+        return rgb
 
     def destroy(self):
         if self.h:

@@ -2748,3 +2748,112 @@ MXO_EXPORT int mxo_dbl_setup(const char* cre, const char* cim, const char* scale
     grid_clear(&g);
     return ri;
 }
+
+/* ========================================================================= */
+/* colour pass (tools/render.cpp:21-35, 69-91, 252-302; engine/interpolate.cpp */
+/* 7-65), float arithmetic as there                                           */
+/* ========================================================================= */
+
+#define MXO_COLP 2048
+
+/* mono_cubic_interpolate (interpolate.cpp:7-65) */
+static void mxo_mono_cubic(const float* xs, const float* ys, int n, float* out, int m) {
+    float dxs[16], ms[16], c1[16], c2[16], c3[16];
+    for (int i = 0; i < n - 1; ++i) {
+        dxs[i] = xs[i + 1] - xs[i];
+        ms[i] = (ys[i + 1] - ys[i]) / dxs[i];
+    }
+    c1[0] = ms[0];
+    for (int i = 0; i < n - 2; ++i) {
+        const float a = ms[i], b = ms[i + 1];
+        if (a * b <= 0) {
+            c1[i + 1] = 0;
+        } else {
+            const float common = dxs[i] + dxs[i + 1];
+            c1[i + 1] = 3 * common / ((common + dxs[i + 1]) / a + (common + dxs[i]) / b);
+        }
+    }
+    c1[n - 1] = ms[n - 2];
+    for (int i = 0; i < n - 1; ++i) {
+        const float inv = 1 / dxs[i], common = c1[i] + c1[i + 1] - ms[i] - ms[i];
+        c2[i] = (ms[i] - c1[i] - common) * inv;
+        c3[i] = common * inv * inv;
+    }
+    for (int k = 0; k < m; ++k) {
+        const float x = 1.f * k / m;
+        if (x == xs[n - 1]) {
+            out[k] = ys[n - 1];
+            continue;
+        }
+        int low = 0, high = n - 2, mid, found = -1;
+        while (low <= high) {
+            mid = (low + high) / 2;
+            if (xs[mid] < x)
+                low = mid + 1;
+            else if (xs[mid] > x)
+                high = mid - 1;
+            else {
+                found = mid;
+                break;
+            }
+        }
+        if (found >= 0) {
+            out[k] = ys[found];
+            continue;
+        }
+        const int i = high > 0 ? high : 0;
+        const float d = x - xs[i], d2 = d * d;
+        out[k] = ys[i] + c1[i] * d + c2[i] * d2 + c3[i] * d * d2;
+    }
+}
+
+/* colorMapInit (render.cpp:25-30) */
+MXO_EXPORT void mxo_palette(float* pal) {
+    static const float pos[6] = {0.0f, 0.16f, 0.42f, 0.6425f, 0.8575f, 1.0f};
+    static const float col[3][6] = {{0, 32, 237, 255, 0, 0}, {70, 107, 255, 170, 2, 70}, {100, 203, 255, 0, 0, 100}};
+    for (int c = 0; c < 3; ++c) mxo_mono_cubic(pos, col[c], 6, pal + c * MXO_COLP, MXO_COLP);
+}
+
+/* getColor's palette index (render.cpp:80-81); a product out of int range converts as
+ * x86's cvttss2si does (INT_MIN) */
+static int mxo_palette_index(float f) {
+    const float v = f * (float)MXO_COLP;
+    int x = (v < 2147483648.0f && v > -2147483904.0f) ? (int)v : INT_MIN;
+    x %= MXO_COLP;
+    if (x < 0) x += MXO_COLP;
+    return x;
+}
+
+/* the colour loop of render.cpp:252-302 without the DE overlay, normal map and relief */
+MXO_EXPORT void mxo_colorize(const float* iter, int W, int H, int ss, int ede, float density, unsigned char* rgb) {
+    float pal[3 * MXO_COLP];
+    mxo_palette(pal);
+    const size_t Ws = (size_t)W * ss;
+    for (int i = 0; i < H; ++i)
+        for (int j = 0; j < W; ++j) {
+            float lr = 0, lg = 0, lb = 0;
+            for (int a = 0; a < ss; ++a)
+                for (int b = 0; b < ss; ++b) {
+                    const float it = iter[((size_t)i * ss + a) * Ws + (size_t)j * ss + b];
+                    float r = 0, g = 0, bb = 0;
+                    if (!(it < 0)) {
+                        const float f = ede ? tanhf(it * density / 3600.0f * 5.0f)
+                                            : powf(logf(it + 2), logf(it + 2) * logf(it + 2) * density / 3600.0f);
+                        const int x = mxo_palette_index(f);
+                        r = pal[x];
+                        g = pal[MXO_COLP + x];
+                        bb = pal[2 * MXO_COLP + x];
+                    }
+                    const double clr = powf(r / 255.f, 2.2f), clg = powf(g / 255.f, 2.2f), clb = powf(bb / 255.f, 2.2f);
+                    lr += clr * 1.0f;
+                    lg += clg * 1.0f;
+                    lb += clb * 1.0f;
+                }
+            const int n = ss * ss;
+            unsigned char* p = rgb + ((size_t)i * W + j) * 3;
+            const float gr = lr / n, gg = lg / n, gb = lb / n;
+            p[0] = (unsigned char)(255.f * powf(gr < 0 ? 0 : gr, 1.f / 2.2f) + 0.5f);
+            p[1] = (unsigned char)(255.f * powf(gg < 0 ? 0 : gg, 1.f / 2.2f) + 0.5f);
+            p[2] = (unsigned char)(255.f * powf(gb < 0 ? 0 : gb, 1.f / 2.2f) + 0.5f);
+        }
+}

@@ -448,3 +448,20 @@ def grid_info(cre, cim, scale, W, H, prec_bits=None, full_h=0, row_base=0):
                         C.cast(C.byref(dxe), C.c_void_p), C.cast(C.byref(dym), C.c_void_p),
                         C.cast(C.byref(dye), C.c_void_p))
     return (dxm.value, dxe.value), (dym.value, dye.value)
+
+
+def palette():
+    """The oracle's restatement of the engine's palette, (3, 2048) float32."""
+    pal = np.zeros(3 * 2048, np.float32)
+    lib().mxo_palette(_p(pal))
+    return pal.reshape(3, 2048)
+
+
+def colorize(it, ss=1, ede=False, density=60.0):
+    """render.cpp's colour pass on the CPU: (H ss, W ss) floats -> (H, W, 3) bytes."""
+    it = np.ascontiguousarray(it, np.float32)
+    Hs, Ws = it.shape
+    W, H = Ws // ss, Hs // ss
+    rgb = np.zeros((H, W, 3), np.uint8)
+    lib().mxo_colorize(_p(it), W, H, ss, int(ede), C.c_float(density), _p(rgb))
+    return rgb
