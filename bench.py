@@ -37,8 +37,12 @@ import mxhip  # noqa: E402
 
 # MI355X published vector peaks.  Neither form issues packed FP32 (v_pk_*) or MFMA:
 #   FP64 vector 78.6 TFLOP/s counts an FMA as 2; the plain form has no FMA -> 39.3 T ops/s
-#   FP32 vector 157.3 TFLOP/s is the packed rate; unpacked with FMA -> 78.6 TFLOP/s
-PEAK = {"fe64": 78.6e12 / 2, "fe32": 157.3e12 / 2, "sc64": 78.6e12}
+#   FP32 vector 157.3 TFLOP/s (FMA as 2).  The mixed form's FMAs are unpacked, yet its
+#   main pass issues 1.373e12 VALU wave instructions in 1.456 s (rocprofv3 SQ_INSTS_VALU
+#   on the 5760x3240 frame): 6.0e13 lane instructions/s, above the 3.9e13 a 64-lane-per-
+#   clock CU would allow, so the unpacked rate is not half the published one and the
+#   published figure is the roof.
+PEAK = {"fe64": 78.6e12 / 2, "fe32": 157.3e12, "sc64": 78.6e12}
 # Algorithmic FP add/mul per executed step, counted from the plain kernel (mx_fe64.hip
 # trip()): fused step 14, reconstruction + |z|^2 7, |w|^2 3, |S w|^2 5 -> 29; the
 # carried derivative adds zm (4), J' (9) and |J|^2 (3) -> 16.  A BLA apply: the linear

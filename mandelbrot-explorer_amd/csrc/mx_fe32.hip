@@ -744,8 +744,10 @@ MX_HD void fast_steps(const MxRefDev& R, const MxTileDev& T, P32<F>& p, C32& c, 
             p.jr = go ? ja : p.jr;
             p.ji = go ? jb : p.ji;
         }
-        p.Xc = go ? Xn : p.Xc;
-        p.Xn = go ? Xpf : p.Xn;
+        // the entries rotate on every step, so that the load above feeds the next step
+        // straight from its register; a lane that stops reloads both (xv = 0 below)
+        p.Xc = Xn;
+        p.Xn = Xpf;
         if (INTERIOR && go && save) {
             p.zsr = zr;
             p.zsi = zi;
@@ -754,7 +756,10 @@ MX_HD void fast_steps(const MxRefDev& R, const MxTileDev& T, P32<F>& p, C32& c, 
         }
         w2 = w2n;
         n += go;
-        if (stop) break;
+        if (stop) {
+            p.xv = 0;
+            break;
+        }
     }
     p.m = m0 + n;
     p.rm = rm0 + n;
