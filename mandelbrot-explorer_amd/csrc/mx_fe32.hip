@@ -353,11 +353,14 @@ MX_HD bool step_tiny(const P32<F>& p, F nwr, F nwi) {
     return fabs_t(nwr) < TINY && fabs_t(nwi) < TINY && (nwr != (F)0 || nwi != (F)0 || p.wr != (F)0 || p.wi != (F)0 || p.dcr != (F)0 || p.dci != (F)0);
 }
 
-// J' = 2 z J + 1 in J's scale, z = X + s w (the pre-step delta)
+// J' = 2 z J + 1 in J's scale, z = X + s w (the pre-step delta).  Both components
+// take the doubled z, so the imaginary part needs no doubling of its own: the result is
+// the same as doubling fma(zr, ji, zi jr) except where that is subnormal.
 template <typename F>
 MX_HD void step_j(const P32<F>& p, F zmr, F zmi, F& a, F& b) {
-    a = fma_t((F)2 * zmr, p.jr, fma_t((F)-2 * zmi, p.ji, p.jone));
-    b = (F)2 * fma_t(zmr, p.ji, zmi * p.jr);
+    const F zr2 = zmr + zmr, zi2 = zmi + zmi;
+    a = fma_t(zr2, p.jr, fma_t(-zi2, p.ji, p.jone));
+    b = fma_t(zr2, p.ji, zi2 * p.jr);
 }
 
 template <typename F>
