@@ -1,9 +1,10 @@
 // HipComputeBackend: the IComputeBackend plugin over the C ABI (include/mx.h), the
 // slot the D3D11 backend occupies in the engine (compute_backend_d3d11.cpp:209-635,
 // compute_backend.cpp:277-290).  Validation, busy and cancellation follow
-// CpuComputeBackend (compute_backend.cpp:44-128, 263-272).
-#include <strings.h>
-
+// CpuComputeBackend (compute_backend.cpp:44-128, 263-272); requests outside this path
+// (shallow or plain-double adaptive supersampling frames, Julia and Expression modes,
+// other colorings) and device failures go to the CPU backend it was given, as the
+// D3D11 backend does (compute_backend_d3d11.cpp:289-315).
 #include <cstdlib>
 #include <cstring>
 #include <mutex>
@@ -11,11 +12,9 @@
 #include <vector>
 
 #include "../../include/mx.h"
-#include "compute_backend.h"
+#include "hip_compute_backend.h"
 
 namespace {
-
-std::string g_create_error;
 
 // decimal string of an mpf value with all its digits ("0.DDDDe<exp>" or "0")
 std::string mpf_decimal(mpf_srcptr x) {
@@ -37,9 +36,10 @@ std::string mpf_decimal(mpf_srcptr x) {
 
 class HipComputeBackend final : public IComputeBackend {
   public:
-    explicit HipComputeBackend(mx_ctx* ctx) : _ctx(ctx) {
+    HipComputeBackend(mx_ctx* ctx, std::unique_ptr<IComputeBackend> cpu) : _ctx(ctx), _cpu(std::move(cpu)) {
         _info.name = "HIP";
         _info.detail = "MI355X (gfx950) perturbation kernels";
+        if (_cpu) _info.detail += " + " + _cpu->info().name + " fallback";
         _info.hardwareAccelerated = true;
         _info.fallback = false;
     }
@@ -47,9 +47,13 @@ class HipComputeBackend final : public IComputeBackend {
 
     const ComputeBackendInfo& info() const override { return _info; }
     bool lastComputeUsedGpuPath() const override { return _lastGpu.load(std::memory_order_acquire); }
+    bool lastComputeUsedCustomDeepPath() const override { return ranCpu() && _cpu->lastComputeUsedCustomDeepPath(); }
+    bool lastComputeUsedGenericDeepPath() const override { return ranCpu() && _cpu->lastComputeUsedGenericDeepPath(); }
+    GenericDeepInfo lastGenericDeepInfo() const override { return ranCpu() ? _cpu->lastGenericDeepInfo() : GenericDeepInfo{}; }
 
     bool compute(const ComputeRequest& request) override {
         _lastGpu.store(false, std::memory_order_release);
+        _lastCpu.store(false, std::memory_order_release);
         if (!valid(request)) return false;
         if (_cancelRequested.load(std::memory_order_acquire)) return false;
         {
@@ -57,6 +61,14 @@ class HipComputeBackend final : public IComputeBackend {
             if (_active) return false; // busy
             _active = true;
         }
+        struct Release {
+            HipComputeBackend* b;
+            ~Release() {
+                std::lock_guard<std::mutex> g(b->_mutex);
+                b->_active = false;
+            }
+        } release{this};
+        if (!eligible(request)) return computeCpu(request);
         const std::string cre = mpf_decimal(request.centerRe), cim = mpf_decimal(request.centerIm);
         // the engine reads the scale as a decimal magnitude; its precision follows the
         // scale's digit count (render.cpp:170), so pass it as an integer string when it
@@ -87,7 +99,7 @@ class HipComputeBackend final : public IComputeBackend {
         r.row_base = request.rowBase;
         r.sub = request.sub;
         r.max_iterations = request.maxIterations;
-        r.coloring_method = request.coloringMethod & ~MX_SUPER_SAMPLING;
+        r.coloring_method = request.coloringMethod;
         r.escape_radius = 0.0f;
         if (const char* e = std::getenv("MANDEL_BAILOUT")) { // Mandel::Mandel, mandel_perturbation.cpp:232-233
             const double b = std::atof(e);
@@ -112,63 +124,75 @@ class HipComputeBackend final : public IComputeBackend {
         if (request.progress) request.progress->store(0.0f, std::memory_order_relaxed);
         mx_frame_info info;
         const int rc = mx_compute(_ctx, &r, &info);
-        {
-            std::lock_guard<std::mutex> g(_mutex);
-            _active = false;
+        if (rc == MX_OK) {
+            _lastGpu.store(true, std::memory_order_release);
+            const bool ok = !_cancelRequested.load(std::memory_order_acquire);
+            if (ok && request.progress) request.progress->store(1.0f, std::memory_order_relaxed);
+            return ok;
         }
-        if (rc != MX_OK) return false;
-        _lastGpu.store(true, std::memory_order_release);
-        if (request.progress) request.progress->store(1.0f, std::memory_order_relaxed);
-        return !_cancelRequested.load(std::memory_order_acquire);
+        if (rc == MX_E_INVALID || rc == MX_E_CANCELLED || _cancelRequested.load(std::memory_order_acquire)) return false;
+        // outside this path (MX_E_UNSUPPORTED) or a device failure: the CPU backend
+        return computeCpu(request);
     }
 
     void cancel() override {
         _cancelRequested.store(true, std::memory_order_release);
         mx_cancel(_ctx);
+        if (_cpu) _cpu->cancel();
     }
 
     void resetCancellation() override {
-        std::lock_guard<std::mutex> g(_mutex);
-        if (!_active) {
-            _cancelRequested.store(false, std::memory_order_release);
-            mx_reset_cancellation(_ctx);
+        {
+            std::lock_guard<std::mutex> g(_mutex);
+            if (!_active) {
+                _cancelRequested.store(false, std::memory_order_release);
+                mx_reset_cancellation(_ctx);
+            }
         }
+        if (_cpu) _cpu->resetCancellation();
     }
 
   private:
-    // compute_backend.cpp:263-272 for the Mandelbrot mode
+    // compute_backend.cpp:263-266: the fields every mode needs (the mode-specific checks
+    // belong to the backend that serves the mode)
     static bool valid(const ComputeRequest& q) {
         const int fullHeight = q.fullHeight > 0 ? q.fullHeight : q.height;
-        if (!q.centerRe || !q.centerIm || !q.scale || mpf_sgn(q.scale) <= 0 || q.width < 2 || q.height < 1 || fullHeight < 2 ||
-            q.rowBase < 0 || q.rowBase > fullHeight - q.height || q.sub < 1 || q.maxIterations < 1 || !q.iterations)
-            return false;
-        return q.mode == ComputeMode::Mandelbrot;
+        return q.centerRe && q.centerIm && q.scale && mpf_sgn(q.scale) > 0 && q.width >= 2 && q.height >= 1 && fullHeight >= 2 &&
+               q.rowBase >= 0 && q.rowBase <= fullHeight - q.height && q.sub >= 1 && q.maxIterations >= 1 && q.iterations;
     }
 
+    // the Mandelbrot mode with the smooth or distance-estimate output (optionally
+    // adaptive supersampling); whether the depth is on this path mx_compute decides
+    static bool eligible(const ComputeRequest& q) {
+        return q.mode == ComputeMode::Mandelbrot && (q.coloringMethod & ~(MX_SUPER_SAMPLING | MX_EXTERIOR_DIST_EST)) == 0;
+    }
+
+    bool computeCpu(const ComputeRequest& request) {
+        if (!_cpu) return false;
+        _lastCpu.store(true, std::memory_order_release);
+        return _cpu->compute(request);
+    }
+
+    bool ranCpu() const { return _cpu && _lastCpu.load(std::memory_order_acquire); }
+
     mx_ctx* _ctx;
+    std::unique_ptr<IComputeBackend> _cpu;
     ComputeBackendInfo _info;
     std::mutex _mutex;
     bool _active = false;
     std::atomic_bool _cancelRequested{false};
     std::atomic_bool _lastGpu{false};
+    std::atomic_bool _lastCpu{false};
 };
 
 } // namespace
 
-std::unique_ptr<IComputeBackend> createComputeBackend(const char* requested) {
-    g_create_error.clear();
-    const bool hip = requested && (strcasecmp(requested, "hip") == 0 || strcasecmp(requested, "gpu") == 0 || strcasecmp(requested, "auto") == 0);
-    if (!hip) {
-        g_create_error = "unknown backend name";
-        return nullptr;
-    }
+std::unique_ptr<IComputeBackend> createHipComputeBackend(std::unique_ptr<IComputeBackend> cpuFallback, std::string* error) {
     mx_ctx* ctx = nullptr;
     const int rc = mx_ctx_create(0, &ctx);
     if (rc != MX_OK) {
-        g_create_error = rc == MX_E_NODEVICE ? "no HIP device" : "HIP device initialisation failed";
+        if (error) *error = rc == MX_E_NODEVICE ? "no HIP device" : "HIP device initialisation failed";
         return nullptr;
     }
-    return std::make_unique<HipComputeBackend>(ctx);
+    return std::make_unique<HipComputeBackend>(ctx, std::move(cpuFallback));
 }
-
-const char* computeBackendError() { return g_create_error.c_str(); }

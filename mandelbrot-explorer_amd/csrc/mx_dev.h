@@ -23,7 +23,20 @@ enum MxFlags {
     MX_F_INTERIOR = 2,   // periodicity-based interior detection
     MX_F_INT_REP = 4,    // exact state-repeat interior detection
     MX_F_BLA = 8,        // BLA skipping
+    MX_F_NOBLA_LANE = 16 // the engine's no-BLA vector loop (see mx_tile_flags)
 };
+
+// Flags of a floatexp tile.  With BLA off and no derivative the engine runs its no-BLA
+// vector loop (Mandel::solveRescaledSimd8NoBla, mandel_perturbation.cpp:3620-3816,
+// selected by stepParallel at 2148-2149 and 2172-2181): no interior detection, and the
+// step-underflow recovery taken when |w'|^2 is exactly 0, with z then formed from the
+// FloatExp orbit.  Everything else runs pixelRescaled's loop, including a periodic
+// reference without BLA (the engine builds periodic references only with BLA on).
+inline int mx_tile_flags(bool ede, bool interior, bool int_rep, bool bla, bool periodic) {
+    const bool nobla_lane = !bla && !ede && !periodic;
+    return (ede ? MX_F_EDE : 0) | (interior && !nobla_lane ? MX_F_INTERIOR : 0) |
+           (int_rep && interior && !nobla_lane ? MX_F_INT_REP : 0) | (bla ? MX_F_BLA : 0) | (nobla_lane ? MX_F_NOBLA_LANE : 0);
+}
 
 struct MxRefDev {
     const double2* zf;

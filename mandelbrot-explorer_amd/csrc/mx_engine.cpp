@@ -15,6 +15,7 @@
 #include <vector>
 
 #include "mx_internal.h"
+#include "mx_tables.h"
 
 namespace {
 
@@ -174,21 +175,12 @@ extern "C" int mx_ref_upload(mx_ctx* ctx, const mx_ref_desc* d, mx_ref** out) {
         return MX_E_NOMEM;
     }
     char* b = (char*)ref->block;
+    // the mixed form's FP32 orbit and FP32-floatexp coefficients
     std::vector<float> zf32(2 * nent);
-    for (size_t k = 0; k < 2 * nent; ++k) zf32[k] = (float)d->zf[k % (2 * len)];
-    // FP32-floatexp copy of the coefficients for the mixed form: A and B each get one
-    // exponent (that of their larger component) and FP32 mantissas in [0.5, 1)
+    mx::fe32_orbit(d->zf, len, nent, zf32.data());
     std::vector<float4> cf((size_t)total);
     std::vector<int2> ce((size_t)total);
-    for (int64_t q = 0; q < total; ++q) {
-        const double* c = d->bla_coef + 4 * q;
-        int ea = 0, eb = 0;
-        const double a = std::max(std::fabs(c[0]), std::fabs(c[1])), bb = std::max(std::fabs(c[2]), std::fabs(c[3]));
-        if (a != 0.0 && std::isfinite(a)) (void)std::frexp(a, &ea);
-        if (bb != 0.0 && std::isfinite(bb)) (void)std::frexp(bb, &eb);
-        cf[q] = make_float4((float)std::ldexp(c[0], -ea), (float)std::ldexp(c[1], -ea), (float)std::ldexp(c[2], -eb), (float)std::ldexp(c[3], -eb));
-        ce[q] = make_int2(ea, eb);
-    }
+    if (d->nlev > 0) mx::fe32_coefs(d->bla_coef, total, cf.data(), ce.data());
     std::vector<int2> lev(MX_MAXLEV, make_int2(0, 0));
     for (int p = 0; p < d->nlev; ++p) lev[p] = make_int2(d->lev_off[p], d->lev_size[p]);
     hipError_t e = hipSuccess;
@@ -363,7 +355,8 @@ extern "C" int mx_plan_create(mx_ctx* ctx, const mx_ref* ref, const mx_tile* t, 
     T.tiles_x = (T.nx + 7) / 8;
     T.ntiles = sdc ? (int)((n + 63) / 64) : T.tiles_x * ((T.ny + 7) / 8);
     T.mxit = t->max_iter;
-    T.flags = (t->ede ? MX_F_EDE : 0) | (t->interior ? MX_F_INTERIOR : 0) | (t->int_rep ? MX_F_INT_REP : 0) | (t->bla ? MX_F_BLA : 0);
+    T.flags = dbl ? (t->ede ? MX_F_EDE : 0) | (t->interior ? MX_F_INTERIOR : 0) | (t->int_rep ? MX_F_INT_REP : 0) | (t->bla ? MX_F_BLA : 0)
+                  : mx_tile_flags(t->ede != 0, t->interior != 0, t->int_rep != 0, t->bla != 0, ref->dev.period > 0);
     T.esc2 = t->esc2;
     T.interior_eps2 = t->interior_eps2;
     T.interior_confirm = t->interior_confirm;
@@ -396,7 +389,7 @@ extern "C" int mx_plan_create(mx_ctx* ctx, const mx_ref* ref, const mx_tile* t, 
     plan->minlevel = t->bla_minlevel;
     plan->ede = t->ede != 0;
     plan->grid_request = t->grid_blocks;
-    plan->interior = t->interior != 0;
+    plan->interior = (T.flags & MX_F_INTERIOR) != 0;
     if (!dbl) {
         // the plan's launch is a one-job table resident beside its tile
         MxJob J;
@@ -914,6 +907,11 @@ int compute_dbl(mx_ctx* ctx, mx::Frame& F, const mx_request* req, mx_frame_info*
         mx_ref_release(ref);
         ref = nullptr;
         F.set_pixel_reference(p / F.W, p % F.W);
+        if (F.halted()) {
+            ctx->set_err("cancelled");
+            rc = MX_E_CANCELLED;
+            break;
+        }
         refpix = (size_t)p;
         nesc[refpix] = F.ref_pixel_n;
         val[refpix] = F.ref_pixel_value;
@@ -979,6 +977,7 @@ extern "C" int mx_compute(mx_ctx* ctx, const mx_request* req, mx_frame_info* inf
     const double t0 = now_s();
     mx_frame frame;
     mx::Frame& F = frame.impl;
+    F.halt = ctx->cancel_host;
     std::string err;
     int rc = F.prepare(*req, err);
     if (rc != MX_OK) {

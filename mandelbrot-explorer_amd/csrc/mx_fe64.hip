@@ -348,8 +348,14 @@ MX_HD bool trip(const MxRefDev& R, const int2* lev, const MxTileDev& T, Px& p, d
     }
     double nwr, nwi;
     step_w(p, X, nwr, nwi);
-    const bool su = fabs(nwr) < 0x1p-1056 && fabs(nwi) < 0x1p-1056;
+    // step underflow: pixelRescaled recovers when both parts of w' are below 2^-1056
+    // (mandel_perturbation.cpp:3459-3460), the no-BLA vector loop when |w'|^2 is 0
+    // (3741), and then forms the escape test's z from the FloatExp orbit (3749-3752)
+    const bool nobla_lane = (T.flags & MX_F_NOBLA_LANE) != 0;
+    const bool su = nobla_lane ? nwr * nwr + nwi * nwi == 0.0 : fabs(nwr) < 0x1p-1056 && fabs(nwi) < 0x1p-1056;
+    bool z_fe = false;
     if (su && (nwr != 0.0 || nwi != 0.0 || p.wr != 0.0 || p.wi != 0.0 || p.dcr.m != 0.0 || p.dci.m != 0.0)) {
+        z_fe = nobla_lane;
         Fe Xrfe = fe_make(0.0, 0), Xife = fe_make(0.0, 0);
         if (p.m) {
             const double2 mm = R.zfe_m[p.rm];
@@ -402,7 +408,14 @@ MX_HD bool trip(const MxRefDev& R, const int2* lev, const MxTileDev& T, Px& p, d
     p.Xn = Xpf;
     p.xv = 1;
     const double zr = X.x + p.s * p.wr, zi = X.y + p.s * p.wi;
-    const double zrad = zr * zr + zi * zi;
+    double zrad = zr * zr + zi * zi;
+    if (z_fe) {
+        // X_{m} of the new m from the FloatExp orbit (the no-BLA loop's `current` index)
+        const double2 mm = R.zfe_m[p.rm];
+        const int2 ee = R.zfe_e[p.rm];
+        const Fe ar = fe_add(fe_make(mm.x, ee.x), fe_mul_d(p.S, p.wr)), ai = fe_add(fe_make(mm.y, ee.y), fe_mul_d(p.S, p.wi));
+        zrad = fe_to_double(fe_add(fe_mul(ar, ar), fe_mul(ai, ai)));
+    }
     if (zrad > ESC2) {
         n = p.iter;
         if constexpr (EDE) {
@@ -578,7 +591,7 @@ MX_HD void fast_steps(const MxRefDev& R, const MxTileDev& T, Px& p, Counters& c,
         // is there) and its step-underflow test
         const double w2n = nwr * nwr + nwi * nwi;
         stop |= (int)(dbits(w2n) - mlo > mspan) & (int)(w2n != 0.0);
-        stop |= (int)(fabs(nwr) < 0x1p-1056) & (int)(fabs(nwi) < 0x1p-1056) &
+        stop |= (((int)(fabs(nwr) < 0x1p-1056) & (int)(fabs(nwi) < 0x1p-1056)) | (int)(w2n == 0.0)) &
                 ((int)(nwr != 0.0) | (int)(nwi != 0.0) | (int)(p.wr != 0.0) | (int)(p.wi != 0.0) | (int)(p.dcr.m != 0.0) | (int)(p.dci.m != 0.0));
         double ja = 0.0, jb = 0.0;
         if (EDE) {

@@ -47,6 +47,7 @@ struct Gmp {
     Gmp(const Gmp&) = delete;
     Gmp& operator=(const Gmp&) = delete;
     operator mpf_ptr() { return v; }
+    operator mpf_srcptr() const { return v; }
 };
 
 // Reference orbit: createRef + calCoefficient (mandel_perturbation.cpp:4014-4187,
@@ -104,6 +105,10 @@ void Frame::build_orbit(mpf_srcptr cre, mpf_srcptr cim, int period_len) {
     ref_it = mxit;
     const double R = (double)escape_radius;
     for (int i = 1; i <= mxit; ++i) {
+        if ((i & 1023) == 0 && halted()) {
+            ref_it = i - 1;
+            break;
+        }
         mpf_add(t1, zr, zi);
         mpf_sub(t2, zr, zi);
         mpf_mul(nzi, zr, zi);
@@ -186,6 +191,10 @@ void Frame::build_orbit_fixed(mpf_srcptr cre, mpf_srcptr cim) {
     ref_it = mxit;
     const double R = (double)escape_radius;
     for (int i = 1; i <= mxit; ++i) {
+        if ((i & 1023) == 0 && halted()) {
+            ref_it = i - 1;
+            break;
+        }
         fx.add(t1, zr, zi);
         fx.sub(t2, zr, zi);
         fx.mul(ab, zr, zi);
@@ -217,236 +226,315 @@ void Frame::build_orbit_fixed(mpf_srcptr cre, mpf_srcptr cim) {
 }
 
 // ---- periodic-nucleus reference (mandel_perturbation.cpp:76-224, 1072-1219) ----
+//
+// Three stages on mpf numbers at the frame's precision:
+//   1. DiskOrbit: the disk of c-values around the view centre, iterated as a disk of
+//      z-values (centre z_n, radius R_n), until it contains 0: the period p;
+//   2. Newton on z_p(c) = 0 starting at the view centre, with z_p and dz_p/dc from the
+//      orbit-derivative recurrence (split over three threads for long periods);
+//   3. the island's size from the one-period orbit, in FloatExp.
+// The sequence of mpf operations in each recurrence is the engine's, operand for
+// operand: the nucleus's last bits, and with them the FloatExp entries of the
+// periodic orbit next to its return to 0, depend on it.
 
 namespace {
 
-// z = z^2 + c with the Karatsuba square (g_sqadd)
-void nuc_sqadd(mpf_ptr zr, mpf_ptr zi, mpf_srcptr cr, mpf_srcptr ci, mpf_ptr t1, mpf_ptr t2, mpf_ptr t3) {
-    mpf_add(t1, zr, zi);
-    mpf_sub(t2, zr, zi);
-    mpf_mul(t3, zr, zi);
-    mpf_mul(zr, t1, t2);
-    mpf_add(zr, zr, cr);
-    mpf_mul_ui(zi, t3, 2);
-    mpf_add(zi, zi, ci);
+struct Scratch3 {
+    Gmp a, b, c;
+    explicit Scratch3(mp_bitcnt_t p) : a(p), b(p), c(p) {}
+};
+
+struct MpfPair {
+    Gmp re, im;
+    explicit MpfPair(mp_bitcnt_t p) : re(p), im(p) {}
+    void zero() {
+        mpf_set_ui(re, 0);
+        mpf_set_ui(im, 0);
+    }
+};
+
+// z <- z^2 + c as (zr + zi)(zr - zi) + cr, 2 zr zi + ci: two products
+void sq_add(MpfPair& z, mpf_srcptr cr, mpf_srcptr ci, Scratch3& s) {
+    mpf_add(s.a, z.re, z.im);
+    mpf_sub(s.b, z.re, z.im);
+    mpf_mul(s.c, z.re, z.im);
+    mpf_mul(z.re, s.a, s.b);
+    mpf_add(z.re, z.re, cr);
+    mpf_mul_ui(z.im, s.c, 2);
+    mpf_add(z.im, z.im, ci);
 }
 
-// Period of the disk |c - (cr, ci)| <= r by ball arithmetic (g_ballPeriod): the disk
-// of z_n has radius R' = (2|z| + R) R + r plus a roundoff allowance; the first n with
-// |z_n| <= R_n; 0 when the disk leaves the bailout region or n passes maxp.
-int ball_period(mpf_srcptr cr, mpf_srcptr ci, mpf_srcptr r, int maxp, mp_bitcnt_t prec) {
-    Gmp zr(prec), zi(prec), R(prec), R2m(prec), zabs2(prec), nb(prec), ru(prec), t(prec), t1(prec), t2(prec), t3(prec);
-    mpf_set_ui(zr, 0);
-    mpf_set_ui(zi, 0);
-    mpf_set_ui(R, 0);
-    const double cab = std::hypot(mpf_get_d(cr), mpf_get_d(ci));
-    mpf_set_ui(ru, 1);
-    mpf_div_2exp(ru, ru, prec > 64 ? prec - 32 : prec / 2);
-    for (int p = 1; p <= maxp; ++p) {
-        const double za = std::hypot(mpf_get_d(zr), mpf_get_d(zi));
-        mpf_set_d(t, 2.0 * za);
-        mpf_add(nb, t, R);
-        mpf_mul(nb, nb, R);
-        mpf_add(nb, nb, r);
-        mpf_set_d(t, za * za + cab + 1.0);
-        mpf_mul(t, t, ru);
-        mpf_add(nb, nb, t);
-        mpf_set(R, nb);
-        nuc_sqadd(zr, zi, cr, ci, t1, t2, t3);
-        mpf_mul(t1, zr, zr);
-        mpf_mul(t2, zi, zi);
-        mpf_add(zabs2, t1, t2);
-        mpf_mul(R2m, R, R);
-        if (mpf_cmp(zabs2, R2m) <= 0) return p;
-        const double zb = std::hypot(mpf_get_d(zr), mpf_get_d(zi));
-        if (zb > cab + 2.0 + mpf_get_d(R)) return 0;
+// out <- |z|^2
+void norm2(mpf_ptr out, const MpfPair& z, Scratch3& s) {
+    mpf_mul(s.a, z.re, z.re);
+    mpf_mul(s.b, z.im, z.im);
+    mpf_add(out, s.a, s.b);
+}
+
+// The two halves of d <- 2 z d + 1 (d = dz/dc), each reading z = (zr, zi) and the
+// previous d = (dr, di): the real part, and the imaginary part (which may overwrite
+// di itself once the real half has read it).
+void deriv_real(mpf_ptr out, mpf_srcptr zr, mpf_srcptr zi, mpf_srcptr dr, mpf_srcptr di, Scratch3& s) {
+    mpf_mul(s.a, zr, dr);
+    mpf_mul(s.b, zi, di);
+    mpf_sub(out, s.a, s.b);
+    mpf_mul_ui(out, out, 2);
+    mpf_add_ui(out, out, 1);
+}
+void deriv_imag(mpf_ptr out, mpf_srcptr zr, mpf_srcptr zi, mpf_srcptr dr, mpf_srcptr di, Scratch3& s) {
+    mpf_mul(s.a, zr, di);
+    mpf_mul(s.b, zi, dr);
+    mpf_add(out, s.a, s.b);
+    mpf_mul_ui(out, out, 2);
+}
+
+// The disk |c - centre| <= r under z -> z^2 + c, as a disk of z: the radius recurrence
+// R' = (2|z| + R) R + r plus a roundoff allowance (|z|^2 + |c| + 1) 2^-(prec-32),
+// with |z| taken in double (orbit values are O(1); R stays in mpf, far below double's
+// range at depth) (g_ballPeriod, mandel_perturbation.cpp:115-161).
+class DiskOrbit {
+  public:
+    DiskOrbit(mpf_srcptr cr, mpf_srcptr ci, mpf_srcptr r, mp_bitcnt_t prec)
+        : _cr(cr), _ci(ci), _r(r), _z(prec), _R(prec), _R2(prec), _z2(prec), _next(prec), _ulp(prec), _t(prec), _s(prec) {
+        _z.zero();
+        mpf_set_ui(_R, 0);
+        _cabs = std::hypot(mpf_get_d(cr), mpf_get_d(ci));
+        mpf_set_ui(_ulp, 1);
+        mpf_div_2exp(_ulp, _ulp, prec > 64 ? prec - 32 : prec / 2);
+    }
+    // one iteration: +1 once the disk contains 0, -1 once it has left the bailout
+    // region, 0 otherwise
+    int step() {
+        const double za = std::hypot(mpf_get_d(_z.re), mpf_get_d(_z.im));
+        mpf_set_d(_t, 2.0 * za);
+        mpf_add(_next, _t, _R);
+        mpf_mul(_next, _next, _R);
+        mpf_add(_next, _next, _r);
+        mpf_set_d(_t, za * za + _cabs + 1.0);
+        mpf_mul(_t, _t, _ulp);
+        mpf_add(_next, _next, _t);
+        mpf_set(_R, _next);
+        sq_add(_z, _cr, _ci, _s);
+        norm2(_z2, _z, _s);
+        mpf_mul(_R2, _R, _R);
+        if (mpf_cmp(_z2, _R2) <= 0) return 1;
+        return std::hypot(mpf_get_d(_z.re), mpf_get_d(_z.im)) > _cabs + 2.0 + mpf_get_d(_R) ? -1 : 0;
+    }
+
+  private:
+    mpf_srcptr _cr, _ci, _r;
+    MpfPair _z;
+    Gmp _R, _R2, _z2, _next, _ulp, _t;
+    Scratch3 _s;
+    double _cabs;
+};
+
+// the first n <= maxp at which the disk contains 0; 0 when it escapes first, when
+// maxp is reached, or when `halt` is raised
+int disk_period(mpf_srcptr cr, mpf_srcptr ci, mpf_srcptr r, int maxp, mp_bitcnt_t prec, const volatile int* halt) {
+    DiskOrbit disk(cr, ci, r, prec);
+    for (int n = 1; n <= maxp; ++n) {
+        if ((n & 1023) == 0 && halt && *halt) return 0;
+        const int st = disk.step();
+        if (st > 0) return n;
+        if (st < 0) return 0;
     }
     return 0;
 }
 
-// z_p and z_p' = dz_p/dc from (0, 0) for one Newton iteration, with the serial loop's
-// operations on the same values: the orbit on this thread, the two components of the
-// derivative on two more, reading z_i through a ring.  Step i of either component
-// reads both components of step i-1 (buffers by parity) and starts once the other
-// has finished step i-1, so neither overwrites a value still being read; a ring slot
-// is refilled once both have read it.
-void nucleus_orbit_deriv(mpf_srcptr cr, mpf_srcptr ci, int period, mp_bitcnt_t prec, mpf_ptr zr, mpf_ptr zi, mpf_ptr dr_out,
-                         mpf_ptr di_out) {
-    const int RB = 1024;
-    std::vector<mpf_t> qr(RB), qi(RB);
-    for (int k = 0; k < RB; ++k) {
-        mpf_init2(qr[k], prec);
-        mpf_init2(qi[k], prec);
+// z_p and d_p = dz_p/dc from z_0 = d_0 = 0, one thread per recurrence: this thread
+// iterates z and publishes z_i in a ring; one worker computes the real part of d,
+// another the imaginary part.  Step i of either half reads both halves of d_{i-1}
+// (double-buffered by parity), so it starts once the other half has finished step
+// i-1; a ring slot is reused once both halves have read it.  On `halt` the ring stops
+// and the workers return early (the result is then not used).
+class OrbitDerivative {
+  public:
+    OrbitDerivative(mpf_srcptr cr, mpf_srcptr ci, long period, mp_bitcnt_t prec, const volatile int* halt)
+        : _cr(cr), _ci(ci), _period(period), _prec(prec), _halt(halt), _ring(RING), _d{MpfPair(prec), MpfPair(prec)} {
+        for (auto& slot : _ring) {
+            mpf_init2(slot.re, prec);
+            mpf_init2(slot.im, prec);
+        }
     }
-    std::atomic<long> count{0}, used[2], done[2];
-    for (int k = 0; k < 2; ++k) {
-        used[k].store(0);
-        done[k].store(0);
+    ~OrbitDerivative() {
+        for (auto& slot : _ring) {
+            mpf_clear(slot.re);
+            mpf_clear(slot.im);
+        }
     }
-    Gmp dr0(prec), dr1(prec), di0(prec), di1(prec);
-    mpf_ptr DR[2] = {dr0.v, dr1.v}, DI[2] = {di0.v, di1.v};
-    mpf_set_ui(DR[0], 0);
-    mpf_set_ui(DI[0], 0);
-    auto wait_for = [](const std::atomic<long>& a, long v) {
+    // false when halted
+    bool run(MpfPair& z, MpfPair& d) {
+        _d[0].zero();
+        std::thread wr([this] { half(0); }), wi([this] { half(1); });
+        Scratch3 s(_prec);
+        z.zero();
+        bool ok = true;
+        for (long i = 0; i < _period; ++i) {
+            if ((i & 1023) == 0 && _halt && *_halt) {
+                ok = false;
+                _abort.store(true, std::memory_order_release);
+                break;
+            }
+            while (std::min(_read[0].load(std::memory_order_acquire), _read[1].load(std::memory_order_acquire)) <= i - RING)
+                std::this_thread::yield();
+            mpf_set(_ring[i % RING].re, z.re);
+            mpf_set(_ring[i % RING].im, z.im);
+            _published.store(i + 1, std::memory_order_release);
+            sq_add(z, _cr, _ci, s);
+        }
+        wr.join();
+        wi.join();
+        if (!ok) return false;
+        mpf_set(d.re, _d[_period & 1].re);
+        mpf_set(d.im, _d[_period & 1].im);
+        return true;
+    }
+
+  private:
+    static constexpr int RING = 1024;
+    struct Slot {
+        mpf_t re, im;
+    };
+    bool wait_until(const std::atomic<long>& a, long v) const {
         for (int spin = 0; a.load(std::memory_order_acquire) < v; ++spin) {
+            if (_abort.load(std::memory_order_acquire)) return false;
             if (spin < 4096)
                 __builtin_ia32_pause();
             else
                 std::this_thread::yield();
         }
-    };
-    auto dloop = [&](int which) {
-        Gmp t1(prec), t2(prec), t3(prec);
-        for (long i = 0; i < period; ++i) {
-            wait_for(count, i + 1);
-            wait_for(done[1 - which], i);
-            mpf_srcptr z_r = qr[i % RB], z_i = qi[i % RB];
-            mpf_srcptr pr = DR[i & 1], pi = DI[i & 1];
-            if (which == 0) {
-                // dr' = 2 (zr dr - zi di) + 1
-                mpf_mul(t1, z_r, pr);
-                mpf_mul(t2, z_i, pi);
-                used[0].store(i + 1, std::memory_order_release);
-                mpf_sub(t3, t1, t2);
-                mpf_mul_ui(t3, t3, 2);
-                mpf_add_ui(DR[(i + 1) & 1], t3, 1);
-            } else {
-                // di' = 2 (zr di + zi dr)
-                mpf_mul(t1, z_r, pi);
-                mpf_mul(t2, z_i, pr);
-                used[1].store(i + 1, std::memory_order_release);
-                mpf_add(t3, t1, t2);
-                mpf_mul_ui(DI[(i + 1) & 1], t3, 2);
-            }
-            done[which].store(i + 1, std::memory_order_release);
-        }
-    };
-    std::thread th0(dloop, 0), th1(dloop, 1);
-    {
-        Gmp t1(prec), t2(prec), t3(prec);
-        mpf_set_ui(zr, 0);
-        mpf_set_ui(zi, 0);
-        for (long i = 0; i < period; ++i) {
-            while (std::min(used[0].load(std::memory_order_acquire), used[1].load(std::memory_order_acquire)) <= i - RB)
-                std::this_thread::yield();
-            mpf_set(qr[i % RB], zr);
-            mpf_set(qi[i % RB], zi);
-            count.store(i + 1, std::memory_order_release);
-            nuc_sqadd(zr, zi, cr, ci, t1, t2, t3);
+        return true;
+    }
+    void half(int which) {
+        Scratch3 s(_prec);
+        for (long i = 0; i < _period; ++i) {
+            if (!wait_until(_published, i + 1) || !wait_until(_finished[1 - which], i)) return;
+            const Slot& slot = _ring[i % RING];
+            const MpfPair& prev = _d[i & 1];
+            if (which == 0)
+                deriv_real(_d[(i + 1) & 1].re, slot.re, slot.im, prev.re, prev.im, s);
+            else
+                deriv_imag(_d[(i + 1) & 1].im, slot.re, slot.im, prev.re, prev.im, s);
+            _read[which].store(i + 1, std::memory_order_release);
+            _finished[which].store(i + 1, std::memory_order_release);
         }
     }
-    th0.join();
-    th1.join();
-    mpf_set(dr_out, DR[period & 1]);
-    mpf_set(di_out, DI[period & 1]);
-    for (int k = 0; k < RB; ++k) {
-        mpf_clear(qr[k]);
-        mpf_clear(qi[k]);
-    }
-}
+    mpf_srcptr _cr, _ci;
+    long _period;
+    mp_bitcnt_t _prec;
+    const volatile int* _halt;
+    std::vector<Slot> _ring;
+    MpfPair _d[2];
+    std::atomic<long> _published{0}, _read[2] = {{0}, {0}}, _finished[2] = {{0}, {0}};
+    std::atomic<bool> _abort{false};
+};
 
-// Newton's method for the period-p nucleus, c <- c - z_p / (dz_p/dc) from z_0 = 0
-// (g_newton); true once a step is below the tolerance 2^-(prec-48) in modulus.  Long
-// periods run the orbit and the derivative on three threads (nucleus_orbit_deriv).
-bool newton_nucleus(mpf_ptr cr, mpf_ptr ci, int period, mp_bitcnt_t prec, int maxit) {
-    Gmp zr(prec), zi(prec), dr(prec), di(prec), den(prec), nr(prec), ni(prec), tol(prec), step2(prec), t1(prec), t2(prec), t3(prec);
+// Newton's method for the period-p nucleus from the start value in c: the step
+// c <- c - z_p / d_p; converged once |step|^2 <= (2^-(prec-48))^2 (g_newton,
+// mandel_perturbation.cpp:163-218).  False on no convergence in `maxit` steps, a
+// vanishing derivative, or `halt`.
+bool newton_nucleus(MpfPair& c, long period, mp_bitcnt_t prec, int maxit, const volatile int* halt) {
+    MpfPair z(prec), d(prec), step(prec);
+    Gmp den(prec), tol(prec), step2(prec), dr_next(prec);
+    Scratch3 s(prec);
     mpf_set_ui(tol, 1);
     mpf_div_2exp(tol, tol, prec > 96 ? prec - 48 : prec / 2);
     mpf_mul(tol, tol, tol);
     for (int it = 0; it < maxit; ++it) {
-        mpf_set_ui(zr, 0);
-        mpf_set_ui(zi, 0);
-        mpf_set_ui(dr, 0);
-        mpf_set_ui(di, 0);
-        if (period >= 256)
-            nucleus_orbit_deriv(cr, ci, period, prec, zr, zi, dr, di);
-        else
-            for (int i = 0; i < period; ++i) {
-                mpf_mul(t1, zr, dr);
-                mpf_mul(t2, zi, di);
-                mpf_sub(t3, t1, t2);
-                mpf_mul_ui(t3, t3, 2);
-                mpf_add_ui(t3, t3, 1);
-                mpf_mul(t1, zr, di);
-                mpf_mul(t2, zi, dr);
-                mpf_add(di, t1, t2);
-                mpf_mul_ui(di, di, 2);
-                mpf_set(dr, t3);
-                nuc_sqadd(zr, zi, cr, ci, t1, t2, t3);
+        if (halt && *halt) return false;
+        z.zero();
+        d.zero();
+        if (period >= 256) {
+            OrbitDerivative od(c.re, c.im, period, prec, halt);
+            if (!od.run(z, d)) return false;
+        } else {
+            for (long i = 0; i < period; ++i) {
+                deriv_real(dr_next, z.re, z.im, d.re, d.im, s);
+                deriv_imag(d.im, z.re, z.im, d.re, d.im, s);
+                mpf_set(d.re, dr_next);
+                sq_add(z, c.re, c.im, s);
             }
-        mpf_mul(t1, dr, dr);
-        mpf_mul(t2, di, di);
-        mpf_add(den, t1, t2);
+        }
+        norm2(den, d, s);
         if (mpf_sgn(den.v) == 0) return false;
-        mpf_mul(t1, zr, dr);
-        mpf_mul(t2, zi, di);
-        mpf_add(t3, t1, t2);
-        mpf_div(nr, t3, den);
-        mpf_mul(t1, zi, dr);
-        mpf_mul(t2, zr, di);
-        mpf_sub(t3, t1, t2);
-        mpf_div(ni, t3, den);
-        mpf_sub(cr, cr, nr);
-        mpf_sub(ci, ci, ni);
-        mpf_mul(t1, nr, nr);
-        mpf_mul(t2, ni, ni);
-        mpf_add(step2, t1, t2);
+        // z / d = z conj(d) / |d|^2
+        mpf_mul(s.a, z.re, d.re);
+        mpf_mul(s.b, z.im, d.im);
+        mpf_add(s.c, s.a, s.b);
+        mpf_div(step.re, s.c, den);
+        mpf_mul(s.a, z.im, d.re);
+        mpf_mul(s.b, z.re, d.im);
+        mpf_sub(s.c, s.a, s.b);
+        mpf_div(step.im, s.c, den);
+        mpf_sub(c.re, c.re, step.re);
+        mpf_sub(c.im, c.im, step.im);
+        norm2(step2, step, s);
         if (mpf_cmp(step2, tol) <= 0) return true;
     }
     return false;
 }
 
+// complex FloatExp
+struct FeZ {
+    Fe re, im;
+};
+FeZ fz_mul(FeZ a, FeZ b) { return {fe_sub(fe_mul(a.re, b.re), fe_mul(a.im, b.im)), fe_add(fe_mul(a.re, b.im), fe_mul(a.im, b.re))}; }
+Fe fz_norm2(FeZ a) { return fe_add(fe_mul(a.re, a.re), fe_mul(a.im, a.im)); }
+
 } // namespace
 
-// findNucleus (mandel_perturbation.cpp:1072-1106): the period of the disk around the
-// view centre of radius 2/scale, refined by Newton; the period when the nucleus lies
-// in that disk, else 0
+// findNucleus (mandel_perturbation.cpp:1072-1106): the disk around the view centre of
+// radius 2/scale; its period, refined by Newton to the nucleus, which must lie in the
+// disk.  Returns the period, or 0 (no period, no convergence, outside the disk,
+// cancelled).
 int Frame::find_nucleus(mpf_ptr nre, mpf_ptr nim, mpf_srcptr scale, int maxp) {
     const mp_bitcnt_t p = mpf_get_prec(c0_re.v);
-    Gmp cre(p), cim(p), r(p), dh(p), dist(p), dr(p), di(p);
+    MpfPair centre(p), c(p);
+    Gmp r(p), dh(p), dist(p);
     mpf_set_ui(r, 2);
     mpf_div(r, r, scale);
-    mpf_add(cre, c0_re.v, r);
+    mpf_add(centre.re, c0_re.v, r);
     mpf_mul_ui(dh, r, H);
     mpf_div_ui(dh, dh, W);
-    mpf_add(cim, c0_im.v, dh);
-    const int period = ball_period(cre, cim, r, maxp, p);
+    mpf_add(centre.im, c0_im.v, dh);
+    const int period = disk_period(centre.re, centre.im, r, maxp, p, halt);
     if (period <= 0) return 0;
-    mpf_set(nre, cre);
-    mpf_set(nim, cim);
-    if (!newton_nucleus(nre, nim, period, p, 200)) return 0;
-    mpf_sub(dr, nre, cre);
-    mpf_sub(di, nim, cim);
-    mpf_mul(dr, dr, dr);
-    mpf_mul(di, di, di);
-    mpf_add(dist, dr, di);
+    mpf_set(c.re, centre.re);
+    mpf_set(c.im, centre.im);
+    if (!newton_nucleus(c, period, p, 200, halt)) return 0;
+    mpf_set(nre, c.re);
+    mpf_set(nim, c.im);
+    // distance to the disk centre
+    MpfPair off(p);
+    mpf_sub(off.re, nre, centre.re);
+    mpf_sub(off.im, nim, centre.im);
+    mpf_mul(off.re, off.re, off.re);
+    mpf_mul(off.im, off.im, off.im);
+    mpf_add(dist, off.re, off.im);
     mpf_sqrt(dist, dist);
     return mpf_cmp(dist, r) <= 0 ? period : 0;
 }
 
 // nucleusSizeMag (mandel_perturbation.cpp:1196-1219): the island's linear size
-// 1 / (|b| |l|^2), l = prod 2 z_k, b = 1 + sum 1/l_k over the one-period orbit
+// 1 / (|b| |l|^2) over the one-period orbit z_1 .. z_{p-1}, with the multiplier
+// l = prod 2 z_k and b = 1 + sum conj(l_k) / |l_k|^2; "huge" without a period.
 Fe Frame::nucleus_size() const {
-    const int p = period;
-    if (p < 2) return fe_make(1.0, 1000000);
-    Fe lr = fe_make(1.0, 0), li = fe_make(0.0, 0), br = fe_make(1.0, 0), bi = fe_make(0.0, 0);
-    for (int k = 1; k < p; ++k) {
-        const Fe zr = fe_make(zfe_m[2 * (size_t)(k - 1)], zfe_e[2 * (size_t)(k - 1)]);
-        const Fe zi = fe_make(zfe_m[2 * (size_t)(k - 1) + 1], zfe_e[2 * (size_t)(k - 1) + 1]);
-        const Fe nr = fe_mul_d(fe_sub(fe_mul(zr, lr), fe_mul(zi, li)), 2.0);
-        const Fe ni = fe_mul_d(fe_add(fe_mul(zr, li), fe_mul(zi, lr)), 2.0);
-        lr = nr;
-        li = ni;
-        const Fe l2 = fe_add(fe_mul(lr, lr), fe_mul(li, li));
+    const Fe huge = fe_make(1.0, 1000000);
+    if (period < 2) return huge;
+    FeZ l{fe_make(1.0, 0), fe_make(0.0, 0)}, b{fe_make(1.0, 0), fe_make(0.0, 0)};
+    for (int k = 1; k < period; ++k) {
+        const size_t q = 2 * (size_t)(k - 1);
+        const FeZ z{fe_make(zfe_m[q], zfe_e[q]), fe_make(zfe_m[q + 1], zfe_e[q + 1])};
+        const FeZ zl = fz_mul(z, l);
+        l = {fe_mul_d(zl.re, 2.0), fe_mul_d(zl.im, 2.0)};
+        const Fe l2 = fz_norm2(l);
         if (l2.m == 0.0) continue;
-        br = fe_add(br, fe_div(lr, l2));
-        bi = fe_sub(bi, fe_div(li, l2));
+        b = {fe_add(b.re, fe_div(l.re, l2)), fe_sub(b.im, fe_div(l.im, l2))};
     }
-    const Fe lmag = fe_sqrt(fe_add(fe_mul(lr, lr), fe_mul(li, li)));
-    const Fe bmag = fe_sqrt(fe_add(fe_mul(br, br), fe_mul(bi, bi)));
+    const Fe lmag = fe_sqrt(fz_norm2(l)), bmag = fe_sqrt(fz_norm2(b));
     const Fe denom = fe_mul(bmag, fe_mul(lmag, lmag));
-    if (denom.m == 0.0) return fe_make(1.0, 1000000);
-    return fe_div(fe_make(1.0, 0), denom);
+    return denom.m == 0.0 ? huge : fe_div(fe_make(1.0, 0), denom);
 }
 
 // The periodic branch of buildReferenceOrbit (mandel_perturbation.cpp:1238-1324): a
@@ -1075,7 +1163,11 @@ int Frame::prepare(const mx_request& r, std::string& err) {
     interior = r.interior != 0;
     escalated = false;
     if (dbl) {
-        const int rc = prepare_dbl(scale, err);
+        int rc = prepare_dbl(scale, err);
+        if (rc == MX_OK && halted()) {
+            err = "cancelled";
+            rc = MX_E_CANCELLED;
+        }
         if (rc != MX_OK || !escalated) {
             prepare_seconds = now_s() - t_start;
             return rc;
@@ -1138,6 +1230,10 @@ int Frame::prepare(const mx_request& r, std::string& err) {
         // value of the reference pixel itself (mandel_perturbation.cpp:4145-4160, 4179-4185)
         if (!ref_virtual) ref_pixel(true);
         ref_seconds = now_s() - t_ref;
+    }
+    if (halted()) {
+        err = "cancelled";
+        return MX_E_CANCELLED;
     }
     // a sub-pixel nucleus: BLA off for the frame (Compute, mandel_perturbation.cpp:1624-1629)
     if (subpixel) use_bla = false;

@@ -29,6 +29,13 @@ struct Frame {
     Frame& operator=(const Frame&) = delete;
 
     int prepare(const mx_request& r, std::string& err);
+
+    // host-mapped cancellation flag of the calling context (null: not cancellable); the
+    // long host loops (reference orbit, nucleus search) poll it, as the engine's
+    // createRef polls _flag_halt (mandel_perturbation.cpp:4140), and prepare() then
+    // returns MX_E_CANCELLED
+    const volatile int* halt = nullptr;
+    bool halted() const { return halt && *halt; }
     void fill_view(mx_frame_view& v) const;
 
     // inputs

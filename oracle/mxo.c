@@ -1087,6 +1087,113 @@ static double pixel_rescaled(const mxo_ctx* c, mxo_fe dcr, mxo_fe dci, int mxit,
     return -2.0;
 }
 
+/* One lane of Mandel::solveRescaledSimd8NoBla (MP.cpp:3620-3816): the engine's
+ * floatexp loop whenever BLA is off and no derivative is carried, as stepParallel
+ * selects it by default (MP.cpp:2148-2149, 2172-2181: MANDEL_FESIMD=1, MANDEL_FE_DUAL=1;
+ * a sub-pixel nucleus turns BLA off, MP.cpp:1629).  Against pixel_rescaled: no
+ * interior detection (a sample that does not escape runs to mxit), the step-underflow
+ * recovery fires when |w'|^2 is exactly 0 (MP.cpp:3741) instead of when both parts of
+ * w' lie below 2^-1056, and after a recovery z is formed in FloatExp from the
+ * FloatExp orbit (MP.cpp:3749-3752).  The step is the fused form (MP.cpp:3693-3701). */
+static double pixel_rescaled_nobla(const mxo_ctx* c, mxo_fe dcr, mxo_fe dci, int mxit, int* n_out, mxo_stats* st) {
+    const double ESC2 = c->esc2;
+    const double LG2 = log(2.0);
+    const double* zfp = c->zf;
+    const int mx_ref_it = c->mx_ref_it, reflen = mx_ref_it + 1;
+    mxo_fe S = mxo_fe_sqrt(mxo_fe_add(mxo_fe_mul(dcr, dcr), mxo_fe_mul(dci, dci)));
+    double wr = 0.0, wi = 0.0, dr = 0.0, di = 0.0;
+    if (S.m == 0.0) {
+        S = mxo_fe_make(1.0, 0);
+    } else {
+        wr = mxo_fe_to_double(mxo_fe_div(dcr, S));
+        wi = mxo_fe_to_double(mxo_fe_div(dci, S));
+        dr = wr;
+        di = wi;
+    }
+    double s = mxo_fe_to_double(S);
+    int idx = 1, iter = 1;
+    while (iter < mxit) {
+        const double pr = idx > 0 ? zfp[2 * (idx - 1)] : 0.0, pi = idx > 0 ? zfp[2 * (idx - 1) + 1] : 0.0;
+        int cur = idx > mx_ref_it ? mx_ref_it : idx;
+        const double fR = (pr + pr) + s * wr, fI = (pi + pi) + s * wi;
+        const double nwr = (fR * wr - fI * wi) + dr, nwi = (fR * wi + fI * wr) + di;
+        double amag;
+        {
+            const double ar = zfp[2 * cur] + s * nwr, ai = zfp[2 * cur + 1] + s * nwi;
+            amag = ar * ar + ai * ai;
+        }
+        double dmag = nwr * nwr + nwi * nwi;
+        if (dmag == 0.0 && (nwr != 0.0 || nwi != 0.0 || wr != 0.0 || wi != 0.0 || dcr.m != 0.0 || dci.m != 0.0)) {
+            const mxo_fe Xr = idx ? c->zre_fe[idx - 1] : mxo_fe_make(0.0, 0);
+            const mxo_fe Xi = idx ? c->zim_fe[idx - 1] : mxo_fe_make(0.0, 0);
+            fe_step_underflow(Xr, Xi, dcr, dci, &S, &wr, &wi);
+            s = mxo_fe_to_double(S);
+            dr = mxo_fe_to_double(mxo_fe_div(dcr, S));
+            di = mxo_fe_to_double(mxo_fe_div(dci, S));
+            const mxo_fe ar = mxo_fe_add(c->zre_fe[cur], mxo_fe_mul_d(S, wr));
+            const mxo_fe ai = mxo_fe_add(c->zim_fe[cur], mxo_fe_mul_d(S, wi));
+            amag = mxo_fe_to_double(mxo_fe_add(mxo_fe_mul(ar, ar), mxo_fe_mul(ai, ai)));
+            dmag = wr * wr + wi * wi;
+            st->underflow_steps++;
+        } else {
+            wr = nwr;
+            wi = nwi;
+        }
+        ++idx;
+        ++iter;
+        st->steps++;
+        if (amag > ESC2) {
+            *n_out = iter;
+            return (double)iter - log(log(amag) / 2.0 / LG2) / LG2;
+        }
+        if (dmag > 1e16 || (dmag < 1e-16 && dmag > 0.0)) {
+            const mxo_fe nm = mxo_fe_sqrt(mxo_fe_from(dmag));
+            S = mxo_fe_mul(S, nm);
+            s = mxo_fe_to_double(S);
+            const double inv = 1.0 / mxo_fe_to_double(nm);
+            wr *= inv;
+            wi *= inv;
+            dr = mxo_fe_to_double(mxo_fe_div(dcr, S));
+            di = mxo_fe_to_double(mxo_fe_div(dci, S));
+            st->rescales++;
+        }
+        const double adz = s * s * (wr * wr + wi * wi);
+        if (amag < 1e-8 || amag < adz || idx >= reflen) {
+            const mxo_fe Xr = idx ? c->zre_fe[idx - 1] : mxo_fe_make(0.0, 0);
+            const mxo_fe Xi = idx ? c->zim_fe[idx - 1] : mxo_fe_make(0.0, 0);
+            const mxo_fe Swr = mxo_fe_mul_d(S, wr), Swi = mxo_fe_mul_d(S, wi);
+            const mxo_fe zr = mxo_fe_add(Xr, Swr), zi = mxo_fe_add(Xi, Swi);
+            const mxo_fe zsq = mxo_fe_add(mxo_fe_mul(zr, zr), mxo_fe_mul(zi, zi));
+            const mxo_fe dsq = mxo_fe_add(mxo_fe_mul(Swr, Swr), mxo_fe_mul(Swi, Swi));
+            if (idx >= reflen || mxo_fe_abs_less(zsq, dsq)) {
+                const mxo_fe Sn = mxo_fe_sqrt(zsq);
+                if (Sn.m == 0.0) {
+                    wr = wi = 0.0;
+                } else {
+                    S = Sn;
+                    s = mxo_fe_to_double(S);
+                    wr = mxo_fe_to_double(mxo_fe_div(zr, S));
+                    wi = mxo_fe_to_double(mxo_fe_div(zi, S));
+                    dr = mxo_fe_to_double(mxo_fe_div(dcr, S));
+                    di = mxo_fe_to_double(mxo_fe_div(dci, S));
+                }
+                idx = 0;
+                st->rebases++;
+            }
+        }
+    }
+    *n_out = -1;
+    return -2.0;
+}
+
+/* The engine's per-sample loop for a context: the no-BLA vector loop's semantics when
+ * BLA is off and no derivative is carried on a full reference, pixelRescaled
+ * otherwise (the engine builds periodic references only with BLA on). */
+static double pixel_fe(const mxo_ctx* c, mxo_fe dcr, mxo_fe dci, int mxit, int* n_out, mxo_stats* st) {
+    if (!c->bla && !c->ede && !c->per) return pixel_rescaled_nobla(c, dcr, dci, mxit, n_out, st);
+    return pixel_rescaled(c, dcr, dci, mxit, n_out, st);
+}
+
 /* ------------------------------------------------------------------------- */
 /* frame (floatexp path): Compute -> buildReferenceOrbit -> buildBLA -> probe  */
 /* -> main pass (MP.cpp:1520-1761, 2128-2229)                                  */
@@ -1219,7 +1326,7 @@ static void run_pixels(const mxo_ctx* ctx, const mxo_frame* f, const mxo_fe* dco
         int i = p / f->W, j = p % f->W;
         mxo_stats st = {0, 0, 0, 0, 0, 0};
         int ne;
-        double v = pixel_rescaled(ctx, dcol[j], drow[i], f->mxit, &ne, &st);
+        double v = pixel_fe(ctx, dcol[j], drow[i], f->mxit, &ne, &st);
         val[p] = v;
         nesc[p] = ne;
         steps += st.steps;
@@ -1648,7 +1755,7 @@ MXO_EXPORT int mxo_compute_fe(mxo_frame* f) {
                 const double px = j + (double)xi / S, py = i + (double)yi / S;
                 mxo_stats st = {0, 0, 0, 0, 0, 0};
                 int ne;
-                const double v = pixel_rescaled(&ctx, mxo_fe_mul_d(dxfe, px - ref_x), mxo_fe_mul_d(dyfe, py - ref_y), f->mxit, &ne, &st);
+                const double v = pixel_fe(&ctx, mxo_fe_mul_d(dxfe, px - ref_x), mxo_fe_mul_d(dyfe, py - ref_y), f->mxit, &ne, &st);
                 const size_t c = ss_cell(W, S, i, j, yi, xi);
                 f->cell_val[c] = v;
                 f->cell_nesc[c] = ne;
@@ -1881,7 +1988,7 @@ MXO_EXPORT void mxo_pixels_fe(const mxo_pixels_in* in, int n, const double* dcr_
     for (int q = 0; q < n; ++q) {
         mxo_stats st = {0, 0, 0, 0, 0, 0};
         int ne;
-        val[q] = pixel_rescaled(&ctx, mxo_fe_make(dcr_m[q], dcr_e[q]), mxo_fe_make(dci_m[q], dci_e[q]), in->mxit, &ne, &st);
+        val[q] = pixel_fe(&ctx, mxo_fe_make(dcr_m[q], dcr_e[q]), mxo_fe_make(dci_m[q], dci_e[q]), in->mxit, &ne, &st);
         nesc[q] = ne;
         steps += st.steps;
         ap += st.bla_applies;

@@ -1,16 +1,21 @@
 // Exercises the C++ plugin interface (IComputeBackend over libmxhip.so) the way the
 // engine's verify tool does in runBackendCase (src/tools/verify.cpp:9668-10069).
-//   backend_driver names            unknown names are refused (no GPU needed)
-//   backend_driver checks           invalid requests, cancellation (GPU)
-//   backend_driver frame CRE CIM SCALE W H MXIT CMETHOD OUT.f32   one frame (GPU)
+//   backend_driver nodevice        no usable device: the factory declines (no GPU)
+//   backend_driver checks          validation, cancellation, strips, CPU fallback (GPU)
+//   backend_driver cancel          cancel from another thread mid-frame (GPU)
+//   backend_driver frame CRE CIM SCALE W H MXIT CMETHOD SUB OUT.f32   one frame (GPU)
+// The CPU backend the engine would hand the factory is a stand-in here: it fills the
+// buffer with 7 and counts its calls, so the tests see which backend served a request.
 #include <gmp.h>
 
+#include <chrono>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <thread>
 #include <vector>
 
-#include "../mandelbrot-explorer_amd/csrc/compute_backend.h"
+#include "../mandelbrot-explorer_amd/csrc/hip_compute_backend.h"
 
 static int fail(const char* what) {
     std::fprintf(stderr, "FAIL: %s\n", what);
@@ -26,31 +31,59 @@ struct Mpf {
     ~Mpf() { mpf_clear(v); }
 };
 
+class StandInCpu final : public IComputeBackend {
+  public:
+    StandInCpu() {
+        _info.name = "CPU";
+        _info.detail = "stand-in";
+    }
+    const ComputeBackendInfo& info() const override { return _info; }
+    bool lastComputeUsedGpuPath() const override { return false; }
+    bool compute(const ComputeRequest& q) override {
+        ++calls;
+        const size_t n = (size_t)q.width * q.height * q.sub * q.sub;
+        for (size_t k = 0; k < n; ++k) q.iterations[k] = 7.0f;
+        return !cancelled;
+    }
+    void cancel() override { cancelled = true; }
+    void resetCancellation() override { cancelled = false; }
+    int calls = 0;
+    bool cancelled = false;
+
+  private:
+    ComputeBackendInfo _info;
+};
+
+static bool same_bits(const float* a, const float* b, size_t n) { return std::memcmp(a, b, n * sizeof(float)) == 0; }
+
 int main(int argc, char** argv) {
     if (argc < 2) return fail("usage");
     const std::string mode = argv[1];
-    if (mode == "names") {
-        if (createComputeBackend("d3d11-warp")) return fail("unknown name accepted");
-        if (std::strcmp(computeBackendError(), "unknown backend name") != 0) return fail("error text");
-        if (createComputeBackend(nullptr)) return fail("null name accepted");
-        std::printf("names ok\n");
+    auto owned = std::make_unique<StandInCpu>();
+    StandInCpu* cpu = owned.get();
+    std::string error;
+    auto be = createHipComputeBackend(std::move(owned), &error);
+    if (mode == "nodevice") {
+        if (be) return fail("backend created without a device");
+        if (error != "no HIP device") return fail("error text");
+        std::printf("nodevice ok\n");
         return 0;
     }
-    auto be = createComputeBackend("hip");
     if (!be) {
-        std::fprintf(stderr, "no backend: %s\n", computeBackendError());
+        std::fprintf(stderr, "no backend: %s\n", error.c_str());
         return 2;
     }
     if (!be->info().hardwareAccelerated || be->info().fallback) return fail("info");
     if (mode == "checks") {
         Mpf cre("0", 4000), cim("1", 4000), sc("1e1000", 4000);
-        std::vector<float> it(64 * 48, -10.f);
+        const int W = 64, H = 48;
+        std::vector<float> it((size_t)W * H, -10.f);
         ComputeRequest q;
         q.centerRe = cre.v;
         q.centerIm = cim.v;
         q.scale = sc.v;
-        q.width = 64;
-        q.height = 48;
+        q.width = W;
+        q.height = H;
         q.maxIterations = 10000;
         q.iterations = it.data();
         // invalid requests return false (verify.cpp:9803-9816)
@@ -63,31 +96,76 @@ int main(int argc, char** argv) {
         bad = q;
         bad.iterations = nullptr;
         if (be->compute(bad)) return fail("null buffer accepted");
-        bad = q;
-        bad.mode = ComputeMode::Julia;
-        if (be->compute(bad)) return fail("Julia accepted");
+        if (cpu->calls != 0) return fail("invalid request handed to the CPU backend");
         // a cancelled backend refuses work until reset (verify.cpp:9817-9818)
         be->cancel();
         if (be->compute(q)) return fail("compute after cancel");
         be->resetCancellation();
         if (!be->compute(q)) return fail("compute after reset");
-        if (!be->lastComputeUsedGpuPath()) return fail("gpu path flag");
+        if (!be->lastComputeUsedGpuPath() || cpu->calls != 0) return fail("gpu path flag");
         for (float v : it)
             if (!(v > 2600.f - 6.f && v < 2700.f)) return fail("c=i frame values");
-        // strip of a taller frame: rows [8, 24) of a 64-row image
-        std::vector<float> strip(64 * 16, -10.f);
-        ComputeRequest s = q;
-        s.height = 16;
-        s.fullHeight = 64;
-        s.rowBase = 8;
-        s.iterations = strip.data();
-        if (!be->compute(s)) return fail("strip");
+        // strips of the same frame: rows [8, 24) of the 48-row image, bit for bit, for
+        // the smooth and the distance-estimate outputs (verify.cpp:9738-9801)
+        for (int method : {0, 2}) {
+            std::vector<float> full((size_t)W * H, -10.f), strip((size_t)W * 16, -10.f);
+            ComputeRequest f = q;
+            f.coloringMethod = method;
+            f.iterations = full.data();
+            if (!be->compute(f)) return fail("full frame");
+            ComputeRequest s = f;
+            s.height = 16;
+            s.fullHeight = H;
+            s.rowBase = 8;
+            s.iterations = strip.data();
+            if (!be->compute(s)) return fail("strip");
+            if (!same_bits(strip.data(), full.data() + (size_t)8 * W, strip.size())) return fail("strip differs from the full frame");
+        }
+        // requests off this path go to the CPU backend (compute_backend_d3d11.cpp:301-314):
+        // the engine's shallow path, the Julia mode, an orbit-trap colouring, adaptive
+        // supersampling on the plain-double path
+        Mpf shallow("1000", 256), mid("1e100", 512);
+        struct Off {
+            const char* what;
+            mpf_ptr scale;
+            ComputeMode mode;
+            int method, sub;
+        } off[] = {{"shallow", shallow.v, ComputeMode::Mandelbrot, 0, 1},
+                   {"julia", sc.v, ComputeMode::Julia, 0, 1},
+                   {"orbit trap", sc.v, ComputeMode::Mandelbrot, 32, 1},
+                   {"double-path supersampling", mid.v, ComputeMode::Mandelbrot, 1, 3}};
+        for (const Off& o : off) {
+            std::vector<float> buf((size_t)W * H * o.sub * o.sub, -10.f);
+            ComputeRequest r = q;
+            r.scale = o.scale;
+            r.mode = o.mode;
+            r.coloringMethod = o.method;
+            r.sub = o.sub;
+            r.iterations = buf.data();
+            const int before = cpu->calls;
+            if (!be->compute(r) || cpu->calls != before + 1 || be->lastComputeUsedGpuPath()) return fail(o.what);
+            for (float v : buf)
+                if (v != 7.0f) return fail(o.what);
+        }
+        // the CPU backend's cancellation follows the plugin's
+        be->cancel();
+        if (!cpu->cancelled) return fail("cancel not forwarded");
+        be->resetCancellation();
+        if (cpu->cancelled) return fail("reset not forwarded");
         std::printf("checks ok\n");
         return 0;
     }
-    if (mode == "frame" && argc == 10) {
-        Mpf cre(argv[2], 8000), cim(argv[3], 8000), sc(argv[4], 8000);
-        const int W = std::atoi(argv[5]), H = std::atoi(argv[6]);
+    if (mode == "cancel") {
+        // a frame whose kernel runs for seconds: the resolved minibrot at 1e875 with
+        // interior detection off, so its interior samples iterate to max_iter
+        // (verify.cpp:10019-10056 cancels a running compute after 10 ms, expects
+        // false within 2 s and unresolved pixels left empty)
+        if (argc != 4) return fail("usage: cancel CRE CIM");
+        setenv("MANDEL_INTERIOR", "0", 1);
+        Mpf cre(argv[2], 4000), cim(argv[3], 4000);
+        std::string s875 = "1" + std::string(875, '0');
+        Mpf sc(s875.c_str(), 4000);
+        const int W = 256, H = 256;
         std::vector<float> it((size_t)W * H, -10.f);
         ComputeRequest q;
         q.centerRe = cre.v;
@@ -95,11 +173,42 @@ int main(int argc, char** argv) {
         q.scale = sc.v;
         q.width = W;
         q.height = H;
+        q.maxIterations = 50000000;
+        q.iterations = it.data();
+        be->resetCancellation();
+        const auto t0 = std::chrono::steady_clock::now();
+        bool result = true;
+        std::thread running([&] { result = be->compute(q); });
+        std::this_thread::sleep_for(std::chrono::milliseconds(300));
+        be->cancel();
+        running.join();
+        const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        be->resetCancellation();
+        size_t empty = 0;
+        for (float v : it) empty += v == -10.f;
+        std::printf("cancel: result %d after %.3f s, %zu of %d samples empty\n", (int)result, secs, empty, W * H);
+        if (result || secs > 2.0 || empty == 0) return fail("cancel");
+        if (cpu->calls != 0) return fail("cancelled frame handed to the CPU backend");
+        std::printf("cancel ok\n");
+        return 0;
+    }
+    if (mode == "frame" && argc == 11) {
+        Mpf cre(argv[2], 8000), cim(argv[3], 8000), sc(argv[4], 8000);
+        const int W = std::atoi(argv[5]), H = std::atoi(argv[6]), sub = std::atoi(argv[9]);
+        std::vector<float> it((size_t)W * H * sub * sub, -10.f);
+        ComputeRequest q;
+        q.centerRe = cre.v;
+        q.centerIm = cim.v;
+        q.scale = sc.v;
+        q.width = W;
+        q.height = H;
+        q.sub = sub;
         q.maxIterations = std::atoi(argv[7]);
         q.coloringMethod = std::atoi(argv[8]);
         q.iterations = it.data();
         if (!be->compute(q)) return fail("compute");
-        FILE* f = std::fopen(argv[9], "wb");
+        if (!be->lastComputeUsedGpuPath() || cpu->calls != 0) return fail("frame not served by the HIP path");
+        FILE* f = std::fopen(argv[10], "wb");
         if (!f) return fail("open output");
         std::fwrite(it.data(), sizeof(float), it.size(), f);
         std::fclose(f);

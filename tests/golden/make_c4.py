@@ -1,9 +1,9 @@
 """Batched uneven-tile workload: 96 render requests over 12 boundary-walk locations.
 
 Definition (the workload spec's batch, with the choices it leaves open fixed):
-  location k = 1..12 is the boundary walk of seed k (tests/golden/make_walk.py) taken
-  to depth 10^-e_k, e_k = round(E_LO + (E_HI - E_LO) * u) with u from splitmix64
-  seeded 4000 + k (first draw);
+  location k = 1..12 is the boundary walk of seed k (tests/golden/make_walk.py,
+  --rule de) taken to depth 10^-e_k, e_k = round(E_LO + (E_HI - E_LO) * u) with u from
+  splitmix64 seeded 4000 + k (first draw), E_LO = 50, E_HI = 1000;
   request r = 0..95 sits at location 1 + r % 12; its tile is square, side drawn from
   {64, 128, 256, 512, 1024} and its max_iter round(10^(4 + 3 u)), from splitmix64
   seeded 5000 + r (draws: side, max_iter, column x0, row y0);
@@ -12,17 +12,11 @@ Definition (the workload spec's batch, with the choices it leaves open fixed):
   (1025 - side) and row y0 = draw mod (1025 - side), so the requests at one location
   share one reference orbit (built to the largest max_iter among them), one BLA
   table and one pair of dc tables.
-E_HI is 300 here (the spec's range reaches 1e-1000; walks to that depth take hours
-per seed on the CPU).  The location walks are make_walk.py's, probe max_iter capped
-at 2^21 (at 2^16 they stall near 1e-41, every probe of the region exceeding it).
-
-A location whose walk stalls (64 redraws without an escaping probe) takes the walk of
-seed k + 100 j instead, j = 1, 2, 3 (recorded as walk_seed).
 Writes tests/golden/c4_seed<k>.json (one location centre each) and, once all twelve
 exist, tests/golden/c4_requests.json.
-Usage: python tests/golden/make_c4.py [--plan] [--seeds 1,2,...] [--gpu] [--out DIR]
+Usage: python tests/golden/make_c4.py [--plan] [--seeds 1,2,...] [--out DIR]
   --plan: print the draws only; --seeds: walk only these locations (the request file
-  is written once every location's centre exists); --gpu, --out: as make_walk.py
+  is written once every location's centre exists); --out: as make_walk.py
 """
 import json
 import math
@@ -31,13 +25,12 @@ import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, HERE)
-from make_walk import GpuProbe, SplitMix64, option, walk  # noqa: E402
+from make_walk import SplitMix64, option, walk  # noqa: E402
 
-E_LO, E_HI = 50, 300
+E_LO, E_HI = 50, 1000
 SIDES = (64, 128, 256, 512, 1024)
 NLOC, NREQ = 12, 96
 GRID = 1024
-WALK_CAP = 1 << 21
 
 
 def unit(rng):
@@ -71,40 +64,14 @@ def main():
     seeds = option(args, "--seeds")
     seeds = None if seeds is None else {int(x) for x in seeds.split(",")}
     out_dir = option(args, "--out", HERE)
-    gpu = GpuProbe() if "--gpu" in args else None
     out = {"_definition": "tests/golden/make_c4.py (seeded batch of uneven tiles)", "locations": [], "requests": reqs}
     for l in locs:
         path = os.path.join(out_dir, f"c4_seed{l['seed']}.json")
         if not os.path.exists(path) and (seeds is None or l["seed"] in seeds):
-            # a walk that stalls (64 redraws without an escaping probe) is replaced by
-            # the walk of seed k + 100 j, j = 1, 2, 3, for the same location and depth
-            for j in range(4):
-                ws = l["seed"] + 100 * j
-                # checkpoints every 50 decades (resumable: a later run continues from the
-                # deepest one recorded)
-                ck_name = f"c4_seed{l['seed']}_checkpoints.json" if j == 0 else f"c4_seed{l['seed']}_walk{ws}_checkpoints.json"
-                ck_path = os.path.join(out_dir, ck_name)
-                cks = json.load(open(ck_path)) if os.path.exists(ck_path) else {}
-                done = [int(d) for d in cks]
-                resume = cks[str(max(done))] if done else None
-                targets = [d for d in range(50, l["depth"], 50) if not done or d > max(done)] + [l["depth"]]
-
-                def save_ck(res, ck_path=ck_path, cks=cks):
-                    cks.update(res)
-                    json.dump(cks, open(ck_path, "w"), indent=1)
-
-                try:
-                    res = walk(ws, targets, log=lambda s: print(s, flush=True), on_depth=save_ck, resume=resume,
-                               mxit_cap=WALK_CAP, gpu=gpu)
-                except RuntimeError as e:
-                    if "stalled" not in str(e) or j == 3:
-                        raise
-                    print(f"location {l['seed']}: walk of seed {ws} stalled, next seed", flush=True)
-                    continue
-                break
+            res = walk(l["seed"], [l["depth"]], log=lambda s: print(s, flush=True), rule="de")
             c = res[str(l["depth"])]
-            json.dump(dict(seed=l["seed"], walk_seed=ws, depth=l["depth"], re=c["re"], im=c["im"], width=c["width"],
-                           steps=c["steps"], probe_mxit=c["probe_mxit"], walk_mxit_cap=WALK_CAP), open(path, "w"), indent=1)
+            json.dump(dict(seed=l["seed"], rule="de", depth=l["depth"], re=c["re"], im=c["im"], width=c["width"],
+                           steps=c["steps"], probe_mxit=c["probe_mxit"]), open(path, "w"), indent=1)
             print("location", l, "done", flush=True)
         if os.path.exists(path):
             c = json.load(open(path))

@@ -1,33 +1,37 @@
-"""Seeded boundary walk: generates the deep view centres used by the benchmark
-configurations and the large-frame tests; writes tests/golden/walk_seed<S>.json.
+"""Seeded boundary walks: generate the deep view centres used by the benchmark
+configurations and the large-frame tests; write tests/golden/walk_<rule>seed<S>.json
+("walk_seed<S>.json" for the doubling rule).
 
-Definition (the workload spec's walk, with the two choices it leaves open fixed):
-  start c = -0.75 + 0i, view width w = 3; seed s drives splitmix64.
-  repeat until w <= target:
-    render a 64x64 probe of the view (square, the engine's grid: scale = 4/w) at
-    probe mxit (starts at 1024) with the oracle;
-    rank the pixels that escaped by (escape count desc, index asc) -- pixels that
-    did not escape are not candidates (counting them as the maximum would walk into
-    the interior of the set and end on an all-interior view);
-    if fewer than 41 pixels escaped, or the highest escape count exceeds half the
-    probe mxit (the probe would otherwise follow its own mxit level curve instead of
-    the set), double the probe mxit (up to 2^21) and render again;
-    pick uniformly among the top ceil(0.1 * n_escaped) candidates (splitmix64 mod k);
-    recentre on that pixel and halve w;
-    if a probe has no escaping pixel even at the mxit cap (the walk reached a region
-    whose escape counts all exceed 2^21), step back to the previous view and draw
-    again among its candidates (the draw continues the same splitmix64 sequence).
-  MPFR precision of each probe = ceil(-log2 w) + 64 bits.
-Centres are recorded at every requested depth; the recorded decimal strings carry
-ceil(-log10 w) + 30 digits.
+Common to every rule: start c = -0.75 + 0i, view width w = 3; seed s drives
+splitmix64; each step renders a 64x64 probe of the view (square, the engine's grid:
+scale = 4/w) with the oracle (full reference from the centre pixel), picks one probe
+pixel uniformly (splitmix64 mod k) among k candidates, recentres on it and halves w;
+MPFR precision of each probe = ceil(-log2 w) + 64 bits.  Centres are recorded at every
+requested depth; the decimal strings carry ceil(-log10 w) + 30 digits.
 
-Usage: python tests/golden/make_walk.py SEED DEPTH10 [DEPTH10 ...] [--resume DEPTH10] [--gpu] [--out DIR]
-e.g.   python tests/golden/make_walk.py 7 300 1000
---resume continues from the centre recorded at that depth (width 3 / 2^steps, the
-generator advanced by the recorded number of draws; one per step in records without
-a draw count).  --gpu renders the probes above
-scale 1e6 with the product's plain form on the GPU (same escape counts as the oracle,
-so the same walk); --out writes the JSON there instead of beside this script.
+--rule spec (the workload definition as written): probe max_iter 1024; every pixel is a
+  candidate, with count = its escape iteration, or 1024 when it did not escape;
+  ranked by (count desc, index asc); k = ceil(0.1 * 4096) = 410.
+  Below widths of about 2^-1024 the pixels counted 1024 are those beyond the
+  escape-time-1024 level curve, a smooth curve at that scale: the walk follows it and
+  its 1e-1000 view is two flat regions (escape at 1023 / 1024), not the set's boundary.
+--rule rising: the spec rule with probe max_iter 1024 + 4 * step, so that the level
+  curve the walk follows stays within a vanishing fraction of a view width of the set
+  (the escape time near the boundary grows by about one iteration per halving):
+  the views stay boundary views at every depth.
+--rule doubling (the first rule, walk_seed<S>.json): candidates are the escaping probe
+  pixels; the probe mxit doubles whenever the highest escape count exceeds half of it
+  (up to 2^21); k = ceil(0.1 * n_escaped); a probe with no escaping pixel at the cap
+  steps back to the previous view and draws again.  It follows the highest escape
+  counts into nested minibrots: by 1e-600 every probe pixel exceeds 2^21 iterations.
+A resumed walk restores the generator by the recorded draw count.
+
+Usage: python tests/golden/make_walk.py SEED DEPTH10 [DEPTH10 ...] [--rule R] [--resume DEPTH10] [--gpu] [--out DIR]
+e.g.   python tests/golden/make_walk.py 7 300 1000 --rule rising
+--resume continues from the centre recorded at that depth (width 3 / 2^steps).
+--gpu renders the probes above scale 1e6 with the product's plain form on the GPU
+(same escape counts as the oracle, so the same walk); --out writes the JSON there
+instead of beside this script.
 """
 import json
 import math
@@ -42,6 +46,7 @@ import mxo  # noqa: E402
 
 MASK = (1 << 64) - 1
 MXIT_CAP = 1 << 21
+THREADS = int(os.environ.get("WALK_THREADS", os.cpu_count() or 1))
 
 
 class SplitMix64:
@@ -76,7 +81,11 @@ class GpuProbe:
         return val, nesc
 
 
-def walk(seed, depths, log=print, on_depth=None, resume=None, mxit_cap=MXIT_CAP, gpu=None):
+def probe_mxit(rule, step):
+    return 1024 + 4 * step if rule == "rising" else 1024
+
+
+def walk(seed, depths, log=print, on_depth=None, resume=None, mxit_cap=MXIT_CAP, gpu=None, rule="doubling"):
     depths = sorted(depths)
     rng = SplitMix64(seed)
     getcontext().prec = max(depths) + 80
@@ -116,16 +125,32 @@ def walk(seed, depths, log=print, on_depth=None, resume=None, mxit_cap=MXIT_CAP,
         prec = int(math.ceil(-math.log2(float(w)) if w > Decimal("1e-300") else (-w.log10() * Decimal(math.log2(10))))) + 64
         prec = max(prec, 80)
         W = H = 64
+        if rule != "doubling":
+            mxit = probe_mxit(rule, step)
         while True:
             if sc_int is not None and scale > 1e6:
                 if gpu is not None:
                     val, nesc = gpu(format(cx, "f"), format(cy, "f"), sc_int, W, H, mxit, prec, scale <= Decimal("1e280"))
                 else:
                     val, nesc, _, _ = mxo.compute_fe(format(cx, "f"), format(cy, "f"), sc_int, W, H, mxit, prec_bits=prec,
-                                                     threads=os.cpu_count())
+                                                     threads=THREADS, periodic=rule == "doubling",
+                                                     c_method=2 if rule == "de" else 0)
             else:
-                val, nesc = mxo.direct(format(cx, "f"), format(cy, "f"), format(scale, "f"), W, H, mxit, prec_bits=prec, wp_bits=prec, threads=os.cpu_count())
+                val, nesc = mxo.direct(format(cx, "f"), format(cy, "f"), format(scale, "f"), W, H, mxit, prec_bits=prec, wp_bits=prec, threads=THREADS,
+                                       c_method=2 if rule == "de" else 0)
             flat = nesc.ravel()
+            if rule == "de":
+                # escaping pixels ranked by their distance estimate (pixels), closest first
+                fv = val.ravel()
+                cand = [(float(fv[k]), k) for k in range(W * H) if flat[k] > 0]
+                if 10 * len(cand) >= 9 * W * H or mxit >= mxit_cap:
+                    break
+                mxit *= 2
+                continue
+            if rule != "doubling":
+                # every pixel, the ones that did not escape counted as the probe's mxit
+                cand = [(-(int(flat[k]) if flat[k] > 0 else mxit), k) for k in range(W * H)]
+                break
             cand = [(-int(flat[k]), k) for k in range(W * H) if flat[k] > 0]
             top = -min(cand)[0] if cand else 0
             if (len(cand) >= 41 and top <= mxit // 2) or mxit >= mxit_cap:
@@ -163,6 +188,10 @@ def walk(seed, depths, log=print, on_depth=None, resume=None, mxit_cap=MXIT_CAP,
     return out
 
 
+def walk_file(seed, rule):
+    return f"walk_seed{seed}.json" if rule == "doubling" else f"walk_{rule}_seed{seed}.json"
+
+
 def option(args, name, default=None):
     """value of `name VALUE` in args (removed from args)"""
     if name not in args:
@@ -180,19 +209,22 @@ def main():
         args.remove("--gpu")
     resume_depth = option(args, "--resume")
     out_dir = option(args, "--out", HERE)
+    rule = option(args, "--rule", "doubling")
+    if rule not in ("doubling", "spec", "rising", "de"):
+        raise SystemExit(f"unknown rule {rule}")
     seed = int(args[0])
     depths = [int(x) for x in args[1:]]
-    path = os.path.join(out_dir, f"walk_seed{seed}.json")
+    path = os.path.join(out_dir, walk_file(seed, rule))
     resume = json.load(open(path))[resume_depth] if resume_depth else None
 
     def save(res):
         old = json.load(open(path)) if os.path.exists(path) else {}
         old.update(res)
-        old["_definition"] = "tests/golden/make_walk.py (seeded boundary walk)"
+        old["_definition"] = f"tests/golden/make_walk.py --rule {rule} (seeded boundary walk)"
         json.dump(old, open(path, "w"), indent=1)
         print("wrote", path, flush=True)
 
-    save(walk(seed, depths, log=lambda s: print(s, flush=True), on_depth=save, resume=resume, gpu=gpu))
+    save(walk(seed, depths, log=lambda s: print(s, flush=True), on_depth=save, resume=resume, gpu=gpu, rule=rule))
 
 
 if __name__ == "__main__":

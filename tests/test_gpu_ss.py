@@ -53,3 +53,19 @@ def test_explicit_samples_equal_grid(ctx):
                 assert np.array_equal(np.asarray(a).view(np.uint8), np.asarray(b).view(np.uint8))
     finally:
         ref.release()
+
+
+def test_adaptive_gui_golden(ctx):
+    """The reference's adaptive-supersampling golden (runAdaptiveGuiCase,
+    verify.cpp:307-388) through mx_compute: its acceptance, and every cell bit for bit."""
+    from test_oracle_ss import gui875_acceptance, gui875_cells
+
+    print("[ss] gui875 4x4 sub 5", flush=True)
+
+    def compute(*a, **kw):
+        return ctx.compute(*a, form=mxhip.MX_FORM_FE64, **kw)[0]
+
+    cells = gui875_cells(compute)
+    g, done = gui875_acceptance(cells)
+    assert done.sum() == 400
+    assert np.array_equal(cells.view(np.int32), g.view(np.int32))

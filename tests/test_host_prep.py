@@ -119,10 +119,13 @@ def test_invalid_requests_are_refused(kw, rc):
     assert e.value.rc == rc
 
 
-def test_backend_factory_refuses_unknown_names():
+def test_backend_factory_declines_without_a_device():
+    """createHipComputeBackend returns nullptr and says why when no device is usable, so
+    the engine keeps its CPU backend (compute_backend.cpp:281-286)."""
     drv = os.path.join(ROOT, "mandelbrot-explorer_amd", "_build", "backend_driver")
-    r = subprocess.run([drv, "names"], capture_output=True, text=True, timeout=60)
-    assert r.returncode == 0, r.stderr
+    env = dict(os.environ, HIP_VISIBLE_DEVICES="-1")
+    r = subprocess.run([drv, "nodevice"], capture_output=True, text=True, timeout=60, env=env)
+    assert r.returncode == 0, r.stdout + r.stderr
 
 
 DBL_FRAMES = {
@@ -186,3 +189,23 @@ def test_supersampling_prepare_accepted():
     assert fv["path"] == 2
     fv = mxhip.frame_prepare("0", "1", "40000000000", 16, 8, 1000, c_method=1, sub=3, options=mxhip.MX_OPT_FLOATEXP)
     assert fv["path"] == 2
+
+
+REF_ENGINE = "/root/reference/src/engine"
+
+
+@pytest.mark.skipif(not os.path.isdir(REF_ENGINE), reason="the reference tree is not present")
+def test_plugin_abi_matches_the_engine_header(tmp_path):
+    """The plugin built against this library's restated compute_backend.h and against
+    the engine's own header sees the same ComputeRequest field offsets, the same info
+    struct sizes and the same vtable slot for every IComputeBackend virtual
+    (src/engine/compute_backend.h:21-113)."""
+    out = {}
+    for name, extra in (("restated", []), ("engine", ["-I", REF_ENGINE])):
+        exe = str(tmp_path / name)
+        subprocess.run(["g++", "-std=c++17", "-I/opt/conda/include", *extra, os.path.join(ROOT, "tests", "abi_probe.cpp"),
+                        "-o", exe], check=True, capture_output=True)
+        out[name] = subprocess.run([exe], check=True, capture_output=True, text=True).stdout.splitlines()
+    assert out["restated"][0] == "header restated" and out["engine"][0] == "header engine"
+    assert out["restated"][1:] == out["engine"][1:]
+    assert "slot compute 57" in out["engine"]

@@ -70,3 +70,41 @@ def test_subsample_cells_equal_direct_samples():
     cells = [(3 * i + 1 + u, 3 * j + 1 + v) for i, j, u, v in pos]
     assert np.array_equal(n2, np.array([cn[c] for c in cells]))
     assert np.array_equal(v2, np.array([cv[c] for c in cells]))
+
+
+def gui875_cells(compute):
+    """runAdaptiveGuiCase (verify.cpp:307-388): deep1 at 1e875, 4x4 pixels, sub 5,
+    max_iter 300000, SUPER_SAMPLING; returns the (20, 20) float cells"""
+    return compute(cases.LOC["deep1_x"], cases.LOC["deep1_y"], cases.pow10(875), 4, 4, 300000, c_method=1, sub=5)
+
+
+def gui875_acceptance(cells):
+    """verify.cpp:338-378's acceptance against minibrot875_gui_ss5.f32: every base cell
+    computed, some sub-cells computed, no class mismatch, max difference <= 1.0"""
+    g = cases.golden("minibrot875_gui_ss5.f32", 20, 20)
+    done = cells != -10.0
+    base = np.zeros_like(done)
+    base[2::5, 2::5] = True
+    assert done[base].all() and (done & ~base).any()
+    gd = g[done]
+    ours = cells[done]
+    assert np.array_equal(ours < 0, gd < 0)
+    ext = ours >= 0
+    assert np.abs(ours[ext].astype(np.float64) - gd[ext]).max() <= 1.0
+    return g, done
+
+
+def test_oracle_reproduces_the_adaptive_gui_golden():
+    """The golden of the engine's production GUI path with adaptive supersampling: the
+    nucleus in view is sub-pixel at this size, so BLA is off and the engine's no-BLA
+    vector loop computes every sample (solveRescaledSimd8NoBla, mandel_perturbation.cpp:
+    3620-3816).  The oracle passes verify.cpp's acceptance and reproduces all 400 cells
+    bit for bit, including the sub-samples whose step-underflow recovery follows that
+    loop's rule (|w'|^2 == 0) rather than pixelRescaled's."""
+    def compute(*a, **kw):
+        return mxo.compute_fe(*a, **kw)[3]["cells"][0]
+
+    cells = gui875_cells(compute)
+    g, done = gui875_acceptance(cells)
+    assert done.sum() == 400
+    assert np.array_equal(cells.view(np.int32), g.view(np.int32))
