@@ -3,17 +3,19 @@
 Workload (the headline configuration): a 1920x1080 frame with 3x3 supersampling
 rendered as a 5760x3240 sample grid (uniform SS as a bigger frame, render.cpp:145,
 186-189), view width 1e-1000 (scale 4e1000) around the seed-7 boundary-walk centre
-(tests/golden/walk_seed7.json), max_iter 2e6, BLA on, exterior distance estimate
-output (carried dz/dc).  The reference orbit, BLA table and dc tables are built on
-the host (GMP) and uploaded before any timed region; a step is one main pass of the
-kernel over every sample (the engine's delta-loop phase, mandel_perturbation.cpp:
-1733-1736).  Both forms the metric names run on the same resident inputs:
+(tests/golden/walk_de_seed7.json, tests/golden/make_walk.py --rule de), max_iter 2e6,
+BLA on, exterior distance estimate output (carried dz/dc).  The reference orbit, BLA
+table and dc tables are built on the host (GMP) and uploaded before any timed region;
+a step is one main pass of the kernel over every sample (the engine's delta-loop
+phase, mandel_perturbation.cpp:1733-1736).  The forms the metric names run on the same
+resident inputs:
   fe64  plain form: IEEE FP64, the engine's operation order (escape counts identical
-        to the engine's CPU path);
-  fe32  mixed form: FP32-mantissa floatexp delta, FP64 radii / rebase / logarithm.
-`value` is the mixed form's throughput; the plain form's figures and the ratio are
-reported beside it, with the mixed form's agreement with the plain form over the
-whole frame and with per-pixel MPFR iteration on random samples.
+        to the engine's CPU path) -- `value`;
+  fe32  mixed form: FP32-mantissa floatexp delta, FP64 radii / rebase / logarithm;
+  sc64  the mixed form's algorithm with an FP64 delta.
+The mixed forms' figures, their ratio to the plain form and their agreement with it
+(whole frame) and with per-pixel MPFR iteration (committed samples) are reported
+beside `value`.
 
 Metric: nominal giga pixel-iterations per second (sum of escape counts, max_iter for
 interior samples: verify.cpp:261-273) and the frame time.
@@ -53,32 +55,26 @@ F_APPLY, F_APPLY_DE = 28, 20
 FORMS = {"fe64": mxhip.MX_FORM_FE64, "fe32": mxhip.MX_FORM_FE32, "sc64": mxhip.MX_FORM_SC64}
 
 
+WALK = os.path.join(ROOT, "tests", "golden", "walk_de_seed7.json")
+WORKLOADS = {
+    # name: (walk depth, W, H, max_iter, coloring, label)
+    "c3": ("1000", 5760, 3240, 2000000, mxhip.MX_EXTERIOR_DIST_EST,
+           "1920x1080 3x3 SS (5760x3240 samples) @ width 1e-1000, seed-7 walk centre, mxit 2e6, BLA, EDE"),
+    "c3-1080": ("1000", 1920, 1080, 2000000, mxhip.MX_EXTERIOR_DIST_EST,
+                "1920x1080 (no SS) @ width 1e-1000, seed-7 walk centre, mxit 2e6, BLA, EDE"),
+    "c2": ("300", 1920, 1080, 1000000, 0, "1920x1080 @ width 1e-300, seed-7 walk centre, mxit 1e6, BLA, smooth"),
+}
+
+
 def load_workload(name):
-    path = os.path.join(ROOT, "tests", "golden", "walk_seed7.json")
-    walk = json.load(open(path)) if os.path.exists(path) else {}
-    if name == "c3":
-        depth, W, H, mxit, cm = "1000", 5760, 3240, 2000000, mxhip.MX_EXTERIOR_DIST_EST
-        label = "1920x1080 3x3 SS (5760x3240 samples) @ width 1e-1000, seed-7 walk centre, mxit 2e6, BLA, EDE"
-    elif name == "c3-1080":
-        depth, W, H, mxit, cm = "1000", 1920, 1080, 2000000, mxhip.MX_EXTERIOR_DIST_EST
-        label = "1920x1080 (no SS) @ width 1e-1000, seed-7 walk centre, mxit 2e6, BLA, EDE"
-    elif name == "c2":
-        depth, W, H, mxit, cm = "300", 1920, 1080, 1000000, 0
-        label = "1920x1080 @ width 1e-300, seed-7 walk centre, mxit 1e6, BLA, smooth"
-    else:
+    if name not in WORKLOADS:
         raise SystemExit(f"unknown workload {name}")
-    if depth not in walk:
-        avail = sorted(int(k) for k in walk if not k.startswith("_"))
-        if not avail:
-            raise SystemExit("tests/golden/walk_seed7.json is missing: run tests/golden/make_walk.py 7 300 1000")
-        # the 1e-500 centre views a boundary region; the walk's 1e-600 centre lies in a
-        # view that is all interior at mxit 2e6
-        new = "500" if "500" in walk else str(avail[-1])
-        label = label.replace(f"1e-{depth}", f"1e-{new}") + f" (walk centre at 1e-{depth} not generated yet)"
-        depth = new
-    c = walk[depth]
-    return dict(name=name, cre=c["re"], cim=c["im"], scale="4" + "0" * int(depth), W=W, H=H, mxit=mxit,
-                c_method=cm, depth=int(depth), label=label)
+    depth, W, H, mxit, cm, label = WORKLOADS[name]
+    c = json.load(open(WALK))[depth]
+    # view width 10^-depth: scale 4 / width (mandel_perturbation.cpp:1403-1416)
+    return dict(name=name, cre=c["re"], cim=c["im"], scale="4" + "0" * int(depth), W=W, H=H, mxit=mxit, c_method=cm,
+                depth=int(depth),
+                label=label + " (boundary walk, distance-estimate rule: tests/golden/make_walk.py --rule de)")
 
 
 def log(msg):
@@ -129,8 +125,8 @@ def main():
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="c3")
-    ap.add_argument("--form", default="fe32", choices=list(FORMS), help="form reported as value")
-    ap.add_argument("--compare", default="fe64", help="comma list of forms measured beside it ('' for none)")
+    ap.add_argument("--form", default="fe64", choices=list(FORMS), help="form reported as value")
+    ap.add_argument("--compare", default="fe32,sc64", help="comma list of forms measured beside it ('' for none)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--mpfr-samples", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
@@ -227,7 +223,7 @@ def main():
     steps, applies, skipped, rebases = main_r["stats"]
     kname = "mx_fe64_kernel" if args.form == "fe64" else "mx_fe32_kernel"
     line = {
-        "metric": "Gpixel-iter/s + frame ms, 1080p @1e-1000: FP32-floatexp vs FP64 on GPU",
+        "metric": f"Gpixel-iter/s + frame ms, 1080p @1e-{wl['depth']}: FP32-floatexp vs FP64 on GPU",
         "value": s["gpix_iter_s"], "unit": "Gpixel-iter/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": s["ms_per_step"], "frame_ms": s["ms_per_step"] + probe_ms,
         "frame_ms_with_host_prepare": s["ms_per_step"] + probe_ms + prep_s * 1e3,

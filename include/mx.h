@@ -183,7 +183,7 @@ int mx_plan_unresolved(mx_plan* plan, int32_t* idx, double* glitch, int32_t cap,
  * tiles of several plans, each with its own reference, size and max_iter, as one
  * persistent grid: waves walk the plans in the given order (largest work first keeps
  * the tail short) and move on as each plan's queues drain.  The plans must be of one
- * floatexp form (FE64, FE32 or SC64) with the same ede and interior flags, each plan
+ * floatexp form (FE64, FE32 or SC64) with the same ede flag, each plan
  * at most once; they stay
  * owned by the caller, must outlive the batch, and their results are read with
  * mx_plan_read.  A batch launch equals launching its plans one by one, sample for

@@ -1,11 +1,13 @@
 """Launch-parameter autotuner for the floatexp kernels.
 
-Three knobs, timed on the device on a strided sample of the frame to be rendered:
+Knobs, timed on the device on a window of the frame to be rendered:
   chunk        lean steps per trip between the full step's checks (mx_tile.chunk);
   grid_blocks  workgroups of 4 waves in the persistent grid, i.e. waves per CU;
-  bla_minlevel the lowest BLA level tried (the BLA level cutoff), tuned only for the
-               mixed FP32 form: it changes the rounding path, so the plain form keeps
-               the engine's cutoff (minimum skip 8, mandel_perturbation.cpp:1520-1530).
+  bla_minlevel the lowest BLA level tried (the BLA level cutoff), only with
+               tune_cutoff and only for the mixed FP32 form.  Unlike the other two it
+               changes results (another sequence of skips and roundings), so by
+               default every form keeps the engine's cutoff (minimum skip 8,
+               mandel_perturbation.cpp:1520-1530) and the forms stay comparable.
 chunk and grid_blocks do not change any result.
 
 The search is coordinate descent from the defaults (chunk 32, 8 blocks per CU, the
@@ -68,7 +70,8 @@ def _time(ctx, ref, fv, form, ede, interior, mxit, knobs, reps):
         p.destroy()
 
 
-def tune(ctx, ref, fv, form, ede=False, interior=True, mxit=None, target_samples=2_000_000, reps=1, log=None):
+def tune(ctx, ref, fv, form, ede=False, interior=True, mxit=None, target_samples=2_000_000, reps=1, log=None,
+         tune_cutoff=False):
     """Best knobs for rendering frame `fv` (a frame_prepare dict whose reference is
     uploaded as `ref`) in `form`.  Returns dict(chunk, blocks_per_cu, bla_minlevel,
     window, ms, trials)."""
@@ -85,7 +88,7 @@ def tune(ctx, ref, fv, form, ede=False, interior=True, mxit=None, target_samples
 
     best = run(knobs)
     axes = [("chunk", CHUNKS), ("blocks_per_cu", BLOCKS_PER_CU)]
-    if form == mxhip.MX_FORM_FE32 and fv["nlev"] > 0:
+    if tune_cutoff and form == mxhip.MX_FORM_FE32 and fv["nlev"] > 0:
         axes.append(("bla_minlevel", tuple(m for m in MINLEVELS if m < fv["nlev"])))
     for name, values in axes:
         for v in values:

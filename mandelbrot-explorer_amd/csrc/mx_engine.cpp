@@ -502,8 +502,8 @@ extern "C" int mx_batch_create(mx_ctx* ctx, mx_plan* const* plans, int32_t n, mx
             ctx->set_err("a batch takes floatexp-form plans of this context");
             return MX_E_INVALID;
         }
-        if (p->form != p0->form || p->ede != p0->ede || p->interior != p0->interior) {
-            ctx->set_err("the plans of a batch share one form and the same EDE and interior-detection flags");
+        if (p->form != p0->form || p->ede != p0->ede) {
+            ctx->set_err("the plans of a batch share one form and the same EDE flag");
             return MX_E_INVALID;
         }
         for (int32_t m = 0; m < k; ++m)
@@ -520,7 +520,10 @@ extern "C" int mx_batch_create(mx_ctx* ctx, mx_plan* const* plans, int32_t n, mx
     b->plans.assign(plans, plans + n);
     b->form = p0->form;
     b->ede = p0->ede;
-    b->interior = p0->interior;
+    // the interior-detection kernel variant when any plan detects interiors (a job
+    // without the flag skips those tests at run time)
+    b->interior = false;
+    for (int32_t k = 0; k < n; ++k) b->interior = b->interior || plans[k]->interior;
     std::vector<MxJob> jobs((size_t)n);
     for (int32_t k = 0; k < n; ++k) jobs[(size_t)k] = plans[k]->job;
     hipError_t e = hipMalloc(&b->jobs_dev, sizeof(MxJob) * (size_t)n);

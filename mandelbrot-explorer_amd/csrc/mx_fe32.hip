@@ -570,7 +570,7 @@ MX_HD bool trip32(const MxRefDev& R, const int2* lev, const MxTileDev& T, P32<F>
         }
         return true;
     }
-    if (INTERIOR) {
+    if (INTERIOR && (T.flags & MX_F_INTERIOR)) {
         const F eps2 = (F)T.interior_eps2;
         if (p.conf_P > 0) {
             p.conf_D2 = fmin_t(p.conf_D2 * (F)4 * zrad, (F)1e18f);
@@ -701,7 +701,7 @@ MX_HD void fast_steps(const MxRefDev& R, const MxTileDev& T, P32<F>& p, C32& c, 
     const int emax = sizeof(F) == 4 ? 128 : 1024, emin = sizeof(F) == 4 ? -150 : -1075; // inf, 0
     const F wthr = can_bla ? ldexp_t((F)1, max(min(er - 2 * p.k, emax), emin)) : (F)0;
     const int nal = 1 - rm0, nm2 = 2 - m0;
-    const bool chk_ok = INTERIOR && p.conf_giveup < 3;
+    const bool chk_ok = INTERIOR && (T.flags & MX_F_INTERIOR) && p.conf_giveup < 3;
     int n = 0;
     for (int u = 0; u < lim; ++u) {
         const int rm = rm0 + n;

@@ -428,7 +428,7 @@ MX_HD bool trip(const MxRefDev& R, const int2* lev, const MxTileDev& T, Px& p, d
         }
         return true;
     }
-    if (INTERIOR) {
+    if (INTERIOR && (T.flags & MX_F_INTERIOR)) {
         if (p.conf_P > 0) {
             p.conf_D2 *= 4.0 * zrad;
             if (p.conf_D2 > 1e18) p.conf_D2 = 1e18;
@@ -567,7 +567,7 @@ MX_HD void fast_steps(const MxRefDev& R, const MxTileDev& T, Px& p, Counters& c,
     // (equality included: a superset)
     const double wthr = can_bla ? ldexp(1.0, max(min(er + 1 - p.S2.e, 1024), -1075)) : 0.0;
     const int nal = 1 - rm0, nm2 = 2 - m0;
-    const bool chk_ok = INTERIOR && p.conf_giveup < 3;
+    const bool chk_ok = INTERIOR && (T.flags & MX_F_INTERIOR) && p.conf_giveup < 3;
     int n = 0;
     for (int u = 0; u < lim; ++u) {
         const int rm = rm0 + n;

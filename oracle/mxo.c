@@ -2026,6 +2026,25 @@ MXO_EXPORT void mxo_direct(const char* cre, const char* cim, const char* scale, 
     grid_clear(&g);
 }
 
+/* Brute-force samples idx[0..n) (row-major in the W x H grid of the frame) */
+MXO_EXPORT void mxo_direct_list(const char* cre, const char* cim, const char* scale, int W, int H, int prec_bits, int wp_bits, int mxit, int c_method, float escape_radius, const int32_t* idx, int n, int threads, int track_d, double* val, int32_t* nesc) {
+    mxo_grid g;
+    grid_init(&g, cre, cim, scale, W, H, 0, 0, prec_bits > wp_bits ? prec_bits : wp_bits, 0);
+    const double esc2 = (double)(escape_radius * escape_radius);
+#pragma omp parallel for schedule(dynamic, 1) num_threads(threads > 0 ? threads : 1)
+    for (int q = 0; q < n; ++q) {
+        const int i = idx[q] / W, j = idx[q] % W;
+        mpfr_t a, b;
+        mpfr_inits2(g.prec, a, b, (mpfr_ptr)0);
+        grid_pixel_c(&g, i, j, 0, 0, 1, a, b);
+        int ne;
+        val[q] = point_mpfr(a, b, wp_bits, mxit, (c_method & 2) != 0, track_d, esc2, g.dx, &ne);
+        nesc[q] = ne;
+        mpfr_clears(a, b, (mpfr_ptr)0);
+    }
+    grid_clear(&g);
+}
+
 /* Brute-force single point from decimal strings. */
 MXO_EXPORT double mxo_point(const char* cre, const char* cim, int wp_bits, int mxit, int ede, int track_d, double esc2, const char* dx, int* nesc) {
     mpfr_t a, b, d;

@@ -86,6 +86,8 @@ def lib():
         L.mxo_bla_tables.restype = C.c_int
         L.mxo_pixels_fe.argtypes = [C.POINTER(PixelsIn), C.c_int] + [C.c_void_p] * 7
         L.mxo_direct.argtypes = [C.c_char_p, C.c_char_p, C.c_char_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_float, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p]
+        L.mxo_direct_list.argtypes = [C.c_char_p, C.c_char_p, C.c_char_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                                      C.c_int, C.c_float, C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p]
         L.mxo_point.argtypes = [C.c_char_p, C.c_char_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, C.c_char_p, C.c_void_p]
         L.mxo_point.restype = C.c_double
         L.mxo_grid_info.argtypes = [C.c_char_p, C.c_char_p, C.c_char_p] + [C.c_int] * 5 + [C.c_void_p] * 4
@@ -409,6 +411,18 @@ def direct(cre, cim, scale, W, H, mxit, c_method=0, escape_radius=1e8, step=1, p
     lib().mxo_direct(cre.encode(), cim.encode(), scale.encode(), W, H, pb, wp_bits or pb, mxit, c_method,
                      escape_radius, step, threads or os.cpu_count(), int(track_d), _p(val), _p(nesc))
     return val.reshape(H, W), nesc.reshape(H, W)
+
+
+def direct_list(cre, cim, scale, W, H, mxit, idx, c_method=0, escape_radius=1e8, prec_bits=None, wp_bits=None,
+                threads=None, track_d=True):
+    """Brute-force MPFR samples idx (row-major) of the frame's grid -> (val, n_esc)."""
+    pb = prec_bits or precision_bits(scale)
+    idx = np.ascontiguousarray(idx, np.int32)
+    val = np.zeros(len(idx))
+    nesc = np.zeros(len(idx), np.int32)
+    lib().mxo_direct_list(cre.encode(), cim.encode(), scale.encode(), W, H, pb, wp_bits or pb, mxit, c_method,
+                          escape_radius, _p(idx), len(idx), threads or os.cpu_count(), int(track_d), _p(val), _p(nesc))
+    return val, nesc
 
 
 def point(cre, cim, wp_bits, mxit, esc2=float(np.float32(1e8) * np.float32(1e8)), ede=False, track_d=True, dx="1"):

@@ -156,16 +156,16 @@ int main(int argc, char** argv) {
         return 0;
     }
     if (mode == "cancel") {
-        // a frame whose kernel runs for seconds: the resolved minibrot at 1e875 with
-        // interior detection off, so its interior samples iterate to max_iter
-        // (verify.cpp:10019-10056 cancels a running compute after 10 ms, expects
-        // false within 2 s and unresolved pixels left empty)
+        // a frame that takes seconds: deep1 at 1e875 with the nucleus search off, so the
+        // centre pixel's full reference orbit (interior, 3e6 iterations at ~3000 bits) is
+        // built on the host first (verify.cpp:10019-10056 cancels a running compute after
+        // 10 ms, expects false within 2 s and unresolved pixels left empty)
         if (argc != 4) return fail("usage: cancel CRE CIM");
-        setenv("MANDEL_INTERIOR", "0", 1);
+        setenv("MANDEL_PERIODIC", "0", 1);
         Mpf cre(argv[2], 4000), cim(argv[3], 4000);
         std::string s875 = "1" + std::string(875, '0');
         Mpf sc(s875.c_str(), 4000);
-        const int W = 256, H = 256;
+        const int W = 64, H = 64;
         std::vector<float> it((size_t)W * H, -10.f);
         ComputeRequest q;
         q.centerRe = cre.v;
@@ -173,7 +173,7 @@ int main(int argc, char** argv) {
         q.scale = sc.v;
         q.width = W;
         q.height = H;
-        q.maxIterations = 50000000;
+        q.maxIterations = 3000000;
         q.iterations = it.data();
         be->resetCancellation();
         const auto t0 = std::chrono::steady_clock::now();

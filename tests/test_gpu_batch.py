@@ -1,7 +1,8 @@
 """Batches: tiles of several references, sizes and iteration limits in one persistent
 launch give, sample for sample, what launching each tile alone gives; a batch
-relaunches with the same results; mixed forms or flags and repeated plans are
-refused."""
+relaunches with the same results; tiles with and without interior detection mix
+(the BLA-off tile runs the engine's no-BLA loop, which has none); mixed forms or EDE
+flags and repeated plans are refused."""
 import numpy as np
 import pytest
 
@@ -83,7 +84,7 @@ def test_batch_refusals(ctx):
         p2 = ctx.plan(r0, t64)
         with pytest.raises(mxhip.MxError):
             ctx.batch([p0, p2])
-        tno = ctx.tile(mxhip.frame_prepare(*cases.FRAMES["exterior1000"][:6]), form=mxhip.MX_FORM_FE32, interior=False)
+        tno = ctx.tile(mxhip.frame_prepare(*cases.FRAMES["exterior1000"][:6]), form=mxhip.MX_FORM_FE32, ede=True)
         p3 = ctx.plan(r0, tno)
         with pytest.raises(mxhip.MxError):
             ctx.batch([p0, p3])
