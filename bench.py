@@ -84,7 +84,8 @@ def log(msg):
 def run_form(ctx, ref, fv, form, ede, interior, center, steps, warmup, knobs=None):
     k = knobs or {}
     tile = ctx.tile(fv, stride=1, ede=ede, interior=interior, form=FORMS[form], bla_minlevel=k.get("bla_minlevel"),
-                    chunk=k.get("chunk", 0), grid_blocks=k.get("blocks_per_cu", 0) * ctx.cu_count)
+                    chunk=k.get("chunk", 0), grid_blocks=k.get("blocks_per_cu", 0) * ctx.cu_count,
+                    quorum=k.get("quorum", 0))
     tile.skip_sample = -1 if fv["ref_virtual"] else center
     plan = ctx.plan(ref, tile)
     try:
@@ -134,8 +135,11 @@ def main():
     ap.add_argument("--no-e2e", action="store_true", help="skip the end-to-end mx_compute frame")
     ap.add_argument("--c4", action="store_true", help="add the batched uneven-tiles leg (96 tiles, 12 references)")
     ap.add_argument("--no-tune", action="store_true", help="skip the autotuner (chunk / grid size / BLA cutoff)")
-    ap.add_argument("--knobs", default="", help="JSON file of launch knobs per form: read instead of tuning when "
-                    "it exists, written after tuning otherwise (profiled reruns use the timed run's knobs)")
+    ap.add_argument("--knobs", default="auto", help="JSON file of launch knobs per form: read instead of tuning when "
+                    "it exists, written after tuning otherwise; 'auto' (default): profiles/knobs_<workload>.json, "
+                    "the autotuner's committed result that the profiles under profiles/ were taken with")
+    ap.add_argument("--copy-calibration", action="store_true", help="also copy 1 GiB device to device with a "
+                    "kernel (the reading that scales rocprofv3's FETCH_SIZE / WRITE_SIZE)")
     ap.add_argument("--c5", action="store_true", help="add the zoom-sequence leg (hipGraph of 1280x720 frames)")
     ap.add_argument("--c5-frames", type=int, default=240, help="first N frames of the zoom sequence")
     args = ap.parse_args()
@@ -144,6 +148,8 @@ def main():
               f"(multi-GPU sharding is design-only, see DESIGN.md)", file=sys.stderr)
         sys.exit(2)
     wl = load_workload(args.workload)
+    if args.knobs == "auto":
+        args.knobs = os.path.join(ROOT, "profiles", f"knobs_{args.workload}.json")
     ede = bool(wl["c_method"] & 2)
     W, H, mxit = wl["W"], wl["H"], wl["mxit"]
     t0 = time.time()
@@ -174,7 +180,7 @@ def main():
             elif not args.no_tune:
                 for f in [args.form] + compare:
                     r = autotune.tune(ctx, ref, fv, FORMS[f], ede=ede, interior=interior, log=log)
-                    tuned[f] = {k: r[k] for k in ("chunk", "blocks_per_cu", "bla_minlevel", "window", "ms")}
+                    tuned[f] = {k: r[k] for k in ("chunk", "blocks_per_cu", "quorum", "bla_minlevel", "window", "ms")}
                     tuned[f]["trials"] = len(r["trials"])
                     log(f"autotune {f}: {tuned[f]}")
                 if args.knobs:
@@ -195,6 +201,8 @@ def main():
             e2e_ms = (time.time() - t1) * 1e3
             log(f"end-to-end frame (mx_compute): {e2e_ms:.0f} ms")
         double = None if args.no_double else double_leg(ctx, args.no_cpu)
+        if args.copy_calibration:
+            copy_calibration()
         batched = c4_leg(ctx) if args.c4 else None
         zoom = c5_leg(ctx, args.c5_frames) if args.c5 else None
     main_r = res[args.form]
@@ -253,6 +261,20 @@ def main():
     if zoom is not None:
         line["zoom_sequence"] = zoom
     print(json.dumps(line))
+
+
+def copy_calibration(nbytes=1 << 30):
+    """A 1 GiB device-to-device copy by a kernel (torch's elementwise copy): reads and
+    writes nbytes each, so its rocprofv3 FETCH_SIZE / WRITE_SIZE give the factors that
+    turn those counters into bytes."""
+    import torch
+    a = torch.empty(nbytes // 4, dtype=torch.float32, device="cuda")
+    a.fill_(1.0)
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    b.copy_(a)
+    torch.cuda.synchronize()
+    log(f"copy calibration: {nbytes} bytes copied by a kernel")
 
 
 def traffic_bytes(kernel, wl):

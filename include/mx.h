@@ -157,6 +157,10 @@ typedef struct {
     int32_t n_samples;
     const double* sample_dc_m;
     const int32_t* sample_dc_e;
+    /* launch tuning (0: default 32), results do not depend on it: a wave ends a run of
+     * lean steps as soon as fewer than quorum/64 of the lanes that started it are still
+     * stepping (the others would idle until the run ends) */
+    int32_t quorum;
 } mx_tile;
 
 /* A plan holds the device copy of a tile's inputs and outputs so that launches

@@ -3,15 +3,16 @@
 Knobs, timed on the device on a window of the frame to be rendered:
   chunk        lean steps per trip between the full step's checks (mx_tile.chunk);
   grid_blocks  workgroups of 4 waves in the persistent grid, i.e. waves per CU;
+  quorum       a wave ends a lean run once fewer than quorum/64 of its lanes still step;
   bla_minlevel the lowest BLA level tried (the BLA level cutoff), only with
                tune_cutoff and only for the mixed FP32 form.  Unlike the other two it
                changes results (another sequence of skips and roundings), so by
                default every form keeps the engine's cutoff (minimum skip 8,
                mandel_perturbation.cpp:1520-1530) and the forms stay comparable.
-chunk and grid_blocks do not change any result.
+chunk, grid_blocks and quorum do not change any result.
 
-The search is coordinate descent from the defaults (chunk 32, 8 blocks per CU, the
-frame's own cutoff): each knob in turn is swept with the others fixed and the fastest
+The search is coordinate descent from the defaults (chunk 32, 8 blocks per CU, quorum
+32, the frame's own cutoff): each knob in turn is swept with the others fixed and the fastest
 value kept.  The sample is the central window of the frame with about 2 M samples
 (the frame's aspect ratio): enough work to fill the grid, a fraction of the frame's
 cost, and neighbouring samples side by side as in the frame, so a wave's lanes
@@ -24,7 +25,8 @@ import os
 
 import mxhip
 
-CHUNKS = (8, 16, 24, 32, 48, 64)
+CHUNKS = (8, 16, 24, 32, 48, 64, 96, 128)
+QUORUMS = (1, 8, 16, 24, 32, 40, 48, 56)
 BLOCKS_PER_CU = (2, 3, 4, 5, 6, 8)
 MINLEVELS = (1, 2, 3, 4, 5, 6, 7, 8, 9, 10)
 
@@ -56,7 +58,7 @@ def subframe(fv, x0, y0, w, h):
 def _time(ctx, ref, fv, form, ede, interior, mxit, knobs, reps):
     t = ctx.tile(fv, form=form, ede=ede, interior=interior, mxit=mxit,
                  bla_minlevel=knobs["bla_minlevel"], chunk=knobs["chunk"],
-                 grid_blocks=knobs["blocks_per_cu"] * ctx.cu_count)
+                 grid_blocks=knobs["blocks_per_cu"] * ctx.cu_count, quorum=knobs.get("quorum", 0))
     t.skip_sample = fv["skip"]
     p = ctx.plan(ref, t)
     try:
@@ -73,10 +75,10 @@ def _time(ctx, ref, fv, form, ede, interior, mxit, knobs, reps):
 def tune(ctx, ref, fv, form, ede=False, interior=True, mxit=None, target_samples=2_000_000, reps=1, log=None,
          tune_cutoff=False):
     """Best knobs for rendering frame `fv` (a frame_prepare dict whose reference is
-    uploaded as `ref`) in `form`.  Returns dict(chunk, blocks_per_cu, bla_minlevel,
-    window, ms, trials)."""
+    uploaded as `ref`) in `form`.  Returns dict(chunk, blocks_per_cu, quorum,
+    bla_minlevel, window, ms, trials)."""
     sub = window(fv, target_samples)
-    knobs = dict(chunk=32, blocks_per_cu=8, bla_minlevel=fv["bla_minlevel"])
+    knobs = dict(chunk=32, blocks_per_cu=8, quorum=32, bla_minlevel=fv["bla_minlevel"])
     trials = []
 
     def run(k):
@@ -87,7 +89,7 @@ def tune(ctx, ref, fv, form, ede=False, interior=True, mxit=None, target_samples
         return ms
 
     best = run(knobs)
-    axes = [("chunk", CHUNKS), ("blocks_per_cu", BLOCKS_PER_CU)]
+    axes = [("quorum", QUORUMS), ("chunk", CHUNKS), ("blocks_per_cu", BLOCKS_PER_CU)]
     if tune_cutoff and form == mxhip.MX_FORM_FE32 and fv["nlev"] > 0:
         axes.append(("bla_minlevel", tuple(m for m in MINLEVELS if m < fv["nlev"])))
     for name, values in axes:
@@ -114,5 +116,5 @@ def load(path, form, ede, interior, cu_count):
 
 def store(path, form, ede, interior, cu_count, result):
     d = json.load(open(path)) if os.path.exists(path) else {}
-    d[key(form, ede, interior, cu_count)] = {k: result[k] for k in ("chunk", "blocks_per_cu", "bla_minlevel")}
+    d[key(form, ede, interior, cu_count)] = {k: result[k] for k in ("chunk", "blocks_per_cu", "quorum", "bla_minlevel")}
     json.dump(d, open(path, "w"), indent=1)

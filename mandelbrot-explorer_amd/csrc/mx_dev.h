@@ -78,6 +78,7 @@ struct MxTileDev {
     int interior_confirm;
     int skip_sample; // sample index not computed (the reference pixel), -1 none
     int chunk;       // lean steps per trip between full-path checks (results do not depend on it)
+    int quorum;      // a lean run ends once fewer than quorum/64 of its lanes still step (neither)
     Fe dxfe;
     double* val;         // W*H
     int* nesc;           // W*H

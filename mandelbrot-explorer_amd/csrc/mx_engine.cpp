@@ -52,6 +52,29 @@ __global__ void mx_reset_kernel(unsigned* queue, unsigned long long* stats, unsi
     }
 }
 
+// The scalar fields of a tile's device record (sizes, limits, flags, launch knobs);
+// the caller sets its buffer pointers.  W x H: the grid, or n samples x 1.
+void tile_fields(MxTileDev& T, const mx_tile& t, const mx_ref& ref, size_t W, size_t H, bool sdc, bool dbl) {
+    std::memset(&T, 0, sizeof T);
+    T.W = (int)W;
+    T.H = (int)H;
+    T.stride = sdc ? 1 : t.stride;
+    T.nx = (T.W + T.stride - 1) / T.stride;
+    T.ny = (T.H + T.stride - 1) / T.stride;
+    T.tiles_x = (T.nx + 7) / 8;
+    T.ntiles = sdc ? (int)((W * H + 63) / 64) : T.tiles_x * ((T.ny + 7) / 8);
+    T.mxit = t.max_iter;
+    T.flags = dbl ? (t.ede ? MX_F_EDE : 0) | (t.interior ? MX_F_INTERIOR : 0) | (t.int_rep ? MX_F_INT_REP : 0) | (t.bla ? MX_F_BLA : 0)
+                  : mx_tile_flags(t.ede != 0, t.interior != 0, t.int_rep != 0, t.bla != 0, ref.dev.period > 0);
+    T.esc2 = t.esc2;
+    T.interior_eps2 = t.interior_eps2;
+    T.interior_confirm = t.interior_confirm;
+    T.dxfe = fe_make(t.dx_m, t.dx_e);
+    T.skip_sample = t.skip_sample;
+    T.chunk = t.chunk > 0 ? std::min(t.chunk, 4096) : 32;
+    T.quorum = t.quorum > 0 ? std::min(t.quorum, 64) : 32;
+}
+
 struct BusyGuard {
     mx_ctx* ctx;
     bool ok;
@@ -344,25 +367,9 @@ extern "C" int mx_plan_create(mx_ctx* ctx, const mx_ref* ref, const mx_tile* t, 
         return MX_E_HIP;
     }
     MxTileDev& T = plan->tile;
-    std::memset(&T, 0, sizeof T);
+    tile_fields(T, *t, *ref, W, H, sdc, dbl);
     T.dcol = (const Fe*)(b + o_col);
     T.drow = (const Fe*)(b + o_row);
-    T.W = (int)W;
-    T.H = (int)H;
-    T.stride = sdc ? 1 : t->stride;
-    T.nx = (T.W + T.stride - 1) / T.stride;
-    T.ny = (T.H + T.stride - 1) / T.stride;
-    T.tiles_x = (T.nx + 7) / 8;
-    T.ntiles = sdc ? (int)((n + 63) / 64) : T.tiles_x * ((T.ny + 7) / 8);
-    T.mxit = t->max_iter;
-    T.flags = dbl ? (t->ede ? MX_F_EDE : 0) | (t->interior ? MX_F_INTERIOR : 0) | (t->int_rep ? MX_F_INT_REP : 0) | (t->bla ? MX_F_BLA : 0)
-                  : mx_tile_flags(t->ede != 0, t->interior != 0, t->int_rep != 0, t->bla != 0, ref->dev.period > 0);
-    T.esc2 = t->esc2;
-    T.interior_eps2 = t->interior_eps2;
-    T.interior_confirm = t->interior_confirm;
-    T.dxfe = fe_make(t->dx_m, t->dx_e);
-    T.skip_sample = t->skip_sample;
-    T.chunk = t->chunk > 0 ? std::min(t->chunk, 4096) : 32;
     T.val = (double*)(b + o_val);
     T.nesc = (int*)(b + o_nesc);
     T.out = (float*)(b + o_out);

@@ -703,6 +703,11 @@ MX_HD void fast_steps(const MxRefDev& R, const MxTileDev& T, P32<F>& p, C32& c, 
     const int nal = 1 - rm0, nm2 = 2 - m0;
     const bool chk_ok = INTERIOR && (T.flags & MX_F_INTERIOR) && p.conf_giveup < 3;
     int n = 0;
+#ifdef __HIP_DEVICE_COMPILE__
+    // the wave ends the run together once fewer than quorum/64 of the lanes that start
+    // it are still stepping: a lane that has stopped idles until the run ends
+    const int need = (__popcll(__ballot(lim > 0)) * T.quorum + 63) >> 6;
+#endif
     for (int u = 0; u < lim; ++u) {
         const int rm = rm0 + n;
         int stop = 0;
@@ -763,6 +768,9 @@ MX_HD void fast_steps(const MxRefDev& R, const MxTileDev& T, P32<F>& p, C32& c, 
             p.xv = 0;
             break;
         }
+#ifdef __HIP_DEVICE_COMPILE__
+        if (__popcll(__ballot(1)) < need) break;
+#endif
     }
     p.m = m0 + n;
     p.rm = rm0 + n;

@@ -525,9 +525,9 @@ MX_HD unsigned long long dbits(double x) { return __builtin_bit_cast(unsigned lo
 // it (the full path then decides), evaluated without branches and joined into one
 // exit.  Before the step: the iteration limit, an active periodicity confirmation, an
 // exact-repeat save or the orbit index back at the saved one, a BLA-aligned index
-// whose |dz|^2 is not above the largest radius by exponent.  After it: a nonzero
-// |w'|^2 outside [1e-16, 1e16] (renormalisation), a step underflow, the same
-// renormalisation test for the derivative, a derivative step whose terms all vanish,
+// whose |dz|^2 is not above the largest radius by exponent.  After it: |w'|^2
+// outside [1e-16, 1e16] (renormalisation; 0 included), a step underflow, the same
+// renormalisation test for the derivative, a derivative step at z = 0,
 // |z|^2 outside [1e-8, escape
 // radius^2], |z| < |dz|, the end of the reference, a near return on a periodicity
 // check.  A wave executes a branch whenever any of its 64 lanes takes it, so these
@@ -569,6 +569,11 @@ MX_HD void fast_steps(const MxRefDev& R, const MxTileDev& T, Px& p, Counters& c,
     const int nal = 1 - rm0, nm2 = 2 - m0;
     const bool chk_ok = INTERIOR && (T.flags & MX_F_INTERIOR) && p.conf_giveup < 3;
     int n = 0;
+#ifdef __HIP_DEVICE_COMPILE__
+    // the wave ends the run together once fewer than quorum/64 of the lanes that start
+    // it are still stepping: a lane that has stopped idles until the run ends
+    const int need = (__popcll(__ballot(lim > 0)) * T.quorum + 63) >> 6;
+#endif
     for (int u = 0; u < lim; ++u) {
         const int rm = rm0 + n;
         int stop = 0;
@@ -587,18 +592,17 @@ MX_HD void fast_steps(const MxRefDev& R, const MxTileDev& T, Px& p, Counters& c,
         const double2 X = p.Xc;
         double nwr, nwi;
         step_w(p, X, nwr, nwi);
-        // the full path's renormalisation test (|w'|^2 that underflows to 0 is left as it
-        // is there) and its step-underflow test
+        // supersets of the full path's renormalisation test and of both step-underflow
+        // tests: |w'|^2 outside [1e-16, 1e16] (0 included), or both parts of w' below
+        // 2^-1056
         const double w2n = nwr * nwr + nwi * nwi;
-        stop |= (int)(dbits(w2n) - mlo > mspan) & (int)(w2n != 0.0);
-        stop |= (((int)(fabs(nwr) < 0x1p-1056) & (int)(fabs(nwi) < 0x1p-1056)) | (int)(w2n == 0.0)) &
-                ((int)(nwr != 0.0) | (int)(nwi != 0.0) | (int)(p.wr != 0.0) | (int)(p.wi != 0.0) | (int)(p.dcr.m != 0.0) | (int)(p.dci.m != 0.0));
+        stop |= (int)(dbits(w2n) - mlo > mspan) | ((int)(fabs(nwr) < 0x1p-1056) & (int)(fabs(nwi) < 0x1p-1056));
         double ja = 0.0, jb = 0.0;
         if (EDE) {
             const double zmr = X.x + p.s * p.wr, zmi = X.y + p.s * p.wi;
             step_j(p, zmr, zmi, ja, jb);
             const double jm2 = ja * ja + jb * jb;
-            stop |= ((int)(zmr == 0.0) & (int)(zmi == 0.0) & (int)(p.invSJd == 0.0)) | ((int)(dbits(jm2) - mlo > mspan) & (int)(jm2 != 0.0));
+            stop |= ((int)(zmr == 0.0) & (int)(zmi == 0.0)) | (int)(dbits(jm2) - mlo > mspan);
         }
         const double2 Xn = p.Xn;
         const double zr = Xn.x + p.s * nwr, zi = Xn.y + p.s * nwi;
@@ -614,7 +618,8 @@ MX_HD void fast_steps(const MxRefDev& R, const MxTileDev& T, Px& p, Counters& c,
             save = chk & (int)(it1 - p.save_iter >= p.period_win);
         }
         // commit by selects, so that the entry loaded above is consumed only by the
-        // next step and its load overlaps this one
+        // next step and its load overlaps this one; the cached entries rotate on every
+        // step, and a lane that stops has them reloaded by the full step (xv = 0 below)
         const bool go = !stop;
         p.wr = go ? nwr : p.wr;
         p.wi = go ? nwi : p.wi;
@@ -622,10 +627,8 @@ MX_HD void fast_steps(const MxRefDev& R, const MxTileDev& T, Px& p, Counters& c,
             p.jr = go ? ja : p.jr;
             p.ji = go ? jb : p.ji;
         }
-        p.Xc.x = go ? Xn.x : p.Xc.x;
-        p.Xc.y = go ? Xn.y : p.Xc.y;
-        p.Xn.x = go ? Xpf.x : p.Xn.x;
-        p.Xn.y = go ? Xpf.y : p.Xn.y;
+        p.Xc = Xn;
+        p.Xn = Xpf;
         if (INTERIOR && go && save) {
             p.zsr = zr;
             p.zsi = zi;
@@ -634,7 +637,13 @@ MX_HD void fast_steps(const MxRefDev& R, const MxTileDev& T, Px& p, Counters& c,
         }
         w2 = w2n;
         n += go;
-        if (stop) break;
+        if (stop) {
+            p.xv = 0;
+            break;
+        }
+#ifdef __HIP_DEVICE_COMPILE__
+        if (__popcll(__ballot(1)) < need) break;
+#endif
     }
     p.m = m0 + n;
     p.rm = rm0 + n;

@@ -103,6 +103,7 @@ struct HostJob {
         T.dxfe = fe_make(v.dx_m, v.dx_e);
         T.skip_sample = -1;
         T.chunk = o.chunk > 0 ? std::min(o.chunk, 4096) : 32;
+        T.quorum = 32;
     }
 };
 
