@@ -213,6 +213,23 @@ void mx_graph_destroy(mx_graph* graph);
 /* create + launch + wait + read + destroy */
 int mx_tile_render(mx_ctx* ctx, const mx_ref* ref, const mx_tile* tile, int32_t* n_esc, double* value, float* out, uint64_t* stats4);
 
+/* mx_tile_render without allocating: no hipMalloc, hipFree or other allocating call.
+ * No counterpart in the reference; for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid it).  The
+ * tile's dc tables, its outputs and its per-launch state (work queues, counters, the
+ * job record) are staged in a fixed scratch area in device memory, one per device,
+ * reserved when the library is loaded and held until the process ends; the reference
+ * must already be uploaded.  Limits: a grid tile (no sample list, no explicit samples)
+ * of a floatexp form with at most MX_NOALLOC_MAX_SAMPLES samples and at most
+ * MX_NOALLOC_MAX_TABLE columns plus rows; larger tiles are refused with MX_E_INVALID.
+ * Every call on a device shares that device's scratch area (which keeps the last
+ * call's contents): calls that overlap on one device race, and a context runs one
+ * call at a time (MX_E_BUSY otherwise). */
+#define MX_NOALLOC_MAX_SAMPLES (1 << 20)
+#define MX_NOALLOC_MAX_TABLE 16384
+int mx_tile_render_noalloc(mx_ctx* ctx, const mx_ref* ref, const mx_tile* tile, int32_t* n_esc, double* value, float* out,
+                           uint64_t* stats4);
+
 /* ---- whole frame: Mandel::Compute (mandel_perturbation.cpp:1388-1788) ----- */
 
 typedef struct {

@@ -75,6 +75,29 @@ void tile_fields(MxTileDev& T, const mx_tile& t, const mx_ref& ref, size_t W, si
     T.quorum = t.quorum > 0 ? std::min(t.quorum, 64) : 32;
 }
 
+// Fixed scratch area of mx_tile_render_noalloc, one copy per device (held until the
+// process ends).  Host code never uses its name as an address: the device address
+// comes from hipGetSymbolAddress at context creation (mx_ctx::noalloc_scratch).
+// Layout: the job record, the work queues and counters, the dc tables as mantissas,
+// exponents and packed FloatExp values, and the outputs (value, n_esc, engine float).
+constexpr size_t NA_JOB = 0;
+constexpr size_t NA_QUEUE = NA_JOB + ((sizeof(MxJob) + 255) & ~(size_t)255);
+constexpr size_t NA_STATS = NA_QUEUE + 256;
+constexpr size_t NA_DCM = NA_STATS + 256;
+constexpr size_t NA_DCE = NA_DCM + 8 * (size_t)MX_NOALLOC_MAX_TABLE;
+constexpr size_t NA_DCF = NA_DCE + 4 * (size_t)MX_NOALLOC_MAX_TABLE;
+constexpr size_t NA_VAL = NA_DCF + sizeof(Fe) * (size_t)MX_NOALLOC_MAX_TABLE;
+constexpr size_t NA_NESC = NA_VAL + 8 * (size_t)MX_NOALLOC_MAX_SAMPLES;
+constexpr size_t NA_OUT = NA_NESC + 4 * (size_t)MX_NOALLOC_MAX_SAMPLES;
+constexpr size_t NA_BYTES = NA_OUT + 4 * (size_t)MX_NOALLOC_MAX_SAMPLES;
+static __device__ __align__(256) unsigned char scratch[NA_BYTES];
+
+// dc tables from separate mantissa and exponent arrays into FloatExp records
+__global__ void mx_pack_fe_kernel(const double* m, const int* e, Fe* out, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = fe_make(m[i], e[i]);
+}
+
 struct BusyGuard {
     mx_ctx* ctx;
     bool ok;
@@ -118,6 +141,12 @@ extern "C" int mx_ctx_create(int device, mx_ctx** out) {
     if (hipHostMalloc((void**)&ctx->cancel_host, sizeof(int), hipHostMallocMapped) != hipSuccess) return fail(MX_E_HIP);
     *ctx->cancel_host = 0;
     if (hipHostGetDevicePointer((void**)&ctx->cancel_dev, ctx->cancel_host, 0) != hipSuccess) return fail(MX_E_HIP);
+    // this device's copy of the no-alloc scratch area (the device is current)
+    if (hipGetSymbolAddress((void**)&ctx->noalloc_scratch, HIP_SYMBOL(scratch)) != hipSuccess) {
+        (void)hipGetLastError();
+        ctx->noalloc_scratch = nullptr;
+        ctx->set_err("no scratch area for mx_tile_render_noalloc on this device");
+    }
     *out = ctx;
     return MX_OK;
 }
@@ -684,6 +713,79 @@ extern "C" int mx_tile_render(mx_ctx* ctx, const mx_ref* ref, const mx_tile* til
     }
     mx_plan_destroy(plan);
     return rc;
+}
+
+extern "C" int mx_tile_render_noalloc(mx_ctx* ctx, const mx_ref* ref, const mx_tile* t, int32_t* n_esc, double* value, float* out,
+                                      uint64_t* stats4) {
+    if (!ctx || !ref || !t || ref->ctx != ctx) return MX_E_INVALID;
+    BusyGuard busy(ctx);
+    if (!busy.ok) return MX_E_BUSY;
+    if (t->form != MX_FORM_FE64 && t->form != MX_FORM_FE32 && t->form != MX_FORM_SC64) {
+        ctx->set_err("mx_tile_render_noalloc takes a floatexp form");
+        return MX_E_INVALID;
+    }
+    if (t->list || t->sample_dc_m || t->width < 1 || t->height < 1 || t->stride < 1 || t->max_iter < 1 || !t->dcol_m ||
+        !t->dcol_e || !t->drow_m || !t->drow_e || (t->bla && (ref->dev.nlev == 0 || t->bla_minlevel < 0))) {
+        ctx->set_err("mx_tile_render_noalloc takes a grid tile with its dc tables");
+        return MX_E_INVALID;
+    }
+    const size_t W = (size_t)t->width, H = (size_t)t->height, n = W * H;
+    if (n > (size_t)MX_NOALLOC_MAX_SAMPLES || W + H > (size_t)MX_NOALLOC_MAX_TABLE) {
+        ctx->set_err("tile beyond the no-alloc scratch limits (" + std::to_string(MX_NOALLOC_MAX_SAMPLES) + " samples, " +
+                     std::to_string(MX_NOALLOC_MAX_TABLE) + " columns + rows)");
+        return MX_E_INVALID;
+    }
+    if (!ctx->noalloc_scratch) {
+        ctx->set_err("no scratch area for mx_tile_render_noalloc on this device");
+        return MX_E_HIP;
+    }
+    if (__atomic_load_n(ctx->cancel_host, __ATOMIC_ACQUIRE)) {
+        ctx->set_err("cancelled");
+        return MX_E_CANCELLED;
+    }
+    unsigned char* b = ctx->noalloc_scratch;
+    MX_HIP(hipSetDevice(ctx->device));
+    hipStream_t st = ctx->stream;
+    // dc tables: columns then rows, packed into FloatExp records on the device
+    MX_HIP(hipMemcpyAsync(b + NA_DCM, t->dcol_m, 8 * W, hipMemcpyHostToDevice, st));
+    MX_HIP(hipMemcpyAsync(b + NA_DCM + 8 * W, t->drow_m, 8 * H, hipMemcpyHostToDevice, st));
+    MX_HIP(hipMemcpyAsync(b + NA_DCE, t->dcol_e, 4 * W, hipMemcpyHostToDevice, st));
+    MX_HIP(hipMemcpyAsync(b + NA_DCE + 4 * W, t->drow_e, 4 * H, hipMemcpyHostToDevice, st));
+    const int ntab = (int)(W + H);
+    hipLaunchKernelGGL(mx_pack_fe_kernel, dim3((ntab + 255) / 256), dim3(256), 0, st, (const double*)(b + NA_DCM), (const int*)(b + NA_DCE),
+                       (Fe*)(b + NA_DCF), ntab);
+    MX_HIP(hipGetLastError());
+    MxJob J;
+    J.R = ref->dev;
+    J.R.minlevel = t->bla_minlevel;
+    J.lev = ref->lev;
+    MxTileDev& T = J.T;
+    tile_fields(T, *t, *ref, W, H, false, false);
+    T.dcol = (const Fe*)(b + NA_DCF);
+    T.drow = (const Fe*)(b + NA_DCF) + W;
+    T.val = (double*)(b + NA_VAL);
+    T.nesc = (int*)(b + NA_NESC);
+    T.out = (float*)(b + NA_OUT);
+    T.queue = (unsigned*)(b + NA_QUEUE);
+    T.stats = (unsigned long long*)(b + NA_STATS);
+    T.cancel = ctx->cancel_dev;
+    MX_HIP(hipMemcpyAsync(b + NA_JOB, &J, sizeof J, hipMemcpyHostToDevice, st));
+    // the previous call's counters and outputs are cleared by a kernel
+    hipLaunchKernelGGL(mx_reset_kernel, dim3(1024), dim3(256), 0, st, T.queue, T.stats, (unsigned*)nullptr, T.val, T.nesc, T.out, n);
+    MX_HIP(hipGetLastError());
+    const int blocks = std::max(1, std::min(t->grid_blocks > 0 ? t->grid_blocks : ctx->cu_count * 8, T.ntiles));
+    const bool interior = (T.flags & MX_F_INTERIOR) != 0;
+    MX_HIP(launch_jobs(t->form, (const MxJob*)(b + NA_JOB), 1, t->ede != 0, interior, blocks, 256, st));
+    if (n_esc) MX_HIP(hipMemcpyAsync(n_esc, T.nesc, 4 * n, hipMemcpyDeviceToHost, st));
+    if (value) MX_HIP(hipMemcpyAsync(value, T.val, 8 * n, hipMemcpyDeviceToHost, st));
+    if (out) MX_HIP(hipMemcpyAsync(out, T.out, 4 * n, hipMemcpyDeviceToHost, st));
+    if (stats4) MX_HIP(hipMemcpyAsync(stats4, T.stats, 32, hipMemcpyDeviceToHost, st));
+    MX_HIP(hipStreamSynchronize(st));
+    if (__atomic_load_n(ctx->cancel_host, __ATOMIC_ACQUIRE)) {
+        ctx->set_err("cancelled");
+        return MX_E_CANCELLED;
+    }
+    return MX_OK;
 }
 
 namespace {

@@ -126,6 +126,7 @@ struct mx_frame {
 struct mx_ctx {
     int device = 0;
     int cu_count = 0;
+    unsigned char* noalloc_scratch = nullptr; // device address of this device's fixed scratch area
     hipStream_t stream = nullptr;
     int* cancel_host = nullptr;     // pinned, host-mapped flag
     int* cancel_dev = nullptr;      // device view of the same flag
