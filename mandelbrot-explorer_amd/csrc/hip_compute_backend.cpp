@@ -2,7 +2,7 @@
 // slot the D3D11 backend occupies in the engine (compute_backend_d3d11.cpp:209-635,
 // compute_backend.cpp:277-290).  Validation, busy and cancellation follow
 // CpuComputeBackend (compute_backend.cpp:44-128, 263-272); requests outside this path
-// (shallow or plain-double adaptive supersampling frames, Julia and Expression modes,
+// (plain-double adaptive supersampling frames, Julia and Expression modes,
 // other colorings) and device failures go to the CPU backend it was given, as the
 // D3D11 backend does (compute_backend_d3d11.cpp:289-315).
 #include <cstdlib>

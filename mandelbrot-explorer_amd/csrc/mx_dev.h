@@ -108,3 +108,23 @@ struct MxJob {
     MxTileDev T;
 };
 
+// Shallow direct path (views at scale <= 1e6): every sample iterates its own c in
+// double.  Base pass: the W x H grid in 8x8 tiles (nitems = 64 tiles); adaptive pass:
+// the sub*sub - 1 cells of each of the nlist flagged pixels in `list` (nitems =
+// nlist (sub*sub - 1)), outputs indexed by item.
+struct MxShallowDev {
+    double c0r, c0i, dx, dy; // truncated mpf grid values
+    int W, H, sub, mxit;
+    double esc2;
+    const int* list; // flagged pixels (row * W + column), null: the base grid
+    int nlist;
+    int tiles_x;
+    unsigned nitems;
+    int chunk, quorum;
+    double* val;
+    int* nesc;
+    float* out;
+    unsigned* queue;            // MX_NQUEUE counters, zeroed before launch
+    unsigned long long* stats;  // iterations
+    const volatile int* cancel; // host-mapped; nonzero => lanes stop taking work
+};

@@ -1079,6 +1079,149 @@ int compute_dbl(mx_ctx* ctx, mx::Frame& F, const mx_request* req, mx_frame_info*
     return MX_OK;
 }
 
+// One launch of the shallow kernel over `nitems` items (the base grid, or the cells of
+// the flagged pixels in `list`); outputs per item.
+int run_shallow(mx_ctx* ctx, MxShallowDev S, bool ede, const std::vector<int32_t>* list, size_t nout, std::vector<int32_t>& nesc,
+                std::vector<double>& val, uint64_t& iters, double& ms) {
+    const size_t o_val = 0, o_nesc = align_up(8 * nout), o_out = align_up(o_nesc + 4 * nout), o_q = align_up(o_out + 4 * nout);
+    const size_t o_st = align_up(o_q + 4 * MX_NQUEUE), o_list = align_up(o_st + 8 * 4);
+    const size_t bytes = align_up(o_list + (list ? 4 * list->size() : 0));
+    char* b = nullptr;
+    if (hipMalloc(&b, bytes) != hipSuccess) {
+        ctx->set_err("hipMalloc of the shallow frame failed");
+        return MX_E_NOMEM;
+    }
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    S.val = (double*)(b + o_val);
+    S.nesc = (int*)(b + o_nesc);
+    S.out = (float*)(b + o_out);
+    S.queue = (unsigned*)(b + o_q);
+    S.stats = (unsigned long long*)(b + o_st);
+    S.list = list ? (const int*)(b + o_list) : nullptr;
+    S.nlist = list ? (int)list->size() : 0;
+    S.cancel = ctx->cancel_dev;
+    auto run = [&]() -> hipError_t {
+        hipError_t e = hipEventCreate(&e0);
+        if (e == hipSuccess) e = hipEventCreate(&e1);
+        if (e == hipSuccess && list)
+            e = hipMemcpyAsync(b + o_list, list->data(), 4 * list->size(), hipMemcpyHostToDevice, ctx->stream);
+        if (e != hipSuccess) return e;
+        hipLaunchKernelGGL(mx_reset_kernel, dim3(1024), dim3(256), 0, ctx->stream, S.queue, S.stats, (unsigned*)nullptr, S.val, S.nesc,
+                           S.out, nout);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+        if ((e = hipEventRecord(e0, ctx->stream)) != hipSuccess) return e;
+        const int work = (int)std::min<size_t>((S.nitems + 63) / 64, (size_t)INT_MAX);
+        const int blocks = std::max(1, std::min(ctx->cu_count * 8, work));
+        if ((e = mx_launch_shallow(S, ede, blocks, ctx->stream)) != hipSuccess) return e;
+        if ((e = hipEventRecord(e1, ctx->stream)) != hipSuccess) return e;
+        if ((e = hipMemcpyAsync(nesc.data(), S.nesc, 4 * nout, hipMemcpyDeviceToHost, ctx->stream)) != hipSuccess) return e;
+        if ((e = hipMemcpyAsync(val.data(), S.val, 8 * nout, hipMemcpyDeviceToHost, ctx->stream)) != hipSuccess) return e;
+        unsigned long long st = 0;
+        if ((e = hipMemcpyAsync(&st, S.stats, 8, hipMemcpyDeviceToHost, ctx->stream)) != hipSuccess) return e;
+        if ((e = hipStreamSynchronize(ctx->stream)) != hipSuccess) return e;
+        iters = st;
+        float f = 0.f;
+        e = hipEventElapsedTime(&f, e0, e1);
+        ms = f;
+        return e;
+    };
+    const hipError_t e = run();
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    (void)hipFree(b);
+    if (e != hipSuccess) {
+        ctx->set_err(std::string("shallow path: ") + hipGetErrorString(e));
+        return MX_E_HIP;
+    }
+    if (__atomic_load_n(ctx->cancel_host, __ATOMIC_ACQUIRE)) {
+        ctx->set_err("cancelled");
+        return MX_E_CANCELLED;
+    }
+    return MX_OK;
+}
+
+// Shallow direct path (scale <= 1e6; Mandel::Compute's method 0,
+// mandel_perturbation.cpp:1444-1519): every pixel iterates its own c in double on the
+// device, then, with adaptive supersampling, the other cells of each flagged pixel
+// (1933-1965).  No reference orbit, series, BLA or interior probe.
+int compute_shallow(mx_ctx* ctx, mx::Frame& F, const mx_request* req, mx_frame_info* info, double t0) {
+    MxShallowDev S;
+    std::memset(&S, 0, sizeof S);
+    S.c0r = F.sh_c0r;
+    S.c0i = F.sh_c0i;
+    S.dx = F.sh_dx;
+    S.dy = F.sh_dy;
+    S.W = F.W;
+    S.H = F.H;
+    S.sub = F.sub;
+    S.mxit = F.max_iter;
+    S.esc2 = F.esc2;
+    S.tiles_x = (F.W + 7) / 8;
+    S.nitems = (unsigned)S.tiles_x * (unsigned)((F.H + 7) / 8) * 64u;
+    S.chunk = 64;
+    S.quorum = 32;
+    const size_t n = (size_t)F.W * F.H;
+    std::vector<int32_t> nesc(n);
+    std::vector<double> val(n);
+    uint64_t iters = 0;
+    double kernel_ms = 0.0;
+    int rc = run_shallow(ctx, S, F.ede, nullptr, n, nesc, val, iters, kernel_ms);
+    if (rc != MX_OK) return rc;
+    const CellMap cm{F.W, F.sub};
+    uint64_t nominal = 0;
+    for (int i = 0; i < F.H; ++i)
+        for (int j = 0; j < F.W; ++j) {
+            const size_t k = (size_t)i * F.W + j, c = cm(i, j, 0, 0);
+            req->iterations[c] = (float)val[k];
+            if (req->values) req->values[c] = val[k];
+            if (req->n_esc) req->n_esc[c] = nesc[k];
+            nominal += nesc[k] > 0 ? (uint64_t)nesc[k] : (uint64_t)F.max_iter;
+        }
+    int ss_pixels = 0;
+    double ss_ms = 0.0;
+    if (F.ss) {
+        const std::vector<int32_t> flagged = ss_flags(F, req->iterations, cm);
+        ss_pixels = (int)flagged.size();
+        const int SS = F.sub, h = SS / 2, per = SS * SS - 1;
+        const size_t m = flagged.size() * (size_t)per;
+        if (m > 0) {
+            if (m > (size_t)UINT_MAX) return MX_E_INVALID;
+            MxShallowDev T = S;
+            T.nitems = (unsigned)m;
+            std::vector<int32_t> sn(m);
+            std::vector<double> sv(m);
+            uint64_t it2 = 0;
+            rc = run_shallow(ctx, T, F.ede, &flagged, m, sn, sv, it2, ss_ms);
+            if (rc != MX_OK) return rc;
+            iters += it2;
+            for (size_t q = 0; q < m; ++q) {
+                const int pix = flagged[q / per];
+                int t = (int)(q % per);
+                if (t >= h * SS + h) ++t;
+                const size_t c = cm(pix / F.W, pix % F.W, t % SS - h, t / SS - h);
+                req->iterations[c] = (float)sv[q];
+                if (req->values) req->values[c] = sv[q];
+                if (req->n_esc) req->n_esc[c] = sn[q];
+                nominal += sn[q] > 0 ? (uint64_t)sn[q] : (uint64_t)F.max_iter;
+            }
+        }
+    }
+    if (info) {
+        std::memset(info, 0, sizeof *info);
+        info->path = 0;
+        info->ss_pixels = ss_pixels;
+        info->ss_ms = ss_ms;
+        info->precision_bits = (int32_t)F.prec;
+        info->kernel_ms = kernel_ms;
+        info->total_seconds = now_s() - t0;
+        info->steps = iters;
+        info->nominal_iterations = nominal;
+        info->passes = 1;
+        info->ref_virtual = 1;
+    }
+    return MX_OK;
+}
+
 } // namespace
 
 extern "C" int mx_compute(mx_ctx* ctx, const mx_request* req, mx_frame_info* info) {
@@ -1096,6 +1239,7 @@ extern "C" int mx_compute(mx_ctx* ctx, const mx_request* req, mx_frame_info* inf
         ctx->set_err(err);
         return rc;
     }
+    if (F.shallow) return compute_shallow(ctx, F, req, info, t0);
     if (F.dbl) return compute_dbl(ctx, F, req, info, t0);
     if (req->form != MX_FORM_FE64 && req->form != MX_FORM_FE32 && req->form != MX_FORM_SC64) {
         ctx->set_err("form must be MX_FORM_FE64, MX_FORM_FE32 or MX_FORM_SC64 on the floatexp path");

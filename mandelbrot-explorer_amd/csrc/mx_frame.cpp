@@ -1106,12 +1106,10 @@ int Frame::prepare(const mx_request& r, std::string& err) {
         err = "could not parse centre/scale";
         return MX_E_INVALID;
     }
-    // path selection (mandel_perturbation.cpp:1434, 1569-1576)
-    if (mpf_cmp_d(scale, 1e6) <= 0) {
-        err = "scale <= 1e6: the engine's shallow (non-perturbation) path is not on this path";
-        return MX_E_UNSUPPORTED;
-    }
-    dbl = mpf_cmp_d(scale, 1e280) <= 0 && !(r.options & MX_OPT_FLOATEXP);
+    // path selection (mandel_perturbation.cpp:1434-1435, 1569-1576): the shallow direct
+    // path at scale <= 1e6, else perturbation in double up to 1e280, else floatexp
+    shallow = mpf_cmp_d(scale, 1e6) <= 0;
+    dbl = !shallow && mpf_cmp_d(scale, 1e280) <= 0 && !(r.options & MX_OPT_FLOATEXP);
     if (dbl && ss) {
         err = "adaptive supersampling on the plain-double path (scale <= 1e280) is not on this path "
               "(MX_OPT_FLOATEXP runs such frames on the floatexp path)";
@@ -1146,6 +1144,16 @@ int Frame::prepare(const mx_request& r, std::string& err) {
             mpf_mul_ui(off, dy.v, r.row_base);
             mpf_add(c0_im.v, c0_im.v, off);
         }
+    }
+    if (shallow) {
+        // the method-0 loops take c0, dx, dy as doubles (mandel_perturbation.cpp:1445-1448)
+        sh_c0r = mpf_get_ld(c0_re.v);
+        sh_c0i = mpf_get_ld(c0_im.v);
+        sh_dx = mpf_get_ld(dx.v);
+        sh_dy = mpf_get_ld(dy.v);
+        interior = false;
+        prepare_seconds = now_s() - t_start;
+        return MX_OK;
     }
     double m;
     long e;
@@ -1434,6 +1442,7 @@ extern "C" int mx_frame_prepare(const mx_request* req, mx_frame** out, mx_frame_
     if (!f) return MX_E_NOMEM;
     std::string err;
     int rc = f->impl.prepare(*req, err);
+    if (rc == MX_OK && f->impl.shallow) rc = MX_E_UNSUPPORTED; // no reference: mx_compute runs such views
     if (rc != MX_OK) {
         delete f;
         return rc;

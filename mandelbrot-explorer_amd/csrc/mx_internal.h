@@ -75,6 +75,10 @@ struct Frame {
 
     double ref_seconds = 0.0, bla_seconds = 0.0, prepare_seconds = 0.0;
 
+    // shallow direct path (scale <= 1e6): c0, dx, dy truncated to double (mpf_get_ld)
+    bool shallow = false;
+    double sh_c0r = 0.0, sh_c0i = 0.0, sh_dx = 0.0, sh_dy = 0.0;
+
     // plain-double path (1e6 < scale <= 1e280): pixel references, the series
     // approximation, the glitch metric's orbit magnitudes and double dc tables
     bool dbl = false;
@@ -187,3 +191,4 @@ hipError_t mx_launch_fe64(const MxJob* jobs, int njobs, bool ede, bool interior,
 hipError_t mx_launch_fe32(const MxJob* jobs, int njobs, bool ede, bool interior, int blocks, int threads, hipStream_t stream);
 hipError_t mx_launch_sc64(const MxJob* jobs, int njobs, bool ede, bool interior, int blocks, int threads, hipStream_t stream);
 hipError_t mx_launch_dbl(const MxRefDev& R, const int2* lev, const MxTileDev& T, int blocks, int threads, hipStream_t stream);
+hipError_t mx_launch_shallow(const MxShallowDev& S, bool ede, int blocks, hipStream_t stream);

@@ -2045,6 +2045,141 @@ MXO_EXPORT void mxo_direct_list(const char* cre, const char* cim, const char* sc
     grid_clear(&g);
 }
 
+/* ------------------------------------------------------------------------- */
+/* shallow direct path (scale <= 1e6)                                         */
+/* ------------------------------------------------------------------------- */
+
+/* floatPointCompute for smooth and exterior-DE colouring (MP.cpp:330-399): z = c,
+ * d = 2 (dz/dz0, the interior test), dc = 1 (dz/dc, EDE); per iteration all three from
+ * the old z; escape |z|^2 > esc2 first, then |d|^2 < 1e-9 (interior), and -2 once the
+ * counter i reaches mxit.  Returns the double the engine casts to float: the smooth
+ * count i + 1 - log2(log(|z|^2)/2/log 2), or the raw distance estimate, -2 interior
+ * (n = escape count i + 1, -2 interior, -1 max_iter). */
+static double shallow_scalar(double c_re, double c_im, int mxit, double esc2, int ede, int* n) {
+    double z_re = c_re, z_im = c_im, d_re = 2, d_im = 0, dc_re = 1, dc_im = 0, tmp;
+    int i = 1;
+    while (i < mxit) {
+        tmp = 2.0 * (d_re * z_re - d_im * z_im);
+        d_im = 2.0 * (d_re * z_im + d_im * z_re);
+        d_re = tmp;
+        if (ede) {
+            tmp = 2.0 * (dc_re * z_re - dc_im * z_im) + 1;
+            dc_im = 2.0 * (dc_re * z_im + dc_im * z_re);
+            dc_re = tmp;
+        }
+        tmp = z_re * z_re - z_im * z_im + c_re;
+        z_im = 2.0 * z_re * z_im + c_im;
+        z_re = tmp;
+        tmp = z_re * z_re + z_im * z_im;
+        if (tmp > esc2) {
+            *n = i + 1;
+            if (ede) return sqrt(tmp) * log(tmp) / sqrt(dc_re * dc_re + dc_im * dc_im);
+            return i + 1 - log(log(tmp) / 2 / log(2)) / log(2);
+        }
+        tmp = d_re * d_re + d_im * d_im;
+        if (tmp < 0.000000001) {
+            *n = -2;
+            return -2;
+        }
+        ++i;
+    }
+    *n = -1;
+    return -2;
+}
+
+/* One lane of solveShallowSimdList (MP.cpp:416-532), the engine's default for these
+ * colourings: the same recurrences with the counter jv kept in double and the three
+ * exits tested after each iteration (escape, then interior, then jv + 1 >= mxit).
+ * Returns the value before the lane rounds it to float for its float output.  For
+ * mxit >= 2 its values equal shallow_scalar's (the engine checks that on four points,
+ * verify.cpp:9980-9990). */
+static double shallow_lane(double cr, double ci, int mxit, double esc2, int ede, int* n) {
+    const double LG2 = log(2.0);
+    double zr = cr, zi = ci, dr = 2, di = 0, dcr = 1, dci = 0, jv = 1;
+    for (;;) {
+        const double ndr = 2.0 * (dr * zr - di * zi), ndi = 2.0 * (dr * zi + di * zr);
+        const double ndcr = 2.0 * (dcr * zr - dci * zi) + 1.0, ndci = 2.0 * (dcr * zi + dci * zr);
+        const double nzr = (zr * zr - zi * zi) + cr, nzi = 2.0 * (zr * zi) + ci;
+        zr = nzr;
+        zi = nzi;
+        dr = ndr;
+        di = ndi;
+        dcr = ndcr;
+        dci = ndci;
+        const double zrad = zr * zr + zi * zi, dmag = dr * dr + di * di, jnext = jv + 1.0;
+        if (zrad > esc2) {
+            *n = (int)jv + 1;
+            if (ede) return sqrt(zrad) * log(zrad) / sqrt(dcr * dcr + dci * dci);
+            return jv + 1 - log(log(zrad) / 2 / LG2) / LG2;
+        }
+        if (dmag < 1e-9) {
+            *n = -2;
+            return -2.0;
+        }
+        if (jnext >= (double)mxit) {
+            *n = -1;
+            return -2.0;
+        }
+        jv = jnext;
+    }
+}
+
+/* Both loops on explicit points (scalar = 1: floatPointCompute, 0: the lane); val is
+ * the unrounded value. */
+MXO_EXPORT void mxo_shallow_points(const double* cr, const double* ci, int n, int mxit, double esc2, int ede, int scalar, double* val, int32_t* nesc) {
+    for (int k = 0; k < n; ++k) {
+        int ne = 0;
+        val[k] = scalar ? shallow_scalar(cr[k], ci[k], mxit, esc2, ede, &ne) : shallow_lane(cr[k], ci[k], mxit, esc2, ede, &ne);
+        nesc[k] = ne;
+    }
+}
+
+/* A shallow view (Mandel::Compute's method 0, MP.cpp:1444-1519): c0, dx, dy from the
+ * engine's mpf grid truncated to double (mpf_get_ld); base pixel (i, j) at
+ * c0 + (dx j, dy i); with list != NULL the adaptive pass instead (MP.cpp:1933-1965): the
+ * sub*sub - 1 cells of each listed pixel (row-major index), xi outer and yi inner,
+ * at (c0r + dx j) + dx xi / sub, (c0i + dy i) + dy yi / sub.  Outputs per sample: val
+ * is a double whose float is the engine's output (the smooth value; under EDE the
+ * lane's float distance divided by dx in double, MP.cpp:1497-1499, 1952-1954), nesc as
+ * shallow_scalar's n.  grid4 (optional) receives c0r, c0i, dx, dy. */
+MXO_EXPORT void mxo_shallow(const char* cre, const char* cim, const char* scale, int W, int H, int full_h, int row_base, int prec_bits, int mxit, float escape_radius, int ede, int sub, const int32_t* list, int nlist, int threads, double* grid4, double* val, int32_t* nesc) {
+    mxo_grid g;
+    grid_init(&g, cre, cim, scale, W, H, full_h, row_base, mpf_bits(prec_bits), 1);
+    const double c0r = mpf_get_ld_c(g.fc0_re), c0i = mpf_get_ld_c(g.fc0_im), dx = mpf_get_ld_c(g.fdx), dy = mpf_get_ld_c(g.fdy);
+    grid_clear(&g);
+    if (grid4) {
+        grid4[0] = c0r;
+        grid4[1] = c0i;
+        grid4[2] = dx;
+        grid4[3] = dy;
+    }
+    const double esc2 = (double)(escape_radius * escape_radius);
+    const int per = sub * sub - 1, h = sub / 2;
+    const long total = list ? (long)nlist * per : (long)W * H;
+    if (threads > 0) omp_set_num_threads(threads);
+#pragma omp parallel for schedule(dynamic, 64)
+    for (long q = 0; q < total; ++q) {
+        double cr, ci;
+        if (list) {
+            const int pix = list[q / per];
+            int t = (int)(q % per);
+            if (t >= h * sub + h) ++t;
+            const int xi = t / sub - h, yi = t % sub - h, i = pix / W, j = pix % W;
+            cr = c0r + dx * j + dx * xi / sub;
+            ci = c0i + dy * i + dy * yi / sub;
+        } else {
+            const int i = (int)(q / W), j = (int)(q % W);
+            cr = c0r + dx * j;
+            ci = c0i + dy * i;
+        }
+        int ne = 0;
+        double v = shallow_lane(cr, ci, mxit, esc2, ede, &ne);
+        if (ede && ne > 0) v = (double)(float)v / dx;
+        val[q] = v;
+        nesc[q] = ne;
+    }
+}
+
 /* Brute-force single point from decimal strings. */
 MXO_EXPORT double mxo_point(const char* cre, const char* cim, int wp_bits, int mxit, int ede, int track_d, double esc2, const char* dx, int* nesc) {
     mpfr_t a, b, d;

@@ -89,6 +89,11 @@ def lib():
         L.mxo_direct_list.argtypes = [C.c_char_p, C.c_char_p, C.c_char_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
                                       C.c_int, C.c_float, C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p]
         L.mxo_point.argtypes = [C.c_char_p, C.c_char_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, C.c_char_p, C.c_void_p]
+        L.mxo_shallow_points.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_double, C.c_int, C.c_int, C.c_void_p,
+                                         C.c_void_p]
+        L.mxo_shallow.argtypes = [C.c_char_p, C.c_char_p, C.c_char_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                                  C.c_float, C.c_int, C.c_int, C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_void_p,
+                                  C.c_void_p]
         L.mxo_point.restype = C.c_double
         L.mxo_grid_info.argtypes = [C.c_char_p, C.c_char_p, C.c_char_p] + [C.c_int] * 5 + [C.c_void_p] * 4
         _LIB = L
@@ -423,6 +428,34 @@ def direct_list(cre, cim, scale, W, H, mxit, idx, c_method=0, escape_radius=1e8,
     lib().mxo_direct_list(cre.encode(), cim.encode(), scale.encode(), W, H, pb, wp_bits or pb, mxit, c_method,
                           escape_radius, _p(idx), len(idx), threads or os.cpu_count(), int(track_d), _p(val), _p(nesc))
     return val, nesc
+
+
+def shallow_points(cr, ci, mxit, escape_radius=1e8, ede=False, scalar=False):
+    """The shallow path's per-sample loop on explicit c (double): floatPointCompute
+    (scalar) or one lane of solveShallowSimdList -> (unrounded value, n_esc)."""
+    cr = np.ascontiguousarray(cr, np.float64)
+    ci = np.ascontiguousarray(ci, np.float64)
+    val = np.zeros(len(cr))
+    nesc = np.zeros(len(cr), np.int32)
+    esc2 = float(np.float32(escape_radius) * np.float32(escape_radius))
+    lib().mxo_shallow_points(_p(cr), _p(ci), len(cr), mxit, esc2, int(ede), int(scalar), _p(val), _p(nesc))
+    return val, nesc
+
+
+def shallow(cre, cim, scale, W, H, mxit, ede=False, sub=1, flagged=None, escape_radius=1e8, full_h=0, row_base=0,
+            prec_bits=None, threads=None):
+    """A shallow view (scale <= 1e6) as the engine's method 0 computes it -> (val, n_esc,
+    (c0r, c0i, dx, dy)); val's float is the engine's output.  With `flagged` (row-major
+    pixel indices) the adaptive pass: sub*sub - 1 cells per pixel, xi outer, yi inner."""
+    lst = None if flagged is None else np.ascontiguousarray(flagged, np.int32)
+    n = W * H if lst is None else len(lst) * (sub * sub - 1)
+    val = np.zeros(n)
+    nesc = np.zeros(n, np.int32)
+    g4 = np.zeros(4)
+    lib().mxo_shallow(cre.encode(), cim.encode(), scale.encode(), W, H, full_h, row_base, prec_bits or precision_bits(scale),
+                      mxit, escape_radius, int(ede), sub, None if lst is None else _p(lst), 0 if lst is None else len(lst),
+                      threads or os.cpu_count(), _p(g4), _p(val), _p(nesc))
+    return val, nesc, tuple(g4)
 
 
 def point(cre, cim, wp_bits, mxit, esc2=float(np.float32(1e8) * np.float32(1e8)), ede=False, track_d=True, dx="1"):

@@ -121,17 +121,27 @@ int main(int argc, char** argv) {
             if (!be->compute(s)) return fail("strip");
             if (!same_bits(strip.data(), full.data() + (size_t)8 * W, strip.size())) return fail("strip differs from the full frame");
         }
-        // requests off this path go to the CPU backend (compute_backend_d3d11.cpp:301-314):
-        // the engine's shallow path, the Julia mode, an orbit-trap colouring, adaptive
-        // supersampling on the plain-double path
+        // a shallow view (scale <= 1e6, the engine's method 0) runs on the device
         Mpf shallow("1000", 256), mid("1e100", 512);
+        {
+            std::vector<float> buf((size_t)W * H, -10.f);
+            ComputeRequest r = q;
+            r.scale = shallow.v;
+            r.iterations = buf.data();
+            const int before = cpu->calls;
+            if (!be->compute(r) || cpu->calls != before || !be->lastComputeUsedGpuPath()) return fail("shallow view");
+            for (float v : buf)
+                if (v == -10.f || v == 7.0f) return fail("shallow view left a pixel");
+        }
+        // requests off this path go to the CPU backend (compute_backend_d3d11.cpp:301-314):
+        // the Julia mode, an orbit-trap colouring, adaptive supersampling on the
+        // plain-double path
         struct Off {
             const char* what;
             mpf_ptr scale;
             ComputeMode mode;
             int method, sub;
-        } off[] = {{"shallow", shallow.v, ComputeMode::Mandelbrot, 0, 1},
-                   {"julia", sc.v, ComputeMode::Julia, 0, 1},
+        } off[] = {{"julia", sc.v, ComputeMode::Julia, 0, 1},
                    {"orbit trap", sc.v, ComputeMode::Mandelbrot, 32, 1},
                    {"double-path supersampling", mid.v, ComputeMode::Mandelbrot, 1, 3}};
         for (const Off& o : off) {
