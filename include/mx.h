@@ -281,7 +281,8 @@ typedef struct {
     int32_t ref_virtual;     /* exact view-centre reference */
     double ref_x, ref_y;     /* reference position in pixels */
     int32_t interior_used;   /* interior detection after the probe gate */
-    int32_t cutoff_sensitive;/* reference escapes within 16 of mxit (flag only) */
+    int32_t cutoff_sensitive;/* view-centre reference escaping within 16 of mxit, or bounded
+                                but escaping by mxit + 64: the fallback below ran */
     int32_t precision_bits;
     int32_t bla_levels;
     double ref_seconds;      /* host GMP orbit */
@@ -300,6 +301,9 @@ typedef struct {
     double ss_ms;            /* their sub-sample pass (kernel) */
     int32_t period;          /* > 0: the reference was the period-`period` nucleus in view */
     int32_t subpixel;        /* a nucleus was found but spans < 4 pixels: BLA was turned off */
+    int32_t gmp_fallback;    /* cutoff-sensitive frame: cells whose value was negative or above
+                                mxit - 16, recomputed in arbitrary precision on the host
+                                (mandel_perturbation.cpp:2173-2183) */
 } mx_frame_info;
 
 int mx_compute(mx_ctx* ctx, const mx_request* req, mx_frame_info* info);

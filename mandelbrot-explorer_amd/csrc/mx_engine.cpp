@@ -1079,6 +1079,55 @@ int compute_dbl(mx_ctx* ctx, mx::Frame& F, const mx_request* req, mx_frame_info*
     return MX_OK;
 }
 
+// The cutoff-sensitive frame's brute-force pass (finalize, mandel_perturbation.cpp:
+// 2173-2183): every computed cell whose float value is negative or above max_iter - 16
+// (the classification the double-precision deltas cannot be trusted with when the
+// reference itself escapes at the cutoff) is recomputed in arbitrary precision at its
+// own c (accuratePixelCompute, 896-918), on the host's cores as the engine does.
+int cutoff_fallback(mx::Frame& F, const mx_request* req, const CellMap& cm, const std::vector<size_t>& ss_cells, int& count) {
+    struct Job {
+        size_t c;
+        int i, j, u, v;
+    };
+    std::vector<Job> jobs;
+    auto consider = [&](size_t c, int i, int j, int u, int v) {
+        const float x = req->iterations[c];
+        if (x < 0 || x > F.max_iter - 16) jobs.push_back({c, i, j, u, v});
+    };
+    for (int i = 0; i < F.H; ++i)
+        for (int j = 0; j < F.W; ++j) consider(cm(i, j, 0, 0), i, j, 0, 0);
+    const int S = F.sub, h = S / 2;
+    const size_t rowlen = (size_t)F.W * S;
+    for (size_t c : ss_cells) {
+        const size_t row = c / rowlen, col = c % rowlen;
+        consider(c, (int)(row / S), (int)(col / S), (int)(row % S) - h, (int)(col % S) - h);
+    }
+    count = (int)jobs.size();
+    if (jobs.empty()) return MX_OK;
+    std::atomic<size_t> next{0};
+    std::atomic<bool> halted{false};
+    auto work = [&]() {
+        for (size_t q; (q = next.fetch_add(1)) < jobs.size();) {
+            const Job& jb = jobs[q];
+            int n = 0;
+            const double r = F.pixel_value(jb.i, jb.j, jb.u, jb.v, n);
+            if (n == INT_MIN) {
+                halted.store(true);
+                return;
+            }
+            req->iterations[jb.c] = n < 0 ? -2.0f : (float)r;
+            if (req->values) req->values[jb.c] = n < 0 ? -2.0 : r;
+            if (req->n_esc) req->n_esc[jb.c] = n;
+        }
+    };
+    const size_t nt = std::min<size_t>(jobs.size(), std::max(1u, std::thread::hardware_concurrency()));
+    std::vector<std::thread> pool;
+    for (size_t k = 1; k < nt; ++k) pool.emplace_back(work);
+    work();
+    for (auto& t : pool) t.join();
+    return halted.load() ? MX_E_CANCELLED : MX_OK;
+}
+
 // One launch of the shallow kernel over `nitems` items (the base grid, or the cells of
 // the flagged pixels in `list`); outputs per item.
 int run_shallow(mx_ctx* ctx, MxShallowDev S, bool ede, const std::vector<int32_t>* list, size_t nout, std::vector<int32_t>& nesc,
@@ -1233,6 +1282,7 @@ extern "C" int mx_compute(mx_ctx* ctx, const mx_request* req, mx_frame_info* inf
     mx_frame frame;
     mx::Frame& F = frame.impl;
     F.halt = ctx->cancel_host;
+    F.check_cutoff = true;
     std::string err;
     int rc = F.prepare(*req, err);
     if (rc != MX_OK) {
@@ -1350,6 +1400,7 @@ extern "C" int mx_compute(mx_ctx* ctx, const mx_request* req, mx_frame_info* inf
         }
     int ss_pixels = 0;
     double ss_ms = 0.0;
+    std::vector<size_t> ss_cells;
     if (F.ss) {
         // adaptive supersampling (mandel_perturbation.cpp:1789-1981): flag the pixels
         // that differ from a neighbour, then every other cell of each flagged pixel in
@@ -1405,10 +1456,20 @@ extern "C" int mx_compute(mx_ctx* ctx, const mx_request* req, mx_frame_info* inf
                 nominal += sn[k] > 0 ? (uint64_t)sn[k] : (uint64_t)F.max_iter;
             }
             for (int k = 0; k < 4; ++k) st[k] += st2[k];
+            ss_cells = std::move(cells);
+        }
+    }
+    int gmp_fallback = 0;
+    if (F.cutoff_sensitive) {
+        rc = cutoff_fallback(F, req, cm, ss_cells, gmp_fallback);
+        if (rc != MX_OK) {
+            ctx->set_err("cancelled");
+            return rc;
         }
     }
     if (info) {
         std::memset(info, 0, sizeof *info);
+        info->gmp_fallback = gmp_fallback;
         info->ss_pixels = ss_pixels;
         info->ss_ms = ss_ms;
         info->path = 2;

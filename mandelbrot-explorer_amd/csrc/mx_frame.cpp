@@ -698,6 +698,10 @@ double Frame::point_value_serial(mpf_srcptr c_re, mpf_srcptr c_im, int mxit, boo
     double res = -2.0;
     n_out = -1;
     for (int i = 1; i < mxit; ++i) {
+        if ((i & 255) == 0 && halted()) { // the engine's cancellation point (mandel_perturbation.cpp:778-782)
+            n_out = INT_MIN;
+            return -10.0;
+        }
         mpf_mul(t1, dr, zi);
         mpf_mul(dr, dr, zr);
         mpf_mul(t2, di, zi);
@@ -813,7 +817,7 @@ double Frame::point_value(mpf_srcptr c_re, mpf_srcptr c_im, int mxit, bool ede, 
     (void)std::frexp(esc2, &eth); // esc2 >= 2^(eth-1)
     const long last = mxit - 1;   // iterations 1 .. mxit-1
     double rad_esc = 0.0;
-    bool undecided = false;
+    bool undecided = false, stopped_by_halt = false;
 
     // the EDE derivative dc, one thread per component: iteration i of either reads
     // both components of iteration i-1 (buffers by parity) and starts once the other
@@ -888,6 +892,11 @@ double Frame::point_value(mpf_srcptr c_re, mpf_srcptr c_im, int mxit, bool ede, 
         double dm = 4.0; // |d|^2 = dm * 2^de, d_0 = 2
         long de = 0;
         for (long i = 1; i <= last; ++i) {
+            if ((i & 1023) == 0 && halted()) {
+                stopped_by_halt = true;
+                Q.d_at.store(i, std::memory_order_release); // stops the derivative threads
+                break;
+            }
             // |d_i|^2 = 4 |d_{i-1}|^2 |z_{i-1}|^2
             {
                 const int sr = mpf_sgn(zr.v), si = mpf_sgn(zi.v);
@@ -951,6 +960,10 @@ double Frame::point_value(mpf_srcptr c_re, mpf_srcptr c_im, int mxit, bool ede, 
         te0.join();
         te1.join();
     }
+    if (stopped_by_halt) {
+        n_out = INT_MIN;
+        return -10.0;
+    }
     if (undecided) return point_value_serial(c_re, c_im, mxit, ede, n_out);
 
     const long ie = Q.esc_at.load(), id = Q.d_at.load();
@@ -982,6 +995,39 @@ double Frame::point_value(mpf_srcptr c_re, mpf_srcptr c_im, int mxit, bool ede, 
         n_out = -2;
     }
     return res;
+}
+
+// A view-centre reference makes the frame cutoff-sensitive when it escapes within 16
+// of max_iter (mandel_perturbation.cpp:4144) or, bounded, when the brute force sees
+// it escape within max_iter + 64 (4171-4178); the second test runs the whole orbit
+// again, so only when the caller applies the fallback (check_cutoff).
+bool Frame::virtual_ref_sensitive() {
+    if (ref_escaped) return ref_it >= max_iter - 16;
+    if (!check_cutoff || halted()) return false;
+    int n = 0;
+    return point_value(ref_re.v, ref_im.v, max_iter + 64, false, n) >= 0.0 && n > 0;
+}
+
+// accuratePixelCompute (mandel_perturbation.cpp:896-918): the brute-force value of
+// cell (u, v) of pixel (i, j) at the frame's precision, c formed in mpf as the engine
+// does (c0 + d k, then the signed sub-cell offset d |u| / sub).
+double Frame::pixel_value(int i, int j, int u, int v, int& n_out) {
+    Gmp cre(prec), cim(prec), off(prec);
+    mpf_set(cre, c0_re.v);
+    mpf_set(cim, c0_im.v);
+    mpf_mul_ui(off, dx.v, (unsigned long)j);
+    mpf_add(cre, cre, off);
+    mpf_mul_ui(off, dx.v, (unsigned long)std::abs(v));
+    if (v < 0) mpf_neg(off, off);
+    mpf_div_ui(off, off, (unsigned long)sub);
+    mpf_add(cre, cre, off);
+    mpf_mul_ui(off, dy.v, (unsigned long)i);
+    mpf_add(cim, cim, off);
+    mpf_mul_ui(off, dy.v, (unsigned long)std::abs(u));
+    if (u < 0) mpf_neg(off, off);
+    mpf_div_ui(off, off, (unsigned long)sub);
+    mpf_add(cim, cim, off);
+    return point_value_serial(cre, cim, max_iter, ede, n_out);
 }
 
 // buildBLA (mandel_perturbation.cpp:2613-2709) without the EDE derivative couplings,
@@ -1215,7 +1261,7 @@ int Frame::prepare(const mx_request& r, std::string& err) {
             ref_x = (W - 1) * 0.5;
             ref_y = (H - 1) * 0.5;
             build_orbit(pre, pim);
-            cutoff_sensitive = ref_escaped && ref_it >= max_iter - 16;
+            cutoff_sensitive = virtual_ref_sensitive();
         } else {
             build_orbit(pre, pim);
         }
@@ -1232,7 +1278,7 @@ int Frame::prepare(const mx_request& r, std::string& err) {
                 ref_x = (W - 1) * 0.5;
                 ref_y = (H - 1) * 0.5;
                 build_orbit(pre, pim);
-                cutoff_sensitive = ref_escaped && ref_it >= max_iter - 16;
+                cutoff_sensitive = virtual_ref_sensitive();
             }
         }
         // value of the reference pixel itself (mandel_perturbation.cpp:4145-4160, 4179-4185)

@@ -100,6 +100,11 @@ struct Frame {
     // ref_pixel_n / ref_pixel_value are final after this (the floatexp path computes
     // an EDE reference pixel on a host thread while the frame's kernels run)
     void ref_pixel_wait();
+    // the cutoff-sensitive fallback: run the bounded view-centre reference's brute-force
+    // test in prepare() (set by mx_compute, which applies the fallback), and the
+    // brute-force value of one output cell (thread-safe)
+    bool check_cutoff = false;
+    double pixel_value(int i, int j, int u, int v, int& n_out);
 
   private:
     std::thread ref_job;
@@ -116,6 +121,7 @@ struct Frame {
     void sa_step(int i);
     int prepare_dbl(mpf_srcptr scale, std::string& err);
     bool tail_escapes(int extra);
+    bool virtual_ref_sensitive();
     double point_value(mpf_srcptr c_re, mpf_srcptr c_im, int mxit, bool ede, int& n_out);
     double point_value_serial(mpf_srcptr c_re, mpf_srcptr c_im, int mxit, bool ede, int& n_out);
     void build_bla(double dcmax);

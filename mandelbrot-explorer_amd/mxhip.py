@@ -61,7 +61,7 @@ class FrameInfo(C.Structure):
                 ("steps", C.c_uint64), ("bla_applies", C.c_uint64), ("bla_skipped", C.c_uint64),
                 ("rebases", C.c_uint64), ("nominal_iterations", C.c_uint64), ("passes", C.c_int32),
                 ("sa_it", C.c_int32), ("escalated", C.c_int32), ("ss_pixels", C.c_int32), ("ss_ms", C.c_double),
-                ("period", C.c_int32), ("subpixel", C.c_int32)]
+                ("period", C.c_int32), ("subpixel", C.c_int32), ("gmp_fallback", C.c_int32)]
 
 
 class FrameView(C.Structure):

@@ -1247,6 +1247,10 @@ typedef struct {
     int no_periodic;
     int period;
     int subpixel;
+    /* outputs: the frame is cutoff-sensitive (MP.cpp:4144, 4177) and how many samples
+     * its brute-force fallback recomputed (MP.cpp:2173-2183) */
+    int cutoff_sensitive;
+    int gmp_fallback;
 } mxo_frame;
 
 /* output cell of sample (i, j, u, v) (getIndex, MP.cpp:4334-4340) */
@@ -1772,6 +1776,55 @@ MXO_EXPORT int mxo_compute_fe(mxo_frame* f) {
             free(pix);
             free(flags);
         }
+    }
+    /* cutoff-sensitive frames: a view-centre reference that escapes within 16 of
+     * max_iter (MP.cpp:4144), or a bounded one whose brute force escapes within
+     * max_iter + 64 (MP.cpp:4177); every computed sample whose float value is negative
+     * or above max_iter - 16 is then recomputed by the brute force at its own c
+     * (finalize, MP.cpp:2173-2183; accuratePixelCompute, MP.cpp:896-918) */
+    f->cutoff_sensitive = f->cutoff_sensitive_hint == 1;
+    if (f->cutoff_sensitive_hint == 2) {
+        int ne;
+        f->cutoff_sensitive = point_mpfr(rre, rim, pb, f->mxit + 64, 0, 1, esc2, g.dx, &ne) >= 0;
+    }
+    f->gmp_fallback = 0;
+    if (f->cutoff_sensitive) {
+        const int S = f->sub > 1 ? f->sub : 1, h = S / 2;
+        const long ncell = (long)W * H * S * S;
+        int nfb = 0;
+#pragma omp parallel for schedule(dynamic, 1) num_threads(f->threads) reduction(+ : nfb)
+        for (long c = 0; c < ncell; ++c) {
+            const int row = (int)(c / ((long)W * S)), col = (int)(c % ((long)W * S));
+            const int i = row / S, j = col / S, yi = row % S - h, xi = col % S - h;
+            const int base = xi == 0 && yi == 0;
+            float v;
+            if (base)
+                v = f->out[i * W + j];
+            else if (f->sub > 1 && f->cell_nesc[c] != INT32_MIN)
+                v = f->cells[c];
+            else
+                continue;
+            if (!(v < 0 || v > f->mxit - 16)) continue;
+            mpfr_t cr, ci;
+            mpfr_inits2(pb, cr, ci, (mpfr_ptr)0);
+            grid_pixel_c(&g, i, j, yi, xi, S, cr, ci);
+            int ne;
+            const double r = point_mpfr(cr, ci, pb, f->mxit, ede, 1, esc2, g.dx, &ne);
+            mpfr_clears(cr, ci, (mpfr_ptr)0);
+            const float rf = ne < 0 ? -2.f : (float)r;
+            if (base) {
+                f->val[i * W + j] = r;
+                f->n_esc[i * W + j] = ne;
+                f->out[i * W + j] = rf;
+            }
+            if (f->sub > 1) {
+                f->cell_val[c] = r;
+                f->cell_nesc[c] = ne;
+                f->cells[c] = rf;
+            }
+            ++nfb;
+        }
+        f->gmp_fallback = nfb;
     }
     f->steps = tot.steps;
     f->bla_applies = tot.bla_applies;

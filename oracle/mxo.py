@@ -27,6 +27,7 @@ class Frame(C.Structure):
         ("orbit_zf", C.c_void_p), ("orbit_m", C.c_void_p), ("orbit_e", C.c_void_p),
         ("sub", C.c_int), ("cells", C.c_void_p), ("cell_val", C.c_void_p), ("cell_nesc", C.c_void_p),
         ("ss_pixels", C.c_int), ("no_periodic", C.c_int), ("period", C.c_int), ("subpixel", C.c_int),
+        ("cutoff_sensitive", C.c_int), ("gmp_fallback", C.c_int),
     ]
 
 
@@ -149,7 +150,8 @@ def compute_fe(cre, cim, scale, W, H, mxit, c_method=0, escape_radius=1e8, prec_
     lib().mxo_compute_fe(C.byref(f))
     info = {k: getattr(f, k) for k in ("ref_it", "ref_escaped", "ref_virtual", "ref_x", "ref_y", "interior_used",
                                         "cutoff_sensitive_hint", "seconds_pixels", "steps", "bla_applies",
-                                        "bla_skipped", "rebases", "period", "subpixel")}
+                                        "bla_skipped", "rebases", "period", "subpixel", "cutoff_sensitive",
+                                        "gmp_fallback")}
     if want_orbit:
         n = 2 * (f.period if f.period else f.ref_it + 1)
         info["orbit"] = dict(zf=ozf[:n].copy(), zfe_m=om[:n].copy(), zfe_e=oe[:n].copy())
