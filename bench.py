@@ -32,6 +32,10 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 for sub in ("mandelbrot-explorer_amd", "oracle", "tests"):
     sys.path.insert(0, os.path.join(ROOT, sub))
 
+# thread binding of the CPU baseline's OpenMP runtime (read when it first starts)
+os.environ.setdefault("OMP_PROC_BIND", "close")
+os.environ.setdefault("OMP_PLACES", "cores")
+
 import numpy as np  # noqa: E402
 
 import autotune  # noqa: E402
@@ -138,8 +142,8 @@ def main():
     ap.add_argument("--knobs", default="auto", help="JSON file of launch knobs per form: read instead of tuning when "
                     "it exists, written after tuning otherwise; 'auto' (default): profiles/knobs_<workload>.json, "
                     "the autotuner's committed result that the profiles under profiles/ were taken with")
-    ap.add_argument("--copy-calibration", action="store_true", help="also copy 1 GiB device to device with a "
-                    "kernel (the reading that scales rocprofv3's FETCH_SIZE / WRITE_SIZE)")
+    ap.add_argument("--copy-calibration", action="store_true", help="only copy 1 GiB device to device with the "
+                    "library's copy kernel and exit (the reading that scales rocprofv3's FETCH_SIZE / WRITE_SIZE)")
     ap.add_argument("--c5", action="store_true", help="add the zoom-sequence leg (hipGraph of 1280x720 frames)")
     ap.add_argument("--c5-frames", type=int, default=240, help="first N frames of the zoom sequence")
     args = ap.parse_args()
@@ -147,6 +151,10 @@ def main():
         print(f"bench.py: --gpus {args.gpus} refused: this build runs the path on one GPU only "
               f"(multi-GPU sharding is design-only, see DESIGN.md)", file=sys.stderr)
         sys.exit(2)
+    if args.copy_calibration:
+        with mxhip.Context(0) as ctx:
+            copy_calibration(ctx)
+        return
     wl = load_workload(args.workload)
     if args.knobs == "auto":
         args.knobs = os.path.join(ROOT, "profiles", f"knobs_{args.workload}.json")
@@ -201,8 +209,6 @@ def main():
             e2e_ms = (time.time() - t1) * 1e3
             log(f"end-to-end frame (mx_compute): {e2e_ms:.0f} ms")
         double = None if args.no_double else double_leg(ctx, args.no_cpu)
-        if args.copy_calibration:
-            copy_calibration()
         batched = c4_leg(ctx) if args.c4 else None
         zoom = c5_leg(ctx, args.c5_frames) if args.c5 else None
     main_r = res[args.form]
@@ -222,7 +228,10 @@ def main():
                                            "value_err_max": float(err.max()) if err.size else 0.0,
                                            "value_err": "relative" if ede else "absolute smooth iterations"}
         forms[f"speedup_{args.form}_vs_{f}"] = forms[f]["ms_per_step"] / s["ms_per_step"]
-    if args.mpfr_samples > 0:
+    fixture = mpfr_fixture(wl)
+    if fixture is not None:
+        accuracy["vs_mpfr"] = vs_mpfr(wl, res, fixture)
+    elif args.mpfr_samples > 0:
         accuracy["vs_mpfr"] = mpfr_samples(wl, fv, res, args.mpfr_samples, center)
     cpu = None
     if not args.no_cpu:
@@ -263,18 +272,13 @@ def main():
     print(json.dumps(line))
 
 
-def copy_calibration(nbytes=1 << 30):
-    """A 1 GiB device-to-device copy by a kernel (torch's elementwise copy): reads and
-    writes nbytes each, so its rocprofv3 FETCH_SIZE / WRITE_SIZE give the factors that
-    turn those counters into bytes."""
-    import torch
-    a = torch.empty(nbytes // 4, dtype=torch.float32, device="cuda")
-    a.fill_(1.0)
-    b = torch.empty_like(a)
-    torch.cuda.synchronize()
-    b.copy_(a)
-    torch.cuda.synchronize()
-    log(f"copy calibration: {nbytes} bytes copied by a kernel")
+def copy_calibration(ctx, nbytes=1 << 30):
+    """A 1 GiB device-to-device copy by the library's plain copy kernel (mx_copy_probe):
+    it reads and writes nbytes each, so its rocprofv3 FETCH_SIZE / WRITE_SIZE give the
+    factors that turn those counters into bytes."""
+    ms = ctx.copy_probe(nbytes)
+    log(f"copy calibration: {nbytes} bytes copied by mx_copy_kernel in {ms:.3f} ms ({2 * nbytes / ms / 1e6:.0f} GB/s)")
+    return ms
 
 
 def traffic_bytes(kernel, wl):
@@ -313,7 +317,7 @@ def double_leg(ctx, no_cpu, reps=5):
     log(f"double path: frame {out['frame_ms']:.1f} ms, kernels {out['kernel_ms']:.2f} ms")
     if not no_cpu:
         import mxo
-        threads = min(16, os.cpu_count() or 1)
+        threads = int(os.environ.get("OMP_NUM_THREADS") or 0) or os.cpu_count() or 1
         t0 = time.perf_counter()
         _, n_o, _, _ = mxo.compute_dbl(cre, cim, scale, W, H, mxit, escape_radius=R, prec_bits=pb, threads=threads)
         dt = time.perf_counter() - t0
@@ -427,17 +431,26 @@ def c5_leg(ctx, nframes=240, forms=("fe32", "fe64")):
     frames, with one BLA table per reference (the shallowest frame's, valid for the
     deeper ones)."""
     seed = 42
-    path = os.path.join(ROOT, "tests", "golden", "walk_seed42.json")
-    walk = json.load(open(path)) if os.path.exists(path) else {}
-    if "500" not in walk:
-        # until the seed-42 walk reaches 1e-500: the seed-7 centre, said in the output
-        seed = 7
-        walk = json.load(open(os.path.join(ROOT, "tests", "golden", "walk_seed7.json")))
-        log("c5: no seed-42 centre at 1e-500 yet (tests/golden/make_walk.py 42 500); using the seed-7 centre")
+    walk = json.load(open(os.path.join(ROOT, "tests", "golden", "walk_de_seed42.json")))
     cre, cim = walk["500"]["re"], walk["500"]["im"]
     W, H = 1280, 720
     opts = mxhip.MX_OPT_VIEW_CENTER_REF | mxhip.MX_OPT_FLOATEXP
-    frames = [f for f in c5_frames()[:nframes] if f["scale_d"] > 1000000]
+    allf = c5_frames()[:nframes]
+    # frames at scale <= 1e6 take the engine's shallow direct path (method 0): one
+    # mx_compute each, every sample iterating its own c in double on the device
+    shallow = {"frames": 0, "frame_ms": 0.0, "kernel_ms": 0.0, "nominal_pixel_iterations": 0}
+    for f in allf:
+        if f["scale_d"] > 1000000:
+            continue
+        ctx.compute(cre, cim, f["scale"], W, H, f["mxit"])  # warm-up
+        t1 = time.perf_counter()
+        _, _, _, info = ctx.compute(cre, cim, f["scale"], W, H, f["mxit"])
+        shallow["frame_ms"] += (time.perf_counter() - t1) * 1e3
+        shallow["kernel_ms"] += info["kernel_ms"]
+        shallow["nominal_pixel_iterations"] += int(info["nominal_iterations"])
+        shallow["frames"] += 1
+    log(f"c5: {shallow['frames']} shallow frames, {shallow['frame_ms']:.1f} ms ({shallow['kernel_ms']:.2f} ms kernels)")
+    frames = [f for f in allf if f["scale_d"] > 1000000]
     for f in frames:
         f["prec"] = int(len(f["scale"]) * 3.321928094887362) + 64
         f["limbs"] = (f["prec"] + 127) // 64
@@ -464,9 +477,12 @@ def c5_leg(ctx, nframes=240, forms=("fe32", "fe64")):
                 nominal_mxit.append(f["mxit"])
         prep_s = time.time() - t0
         log(f"c5: {len(frames)} frames, {len(groups)} references, host {prep_s:.1f} s")
-        out = {"workload": f"{len(frames)} frames 1280x720, width 4 -> 1e-500 (frames at scale <= 1e6 left out), "
-                           f"seed-{seed} walk centre, max_iter 1e3 -> 5e6 linear, smooth, BLA, interior detection",
-               "frames": len(frames), "references": len(groups), "host_prepare_s": prep_s}
+        out = {"workload": f"{len(allf)} frames 1280x720, width 4 -> 1e-500, seed-{seed} boundary-walk centre "
+                           f"(tests/golden/walk_de_seed42.json), max_iter 1e3 -> 5e6 linear, smooth, BLA, interior "
+                           f"detection; {len(frames)} perturbation frames in one HIP graph per form, "
+                           f"{shallow['frames']} shallow frames (scale <= 1e6) through mx_compute",
+               "frames": len(allf), "graph_frames": len(frames), "shallow": shallow, "references": len(groups),
+               "host_prepare_s": prep_s}
         for fm in forms:
             plans = plans_by_form[fm]
             g = ctx.graph(plans)
@@ -481,7 +497,10 @@ def c5_leg(ctx, nframes=240, forms=("fe32", "fe64")):
                 nesc = p.read()[0]
                 nominal += int(np.where(nesc > 0, nesc, mx).astype(np.int64).sum())
             r = {"graph_ms": wall * 1e3, "graph_gpu_ms": gms, "ms_per_frame": wall * 1e3 / len(plans),
-                 "nominal_pixel_iterations": nominal, "gpix_iter_s": nominal / wall / 1e9}
+                 "nominal_pixel_iterations": nominal, "gpix_iter_s": nominal / wall / 1e9,
+                 "sequence_ms": wall * 1e3 + shallow["frame_ms"],
+                 "sequence_gpix_iter_s": (nominal + shallow["nominal_pixel_iterations"]) /
+                 (wall + shallow["frame_ms"] / 1e3) / 1e9}
             if fm == forms[0]:
                 log(f"c5 {fm}: graph replay {wall * 1e3:.0f} ms; frame by frame next")
                 t1 = time.perf_counter()
@@ -501,6 +520,45 @@ def c5_leg(ctx, nframes=240, forms=("fe32", "fe64")):
                 p.destroy()
         for r_ in refs:
             r_.release()
+
+
+def mpfr_fixture(wl):
+    """The committed per-sample MPFR reference of this workload
+    (tests/golden/mpfr_<name>.json, tests/golden/make_mpfr_samples.py), or None."""
+    path = os.path.join(ROOT, "tests", "golden", f"mpfr_{wl['name']}.json")
+    if not os.path.exists(path):
+        return None
+    d = json.load(open(path))
+    same = (d["W"], d["H"], d["max_iter"], d["c_method"], d["depth"], d["walk"]) == \
+        (wl["W"], wl["H"], wl["mxit"], wl["c_method"], wl["depth"], os.path.basename(WALK))
+    return d if same else None
+
+
+def vs_mpfr(wl, res, d):
+    """Every form against the committed MPFR samples: the fraction at identical escape
+    count (and class), and the value error (relative for distances, absolute smooth
+    iterations otherwise) over the samples both call exterior."""
+    idx = np.asarray(d["idx"], np.int64)
+    mn = np.asarray(d["n_esc"], np.int32)
+    mv = np.asarray(d["value"], np.float64)
+    ede = bool(wl["c_method"] & 2)
+    out = {"samples": int(idx.size), "precision_bits": d["precision_bits"],
+           "fixture": f"tests/golden/mpfr_{wl['name']}.json",
+           "fixture_check": {"precision_bits": d["check_precision_bits"], "samples": d["check_samples"],
+                             "n_esc_identical": d["check_n_esc_identical"]},
+           "mpfr_exterior_frac": float((mn > 0).mean())}
+    for f, rr in res.items():
+        n_f, v_f = rr["nesc"][idx], rr["val"][idx]
+        both = (n_f > 0) & (mn > 0)
+        err = np.abs(v_f[both] - mv[both])
+        if ede:
+            err = err / np.abs(mv[both])
+        out[f] = {"n_esc_identical_frac": float(np.mean(np.where(mn > 0, n_f == mn, n_f < 0))),
+                  "class_identical_frac": float(np.mean((n_f > 0) == (mn > 0))),
+                  "value_err_max": float(err.max()) if err.size else 0.0,
+                  "value_err_median": float(np.median(err)) if err.size else 0.0,
+                  "value_err": "relative" if ede else "absolute smooth iterations"}
+    return out
 
 
 def mpfr_samples(wl, fv, res, n, center):
@@ -542,11 +600,29 @@ def mpfr_samples(wl, fv, res, n, center):
     return r
 
 
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(fv, fe64, mxit, ede, interior, seconds, center):
-    """The oracle (the engine's per-pixel loop restated in C, OpenMP over samples) on
-    the host cores, on a random sample of the same frame's samples, timed; its outputs
-    double as a spot check of the plain form (bit-exact escape counts expected)."""
+    """The CPU path this workload takes in the engine -- per-pixel pixelRescaled with
+    BLA, scalar per pixel, OpenMP over pixels (mandel_perturbation.cpp:2141-2148,
+    2203-2210: the AVX2 lane loops run only with BLA off) -- as the oracle restates it,
+    built with clang -O2 -mavx2 -fopenmp -ffp-contract=off (oracle/libmxo_clang.so,
+    LLVM's OpenMP runtime), on every host core this process may use
+    (OMP_NUM_THREADS, else all), threads bound close to cores; timed on a random
+    sample of the same frame's samples.  Its outputs double as a spot check of the
+    plain form (bit-exact escape counts expected)."""
     import mxo
+
+    build = "libmxo_clang.so" if os.path.exists(os.path.join(ROOT, "oracle", "libmxo_clang.so")) else "libmxo.so"
+    mxo.use_build(build)
 
     rng = np.random.default_rng(7)
     W, H = fv["W"], fv["H"]
@@ -557,7 +633,7 @@ def cpu_baseline(fv, fe64, mxit, ede, interior, seconds, center):
     if fv["nlev"]:
         b = mxo.bla_tables(fv["zf"], fv["len"] - 1, fv["dcmax"])
         bla = dict(nlev=b["nlev"], off=b["off"], size=b["size"], r2=b["r2"], l=b["l"], coef=b["coef"], rmax2=b["rmax2"])
-    threads = min(os.cpu_count() or 1, 16)
+    threads = int(os.environ.get("OMP_NUM_THREADS") or 0) or os.cpu_count() or 1
 
     def run(ix):
         i, j = ix // W, ix % W
@@ -575,7 +651,12 @@ def cpu_baseline(fv, fe64, mxit, ede, interior, seconds, center):
     log(f"cpu oracle: {n} samples in {dt:.1f}s on {threads} threads")
     nominal = int(np.where(ne > 0, ne, mxit).astype(np.int64).sum())
     cpu = {"value": nominal / dt / 1e9, "unit": "Gpixel-iter/s", "cores": threads, "kind": "port",
-           "sample": f"{n} random samples of the same frame ({dt:.1f} s, OpenMP over samples)"}
+           "sample": f"{n} random samples of the same frame ({dt:.1f} s, OpenMP over samples)",
+           "build": ("clang -O2 -mavx2 -fopenmp -ffp-contract=off (oracle/libmxo_clang.so)" if build != "libmxo.so"
+                     else "gcc -O2 -fopenmp -ffp-contract=off (oracle/libmxo.so)"),
+           "cpu_model": cpu_model(), "host_cpus_visible": os.cpu_count(),
+           "omp": {k: os.environ.get(k) for k in ("OMP_NUM_THREADS", "OMP_PROC_BIND", "OMP_PLACES")}}
+    mxo.use_build("libmxo.so")
     par = None
     if fe64 is not None:
         same = float((ne == fe64["nesc"][ix]).mean())

@@ -308,6 +308,12 @@ typedef struct {
 
 int mx_compute(mx_ctx* ctx, const mx_request* req, mx_frame_info* info);
 
+/* Measurement aid, no counterpart in the reference: one plain device-to-device copy
+ * kernel over `bytes` (two fresh device buffers), timed.  Its hardware byte counters
+ * (rocprofv3 FETCH_SIZE / WRITE_SIZE) read bytes / 1024 KiB each for an exact copy:
+ * the factor that turns the same counters of the path's kernels into bytes. */
+int mx_copy_probe(mx_ctx* ctx, size_t bytes, double* kernel_ms);
+
 /* ---- colouring and downsampling (tools/render.cpp:252-302) ------------------ */
 
 /* The engine's palette (render.cpp:21-30: colorMapInit through mono_cubic_interpolate,

@@ -90,7 +90,7 @@ EXPORTS = ["mx_version", "mx_device_count", "mx_ctx_create", "mx_ctx_destroy", "
            "mx_plan_wait", "mx_plan_read", "mx_plan_unresolved", "mx_plan_destroy", "mx_tile_render", "mx_compute",
            "mx_frame_prepare", "mx_frame_free", "mx_batch_create", "mx_batch_launch", "mx_batch_wait",
            "mx_batch_destroy", "mx_ctx_cu_count", "mx_graph_create", "mx_graph_launch", "mx_graph_wait",
-           "mx_graph_destroy", "mx_palette", "mx_colorize", "mx_plan_colorize", "mx_tile_render_noalloc"]
+           "mx_graph_destroy", "mx_palette", "mx_colorize", "mx_plan_colorize", "mx_tile_render_noalloc", "mx_copy_probe"]
 
 _LIB = None
 
@@ -120,6 +120,7 @@ def lib():
         L.mx_plan_unresolved.argtypes = [vp, vp, vp, C.c_int32, C.POINTER(C.c_int32)]
         L.mx_tile_render.argtypes = [vp, vp, C.POINTER(Tile), vp, vp, vp, vp]
         L.mx_tile_render_noalloc.argtypes = [vp, vp, C.POINTER(Tile), vp, vp, vp, vp]
+        L.mx_copy_probe.argtypes = [vp, C.c_size_t, vp]
         L.mx_ctx_cu_count.argtypes = [vp]
         L.mx_graph_create.argtypes = [vp, C.POINTER(vp), C.c_int32, C.POINTER(vp)]
         L.mx_graph_launch.argtypes = [vp]
@@ -329,6 +330,12 @@ class Context:
         rc = lib().mx_tile_render(self.h, ref.h, C.byref(tile), _p(nesc), _p(val), _p(out), _p(st))
         self.check(rc, "mx_tile_render")
         return nesc, val, out, st
+
+    def copy_probe(self, nbytes):
+        """One device copy kernel over nbytes (counter calibration); its time in ms."""
+        ms = C.c_double()
+        self.check(lib().mx_copy_probe(self.h, nbytes, C.cast(C.byref(ms), C.c_void_p)), "mx_copy_probe")
+        return ms.value
 
     def render_noalloc(self, ref, tile):
         """mx_tile_render_noalloc: the same results, staged in the device's fixed scratch area."""

@@ -1251,6 +1251,9 @@ typedef struct {
      * its brute-force fallback recomputed (MP.cpp:2173-2183) */
     int cutoff_sensitive;
     int gmp_fallback;
+    /* input: reference at the exact view centre at any depth (the library's
+     * MX_OPT_VIEW_CENTER_REF; the engine's createRef(view_center), MP.cpp:4021-4036) */
+    int view_center;
 } mxo_frame;
 
 /* output cell of sample (i, j, u, v) (getIndex, MP.cpp:4334-4340) */
@@ -1588,7 +1591,7 @@ MXO_EXPORT int mxo_compute_fe(mxo_frame* f) {
         mpfr_t sc;
         mpfr_init2(sc, pb);
         mpfr_set_str(sc, f->scale, 10, MPFR_RNDN);
-        escalated = mpfr_cmp_d(sc, 1e280) <= 0;
+        escalated = mpfr_cmp_d(sc, 1e280) <= 0 || f->view_center;
         mpfr_clear(sc);
     }
     /* one integer limb: the fixed-point orbit needs |z|^2 + |c| well below 2^64

@@ -9,6 +9,14 @@ import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB = None
+_NAME = "libmxo.so"
+
+
+def use_build(name):
+    """Load another build of the oracle (libmxo_clang.so: the clang + AVX2 + libomp
+    build that times the CPU baseline) from the next call on."""
+    global _LIB, _NAME
+    _LIB, _NAME = None, name
 
 
 class Frame(C.Structure):
@@ -27,7 +35,7 @@ class Frame(C.Structure):
         ("orbit_zf", C.c_void_p), ("orbit_m", C.c_void_p), ("orbit_e", C.c_void_p),
         ("sub", C.c_int), ("cells", C.c_void_p), ("cell_val", C.c_void_p), ("cell_nesc", C.c_void_p),
         ("ss_pixels", C.c_int), ("no_periodic", C.c_int), ("period", C.c_int), ("subpixel", C.c_int),
-        ("cutoff_sensitive", C.c_int), ("gmp_fallback", C.c_int),
+        ("cutoff_sensitive", C.c_int), ("gmp_fallback", C.c_int), ("view_center", C.c_int),
     ]
 
 
@@ -59,9 +67,9 @@ class DFrame(C.Structure):
 def lib():
     global _LIB
     if _LIB is None:
-        path = os.path.join(_HERE, "libmxo.so")
+        path = os.path.join(_HERE, _NAME)
         if not os.path.exists(path):
-            raise RuntimeError("oracle/libmxo.so is not built (run `make -C oracle`)")
+            raise RuntimeError(f"oracle/{_NAME} is not built (run `make -C oracle`)")
         L = C.CDLL(path)
         L.mxo_compute_fe.argtypes = [C.POINTER(Frame)]
         L.mxo_compute_dbl.argtypes = [C.POINTER(DFrame)]
@@ -112,7 +120,7 @@ def precision_bits(scale_str):
 
 def compute_fe(cre, cim, scale, W, H, mxit, c_method=0, escape_radius=1e8, prec_bits=None,
                use_bla=True, bla_minskip=8, interior=True, probe=True, threads=None, full_h=0, row_base=0,
-               want_orbit=False, sub=1, periodic=True):
+               want_orbit=False, sub=1, periodic=True, view_center=False):
     """Whole floatexp frame (Compute -> reference -> BLA -> probe -> pass).  With
     want_orbit, info["orbit"] holds the reference orbit's shadows (zf, zfe_m, zfe_e,
     (re, im) interleaved).  With sub > 1, info["cells"] holds the (H sub) x (W sub)
@@ -140,6 +148,7 @@ def compute_fe(cre, cim, scale, W, H, mxit, c_method=0, escape_radius=1e8, prec_
     f.use_bla, f.bla_minskip, f.bla_eps = int(use_bla), bla_minskip, 0.0
     f.interior, f.probe = int(interior), int(probe)
     f.no_periodic = int(not periodic)
+    f.view_center = int(view_center)
     f.threads = threads or os.cpu_count()
     val = np.zeros(W * H, np.float64)
     nesc = np.zeros(W * H, np.int32)
