@@ -187,7 +187,10 @@ def main():
                 log(f"knobs from {args.knobs}: {tuned}")
             elif not args.no_tune:
                 for f in [args.form] + compare:
-                    r = autotune.tune(ctx, ref, fv, FORMS[f], ede=ede, interior=interior, log=log)
+                    # the whole frame when one launch is cheap (the probe pass stands in)
+                    whole = probe_ms * 64 < 2000.0
+                    r = autotune.tune(ctx, ref, fv, FORMS[f], ede=ede, interior=interior, log=log,
+                                      target_samples=None if whole else 2_000_000)
                     tuned[f] = {k: r[k] for k in ("chunk", "blocks_per_cu", "quorum", "bla_minlevel", "window", "ms")}
                     tuned[f]["trials"] = len(r["trials"])
                     log(f"autotune {f}: {tuned[f]}")

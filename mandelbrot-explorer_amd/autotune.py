@@ -12,10 +12,11 @@ Knobs, timed on the device on a window of the frame to be rendered:
 chunk, grid_blocks and quorum do not change any result.
 
 The search is coordinate descent from the defaults (chunk 32, 8 blocks per CU, quorum
-32, the frame's own cutoff): each knob in turn is swept with the others fixed and the fastest
-value kept.  The sample is the central window of the frame with about 2 M samples
-(the frame's aspect ratio): enough work to fill the grid, a fraction of the frame's
-cost, and neighbouring samples side by side as in the frame, so a wave's lanes
+32, the frame's own cutoff): each knob in turn is swept with the others fixed and the
+fastest value kept, over two rounds.  The sample is the whole frame when a launch is
+cheap (the caller's choice), else the central window of the frame with about 2 M
+samples (the frame's aspect ratio): enough work to fill the grid, a fraction of the
+frame's cost, and neighbouring samples side by side as in the frame, so a wave's lanes
 diverge as they do there (a strided subsample puts distant samples in one wave and
 runs several times slower per sample).  Results can be kept in a JSON file keyed by
 form, flags and CU count.
@@ -25,7 +26,7 @@ import os
 
 import mxhip
 
-CHUNKS = (8, 16, 24, 32, 48, 64, 96, 128)
+CHUNKS = (8, 16, 24, 32, 48, 64, 96, 128, 192, 256)
 QUORUMS = (1, 8, 16, 24, 32, 40, 48, 56)
 BLOCKS_PER_CU = (2, 3, 4, 5, 6, 8)
 MINLEVELS = (1, 2, 3, 4, 5, 6, 7, 8, 9, 10)
@@ -73,11 +74,13 @@ def _time(ctx, ref, fv, form, ede, interior, mxit, knobs, reps):
 
 
 def tune(ctx, ref, fv, form, ede=False, interior=True, mxit=None, target_samples=2_000_000, reps=1, log=None,
-         tune_cutoff=False):
+         tune_cutoff=False, passes=2):
     """Best knobs for rendering frame `fv` (a frame_prepare dict whose reference is
-    uploaded as `ref`) in `form`.  Returns dict(chunk, blocks_per_cu, quorum,
-    bla_minlevel, window, ms, trials)."""
-    sub = window(fv, target_samples)
+    uploaded as `ref`) in `form`; target_samples=None times the whole frame.  `passes`
+    rounds of coordinate descent (the knobs interact: a longer lean run wants another
+    quorum).  Returns dict(chunk, blocks_per_cu, quorum, bla_minlevel, window, ms,
+    trials)."""
+    sub = window(fv, target_samples) if target_samples else subframe(fv, 0, 0, fv["W"], fv["H"])
     knobs = dict(chunk=32, blocks_per_cu=8, quorum=32, bla_minlevel=fv["bla_minlevel"])
     trials = []
 
@@ -92,14 +95,17 @@ def tune(ctx, ref, fv, form, ede=False, interior=True, mxit=None, target_samples
     axes = [("quorum", QUORUMS), ("chunk", CHUNKS), ("blocks_per_cu", BLOCKS_PER_CU)]
     if tune_cutoff and form == mxhip.MX_FORM_FE32 and fv["nlev"] > 0:
         axes.append(("bla_minlevel", tuple(m for m in MINLEVELS if m < fv["nlev"])))
-    for name, values in axes:
-        for v in values:
-            if v == knobs[name]:
-                continue
-            k = dict(knobs, **{name: v})
-            ms = run(k)
-            if ms < best:
-                best, knobs = ms, k
+    for _ in range(passes):
+        for name, values in axes:
+            for v in values:
+                if v == knobs[name]:
+                    continue
+                k = dict(knobs, **{name: v})
+                if any(all(t[a] == k[a] for a in k) for t in trials):
+                    continue
+                ms = run(k)
+                if ms < best:
+                    best, knobs = ms, k
     return dict(knobs, window=[sub["W"], sub["H"]], ms=best, trials=trials)
 
 
