@@ -683,7 +683,7 @@ typedef struct {
 } mxo_ctx;
 
 typedef struct {
-    long long steps, bla_applies, bla_skipped, rebases, rescales, underflow_steps;
+    long long steps, bla_applies, bla_skipped, rebases, rescales, underflow_steps, deriv_underflows;
 } mxo_stats;
 
 static inline int tzcnt32(unsigned x) { return __builtin_ctz(x); }
@@ -975,6 +975,7 @@ static double pixel_rescaled(const mxo_ctx* c, mxo_fe dcr, mxo_fe dci, int mxit,
         if (deriv) {
             if (deriv_underflow) {
                 fe_deriv_underflow(zmrfe, zmife, &SJ, &jr, &ji, &invSJd);
+                st->deriv_underflows++;
             } else {
                 double a = 2.0 * (zmr * jr - zmi * ji) + invSJd;
                 double bb = 2.0 * (zmr * ji + zmi * jr);
@@ -1254,6 +1255,10 @@ typedef struct {
     /* input: reference at the exact view centre at any depth (the library's
      * MX_OPT_VIEW_CENTER_REF; the engine's createRef(view_center), MP.cpp:4021-4036) */
     int view_center;
+    /* outputs (main pass): w rescales, cold step-underflow recoveries
+     * (feRescaledStepUnderflow, MP.cpp:2794-2835) and derivative-underflow recoveries
+     * (feDerivativeStepUnderflow, MP.cpp:2841-2876) */
+    long long rescales, underflow_steps, deriv_underflows;
 } mxo_frame;
 
 /* output cell of sample (i, j, u, v) (getIndex, MP.cpp:4334-4340) */
@@ -1326,12 +1331,12 @@ static double ref_pixel_value(const mxo_orbit* o, int mxit, int c_method, double
 }
 
 static void run_pixels(const mxo_ctx* ctx, const mxo_frame* f, const mxo_fe* dcol, const mxo_fe* drow, const int* idx, int n, double* val, int32_t* nesc, mxo_stats* tot) {
-    long long steps = 0, ap = 0, sk = 0, rb = 0;
-#pragma omp parallel for schedule(dynamic, 64) num_threads(f->threads) reduction(+ : steps, ap, sk, rb)
+    long long steps = 0, ap = 0, sk = 0, rb = 0, rs = 0, us = 0, du = 0;
+#pragma omp parallel for schedule(dynamic, 64) num_threads(f->threads) reduction(+ : steps, ap, sk, rb, rs, us, du)
     for (int q = 0; q < n; ++q) {
         int p = idx[q];
         int i = p / f->W, j = p % f->W;
-        mxo_stats st = {0, 0, 0, 0, 0, 0};
+        mxo_stats st = {0, 0, 0, 0, 0, 0, 0};
         int ne;
         double v = pixel_fe(ctx, dcol[j], drow[i], f->mxit, &ne, &st);
         val[p] = v;
@@ -1340,11 +1345,17 @@ static void run_pixels(const mxo_ctx* ctx, const mxo_frame* f, const mxo_fe* dco
         ap += st.bla_applies;
         sk += st.bla_skipped;
         rb += st.rebases;
+        rs += st.rescales;
+        us += st.underflow_steps;
+        du += st.deriv_underflows;
     }
     tot->steps += steps;
     tot->bla_applies += ap;
     tot->bla_skipped += sk;
     tot->rebases += rb;
+    tot->rescales += rs;
+    tot->underflow_steps += us;
+    tot->deriv_underflows += du;
 }
 
 /* ------------------------------------------------------------------------- */
@@ -1706,13 +1717,13 @@ MXO_EXPORT int mxo_compute_fe(mxo_frame* f) {
     for (int i = 0; i < H; ++i) drow[i] = mxo_fe_mul_d(dyfe, (double)i - ref_y);
 
     int* idx = (int*)malloc(sizeof(int) * (size_t)W * H);
-    mxo_stats tot = {0, 0, 0, 0, 0, 0};
+    mxo_stats tot = {0, 0, 0, 0, 0, 0, 0};
     double t0 = omp_get_wtime();
     if (f->interior && f->probe) {
         int n = 0;
         for (int i = 0; i < H; i += 8)
             for (int j = 0; j < W; j += 8) idx[n++] = i * W + j;
-        mxo_stats pst = {0, 0, 0, 0, 0, 0};
+        mxo_stats pst = {0, 0, 0, 0, 0, 0, 0};
         run_pixels(&ctx, f, dcol, drow, idx, n, f->val, f->n_esc, &pst);
         int nint = 0;
         for (int q = 0; q < n; ++q)
@@ -1760,7 +1771,7 @@ MXO_EXPORT int mxo_compute_fe(mxo_frame* f) {
                 const int yi = k / S - h, xi = k % S - h;
                 const int i = p / W, j = p % W;
                 const double px = j + (double)xi / S, py = i + (double)yi / S;
-                mxo_stats st = {0, 0, 0, 0, 0, 0};
+                mxo_stats st = {0, 0, 0, 0, 0, 0, 0};
                 int ne;
                 const double v = pixel_fe(&ctx, mxo_fe_mul_d(dxfe, px - ref_x), mxo_fe_mul_d(dyfe, py - ref_y), f->mxit, &ne, &st);
                 const size_t c = ss_cell(W, S, i, j, yi, xi);
@@ -1833,6 +1844,9 @@ MXO_EXPORT int mxo_compute_fe(mxo_frame* f) {
     f->bla_applies = tot.bla_applies;
     f->bla_skipped = tot.bla_skipped;
     f->rebases = tot.rebases;
+    f->rescales = tot.rescales;
+    f->underflow_steps = tot.underflow_steps;
+    f->deriv_underflows = tot.deriv_underflows;
     if (f->orbit_zf)
         for (int k = 0; k < o.len; ++k) {
             f->orbit_zf[2 * k] = o.zf[2 * k];
@@ -1876,6 +1890,29 @@ MXO_EXPORT void mxo_fe_op(int op, int n, const double* am, const int64_t* ae, co
         rm[k] = r.m;
         re[k] = r.e;
     }
+}
+
+/* The cold recoveries on explicit inputs, for tests.  step: in X (re, im), dc (re, im),
+ * S, w (re, im) -> out S, w.  deriv: in z (re, im), SJ, j (re, im) -> out SJ, j, 1/SJ.
+ * FloatExp values as (mantissa, exponent) pairs. */
+MXO_EXPORT void mxo_step_underflow(const double* xm, const int64_t* xe, const double* cm, const int64_t* ce, double sm, int64_t se, double wr, double wi, double* out_sm, int64_t* out_se, double* out_w) {
+    mxo_fe S = mxo_fe_make(sm, se);
+    fe_step_underflow(mxo_fe_make(xm[0], xe[0]), mxo_fe_make(xm[1], xe[1]), mxo_fe_make(cm[0], ce[0]), mxo_fe_make(cm[1], ce[1]), &S, &wr, &wi);
+    *out_sm = S.m;
+    *out_se = S.e;
+    out_w[0] = wr;
+    out_w[1] = wi;
+}
+
+MXO_EXPORT void mxo_deriv_underflow(const double* zm, const int64_t* ze, double sjm, int64_t sje, double jr, double ji, double* out_sjm, int64_t* out_sje, double* out_j) {
+    mxo_fe SJ = mxo_fe_make(sjm, sje);
+    double inv = 0.0;
+    fe_deriv_underflow(mxo_fe_make(zm[0], ze[0]), mxo_fe_make(zm[1], ze[1]), &SJ, &jr, &ji, &inv);
+    *out_sjm = SJ.m;
+    *out_sje = SJ.e;
+    out_j[0] = jr;
+    out_j[1] = ji;
+    out_j[2] = inv;
 }
 
 /* Reference orbit shadows for a point given as decimal strings, in MPFR at prec_bits
@@ -2039,10 +2076,10 @@ MXO_EXPORT void mxo_pixels_fe(const mxo_pixels_in* in, int n, const double* dcr_
     ctx.ede = in->ede;
     ctx.dxfe = mxo_fe_make(in->dx_m, in->dx_e);
     ctx.per = in->period > 0 ? in->period : 0;
-    long long steps = 0, ap = 0, sk = 0, rb = 0;
-#pragma omp parallel for schedule(dynamic, 16) num_threads(in->threads > 0 ? in->threads : 1) reduction(+ : steps, ap, sk, rb)
+    long long steps = 0, ap = 0, sk = 0, rb = 0, rs = 0, us = 0, du = 0;
+#pragma omp parallel for schedule(dynamic, 16) num_threads(in->threads > 0 ? in->threads : 1) reduction(+ : steps, ap, sk, rb, rs, us, du)
     for (int q = 0; q < n; ++q) {
-        mxo_stats st = {0, 0, 0, 0, 0, 0};
+        mxo_stats st = {0, 0, 0, 0, 0, 0, 0};
         int ne;
         val[q] = pixel_fe(&ctx, mxo_fe_make(dcr_m[q], dcr_e[q]), mxo_fe_make(dci_m[q], dci_e[q]), in->mxit, &ne, &st);
         nesc[q] = ne;
@@ -2050,12 +2087,18 @@ MXO_EXPORT void mxo_pixels_fe(const mxo_pixels_in* in, int n, const double* dcr_
         ap += st.bla_applies;
         sk += st.bla_skipped;
         rb += st.rebases;
+        rs += st.rescales;
+        us += st.underflow_steps;
+        du += st.deriv_underflows;
     }
-    if (stats4) {
+    if (stats4) { /* steps, applies, skipped, rebases, rescales, step underflows, derivative underflows */
         stats4[0] = steps;
         stats4[1] = ap;
         stats4[2] = sk;
         stats4[3] = rb;
+        stats4[4] = rs;
+        stats4[5] = us;
+        stats4[6] = du;
     }
     free(zre);
     free(zim);
@@ -2794,7 +2837,7 @@ MXO_EXPORT int mxo_compute_dbl(mxo_dframe* f) {
             mxo_c dz = sa_seed(&sa, dc);                                                                      \
             mxo_c dd0 = {0.0, 0.0};                                                                           \
             if (ede) dd0 = sa_horner(sa.B, sa.order, sa.delta, dc);                                           \
-            mxo_stats st = {0, 0, 0, 0, 0, 0};                                                                \
+            mxo_stats st = {0, 0, 0, 0, 0, 0, 0};                                                                \
             int ne;                                                                                           \
             double gpp;                                                                                       \
             double v = pixel_double(&ctx, dc, dz, dd0, mxit, &ne, &gpp, &st);                                 \
@@ -2990,7 +3033,7 @@ MXO_EXPORT void mxo_pixels_dbl(const mxo_pixels_in* in, const double* z_m3, cons
         mxo_c dz = sa_seed(&sa, dc);
         mxo_c dd0 = {0.0, 0.0};
         if (in->ede) dd0 = sa_horner(sa.B, sa.order, sa.delta, dc);
-        mxo_stats st = {0, 0, 0, 0, 0, 0};
+        mxo_stats st = {0, 0, 0, 0, 0, 0, 0};
         int ne;
         double g;
         double v = pixel_double(&ctx, dc, dz, dd0, in->mxit, &ne, &g, &st);

@@ -36,6 +36,7 @@ class Frame(C.Structure):
         ("sub", C.c_int), ("cells", C.c_void_p), ("cell_val", C.c_void_p), ("cell_nesc", C.c_void_p),
         ("ss_pixels", C.c_int), ("no_periodic", C.c_int), ("period", C.c_int), ("subpixel", C.c_int),
         ("cutoff_sensitive", C.c_int), ("gmp_fallback", C.c_int), ("view_center", C.c_int),
+        ("rescales", C.c_longlong), ("underflow_steps", C.c_longlong), ("deriv_underflows", C.c_longlong),
     ]
 
 
@@ -98,6 +99,10 @@ def lib():
         L.mxo_direct_list.argtypes = [C.c_char_p, C.c_char_p, C.c_char_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
                                       C.c_int, C.c_float, C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p]
         L.mxo_point.argtypes = [C.c_char_p, C.c_char_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, C.c_char_p, C.c_void_p]
+        L.mxo_step_underflow.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_double, C.c_int64, C.c_double,
+                                         C.c_double, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.mxo_deriv_underflow.argtypes = [C.c_void_p, C.c_void_p, C.c_double, C.c_int64, C.c_double, C.c_double,
+                                          C.c_void_p, C.c_void_p, C.c_void_p]
         L.mxo_shallow_points.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_double, C.c_int, C.c_int, C.c_void_p,
                                          C.c_void_p]
         L.mxo_shallow.argtypes = [C.c_char_p, C.c_char_p, C.c_char_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
@@ -160,7 +165,7 @@ def compute_fe(cre, cim, scale, W, H, mxit, c_method=0, escape_radius=1e8, prec_
     info = {k: getattr(f, k) for k in ("ref_it", "ref_escaped", "ref_virtual", "ref_x", "ref_y", "interior_used",
                                         "cutoff_sensitive_hint", "seconds_pixels", "steps", "bla_applies",
                                         "bla_skipped", "rebases", "period", "subpixel", "cutoff_sensitive",
-                                        "gmp_fallback")}
+                                        "gmp_fallback", "rescales", "underflow_steps", "deriv_underflows")}
     if want_orbit:
         n = 2 * (f.period if f.period else f.ref_it + 1)
         info["orbit"] = dict(zf=ozf[:n].copy(), zfe_m=om[:n].copy(), zfe_e=oe[:n].copy())
@@ -353,10 +358,11 @@ def pixels_fe(orb, bla, dcr_m, dcr_e, dci_m, dci_e, mxit, esc2, dx_m, dx_e, ede=
     n = len(dcr_m)
     val = np.zeros(n)
     nesc = np.zeros(n, np.int32)
-    st = np.zeros(4, np.int64)
+    st = np.zeros(7, np.int64)
     lib().mxo_pixels_fe(C.byref(pi), n, arr(dcr_m, np.float64), arr(dcr_e, np.int64), arr(dci_m, np.float64),
                         arr(dci_e, np.int64), _p(val), _p(nesc), _p(st))
-    return val, nesc, dict(steps=int(st[0]), bla_applies=int(st[1]), bla_skipped=int(st[2]), rebases=int(st[3]))
+    return val, nesc, dict(steps=int(st[0]), bla_applies=int(st[1]), bla_skipped=int(st[2]), rebases=int(st[3]),
+                           rescales=int(st[4]), underflow_steps=int(st[5]), deriv_underflows=int(st[6]))
 
 
 def pixels_dbl(zf, z_m3, bla, sa_coef, sa_it, sa_order, sa_delta, dcr, dci, mxit, esc2, use_interior=True,
@@ -439,6 +445,27 @@ def direct_list(cre, cim, scale, W, H, mxit, idx, c_method=0, escape_radius=1e8,
     lib().mxo_direct_list(cre.encode(), cim.encode(), scale.encode(), W, H, pb, wp_bits or pb, mxit, c_method,
                           escape_radius, _p(idx), len(idx), threads or os.cpu_count(), int(track_d), _p(val), _p(nesc))
     return val, nesc
+
+
+def step_underflow(X, dc, S, w):
+    """feRescaledStepUnderflow on explicit FloatExp inputs: X, dc = ((m, e), (m, e)),
+    S = (m, e), w = (wr, wi) -> (S', (wr', wi'))."""
+    xm = np.array([X[0][0], X[1][0]]); xe = np.array([X[0][1], X[1][1]], np.int64)
+    cm = np.array([dc[0][0], dc[1][0]]); ce = np.array([dc[0][1], dc[1][1]], np.int64)
+    om, oe, ow = C.c_double(), C.c_int64(), np.zeros(2)
+    lib().mxo_step_underflow(_p(xm), _p(xe), _p(cm), _p(ce), S[0], S[1], w[0], w[1], C.cast(C.byref(om), C.c_void_p),
+                             C.cast(C.byref(oe), C.c_void_p), _p(ow))
+    return (om.value, oe.value), (ow[0], ow[1])
+
+
+def deriv_underflow(z, SJ, j):
+    """feDerivativeStepUnderflow on explicit inputs: z = ((m, e), (m, e)), SJ = (m, e),
+    j = (jr, ji) -> (SJ', (jr', ji'), 1/SJ' as double)."""
+    zm = np.array([z[0][0], z[1][0]]); ze = np.array([z[0][1], z[1][1]], np.int64)
+    om, oe, oj = C.c_double(), C.c_int64(), np.zeros(3)
+    lib().mxo_deriv_underflow(_p(zm), _p(ze), SJ[0], SJ[1], j[0], j[1], C.cast(C.byref(om), C.c_void_p),
+                              C.cast(C.byref(oe), C.c_void_p), _p(oj))
+    return (om.value, oe.value), (oj[0], oj[1]), oj[2]
 
 
 def shallow_points(cr, ci, mxit, escape_radius=1e8, ede=False, scalar=False):
