@@ -271,8 +271,14 @@ typedef struct {
  *                           the plain-double path up to 1e280);
  *   MX_OPT_NO_PERIODIC      no search for a nucleus in view: always the full reference
  *                           from the centre pixel (the engine's MANDEL_PERIODIC=0,
- *                           mandel_perturbation.cpp:1240-1250). */
-enum { MX_OPT_VIEW_CENTER_REF = 1, MX_OPT_FLOATEXP = 2, MX_OPT_NO_PERIODIC = 4 };
+ *                           mandel_perturbation.cpp:1240-1250);
+ *   MX_OPT_PERIODIC         the search at every scale, not only above 1e280 (the
+ *                           engine's MANDEL_PERIODIC=1, mandel_perturbation.cpp:
+ *                           1240-1245); on the floatexp path only (with MX_OPT_FLOATEXP
+ *                           below 1e280): the periodic reference on the plain-double
+ *                           path (its mod-p delta loop, 2345-2349, 2315) is refused with
+ *                           MX_E_UNSUPPORTED. */
+enum { MX_OPT_VIEW_CENTER_REF = 1, MX_OPT_FLOATEXP = 2, MX_OPT_NO_PERIODIC = 4, MX_OPT_PERIODIC = 8 };
 
 typedef struct {
     int32_t path;            /* 1 = plain-double path, 2 = floatexp rescaled path */

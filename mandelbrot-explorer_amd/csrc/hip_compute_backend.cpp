@@ -118,8 +118,18 @@ class HipComputeBackend final : public IComputeBackend {
         if (const char* e = std::getenv("MANDEL_BLA_MINSKIP")) r.bla_minskip = std::atoi(e);
         // MANDEL_PERIODIC=0: no nucleus search (buildReferenceOrbit, mandel_perturbation.cpp:
         // 1240-1250); the search runs by default above 1e280 with BLA on
-        if (const char* e = std::getenv("MANDEL_PERIODIC"))
-            if (std::atoi(e) == 0) r.options |= MX_OPT_NO_PERIODIC;
+        // MANDEL_PERIODIC=1 runs it at every scale; MANDEL_FE=1 takes the floatexp path
+        // at every scale above 1e6 (mandel_perturbation.cpp:1573-1576)
+        if (const char* e = std::getenv("MANDEL_PERIODIC")) r.options |= std::atoi(e) == 0 ? MX_OPT_NO_PERIODIC : std::atoi(e) == 1 ? MX_OPT_PERIODIC : 0;
+        if (const char* e = std::getenv("MANDEL_FE")) {
+            // MANDEL_FE=0 above 1e280 asks for the plain-double path past its range: the
+            // engine's CPU backend runs that
+            if (std::atoi(e) == 0) {
+                if (mpf_cmp_d(request.scale, 1e280) > 0) return computeCpu(request);
+            } else {
+                r.options |= MX_OPT_FLOATEXP;
+            }
+        }
         r.iterations = request.iterations;
         if (request.progress) request.progress->store(0.0f, std::memory_order_relaxed);
         mx_frame_info info;

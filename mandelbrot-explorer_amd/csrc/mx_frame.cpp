@@ -1156,6 +1156,11 @@ int Frame::prepare(const mx_request& r, std::string& err) {
     // path at scale <= 1e6, else perturbation in double up to 1e280, else floatexp
     shallow = mpf_cmp_d(scale, 1e6) <= 0;
     dbl = !shallow && mpf_cmp_d(scale, 1e280) <= 0 && !(r.options & MX_OPT_FLOATEXP);
+    if (dbl && (r.options & MX_OPT_PERIODIC) && !(r.options & MX_OPT_NO_PERIODIC) && r.use_bla != 0) {
+        err = "the periodic (nucleus) reference on the plain-double path (scale <= 1e280) is not on this path "
+              "(MX_OPT_FLOATEXP runs such frames on the floatexp path)";
+        return MX_E_UNSUPPORTED;
+    }
     if (dbl && ss) {
         err = "adaptive supersampling on the plain-double path (scale <= 1e280) is not on this path "
               "(MX_OPT_FLOATEXP runs such frames on the floatexp path)";
@@ -1238,7 +1243,8 @@ int Frame::prepare(const mx_request& r, std::string& err) {
     subpixel = false;
     // a nucleus in view on the deep floatexp path with BLA (buildReferenceOrbit,
     // mandel_perturbation.cpp:1238-1250): a resolved one becomes the reference
-    if (!escalated && !view_ref && use_bla && !(r.options & MX_OPT_NO_PERIODIC) && mpf_cmp_d(scale, 1e280) > 0 && try_periodic(scale)) {
+    const bool periodic_want = (r.options & MX_OPT_PERIODIC) || mpf_cmp_d(scale, 1e280) > 0;
+    if (!escalated && !view_ref && use_bla && !(r.options & MX_OPT_NO_PERIODIC) && periodic_want && try_periodic(scale)) {
         ref_seconds = now_s() - t_ref;
     } else {
         Gmp pre(prec), pim(prec), t(prec);

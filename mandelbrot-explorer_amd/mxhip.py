@@ -12,7 +12,7 @@ LIB_PATH = os.path.join(_HERE, "libmxhip.so")
 MX_OK, MX_E_INVALID, MX_E_NODEVICE, MX_E_HIP, MX_E_CANCELLED, MX_E_BUSY, MX_E_NOMEM, MX_E_UNSUPPORTED = 0, -1, -2, -3, -4, -5, -6, -7
 MX_FORM_FE64, MX_FORM_FE32, MX_FORM_DOUBLE, MX_FORM_SC64 = 0, 1, 2, 3
 MX_EXTERIOR_DIST_EST = 2
-MX_OPT_VIEW_CENTER_REF, MX_OPT_FLOATEXP, MX_OPT_NO_PERIODIC = 1, 2, 4
+MX_OPT_VIEW_CENTER_REF, MX_OPT_FLOATEXP, MX_OPT_NO_PERIODIC, MX_OPT_PERIODIC = 1, 2, 4, 8
 ERRORS = {-1: "invalid", -2: "no device", -3: "hip error", -4: "cancelled", -5: "busy", -6: "out of memory", -7: "unsupported"}
 
 

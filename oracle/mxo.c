@@ -1259,6 +1259,10 @@ typedef struct {
      * (feRescaledStepUnderflow, MP.cpp:2794-2835) and derivative-underflow recoveries
      * (feDerivativeStepUnderflow, MP.cpp:2841-2876) */
     long long rescales, underflow_steps, deriv_underflows;
+    /* inputs: the floatexp path at any scale above 1e6 with the centre-pixel reference
+     * (MANDEL_FE=1, MP.cpp:1573-1576), and the nucleus search at any scale
+     * (MANDEL_PERIODIC=1, MP.cpp:1240-1245) */
+    int force_fe, force_periodic;
 } mxo_frame;
 
 /* output cell of sample (i, j, u, v) (getIndex, MP.cpp:4334-4340) */
@@ -1597,12 +1601,13 @@ MXO_EXPORT int mxo_compute_fe(mxo_frame* f) {
      * is the exact view centre */
     mpfr_t rre, rim;
     mpfr_inits2(pb, rre, rim, (mpfr_ptr)0);
-    int escalated;
+    int escalated, shallow_fe;
     {
         mpfr_t sc;
         mpfr_init2(sc, pb);
         mpfr_set_str(sc, f->scale, 10, MPFR_RNDN);
-        escalated = mpfr_cmp_d(sc, 1e280) <= 0 || f->view_center;
+        shallow_fe = mpfr_cmp_d(sc, 1e280) <= 0;
+        escalated = (shallow_fe && !f->force_fe) || f->view_center;
         mpfr_clear(sc);
     }
     /* one integer limb: the fixed-point orbit needs |z|^2 + |c| well below 2^64
@@ -1618,7 +1623,7 @@ MXO_EXPORT int mxo_compute_fe(mxo_frame* f) {
     /* a nucleus in view on the deep path with BLA (buildReferenceOrbit, MP.cpp:
      * 1238-1324): resolved (at least 4 pixels) it is the reference, its orbit one
      * period long; sub-pixel, the centre pixel's full reference with BLA off */
-    if (!escalated && f->use_bla && !f->no_periodic) {
+    if (!escalated && f->use_bla && !f->no_periodic && (!shallow_fe || f->force_periodic)) {
         const mp_bitcnt_t np = mpf_get_prec(g.fc0_re);
         mpf_t nre, nim, t;
         mpf_init2(nre, np);

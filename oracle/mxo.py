@@ -37,6 +37,7 @@ class Frame(C.Structure):
         ("ss_pixels", C.c_int), ("no_periodic", C.c_int), ("period", C.c_int), ("subpixel", C.c_int),
         ("cutoff_sensitive", C.c_int), ("gmp_fallback", C.c_int), ("view_center", C.c_int),
         ("rescales", C.c_longlong), ("underflow_steps", C.c_longlong), ("deriv_underflows", C.c_longlong),
+        ("force_fe", C.c_int), ("force_periodic", C.c_int),
     ]
 
 
@@ -125,7 +126,7 @@ def precision_bits(scale_str):
 
 def compute_fe(cre, cim, scale, W, H, mxit, c_method=0, escape_radius=1e8, prec_bits=None,
                use_bla=True, bla_minskip=8, interior=True, probe=True, threads=None, full_h=0, row_base=0,
-               want_orbit=False, sub=1, periodic=True, view_center=False):
+               want_orbit=False, sub=1, periodic=True, view_center=False, force_fe=False, force_periodic=False):
     """Whole floatexp frame (Compute -> reference -> BLA -> probe -> pass).  With
     want_orbit, info["orbit"] holds the reference orbit's shadows (zf, zfe_m, zfe_e,
     (re, im) interleaved).  With sub > 1, info["cells"] holds the (H sub) x (W sub)
@@ -154,6 +155,7 @@ def compute_fe(cre, cim, scale, W, H, mxit, c_method=0, escape_radius=1e8, prec_
     f.interior, f.probe = int(interior), int(probe)
     f.no_periodic = int(not periodic)
     f.view_center = int(view_center)
+    f.force_fe, f.force_periodic = int(force_fe), int(force_periodic)
     f.threads = threads or os.cpu_count()
     val = np.zeros(W * H, np.float64)
     nesc = np.zeros(W * H, np.int32)
