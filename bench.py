@@ -316,7 +316,11 @@ def double_leg(ctx, no_cpu, reps=5):
            "gpix_iter_s_frame": nominal / float(np.median(tot)) / 1e9,
            "gpix_iter_s_kernel": nominal / (float(np.median(ker)) / 1e3) / 1e9,
            "nominal_pixel_iterations": nominal, "passes": info["passes"], "sa_it": info["sa_it"],
-           "executed_steps": int(info["steps"]), "bla_applies": int(info["bla_applies"]), "rebases": int(info["rebases"])}
+           "executed_steps": int(info["steps"]), "bla_applies": int(info["bla_applies"]), "rebases": int(info["rebases"]),
+           # the frame's longest dependent chain: a sample that runs from the series skip to
+           # max_iter, one step after the other (its steps cannot overlap)
+           "longest_chain_steps": mxit - 1 - int(info["sa_it"]),
+           "ns_per_chain_step": float(np.median(ker)) * 1e6 / (mxit - 1 - int(info["sa_it"]))}
     log(f"double path: frame {out['frame_ms']:.1f} ms, kernels {out['kernel_ms']:.2f} ms")
     if not no_cpu:
         import mxo

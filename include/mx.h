@@ -161,6 +161,10 @@ typedef struct {
      * lean steps as soon as fewer than quorum/64 of the lanes that started it are still
      * stepping (the others would idle until the run ends) */
     int32_t quorum;
+    /* plain-double form only, results do not depend on it: 0 stages the orbit (all of
+     * it, or the prefix that fits) in LDS unless the tile has both EDE and interior
+     * detection, 1 always, -1 never */
+    int32_t lds_stage;
 } mx_tile;
 
 /* A plan holds the device copy of a tile's inputs and outputs so that launches

@@ -16,6 +16,12 @@
 // Execution model as in the floatexp kernels: a persistent grid whose lanes refill
 // from per-XCD atomic work queues (wave-aggregated fetch-add), samples handed out
 // in 8x8 tiles, or taken from a list of sample indices for a later reference pass.
+//
+// The orbit in LDS: every step reads the next orbit entry, a dependent chain of one
+// load per iteration that the latency of L2 would otherwise set.  When the launch
+// asks for it, each workgroup (1024 lanes, one per CU) first copies the orbit's first
+// `nstage` entries -- the whole orbit when it fits, else the prefix that every rebase
+// restarts in -- into LDS, and the steps read them from there.
 #include <climits>
 
 #include "mx_dbl.h"
@@ -137,11 +143,34 @@ __device__ __noinline__ BlaRes try_bla(const double* __restrict__ bla_r2, const 
     return r;
 }
 
+// The k >= 1 step (mandel_perturbation.cpp:2444-2483), shared by the full and the lean
+// path (this file is built without FP contraction: identical expressions, identical
+// bits): dz' = 2 dz X + dz^2 + dc, and under EDE dd' = 2 (dd X + dz (D + dd)).
+__device__ __forceinline__ void dz_step(double dzr, double dzi, double dcr, double dci, double xr, double xi, double& ndzr,
+                                        double& ndzi) {
+    const double dzr2 = dzr * dzr, dzi2 = dzi * dzi;
+    ndzr = (dzr * xr - dzi * xi) * 2 + dzr2 - dzi2 + dcr;
+    ndzi = (dzr * xi + dzi * xr) * 2 + dzr * dzi * 2 + dci;
+}
+__device__ __forceinline__ void dd_step(double ddr, double ddi, double dzr, double dzi, double xr, double xi, double2 D,
+                                        double& nddr, double& nddi) {
+    nddr = (ddr * xr - ddi * xi + dzr * (D.x + ddr) - dzi * (D.y + ddi)) * 2;
+    nddi = (ddr * xi + ddi * xr + dzr * (D.y + ddi) + dzi * (D.x + ddr)) * 2;
+}
+
+// Orbit entries: the LDS copy of the first `n` entries, the rest from global memory.
+struct OrbitSrc {
+    const double2* lds;
+    int n;
+    const double2* zf;
+    __device__ __forceinline__ double2 operator[](int k) const { return k < n ? lds[k] : zf[k]; }
+};
+
 // One iteration of the stepParallel loop body for one lane.  Returns 1 when the
 // sample is finished (n, v set; n == INT_MIN: unresolved, *gp set).
 template <bool INTERIOR, bool EDE>
-__device__ __forceinline__ int trip(const MxRefDev& R, const int2* __restrict__ lev, const MxTileDev& T, Pd& p, double& v, int& n,
-                                    double& gp, Counters& c) {
+__device__ __forceinline__ int trip(const MxRefDev& R, const int2* __restrict__ lev, const MxTileDev& T, const OrbitSrc& Z, Pd& p,
+                                    double& v, int& n, double& gp, Counters& c) {
     if (p.j >= T.mxit) {
         v = -2.0;
         n = -1;
@@ -187,13 +216,13 @@ __device__ __forceinline__ int trip(const MxRefDev& R, const int2* __restrict__ 
         }
     }
     if (!p.xv) {
-        if (p.k != 0) p.Xc = R.zf[p.k - 1];
-        p.Xn = R.zf[p.k];
+        if (p.k != 0) p.Xc = Z[p.k - 1];
+        p.Xn = Z[p.k];
     }
     // the entry after this step's, issued now so its latency overlaps the step; k can
     // equal mx_ref_it only on the first step after a series skip to the reference's
     // escape, and then the next step rebases, so the clamped entry goes unused
-    const double2 Xpf = R.zf[min(p.k + 1, R.mx_ref_it)];
+    const double2 Xpf = Z[min(p.k + 1, R.mx_ref_it)];
     double dr = 0.0, di = 0.0;
     if (EDE) {
         // dd' = 2 (dd X + dz (D + dd)) from the dz before this step (2444-2456)
@@ -202,24 +231,20 @@ __device__ __forceinline__ int trip(const MxRefDev& R, const int2* __restrict__ 
             t2 = (p.dzr * (1 + p.ddr) - p.dzi * (1 + p.ddi)) * 2;
             p.ddi = (p.dzr * (1 + p.ddi) + p.dzi * (1 + p.ddr)) * 2;
         } else {
-            const double2 D = R.df[p.k - 1];
-            t2 = (p.ddr * p.Xc.x - p.ddi * p.Xc.y + p.dzr * (D.x + p.ddr) - p.dzi * (D.y + p.ddi)) * 2;
-            p.ddi = (p.ddr * p.Xc.y + p.ddi * p.Xc.x + p.dzr * (D.y + p.ddi) + p.dzi * (D.x + p.ddr)) * 2;
+            dd_step(p.ddr, p.ddi, p.dzr, p.dzi, p.Xc.x, p.Xc.y, R.df[p.k - 1], t2, p.ddi);
         }
         p.ddr = t2;
         const double2 Dk = R.df[p.k];
         dr = p.ddr + Dk.x;
         di = p.ddi + Dk.y;
     }
-    const double dzr2 = p.dzr * p.dzr, dzi2 = p.dzi * p.dzi;
     double tmp, dzi;
     if (p.k == 0) {
+        const double dzr2 = p.dzr * p.dzr, dzi2 = p.dzi * p.dzi;
         tmp = dzr2 - dzi2 + p.dcr;
         dzi = p.dzr * p.dzi * 2 + p.dci;
     } else {
-        const double xr = p.Xc.x, xi = p.Xc.y;
-        tmp = (p.dzr * xr - p.dzi * xi) * 2 + dzr2 - dzi2 + p.dcr;
-        dzi = (p.dzr * xi + p.dzi * xr) * 2 + p.dzr * p.dzi * 2 + p.dci;
+        dz_step(p.dzr, p.dzi, p.dcr, p.dci, p.Xc.x, p.Xc.y, tmp, dzi);
     }
     const double dzr = tmp;
     const double2 Xn = p.Xn;
@@ -310,6 +335,94 @@ __device__ __forceinline__ int trip(const MxRefDev& R, const int2* __restrict__ 
     return 0;
 }
 
+// Lean steps: the trip's k >= 1 step for up to T.chunk iterations while it is provably
+// the whole iteration, so results are bit-identical to trips alone.  A lane stops, its
+// state untouched, before a step at which the trip would do more: the iteration
+// limit, the end of the reference (k + 1 >= mx_ref_it, the rebase there), an
+// exact-repeat save or the index back at the saved one, an active periodicity
+// confirmation, a BLA try (|dz|^2 below the largest radius at an index whose level
+// search is not empty: k == 2 or k - 2 a multiple of 2^minlevel); and after a step
+// whose outcome is not plain: escape, a close return on a periodicity check,
+// |z| < |dz|.  The periodicity check's save (every 16 iterations) is done here as the
+// trip does it.  The wave leaves a run together once fewer than quorum/64 of the lanes
+// that started it still step.
+template <bool INTERIOR, bool EDE>
+__device__ __forceinline__ void fast_steps(const MxRefDev& R, const MxTileDev& T, const OrbitSrc& Z, Pd& p, Counters& c) {
+    int lim = min(T.chunk, T.mxit - p.j);
+    lim = min(lim, R.mx_ref_it - 1 - p.k);
+    if (p.k == 0 || !p.xv) lim = 0;
+    const bool int_rep = INTERIOR && (T.flags & MX_F_INT_REP);
+    if (int_rep) {
+        lim = min(lim, p.save_j2 + p.twin2 - p.j);
+        if (p.sk - p.k >= 0) lim = min(lim, p.sk - p.k);
+    }
+    if (INTERIOR && p.conf_P > 0) lim = 0;
+    const bool can_bla = (T.flags & MX_F_BLA) != 0;
+    const int amask = (1 << R.minlevel) - 1;
+    const bool chk_ok = INTERIOR && (T.flags & MX_F_INTERIOR) && p.conf_giveup < 3;
+    const int need = (__popcll(__ballot(lim > 0)) * T.quorum + 63) >> 6;
+    if (lim <= 0) return;
+    double dzr = p.dzr, dzi = p.dzi, ddr = p.ddr, ddi = p.ddi;
+    double w2 = dzr * dzr + dzi * dzi;
+    double2 Xc = p.Xc, Xn = p.Xn;
+    double2 D = EDE ? R.df[p.k - 1] : make_double2(0.0, 0.0);
+    int k = p.k, j = p.j, n = 0;
+    while (n < lim) {
+        if (can_bla && w2 < R.rmax2 && k >= 2 && (k == 2 || ((k - 2) & amask) == 0)) break;
+        const double2 Xpf = Z[k + 1];
+        double nddr = 0.0, nddi = 0.0;
+        double2 Dk = D;
+        if (EDE) {
+            dd_step(ddr, ddi, dzr, dzi, Xc.x, Xc.y, D, nddr, nddi);
+            Dk = R.df[k];
+        }
+        double ndzr, ndzi;
+        dz_step(dzr, dzi, p.dcr, p.dci, Xc.x, Xc.y, ndzr, ndzi);
+        const double zr = ndzr + Xn.x, zi = ndzi + Xn.y;
+        const double zrad = zr * zr + zi * zi;
+        const double dz2 = ndzr * ndzr + ndzi * ndzi;
+        bool stop = zrad > T.esc2 || zrad < dz2;
+        bool save = false;
+        if (INTERIOR && chk_ok && (j & 15) == 0) {
+            const double pr = zr - p.zsr, pi = zi - p.zsi;
+            stop = stop || pr * pr + pi * pi < T.interior_eps2 * zrad;
+            save = j - p.save_j >= p.period_win;
+        }
+        if (stop) break;
+        if (INTERIOR && save) {
+            p.zsr = zr;
+            p.zsi = zi;
+            p.save_j = j;
+            p.period_win += p.period_win;
+        }
+        dzr = ndzr;
+        dzi = ndzi;
+        if (EDE) {
+            ddr = nddr;
+            ddi = nddi;
+            D = Dk;
+        }
+        w2 = dz2;
+        Xc = Xn;
+        Xn = Xpf;
+        ++k;
+        ++j;
+        ++n;
+        if (__popcll(__ballot(1)) < need) break;
+    }
+    p.dzr = dzr;
+    p.dzi = dzi;
+    if (EDE) {
+        p.ddr = ddr;
+        p.ddi = ddi;
+    }
+    p.Xc = Xc;
+    p.Xn = Xn;
+    p.k = k;
+    p.j = j;
+    c.steps += (unsigned long long)n;
+}
+
 // Map a queue item to a sample index; false for the padding lanes of edge tiles.
 __device__ __forceinline__ bool item_to_sample(const MxTileDev& T, unsigned item, int& sidx) {
     if (T.list) {
@@ -324,8 +437,15 @@ __device__ __forceinline__ bool item_to_sample(const MxTileDev& T, unsigned item
     return sidx != T.skip_sample;
 }
 
-template <bool INTERIOR, bool EDE>
-__global__ __launch_bounds__(256) void mx_dbl_kernel(MxRefDev R, const int2* __restrict__ lev, MxTileDev T) {
+template <bool INTERIOR, bool EDE, bool STAGE>
+__global__ __launch_bounds__(STAGE ? 1024 : 256) void mx_dbl_kernel(MxRefDev R, const int2* __restrict__ lev, MxTileDev T, int nstage) {
+    extern __shared__ double2 orbit_lds[];
+    OrbitSrc Z{orbit_lds, 0, R.zf};
+    if (STAGE) {
+        for (int k = (int)threadIdx.x; k < nstage; k += (int)blockDim.x) orbit_lds[k] = R.zf[k];
+        __syncthreads();
+        Z.n = nstage;
+    }
     const int lane = __lane_id();
     const unsigned long long lanemask_lt = (1ull << lane) - 1ull;
     int q = blockIdx.x % MX_NQUEUE;
@@ -368,7 +488,7 @@ __global__ __launch_bounds__(256) void mx_dbl_kernel(MxRefDev R, const int2* __r
         if (active) {
             double v = 0.0, gp = 0.0;
             int n = 0;
-            if (trip<INTERIOR, EDE>(R, lev, T, p, v, n, gp, c)) {
+            if (trip<INTERIOR, EDE>(R, lev, T, Z, p, v, n, gp, c)) {
                 if (n == INT_MIN) {
                     const unsigned slot = atomicAdd(T.unres_count, 1u);
                     T.unres_idx[slot] = sidx;
@@ -379,6 +499,8 @@ __global__ __launch_bounds__(256) void mx_dbl_kernel(MxRefDev R, const int2* __r
                     if (T.out) T.out[sidx] = n < 0 ? -2.0f : (float)v;
                 }
                 active = false;
+            } else {
+                fast_steps<INTERIOR, EDE>(R, T, Z, p, c);
             }
         }
     }
@@ -393,15 +515,29 @@ __global__ __launch_bounds__(256) void mx_dbl_kernel(MxRefDev R, const int2* __r
 
 } // namespace
 
-hipError_t mx_launch_dbl(const MxRefDev& R, const int2* lev, const MxTileDev& T, int blocks, int threads, hipStream_t stream) {
-    const bool ede = T.flags & MX_F_EDE, interior = T.flags & MX_F_INTERIOR;
-    if (ede && interior)
-        hipLaunchKernelGGL((mx_dbl_kernel<true, true>), dim3(blocks), dim3(threads), 0, stream, R, lev, T);
-    else if (ede)
-        hipLaunchKernelGGL((mx_dbl_kernel<false, true>), dim3(blocks), dim3(threads), 0, stream, R, lev, T);
-    else if (interior)
-        hipLaunchKernelGGL((mx_dbl_kernel<true, false>), dim3(blocks), dim3(threads), 0, stream, R, lev, T);
-    else
-        hipLaunchKernelGGL((mx_dbl_kernel<false, false>), dim3(blocks), dim3(threads), 0, stream, R, lev, T);
+namespace {
+
+template <bool INTERIOR, bool EDE>
+hipError_t launch_dbl(const MxRefDev& R, const int2* lev, const MxTileDev& T, int blocks, int threads, int nstage, hipStream_t stream) {
+    if (nstage > 0) {
+        const size_t bytes = (size_t)nstage * sizeof(double2);
+        hipError_t e = hipFuncSetAttribute((const void*)mx_dbl_kernel<INTERIOR, EDE, true>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                           (int)bytes);
+        if (e != hipSuccess) return e;
+        hipLaunchKernelGGL((mx_dbl_kernel<INTERIOR, EDE, true>), dim3(blocks), dim3(1024), bytes, stream, R, lev, T, nstage);
+    } else {
+        hipLaunchKernelGGL((mx_dbl_kernel<INTERIOR, EDE, false>), dim3(blocks), dim3(threads), 0, stream, R, lev, T, 0);
+    }
     return hipGetLastError();
+}
+
+} // namespace
+
+// nstage > 0: the orbit's first nstage entries staged in LDS, blocks of 1024 lanes
+hipError_t mx_launch_dbl(const MxRefDev& R, const int2* lev, const MxTileDev& T, int blocks, int threads, hipStream_t stream, int nstage) {
+    const bool ede = T.flags & MX_F_EDE, interior = T.flags & MX_F_INTERIOR;
+    if (ede && interior) return launch_dbl<true, true>(R, lev, T, blocks, threads, nstage, stream);
+    if (ede) return launch_dbl<false, true>(R, lev, T, blocks, threads, nstage, stream);
+    if (interior) return launch_dbl<true, false>(R, lev, T, blocks, threads, nstage, stream);
+    return launch_dbl<false, false>(R, lev, T, blocks, threads, nstage, stream);
 }

@@ -137,6 +137,7 @@ extern "C" int mx_ctx_create(int device, mx_ctx** out) {
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device) != hipSuccess) return fail(MX_E_HIP);
     ctx->cu_count = prop.multiProcessorCount;
+    ctx->lds_max = (int)prop.sharedMemPerBlock;
     if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) return fail(MX_E_HIP);
     if (hipHostMalloc((void**)&ctx->cancel_host, sizeof(int), hipHostMallocMapped) != hipSuccess) return fail(MX_E_HIP);
     *ctx->cancel_host = 0;
@@ -448,6 +449,14 @@ extern "C" int mx_plan_create(mx_ctx* ctx, const mx_ref* ref, const mx_tile* t, 
     const int work_tiles = t->list ? (int)((nlist + 63) / 64) : T.ntiles;
     plan->work_tiles = work_tiles;
     plan->blocks = std::max(1, std::min(t->grid_blocks > 0 ? t->grid_blocks : waves / 4, work_tiles));
+    if (dbl && (t->lds_stage > 0 || (t->lds_stage == 0 && !(t->ede && plan->interior)))) {
+        // the orbit in LDS: one workgroup of 16 waves per CU
+        plan->nstage = std::min(ref->dev.nent, ctx->lds_max / (int)sizeof(double2));
+        if (plan->nstage > 0) {
+            plan->threads = 1024;
+            plan->blocks = std::max(1, std::min(t->grid_blocks > 0 ? t->grid_blocks : ctx->cu_count, (work_tiles + 15) / 16));
+        }
+    }
     *out = plan;
     return MX_OK;
 }
@@ -463,7 +472,7 @@ extern "C" int mx_plan_launch(mx_plan* plan) {
     if (plan->form == MX_FORM_DOUBLE) {
         MxRefDev R = plan->ref->dev;
         R.minlevel = plan->minlevel;
-        MX_HIP(mx_launch_dbl(R, plan->ref->lev, T, plan->blocks, plan->threads, ctx->stream));
+        MX_HIP(mx_launch_dbl(R, plan->ref->lev, T, plan->blocks, plan->threads, ctx->stream, plan->nstage));
     } else {
         MX_HIP(launch_jobs(plan->form, plan->job_dev, 1, plan->ede, plan->interior, plan->blocks, plan->threads, ctx->stream));
     }

@@ -136,6 +136,7 @@ struct mx_frame {
 struct mx_ctx {
     int device = 0;
     int cu_count = 0;
+    int lds_max = 0;                          // dynamic LDS bytes a workgroup may use
     unsigned char* noalloc_scratch = nullptr; // device address of this device's fixed scratch area
     hipStream_t stream = nullptr;
     int* cancel_host = nullptr;     // pinned, host-mapped flag
@@ -173,6 +174,7 @@ struct mx_plan {
     MxJob* job_dev = nullptr;   // and its device copy, inside `block`
     int work_tiles = 0;         // 64-sample work items of one launch
     int grid_request = 0;       // caller's grid size (0: default)
+    int nstage = 0;             // plain-double form: orbit entries staged in LDS
 };
 
 struct mx_batch {
@@ -196,5 +198,5 @@ struct mx_graph {
 hipError_t mx_launch_fe64(const MxJob* jobs, int njobs, bool ede, bool interior, int blocks, int threads, hipStream_t stream);
 hipError_t mx_launch_fe32(const MxJob* jobs, int njobs, bool ede, bool interior, int blocks, int threads, hipStream_t stream);
 hipError_t mx_launch_sc64(const MxJob* jobs, int njobs, bool ede, bool interior, int blocks, int threads, hipStream_t stream);
-hipError_t mx_launch_dbl(const MxRefDev& R, const int2* lev, const MxTileDev& T, int blocks, int threads, hipStream_t stream);
+hipError_t mx_launch_dbl(const MxRefDev& R, const int2* lev, const MxTileDev& T, int blocks, int threads, hipStream_t stream, int nstage);
 hipError_t mx_launch_shallow(const MxShallowDev& S, bool ede, int blocks, hipStream_t stream);

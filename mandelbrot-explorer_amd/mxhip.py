@@ -40,7 +40,8 @@ class Tile(C.Structure):
                 ("dx_m", C.c_double), ("dx_e", C.c_int32), ("skip_sample", C.c_int32),
                 ("dcol_d", C.c_void_p), ("drow_d", C.c_void_p), ("list", C.c_void_p), ("list_len", C.c_int32),
                 ("chunk", C.c_int32), ("grid_blocks", C.c_int32), ("n_samples", C.c_int32),
-                ("sample_dc_m", C.c_void_p), ("sample_dc_e", C.c_void_p), ("quorum", C.c_int32)]
+                ("sample_dc_m", C.c_void_p), ("sample_dc_e", C.c_void_p), ("quorum", C.c_int32),
+                ("lds_stage", C.c_int32)]
 
 
 class Request(C.Structure):
@@ -294,9 +295,9 @@ class Context:
         return Ref(ref)
 
     def tile(self, fv, stride=1, form=MX_FORM_FE64, ede=False, interior=True, int_rep=None, bla=True,
-             bla_minlevel=None, mxit=None, samples=None, chunk=0, grid_blocks=0, quorum=0):
+             bla_minlevel=None, mxit=None, samples=None, chunk=0, grid_blocks=0, quorum=0, lds_stage=0):
         t = Tile()
-        t.chunk, t.grid_blocks, t.quorum = chunk, grid_blocks, quorum
+        t.chunk, t.grid_blocks, t.quorum, t.lds_stage = chunk, grid_blocks, quorum, lds_stage
         t.width, t.height, t.stride = fv["W"], fv["H"], stride
         t.max_iter = mxit or fv["mxit"]
         t.esc2, t.form, t.ede = fv["esc2"], form, int(ede)

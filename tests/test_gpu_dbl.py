@@ -121,3 +121,31 @@ def test_escalated_frame_matches_oracle(ctx):
     _check(val, nesc, val_o, nesc_o, "escalated")
     for k in ("steps", "bla_applies", "bla_skipped", "rebases"):
         assert info[k] == info_o[k], k
+
+
+@pytest.mark.parametrize("name,ede", [("c1", False), ("ticktock", False), ("point31", True), ("c1", True)])
+def test_orbit_in_lds_changes_nothing(ctx, name, ede):
+    """The orbit staged in LDS (all of it for C1's 8,193 entries; the prefix for
+    ticktock's longer orbit) gives the global-memory launch's results and counters bit
+    for bit, glitched samples included."""
+    cre, cim, scale, W, H, mxit, R, pb = FRAMES[name]
+    print(f"[dbl lds] {name} ede={ede}", flush=True)
+    fv = mxhip.frame_prepare(cre, cim, scale, W, H, mxit, escape_radius=R, prec_bits=pb, c_method=2 if ede else 0)
+    ref = ctx.upload(fv)
+    try:
+        outs = []
+        for stage in (-1, 1):
+            plan = ctx.plan(ref, ctx.tile(fv, form=mxhip.MX_FORM_DOUBLE, interior=True, ede=ede, lds_stage=stage))
+            try:
+                plan.launch()
+                ms = plan.wait()
+                outs.append(plan.read() + plan.unresolved() + (ms,))
+            finally:
+                plan.destroy()
+    finally:
+        ref.release()
+    (n0, v0, o0, s0, u0, g0, ms0), (n1, v1, o1, s1, u1, g1, ms1) = outs
+    print(f"[dbl lds] {name}: global {ms0:.3f} ms, LDS {ms1:.3f} ms (orbit {fv['len']} entries)", flush=True)
+    assert np.array_equal(n0, n1) and np.array_equal(v0.view(np.int64), v1.view(np.int64))
+    assert np.array_equal(s0, s1)
+    assert np.array_equal(np.sort(u0), np.sort(u1))
