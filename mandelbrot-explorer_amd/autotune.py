@@ -26,7 +26,7 @@ import os
 
 import mxhip
 
-CHUNKS = (8, 16, 24, 32, 48, 64, 96, 128, 192, 256)
+CHUNKS = (8, 16, 24, 32, 48, 64, 96, 128, 192, 256, 384, 512)
 QUORUMS = (1, 8, 16, 24, 32, 40, 48, 56)
 BLOCKS_PER_CU = (2, 3, 4, 5, 6, 8)
 MINLEVELS = (1, 2, 3, 4, 5, 6, 7, 8, 9, 10)
