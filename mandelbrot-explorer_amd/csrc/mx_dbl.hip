@@ -52,7 +52,7 @@ __device__ __forceinline__ unsigned long long wave_sum(unsigned long long v) {
 
 // Series seeds (the Horner loops of mandel_perturbation.cpp:2319-2330) with the
 // explicit complex formulas of mx_dbl.h.
-__device__ __forceinline__ MxC sa_horner(const double2* __restrict__ C, int order, MxC delta, MxC dc) {
+__device__ __forceinline__ MxC sa_horner(GPtr<const double2> C, int order, MxC delta, MxC dc) {
     MxC z = {0.0, 0.0};
     for (int x = order; x >= 0; --x) {
         const double2 a = C[x];
@@ -103,9 +103,9 @@ struct BlaRes {
 };
 
 template <bool EDE>
-__device__ __noinline__ BlaRes try_bla(const double* __restrict__ bla_r2, const double4* __restrict__ bla_coef,
-                                       const double4* __restrict__ bla_der, const double2* __restrict__ zf,
-                                       const int2* __restrict__ lev, int nlev, int minlevel, int mx_ref_it, int s, double dzr,
+__device__ __noinline__ BlaRes try_bla(GPtr<const double> bla_r2, GPtr<const double4> bla_coef,
+                                       GPtr<const double4> bla_der, GPtr<const double2> zf,
+                                       GPtr<const int2> lev, int nlev, int minlevel, int mx_ref_it, int s, double dzr,
                                        double dzi, double ddr, double ddi, double dcr, double dci, double esc2) {
     BlaRes r = {0.0, 0.0, 0.0, 0.0, 0};
     if (s < 1) return r;
@@ -160,16 +160,16 @@ __device__ __forceinline__ void dd_step(double ddr, double ddi, double dzr, doub
 
 // Orbit entries: the LDS copy of the first `n` entries, the rest from global memory.
 struct OrbitSrc {
-    const double2* lds;
+    const double2 MX_AS_LDS* lds;
     int n;
-    const double2* zf;
-    __device__ __forceinline__ double2 operator[](int k) const { return k < n ? lds[k] : zf[k]; }
+    GPtr<const double2> zf;
+    __device__ __forceinline__ double2 operator[](int k) const { return k < n ? double2(lds[k]) : double2(zf[k]); }
 };
 
 // One iteration of the stepParallel loop body for one lane.  Returns 1 when the
 // sample is finished (n, v set; n == INT_MIN: unresolved, *gp set).
 template <bool INTERIOR, bool EDE>
-__device__ __forceinline__ int trip(const MxRefDev& R, const int2* __restrict__ lev, const MxTileDev& T, const OrbitSrc& Z, Pd& p,
+__device__ __forceinline__ int trip(const MxRefDev& R, GPtr<const int2> lev, const MxTileDev& T, const OrbitSrc& Z, Pd& p,
                                     double& v, int& n, double& gp, Counters& c) {
     if (p.j >= T.mxit) {
         v = -2.0;
@@ -365,7 +365,7 @@ __device__ __forceinline__ void fast_steps(const MxRefDev& R, const MxTileDev& T
     double dzr = p.dzr, dzi = p.dzi, ddr = p.ddr, ddi = p.ddi;
     double w2 = dzr * dzr + dzi * dzi;
     double2 Xc = p.Xc, Xn = p.Xn;
-    double2 D = EDE ? R.df[p.k - 1] : make_double2(0.0, 0.0);
+    double2 D = EDE ? double2(R.df[p.k - 1]) : make_double2(0.0, 0.0);
     int k = p.k, j = p.j, n = 0;
     while (n < lim) {
         if (can_bla && w2 < R.rmax2 && k >= 2 && (k == 2 || ((k - 2) & amask) == 0)) break;
@@ -438,9 +438,9 @@ __device__ __forceinline__ bool item_to_sample(const MxTileDev& T, unsigned item
 }
 
 template <bool INTERIOR, bool EDE, bool STAGE>
-__global__ __launch_bounds__(STAGE ? 1024 : 256) void mx_dbl_kernel(MxRefDev R, const int2* __restrict__ lev, MxTileDev T, int nstage) {
+__global__ __launch_bounds__(STAGE ? 1024 : 256) void mx_dbl_kernel(MxRefDev R, GPtr<const int2> lev, MxTileDev T, int nstage) {
     extern __shared__ double2 orbit_lds[];
-    OrbitSrc Z{orbit_lds, 0, R.zf};
+    OrbitSrc Z{(const double2 MX_AS_LDS*)orbit_lds, 0, R.zf};
     if (STAGE) {
         for (int k = (int)threadIdx.x; k < nstage; k += (int)blockDim.x) orbit_lds[k] = R.zf[k];
         __syncthreads();

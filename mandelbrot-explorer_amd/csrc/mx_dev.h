@@ -15,6 +15,32 @@
 
 #include "mx_fe.h"
 
+// A pointer into device (global) memory.  The layouts below are written by host code
+// and read by the kernels from a job table in memory, where a plain pointer is
+// generic: every access through it would be a flat_* instruction with a 64-bit VGPR
+// address, which also waits on the LDS counter.  In the device compile this wrapper
+// hands out address-space-1 lvalues, so the same accesses become global_* loads and
+// stores with an SGPR base and a 32-bit offset.  It holds one plain pointer (same
+// size and layout in host and device code) and converts to and from T*.
+#if defined(__HIP_DEVICE_COMPILE__)
+#define MX_AS_GLOBAL __attribute__((address_space(1)))
+#define MX_AS_LDS __attribute__((address_space(3)))
+#else
+#define MX_AS_GLOBAL
+#define MX_AS_LDS
+#endif
+template <class T>
+struct GPtr {
+    T* p;
+    GPtr() = default;
+    __host__ __device__ constexpr GPtr(T* q) : p(q) {}
+    __host__ __device__ constexpr GPtr(decltype(nullptr)) : p(nullptr) {}
+    __host__ __device__ operator T*() const { return p; }
+    __host__ __device__ T MX_AS_GLOBAL& operator[](long long i) const { return ((T MX_AS_GLOBAL*)p)[i]; }
+    __host__ __device__ T MX_AS_GLOBAL& operator*() const { return *(T MX_AS_GLOBAL*)p; }
+    __host__ __device__ GPtr operator+(long long i) const { return GPtr(p + i); }
+};
+
 #define MX_MAXLEV 32
 #define MX_NQUEUE 8 // one work queue per XCD
 
@@ -39,63 +65,63 @@ inline int mx_tile_flags(bool ede, bool interior, bool int_rep, bool bla, bool p
 }
 
 struct MxRefDev {
-    const double2* zf;
-    const double2* zfe_m;
-    const int2* zfe_e;
-    const float2* zf32;
-    const double* bla_r2;
-    const double4* bla_coef;
-    const float4* bla_coeff; // mixed form: coefficient mantissas (ar, ai) and (br, bi), FP32
-    const int2* bla_coefe;   // mixed form: exponents of A and of B
+    GPtr<const double2> zf;
+    GPtr<const double2> zfe_m;
+    GPtr<const int2> zfe_e;
+    GPtr<const float2> zf32;
+    GPtr<const double> bla_r2;
+    GPtr<const double4> bla_coef;
+    GPtr<const float4> bla_coeff;    // mixed form: coefficient mantissas (ar, ai) and (br, bi), FP32
+    GPtr<const int2> bla_coefe;      // mixed form: exponents of A and of B
     int mx_ref_it;
     int nlev;
     int minlevel;
-    int period; // > 0: periodic reference, orbit read at (m-1) mod period
-    int nent;   // entries in zf/zfe/zf32: mx_ref_it+1, or period+2 (two wrap copies)
+    int period;                      // > 0: periodic reference, orbit read at (m-1) mod period
+    int nent;                        // entries in zf/zfe/zf32: mx_ref_it+1, or period+2 (two wrap copies)
     double rmax2;
     // plain-double path
-    const double* z_m3;    // |X|^2 / 1e6 per entry (glitch metric)
-    const double2* sa_A;   // series coefficients A_0..A_29
+    GPtr<const double> z_m3;         // |X|^2 / 1e6 per entry (glitch metric)
+    GPtr<const double2> sa_A;        // series coefficients A_0..A_29
     double2 sa_delta;
     int sa_it, sa_order;
-    const double2* df;       // reference derivative D (EDE)
-    const double4* bla_der;  // BLA derivative couplings (cr, ci, er, ei) (EDE)
-    const double2* sa_B;     // derivative-delta series (EDE)
+    GPtr<const double2> df;          // reference derivative D (EDE)
+    GPtr<const double4> bla_der;     // BLA derivative couplings (cr, ci, er, ei) (EDE)
+    GPtr<const double2> sa_B;        // derivative-delta series (EDE)
 };
 
 struct MxTileDev {
-    const Fe* dcol; // per column dc (real part)
-    const Fe* drow; // per row dc (imag part)
-    int W, H;       // sample grid of the tile
-    int stride;     // 1: every sample; 8: the interior-gate probe grid
-    int nx, ny;     // samples in this launch (after stride)
-    int tiles_x;    // 8x8 lane tiles per launch row
+    GPtr<const Fe> dcol;             // per column dc (real part)
+    GPtr<const Fe> drow;             // per row dc (imag part)
+    int W, H;                        // sample grid of the tile
+    int stride;                      // 1: every sample; 8: the interior-gate probe grid
+    int nx, ny;                      // samples in this launch (after stride)
+    int tiles_x;                     // 8x8 lane tiles per launch row
     int ntiles;
     int mxit;
     int flags;
     double esc2;
     double interior_eps2;
     int interior_confirm;
-    int skip_sample; // sample index not computed (the reference pixel), -1 none
-    int chunk;       // lean steps per trip between full-path checks (results do not depend on it)
-    int quorum;      // a lean run ends once fewer than quorum/64 of its lanes still step (neither)
+    int skip_sample;                 // sample index not computed (the reference pixel), -1 none
+    int chunk;                       // lean steps per trip between full-path checks (results do not depend on it)
+    int quorum;                      // a lean run ends once fewer than quorum/64 of its lanes still step (neither)
     Fe dxfe;
-    double* val;         // W*H
-    int* nesc;           // W*H
-    float* out;          // W*H (reference-compatible float buffer), may be null
-    unsigned* queue;     // MX_NQUEUE counters, zeroed before launch
-    unsigned long long* stats; // steps, bla applies, bla skipped iterations, rebases
-    const volatile int* cancel; // host-mapped; nonzero => lanes stop taking work
+    GPtr<double> val;                // W*H
+    GPtr<int> nesc;                  // W*H
+    GPtr<float> out;                 // W*H (reference-compatible float buffer), may be null
+    GPtr<unsigned> queue;            // MX_NQUEUE counters, zeroed before launch
+    GPtr<unsigned long long> stats;  // steps, bla applies, bla skipped iterations, rebases
+    GPtr<const volatile int> cancel; // host-mapped; nonzero => lanes stop taking work
     // plain-double path
-    const double* dcol_d;  // per column dc (double)
-    const double* drow_d;  // per row dc (double)
-    const int* list;       // sample indices to compute (null: the grid)
-    int list_len;          // entries of `list`, or of `sdc`
-    const Fe* sdc;         // floatexp forms: explicit (re, im) dc per sample (null: the grid);
-                           // outputs are then indexed by list entry
-    int* unres_idx;        // unresolved samples of this launch
-    double* unres_gp;      // their glitch metric
-    unsigned* unres_count;
+    GPtr<const double> dcol_d;       // per column dc (double)
+    GPtr<const double> drow_d;       // per row dc (double)
+    GPtr<const int> list;            // sample indices to compute (null: the grid)
+    int list_len;                    // entries of `list`, or of `sdc`
+    GPtr<const Fe> sdc;              // floatexp forms: explicit (re, im) dc per sample (null: the grid);
+                                     // outputs are then indexed by list entry
+    GPtr<int> unres_idx;             // unresolved samples of this launch
+    GPtr<double> unres_gp;           // their glitch metric
+    GPtr<unsigned> unres_count;
 };
 
 // One tile of a launch and the reference it is computed against.  A launch runs its
@@ -104,7 +130,7 @@ struct MxTileDev {
 // different size, depth and iteration limit share one persistent grid.
 struct MxJob {
     MxRefDev R;
-    const int2* lev;
+    GPtr<const int2> lev;
     MxTileDev T;
 };
 
@@ -113,18 +139,18 @@ struct MxJob {
 // the sub*sub - 1 cells of each of the nlist flagged pixels in `list` (nitems =
 // nlist (sub*sub - 1)), outputs indexed by item.
 struct MxShallowDev {
-    double c0r, c0i, dx, dy; // truncated mpf grid values
+    double c0r, c0i, dx, dy;         // truncated mpf grid values
     int W, H, sub, mxit;
     double esc2;
-    const int* list; // flagged pixels (row * W + column), null: the base grid
+    GPtr<const int> list;            // flagged pixels (row * W + column), null: the base grid
     int nlist;
     int tiles_x;
     unsigned nitems;
     int chunk, quorum;
-    double* val;
-    int* nesc;
-    float* out;
-    unsigned* queue;            // MX_NQUEUE counters, zeroed before launch
-    unsigned long long* stats;  // iterations
-    const volatile int* cancel; // host-mapped; nonzero => lanes stop taking work
+    GPtr<double> val;
+    GPtr<int> nesc;
+    GPtr<float> out;
+    GPtr<unsigned> queue;            // MX_NQUEUE counters, zeroed before launch
+    GPtr<unsigned long long> stats;  // iterations
+    GPtr<const volatile int> cancel; // host-mapped; nonzero => lanes stop taking work
 };

@@ -87,6 +87,9 @@ struct C32 {
 
 // minimum waves per SIMD requested from the register allocator (0: its own choice);
 // the few spills this costs sit on rare paths and the added occupancy hides latency
+#ifndef MX_SC64_WAVES
+#define MX_SC64_WAVES 5
+#endif
 #ifndef MX_FE32_WAVES
 #define MX_FE32_WAVES 5
 #endif
@@ -214,11 +217,11 @@ struct OrbitLoader<float> {
     int n;
     MX_HD explicit OrbitLoader(const MxRefDev& R) : n(R.nent) {
 #ifdef __HIP_DEVICE_COMPILE__
-        r = __builtin_amdgcn_make_buffer_rsrc((void*)R.zf32, 0, R.nent * 8, 0x00020000);
+        r = __builtin_amdgcn_make_buffer_rsrc((void*)R.zf32.p, 0, R.nent * 8, 0x00020000);
 #endif
         zf = R.zf32;
     }
-    const float2* zf;
+    GPtr<const float2> zf;
     MX_HD float2 load(int i) const {
 #ifdef __HIP_DEVICE_COMPILE__
         return __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(r, i * 8, 0, 0));
@@ -229,7 +232,7 @@ struct OrbitLoader<float> {
 };
 template <>
 struct OrbitLoader<double> {
-    const double2* zf;
+    GPtr<const double2> zf;
     int n;
     MX_HD explicit OrbitLoader(const MxRefDev& R) : zf(R.zf), n(R.nent) {}
     MX_HD double2 load(int i) const { return zf[min(i, n - 1)]; }
@@ -369,7 +372,7 @@ MX_HD bool renorm_due(F m2) { return m2 > (F)0x1p40f || (m2 < (F)0x1p-40f && m2 
 // BLA skip from table index s = m-1 (tryBLAfe, mandel_perturbation.cpp:2883-2970)
 // with floatexp coefficients (mantissa c, exponents ce).  Updates (w, k) and J.
 template <typename F>
-MX_HD int try_bla32(const MxRefDev& R, const int2* lev, int s, P32<F>& p, int reflen, F esc2f, bool ede) {
+MX_HD int try_bla32(const MxRefDev& R, GPtr<const int2> lev, int s, P32<F>& p, int reflen, F esc2f, bool ede) {
     using V = typename Vec2<F>::T;
     using V4 = typename Vec4<F>::T;
     if (s < 1) return 0;
@@ -380,85 +383,91 @@ MX_HD int try_bla32(const MxRefDev& R, const int2* lev, int s, P32<F>& p, int re
     const F w2 = wmag2(p.wr, p.wi);
     const int k2 = 2 * p.k;
     if (!(R.rmax2 > 0.0) || !dz2_less(w2, k2, R.rmax2)) return 0;
-    for (int q = startp; q >= R.minlevel; --q) {
+    // the first level whose entry is valid decides: it is applied, or the search ends
+    // with no skip (an escaping or unrepresentable landing does not fall back to a
+    // shorter level), so the level search is a loop of its own
+    int q = startp, e = -1;
+    for (; q >= R.minlevel; --q) {
         const int2 ls = lev[q];
         const int i = (s - 1) >> q;
         if (i >= ls.y) continue;
-        const int e = ls.x + i;
-        const double r2 = R.bla_r2[e];
+        const int ee = ls.x + i;
+        const double r2 = R.bla_r2[ee];
         if (!(r2 > 0.0) || !dz2_less(w2, k2, r2)) continue;
-        const int land = s + (1 << q);
-        if (land >= reflen - 1) continue;
-        V4 c;
-        int2 ce;
-        load_coef(R, e, c, ce);
-        // A w and B d with their exponents, then a common exponent
-        const F ar = c.x * p.wr - c.y * p.wi, ai = c.x * p.wi + c.y * p.wr;
-        const F br = c.z * p.dr - c.w * p.di, bi = c.z * p.di + c.w * p.dr;
-        const F aa = fmax_t(fabs_t(ar), fabs_t(ai)), bb = fmax_t(fabs_t(br), fabs_t(bi));
-        int ea = INT_MIN / 2, eb = INT_MIN / 2;
-        if (aa != (F)0) {
-            (void)frexp_t(aa, &ea);
-            ea += ce.x;
-        }
-        if (bb != (F)0) {
-            (void)frexp_t(bb, &eb);
-            eb += ce.y;
-        }
-        const int es = max(ea, eb);
-        F nwr, nwi;
-        if (es == INT_MIN / 2) {
-            nwr = nwi = (F)0;
-        } else {
-            nwr = ldexp_t(ar, ce.x - es) + ldexp_t(br, ce.y - es);
-            nwi = ldexp_t(ai, ce.x - es) + ldexp_t(bi, ce.y - es);
-        }
-        const int nk = p.k + es;
-        // never land past an escape: z = X_land + 2^nk w'
-        const V X = load_X(R, land, (F)0);
-        const F sn = pow2t<F>(nk);
-        const F fr = X.x + sn * nwr, fi = X.y + sn * nwi;
-        if (fr * fr + fi * fi > esc2f) return 0;
-        // unrepresentable landing: rejected on a full reference, kept as a zero delta at
-        // unit scale on a periodic one (as the plain form)
-        const bool wzero = nwr == (F)0 && nwi == (F)0;
-        if (wzero && !R.period) return 0;
-        if (ede) {
-            // J -> A J + B (mandel_perturbation.cpp:3399-3414): A J at 2^(kj + ce.x), B at 2^ce.y
-            const F tr = c.x * p.jr - c.y * p.ji, ti = c.x * p.ji + c.y * p.jr;
-            const F ta = fmax_t(fabs_t(tr), fabs_t(ti));
-            int ej = INT_MIN / 2, eB = INT_MIN / 2;
-            if (ta != (F)0) {
-                (void)frexp_t(ta, &ej);
-                ej += p.kj + ce.x;
-            }
-            const F tb = fmax_t(fabs_t(c.z), fabs_t(c.w));
-            if (tb != (F)0) {
-                (void)frexp_t(tb, &eB);
-                eB += ce.y;
-            }
-            const int em = max(ej, eB);
-            if (em == INT_MIN / 2) {
-                p.jr = p.ji = (F)0;
-                p.kj = 0;
-            } else {
-                p.jr = ldexp_t(tr, p.kj + ce.x - em) + ldexp_t(c.z, ce.y - em);
-                p.ji = ldexp_t(ti, p.kj + ce.x - em) + ldexp_t(c.w, ce.y - em);
-                p.kj = em;
-            }
-            p.jone = pow2t<F>(-p.kj);
-        }
-        p.wr = nwr;
-        p.wi = nwi;
-        p.k = wzero ? 0 : nk;
-        set_scale(p);
-        return 1 << q;
+        if (s + (1 << q) >= reflen - 1) continue;
+        e = ee;
+        break;
     }
-    return 0;
+    if (e < 0) return 0;
+    const int land = s + (1 << q);
+    V4 c;
+    int2 ce;
+    load_coef(R, e, c, ce);
+    // A w and B d with their exponents, then a common exponent
+    const F ar = c.x * p.wr - c.y * p.wi, ai = c.x * p.wi + c.y * p.wr;
+    const F br = c.z * p.dr - c.w * p.di, bi = c.z * p.di + c.w * p.dr;
+    const F aa = fmax_t(fabs_t(ar), fabs_t(ai)), bb = fmax_t(fabs_t(br), fabs_t(bi));
+    int ea = INT_MIN / 2, eb = INT_MIN / 2;
+    if (aa != (F)0) {
+        (void)frexp_t(aa, &ea);
+        ea += ce.x;
+    }
+    if (bb != (F)0) {
+        (void)frexp_t(bb, &eb);
+        eb += ce.y;
+    }
+    const int es = max(ea, eb);
+    F nwr, nwi;
+    if (es == INT_MIN / 2) {
+        nwr = nwi = (F)0;
+    } else {
+        nwr = ldexp_t(ar, ce.x - es) + ldexp_t(br, ce.y - es);
+        nwi = ldexp_t(ai, ce.x - es) + ldexp_t(bi, ce.y - es);
+    }
+    const int nk = p.k + es;
+    // never land past an escape: z = X_land + 2^nk w'
+    const V X = load_X(R, land, (F)0);
+    const F sn = pow2t<F>(nk);
+    const F fr = X.x + sn * nwr, fi = X.y + sn * nwi;
+    if (fr * fr + fi * fi > esc2f) return 0;
+    // unrepresentable landing: rejected on a full reference, kept as a zero delta at
+    // unit scale on a periodic one (as the plain form)
+    const bool wzero = nwr == (F)0 && nwi == (F)0;
+    if (wzero && !R.period) return 0;
+    if (ede) {
+        // J -> A J + B (mandel_perturbation.cpp:3399-3414): A J at 2^(kj + ce.x), B at 2^ce.y
+        const F tr = c.x * p.jr - c.y * p.ji, ti = c.x * p.ji + c.y * p.jr;
+        const F ta = fmax_t(fabs_t(tr), fabs_t(ti));
+        int ej = INT_MIN / 2, eB = INT_MIN / 2;
+        if (ta != (F)0) {
+            (void)frexp_t(ta, &ej);
+            ej += p.kj + ce.x;
+        }
+        const F tb = fmax_t(fabs_t(c.z), fabs_t(c.w));
+        if (tb != (F)0) {
+            (void)frexp_t(tb, &eB);
+            eB += ce.y;
+        }
+        const int em = max(ej, eB);
+        if (em == INT_MIN / 2) {
+            p.jr = p.ji = (F)0;
+            p.kj = 0;
+        } else {
+            p.jr = ldexp_t(tr, p.kj + ce.x - em) + ldexp_t(c.z, ce.y - em);
+            p.ji = ldexp_t(ti, p.kj + ce.x - em) + ldexp_t(c.w, ce.y - em);
+            p.kj = em;
+        }
+        p.jone = pow2t<F>(-p.kj);
+    }
+    p.wr = nwr;
+    p.wi = nwi;
+    p.k = wzero ? 0 : nk;
+    set_scale(p);
+    return 1 << q;
 }
 
 template <typename F, bool EDE, bool INTERIOR>
-MX_HD bool trip32(const MxRefDev& R, const int2* lev, const MxTileDev& T, P32<F>& p, double& v, int& n, C32& c, F esc2f) {
+MX_HD bool trip32(const MxRefDev& R, GPtr<const int2> lev, const MxTileDev& T, P32<F>& p, double& v, int& n, C32& c, F esc2f) {
     using V = typename Vec2<F>::T;
     const int reflen = R.mx_ref_it + 1;
     // a periodic reference wraps instead of ending (mandel_perturbation.cpp:3308-3313)
@@ -812,7 +821,7 @@ MX_HD bool take_sample(const MxTileDev& T, unsigned item, int& sidx, Fe& dcr, Fe
 }
 
 template <typename F, bool EDE, bool INTERIOR>
-__device__ __forceinline__ void run_job32(const MxRefDev& R, const int2* __restrict__ lev, const MxTileDev& T) {
+__device__ __forceinline__ void run_job32(const MxRefDev& R, GPtr<const int2> lev, const MxTileDev& T) {
     const int lane = __lane_id();
     const unsigned long long lanemask_lt = (1ull << lane) - 1ull;
     const F esc2f = (F)T.esc2;
@@ -820,8 +829,14 @@ __device__ __forceinline__ void run_job32(const MxRefDev& R, const int2* __restr
     int exhausted = 0;
     bool alive = true, active = false;
     P32<F> p;
-    C32 c = {0, 0, 0, 0};
-    int sidx = 0;
+    // each lane's work counters and output index live in LDS, not in registers: they
+    // are touched once per trip and would otherwise hold seven VGPRs across the loop
+    __shared__ C32 lds_counters[256];
+    __shared__ int lds_sidx[256];
+    C32& c = lds_counters[threadIdx.x];
+    int& sidx = lds_sidx[threadIdx.x];
+    c = C32{0, 0, 0, 0};
+    sidx = 0;
     unsigned trips = 0;
     const unsigned ntiles = (unsigned)T.ntiles;
     while (true) {
@@ -878,7 +893,7 @@ __device__ __forceinline__ void run_job32(const MxRefDev& R, const int2* __restr
 }
 
 template <typename F, bool EDE, bool INTERIOR>
-__global__ __launch_bounds__(256) MX_WAVES_ATTR(MX_FE32_WAVES) void mx_fe32_kernel(const MxJob* __restrict__ jobs, int njobs) {
+__global__ __launch_bounds__(256) MX_WAVES_ATTR(sizeof(F) == 8 ? MX_SC64_WAVES : MX_FE32_WAVES) void mx_fe32_kernel(const MxJob* __restrict__ jobs, int njobs) {
     for (int j = 0; j < njobs; ++j) run_job32<F, EDE, INTERIOR>(jobs[j].R, jobs[j].lev, jobs[j].T);
 }
 

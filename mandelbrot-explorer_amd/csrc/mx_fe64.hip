@@ -163,7 +163,7 @@ __host__ __device__ __noinline__ SJW deriv_underflow(Fe zr, Fe zi, Fe SJ, double
 
 // Overflow-safe FloatExp landing of a BLA skip (mandel_perturbation.cpp:2943-2957).
 // Returns false when the landing escapes.
-__host__ __device__ __noinline__ SW bla_land_fe(const double2* zf, int land, double esc2, double4 bc, Fe S, double wr, double wi, double dr, double di) {
+__host__ __device__ __noinline__ SW bla_land_fe(GPtr<const double2> zf, int land, double esc2, double4 bc, Fe S, double wr, double wi, double dr, double di) {
     Fe dzr = fe_mul_d(S, wr), dzi = fe_mul_d(S, wi);
     Fe dcrfe = fe_mul_d(S, dr), dcife = fe_mul_d(S, di);
     Fe nzr = fe_add(fe_sub(fe_mul_d(dzr, bc.x), fe_mul_d(dzi, bc.y)), fe_sub(fe_mul_d(dcrfe, bc.z), fe_mul_d(dcife, bc.w)));
@@ -178,7 +178,7 @@ __host__ __device__ __noinline__ SW bla_land_fe(const double2* zf, int land, dou
 
 // tryBLAfe (mandel_perturbation.cpp:2883-2970).  s = m-1, reflen = mx_ref_it+1, or
 // the period for a periodic reference (landings stay inside one period).
-MX_HD int try_bla(const MxRefDev& R, const int2* lev, int s, Fe& S, Fe S2, double& wr, double& wi, double dr, double di, int reflen, double esc2, double4& ab) {
+MX_HD int try_bla(const MxRefDev& R, GPtr<const int2> lev, int s, Fe& S, Fe S2, double& wr, double& wi, double dr, double di, int reflen, double esc2, int& kbla) {
     if (s < 1) return 0;
     const int maxp = R.nlev - 1;
     int startp = (s == 1) ? maxp : __builtin_ctz((unsigned)(s - 1));
@@ -186,57 +186,63 @@ MX_HD int try_bla(const MxRefDev& R, const int2* lev, int s, Fe& S, Fe S2, doubl
     if (startp < R.minlevel) return 0;
     const Fe dz2 = fe_mul(S2, fe_from(wr * wr + wi * wi));
     if (R.rmax2 <= 0.0 || !fe_abs_less(dz2, fe_from(R.rmax2))) return 0;
-    for (int p = startp; p >= R.minlevel; --p) {
+    // the first level whose entry is valid decides: it is applied, or the search ends
+    // with no skip (an escaping or unrepresentable landing does not fall back to a
+    // shorter level), so the level search is a loop of its own
+    int p = startp, k = -1;
+    for (; p >= R.minlevel; --p) {
         const int2 ls = lev[p];
         const int i = (s - 1) >> p;
         if (i >= ls.y) continue;
-        const int k = ls.x + i;
-        const double r2 = R.bla_r2[k];
+        const int kk = ls.x + i;
+        const double r2 = R.bla_r2[kk];
         if (r2 <= 0 || !fe_abs_less(dz2, fe_from(r2))) continue;
-        const int land = s + (1 << p);
-        if (land >= reflen - 1) continue;
-        const double4 bc = R.bla_coef[k];
-        const double nwr = (bc.x * wr - bc.y * wi) + (bc.z * dr - bc.w * di);
-        const double nwi = (bc.x * wi + bc.y * wr) + (bc.z * di + bc.w * dr);
-        const double BIG = 1e150;
-        if (__builtin_isfinite(nwr) && __builtin_isfinite(nwi) && fabs(nwr) < BIG && fabs(nwi) < BIG) {
-            const double2 X = R.zf[land];
-            const double fr = X.x + fe_to_double(fe_mul_d(S, nwr));
-            const double fi = X.y + fe_to_double(fe_mul_d(S, nwi));
-            if (fr * fr + fi * fi > esc2) return 0;
-            const double wm2 = nwr * nwr + nwi * nwi;
-            if (wm2 == 0.0) {
-                // a full reference rejects the unrepresentable landing; a periodic one
-                // keeps the nonzero unit delta (blaReRescaleUnderflow, 2770-2781)
-                if (!R.period) return 0;
-                if (nwr == 0.0 && nwi == 0.0) {
-                    S = fe_make(1.0, 0);
-                    wr = wi = 0.0;
-                } else {
-                    const Fe wmag = fe_sqrt(fe_add(fe_mul(fe_from(nwr), fe_from(nwr)), fe_mul(fe_from(nwi), fe_from(nwi))));
-                    S = fe_mul(S, wmag);
-                    const double inv = 1.0 / fe_to_double(wmag);
-                    wr = nwr * inv;
-                    wi = nwi * inv;
-                }
+        if (s + (1 << p) >= reflen - 1) continue;
+        k = kk;
+        break;
+    }
+    if (k < 0) return 0;
+    const int land = s + (1 << p);
+    const double4 bc = R.bla_coef[k];
+    const double nwr = (bc.x * wr - bc.y * wi) + (bc.z * dr - bc.w * di);
+    const double nwi = (bc.x * wi + bc.y * wr) + (bc.z * di + bc.w * dr);
+    const double BIG = 1e150;
+    if (__builtin_isfinite(nwr) && __builtin_isfinite(nwi) && fabs(nwr) < BIG && fabs(nwi) < BIG) {
+        const double2 X = R.zf[land];
+        const double fr = X.x + fe_to_double(fe_mul_d(S, nwr));
+        const double fi = X.y + fe_to_double(fe_mul_d(S, nwi));
+        if (fr * fr + fi * fi > esc2) return 0;
+        const double wm2 = nwr * nwr + nwi * nwi;
+        if (wm2 == 0.0) {
+            // a full reference rejects the unrepresentable landing; a periodic one
+            // keeps the nonzero unit delta (blaReRescaleUnderflow, 2770-2781)
+            if (!R.period) return 0;
+            if (nwr == 0.0 && nwi == 0.0) {
+                S = fe_make(1.0, 0);
+                wr = wi = 0.0;
             } else {
-                const double wmag = sqrt(wm2);
-                S = fe_mul(S, fe_from(wmag));
-                const double inv = 1.0 / wmag;
+                const Fe wmag = fe_sqrt(fe_add(fe_mul(fe_from(nwr), fe_from(nwr)), fe_mul(fe_from(nwi), fe_from(nwi))));
+                S = fe_mul(S, wmag);
+                const double inv = 1.0 / fe_to_double(wmag);
                 wr = nwr * inv;
                 wi = nwi * inv;
             }
         } else {
-            const SW r = bla_land_fe(R.zf, land, esc2, bc, S, wr, wi, dr, di);
-            if (!r.ok) return 0;
-            S = r.S;
-            wr = r.wr;
-            wi = r.wi;
+            const double wmag = sqrt(wm2);
+            S = fe_mul(S, fe_from(wmag));
+            const double inv = 1.0 / wmag;
+            wr = nwr * inv;
+            wi = nwi * inv;
         }
-        ab = bc;
-        return 1 << p;
+    } else {
+        const SW r = bla_land_fe(R.zf, land, esc2, bc, S, wr, wi, dr, di);
+        if (!r.ok) return 0;
+        S = r.S;
+        wr = r.wr;
+        wi = r.wi;
     }
-    return 0;
+    kbla = k;
+    return 1 << p;
 }
 
 // The common-case arithmetic of one step, shared by the full and the lean step
@@ -258,7 +264,7 @@ MX_HD void step_j(const Px& p, double zmr, double zmi, double& a, double& b) {
 // One trip of the pixel loop.  Returns true when the sample is finished (value and
 // n_esc written to v/n).  n: escape iteration, -1 = reached mxit, -2 = detected interior.
 template <bool EDE, bool INTERIOR>
-MX_HD bool trip(const MxRefDev& R, const int2* lev, const MxTileDev& T, Px& p, double& v, int& n, Counters& c) {
+MX_HD bool trip(const MxRefDev& R, GPtr<const int2> lev, const MxTileDev& T, Px& p, double& v, int& n, Counters& c) {
     const double ESC2 = T.esc2;
     const int reflen = R.mx_ref_it + 1;
     // a periodic reference wraps instead of ending (mandel_perturbation.cpp:3308-3313)
@@ -285,10 +291,13 @@ MX_HD bool trip(const MxRefDev& R, const int2* lev, const MxTileDev& T, Px& p, d
         }
     }
     if ((T.flags & MX_F_BLA) && p.m >= 2) {
-        double4 ab;
-        const int skip = try_bla(R, lev, p.rm, p.S, p.S2, p.wr, p.wi, p.dr, p.di, per ? per : reflen, ESC2, ab);
+        int kbla = 0;
+        const int skip = try_bla(R, lev, p.rm, p.S, p.S2, p.wr, p.wi, p.dr, p.di, per ? per : reflen, ESC2, kbla);
         if (skip > 0) {
             if constexpr (EDE) {
+                // the applied entry's coefficients are read again rather than carried
+                // out of the level search (fewer registers live across the landing)
+                const double4 ab = R.bla_coef[kbla];
                 Fe Jr = fe_mul_d(p.SJ, p.jr), Ji = fe_mul_d(p.SJ, p.ji);
                 Fe nJr = fe_add(fe_sub(fe_mul_d(Jr, ab.x), fe_mul_d(Ji, ab.y)), fe_from(ab.z));
                 Fe nJi = fe_add(fe_add(fe_mul_d(Jr, ab.y), fe_mul_d(Ji, ab.x)), fe_from(ab.w));
@@ -328,10 +337,10 @@ MX_HD bool trip(const MxRefDev& R, const int2* lev, const MxTileDev& T, Px& p, d
     // periodic orbit carries copies of its first two entries after the last, so
     // rm+1 and rm+2 read the wrapped entries without a modulo).
     if (!p.xv) {
-        p.Xc = p.m ? R.zf[p.rm] : make_double2(0.0, 0.0);
+        p.Xc = p.m ? double2(R.zf[p.rm]) : make_double2(0.0, 0.0);
         p.Xn = R.zf[p.rm + 1];
     }
-    const double2 Xpf = (p.rm + 2 < R.nent) ? R.zf[p.rm + 2] : make_double2(0.0, 0.0);
+    const double2 Xpf = (p.rm + 2 < R.nent) ? double2(R.zf[p.rm + 2]) : make_double2(0.0, 0.0);
     double2 X = p.Xc;
     double zmr = 0.0, zmi = 0.0;
     bool d_underflow = false;
@@ -551,7 +560,7 @@ MX_HD void fast_steps(const MxRefDev& R, const MxTileDev& T, Px& p, Counters& c,
 #ifdef __HIP_DEVICE_COMPILE__
     // orbit entries through a buffer resource: a 32-bit offset per load, and indices
     // past the end read as 0
-    const __amdgpu_buffer_rsrc_t zrs = __builtin_amdgcn_make_buffer_rsrc((void*)R.zf, 0, R.nent * 16, 0x00020000);
+    const __amdgpu_buffer_rsrc_t zrs = __builtin_amdgcn_make_buffer_rsrc((void*)R.zf.p, 0, R.nent * 16, 0x00020000);
 #endif
     double w2 = p.wr * p.wr + p.wi * p.wi;
     // one counter for the steps taken: orbit index, reference index and iteration
@@ -585,7 +594,7 @@ MX_HD void fast_steps(const MxRefDev& R, const MxTileDev& T, Px& p, Counters& c,
         int stop = 0;
         if (can_bla) stop = (int)(((n - nal) & amask) == 0) & (int)(n >= nm2) & (int)(w2 <= wthr);
         if (!p.xv) {
-            p.Xc = m0 + n ? R.zf[rm] : make_double2(0.0, 0.0);
+            p.Xc = m0 + n ? double2(R.zf[rm]) : make_double2(0.0, 0.0);
             p.Xn = R.zf[rm + 1];
             p.xv = 1;
         }
@@ -693,15 +702,21 @@ MX_HD bool take_sample(const MxTileDev& T, unsigned item, int& sidx, Fe& dcr, Fe
 }
 
 template <bool EDE, bool INTERIOR>
-__device__ __forceinline__ void run_job64(const MxRefDev& R, const int2* __restrict__ lev, const MxTileDev& T) {
+__device__ __forceinline__ void run_job64(const MxRefDev& R, GPtr<const int2> lev, const MxTileDev& T) {
     const int lane = __lane_id();
     const unsigned long long lanemask_lt = (1ull << lane) - 1ull;
     int q = blockIdx.x % MX_NQUEUE;
     int exhausted = 0;
     bool alive = true, active = false;
     Px p;
-    Counters c = {0, 0, 0, 0};
-    int sidx = 0;
+    // each lane's work counters and output index live in LDS, not in registers: they
+    // are touched once per trip and would otherwise hold seven VGPRs across the loop
+    __shared__ Counters lds_counters[256];
+    __shared__ int lds_sidx[256];
+    Counters& c = lds_counters[threadIdx.x];
+    int& sidx = lds_sidx[threadIdx.x];
+    c = Counters{0, 0, 0, 0};
+    sidx = 0;
     unsigned trips = 0;
     const unsigned ntiles = (unsigned)T.ntiles;
     while (true) {
