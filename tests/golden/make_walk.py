@@ -19,6 +19,12 @@ requested depth; the decimal strings carry ceil(-log10 w) + 30 digits.
   curve the walk follows stays within a vanishing fraction of a view width of the set
   (the escape time near the boundary grows by about one iteration per halving):
   the views stay boundary views at every depth.
+--rule de (walk_de_seed<S>.json; the benchmark views): the probe renders the exterior
+  distance estimate; candidates are the escaping probe pixels ranked by their distance
+  estimate in pixels, closest to the set first (index order among equals),
+  k = ceil(0.1 * n_escaped); the probe max_iter (from 1024) doubles, up to 2^21, until
+  at least 90 % of the probe pixels escape.  The views stay on the set's boundary at
+  every depth without following nested minibrots.
 --rule doubling (the first rule, walk_seed<S>.json): candidates are the escaping probe
   pixels; the probe mxit doubles whenever the highest escape count exceeds half of it
   (up to 2^21); k = ceil(0.1 * n_escaped); a probe with no escaping pixel at the cap
