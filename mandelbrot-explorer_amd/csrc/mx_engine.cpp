@@ -884,19 +884,19 @@ int upload_frame_ref(mx_ctx* ctx, const mx::Frame& F, mx_ref** ref) {
 // One launch of the double kernel against `ref` over the grid (list == nullptr) or a
 // list of samples; results merged into nesc/val for the samples it resolved, the
 // unresolved ones returned with their glitch metric.
-int dbl_pass(mx_ctx* ctx, const mx::Frame& F, const mx_ref* ref, mx_tile t, const std::vector<int32_t>* list, int stride,
-             std::vector<int32_t>& nesc, std::vector<double>& val, std::vector<int32_t>& un_idx, std::vector<double>& un_gp,
-             uint64_t st[4], double* kernel_ms) {
+int dbl_pass(mx_ctx* ctx, const std::vector<double>& dcol, const std::vector<double>& drow, const mx_ref* ref, mx_tile t,
+             const std::vector<int32_t>* list, int stride, std::vector<int32_t>& nesc, std::vector<double>& val,
+             std::vector<int32_t>& un_idx, std::vector<double>& un_gp, uint64_t st[4], double* kernel_ms) {
     t.stride = stride;
-    t.dcol_d = F.dcol_d.data();
-    t.drow_d = F.drow_d.data();
+    t.dcol_d = dcol.data();
+    t.drow_d = drow.data();
     t.list = list ? list->data() : nullptr;
     t.list_len = list ? (int32_t)list->size() : 0;
     mx_plan* plan = nullptr;
     int rc = mx_plan_create(ctx, ref, &t, &plan);
     if (rc != MX_OK) return rc;
     double ms = 0.0;
-    const size_t n = (size_t)F.W * F.H;
+    const size_t n = (size_t)t.width * t.height;
     std::vector<int32_t> pn(n);
     std::vector<double> pv(n);
     uint64_t ps[4] = {0, 0, 0, 0};
@@ -978,7 +978,7 @@ int compute_dbl(mx_ctx* ctx, mx::Frame& F, const mx_request* req, mx_frame_info*
         std::vector<int32_t> pn(n, INT_MIN);
         std::vector<double> pv(n, -10.0);
         uint64_t pst[4] = {0, 0, 0, 0};
-        rc = dbl_pass(ctx, F, ref, t, nullptr, 8, pn, pv, un_idx, un_gp, pst, &probe_ms);
+        rc = dbl_pass(ctx, F.dcol_d, F.drow_d, ref, t, nullptr, 8, pn, pv, un_idx, un_gp, pst, &probe_ms);
         if (rc != MX_OK) {
             mx_ref_release(ref);
             return rc;
@@ -1004,7 +1004,7 @@ int compute_dbl(mx_ctx* ctx, mx::Frame& F, const mx_request* req, mx_frame_info*
     std::vector<int32_t> list;
     bool first = true;
     while (true) {
-        rc = dbl_pass(ctx, F, ref, t, first ? nullptr : &list, 1, nesc, val, un_idx, un_gp, st, &kernel_ms);
+        rc = dbl_pass(ctx, F.dcol_d, F.drow_d, ref, t, first ? nullptr : &list, 1, nesc, val, un_idx, un_gp, st, &kernel_ms);
         first = false;
         ++passes;
         if (rc != MX_OK) break;
@@ -1047,8 +1047,10 @@ int compute_dbl(mx_ctx* ctx, mx::Frame& F, const mx_request* req, mx_frame_info*
         t.bla = F.use_bla && !F.lev_off.empty();
         t.skip_sample = (int32_t)refpix;
     }
-    if (ref) mx_ref_release(ref);
-    if (rc != MX_OK) return rc;
+    if (rc != MX_OK) {
+        if (ref) mx_ref_release(ref);
+        return rc;
+    }
     const CellMap cm{F.W, F.sub};
     uint64_t nominal = 0;
     for (int i = 0; i < F.H; ++i)
@@ -1060,6 +1062,105 @@ int compute_dbl(mx_ctx* ctx, mx::Frame& F, const mx_request* req, mx_frame_info*
             if (req->n_esc) req->n_esc[c] = ne;
             nominal += ne > 0 ? (uint64_t)ne : (uint64_t)F.max_iter;
         }
+    int ss_pixels = 0;
+    double ss_ms = 0.0;
+    if (F.ss) {
+        // adaptive supersampling (mandel_perturbation.cpp:1789-2005): the flagged pixels'
+        // other cells against the current reference, the first pass with its BLA table;
+        // a glitched sub-sample becomes the next reference (BLA off from then on) until
+        // every sub-sample resolves.  The cells form an (H sub) x (W sub) grid whose
+        // dc tables are relative to the reference's cell.
+        const std::vector<int32_t> flagged = ss_flags(F, req->iterations, cm);
+        ss_pixels = (int)flagged.size();
+        const int S = F.sub, h = S / 2, Wc = F.W * S, Hc = F.H * S;
+        std::vector<int32_t> cells;
+        for (int32_t pix : flagged) {
+            const int i = pix / F.W, j = pix % F.W;
+            for (int yi = -h; yi <= h; ++yi)
+                for (int xi = -h; xi <= h; ++xi)
+                    if (xi || yi) cells.push_back((int32_t)cm(i, j, yi, xi));
+        }
+        // the engine's std::set order of sub-samples (i, j, yi, xi), for ties
+        auto set_key = [&](int32_t c) {
+            const int r = c / Wc, q = c % Wc;
+            return (((int64_t)(r / S) * F.W + q / S) * S + r % S) * S + q % S;
+        };
+        if (!cells.empty() && !ref) {
+            const double tu = now_s();
+            rc = upload_frame_ref(ctx, F, &ref);
+            upload_s += now_s() - tu;
+        }
+        mx_tile ts = t;
+        ts.width = Wc;
+        ts.height = Hc;
+        ts.skip_sample = -1;
+        ts.bla = F.use_bla && !F.lev_off.empty();
+        std::vector<double> ccol, crow;
+        F.cell_dc(ccol, crow);
+        std::vector<int32_t> cn;
+        std::vector<double> cv;
+        if (!cells.empty()) {
+            cn.assign((size_t)Wc * Hc, INT_MIN);
+            cv.assign((size_t)Wc * Hc, -10.0);
+        }
+        std::vector<int32_t> pending;
+        pending.insert(pending.end(), cells.begin(), cells.end());
+        while (rc == MX_OK && !pending.empty()) {
+            rc = dbl_pass(ctx, ccol, crow, ref, ts, &pending, 1, cn, cv, un_idx, un_gp, st, &ss_ms);
+            if (rc != MX_OK || un_idx.empty()) break;
+            if (__atomic_load_n(ctx->cancel_host, __ATOMIC_ACQUIRE)) {
+                ctx->set_err("cancelled");
+                rc = MX_E_CANCELLED;
+                break;
+            }
+            size_t best = 0;
+            double mn = 1e100;
+            bool found = false;
+            for (size_t q = 0; q < un_idx.size(); ++q) {
+                const bool less = un_gp[q] >= 0 && (un_gp[q] < mn || (found && un_gp[q] == mn && set_key(un_idx[q]) < set_key(un_idx[best])));
+                if (less) {
+                    mn = un_gp[q];
+                    best = q;
+                    found = true;
+                }
+            }
+            if (!found)
+                for (size_t q = 1; q < un_idx.size(); ++q)
+                    if (set_key(un_idx[q]) < set_key(un_idx[best])) best = q;
+            const int32_t bc = un_idx[best];
+            const int br = bc / Wc, bq = bc % Wc;
+            mx_ref_release(ref);
+            ref = nullptr;
+            F.set_pixel_reference(br / S, bq / S, br % S - h, bq % S - h, false);
+            if (F.halted()) {
+                ctx->set_err("cancelled");
+                rc = MX_E_CANCELLED;
+                break;
+            }
+            cn[(size_t)bc] = F.ref_pixel_n;
+            cv[(size_t)bc] = F.ref_pixel_value;
+            pending.clear();
+            for (size_t q = 0; q < un_idx.size(); ++q)
+                if (q != best) pending.push_back(un_idx[q]);
+            if (pending.empty()) break;
+            const double tu = now_s();
+            rc = upload_frame_ref(ctx, F, &ref);
+            upload_s += now_s() - tu;
+            F.cell_dc(ccol, crow);
+            ts.bla = 0;
+        }
+        if (rc == MX_OK)
+            for (int32_t c : cells) {
+                const int32_t ne = cn[(size_t)c];
+                if (ne == INT_MIN) continue;
+                req->iterations[c] = ne < 0 ? -2.0f : (float)cv[(size_t)c];
+                if (req->values) req->values[c] = cv[(size_t)c];
+                if (req->n_esc) req->n_esc[c] = ne;
+                nominal += ne > 0 ? (uint64_t)ne : (uint64_t)F.max_iter;
+            }
+    }
+    if (ref) mx_ref_release(ref);
+    if (rc != MX_OK) return rc;
     if (info) {
         std::memset(info, 0, sizeof *info);
         info->path = 1;
@@ -1084,6 +1185,8 @@ int compute_dbl(mx_ctx* ctx, mx::Frame& F, const mx_request* req, mx_frame_info*
         info->nominal_iterations = nominal;
         info->passes = passes;
         info->sa_it = sa_it0;
+        info->ss_pixels = ss_pixels;
+        info->ss_ms = ss_ms;
     }
     return MX_OK;
 }

@@ -1161,11 +1161,6 @@ int Frame::prepare(const mx_request& r, std::string& err) {
               "(MX_OPT_FLOATEXP runs such frames on the floatexp path)";
         return MX_E_UNSUPPORTED;
     }
-    if (dbl && ss) {
-        err = "adaptive supersampling on the plain-double path (scale <= 1e280) is not on this path "
-              "(MX_OPT_FLOATEXP runs such frames on the floatexp path)";
-        return MX_E_UNSUPPORTED;
-    }
     const bool view_ref = (r.options & MX_OPT_VIEW_CENTER_REF) != 0;
     // depth guard for the int32 exponent of the device FloatExp: |log2 scale| < 2^26
     {
@@ -1368,24 +1363,35 @@ int Frame::prepare_dbl(mpf_srcptr scale, std::string& err) {
     return MX_OK;
 }
 
-void Frame::set_pixel_reference(int pi, int pj) {
+void Frame::set_pixel_reference(int pi, int pj, int u, int v, bool with_bla) {
     const double t_ref = now_s();
     Gmp pre(prec), pim(prec), t(prec);
+    // c0 + dx pj + (+-dx |v|) / sub, c0 + dy pi + (+-dy |u|) / sub (mandel_perturbation.cpp:4057-4074)
     mpf_mul_ui(t, dx.v, pj);
     mpf_add(pre, c0_re.v, t);
+    mpf_mul_ui(t, dx.v, (unsigned long)std::abs(v));
+    if (v < 0) mpf_neg(t, t);
+    mpf_div_ui(t, t, (unsigned long)sub);
+    mpf_add(pre, pre, t);
     mpf_mul_ui(t, dy.v, pi);
     mpf_add(pim, c0_im.v, t);
+    mpf_mul_ui(t, dy.v, (unsigned long)std::abs(u));
+    if (u < 0) mpf_neg(t, t);
+    mpf_div_ui(t, t, (unsigned long)sub);
+    mpf_add(pim, pim, t);
     ref_virtual = false;
     ref_i = pi;
     ref_j = pj;
-    ref_x = pj;
-    ref_y = pi;
+    ref_u = u;
+    ref_v = v;
+    ref_x = pj + (double)v / sub;
+    ref_y = pi + (double)u / sub;
     build_orbit(pre, pim);
     ref_pixel(false);
     ++n_refs;
     ref_seconds += now_s() - t_ref;
     const double t_bla = now_s();
-    if (use_bla) {
+    if (use_bla && with_bla) {
         const double dxf = std::fabs(mpf_get_ld(dx.v)), dyf = std::fabs(mpf_get_ld(dy.v));
         const double mxx = std::max(ref_x, (double)(W - 1) - ref_x) + 0.5;
         const double mxy = std::max(ref_y, (double)(H - 1) - ref_y) + 0.5;
@@ -1412,6 +1418,28 @@ void Frame::set_pixel_reference(int pi, int pj) {
         mpf_mul_ui(t, dy.v, (unsigned long)std::abs(yp));
         if (yp < 0) mpf_neg(t, t);
         drow_d[i] = mpf_get_ld(t);
+    }
+}
+
+void Frame::cell_dc(std::vector<double>& col, std::vector<double>& row) const {
+    const int h = sub / 2;
+    const long rc = (long)ref_j * sub + ref_v + h, rr = (long)ref_i * sub + ref_u + h;
+    Gmp t(prec);
+    col.resize((size_t)W * sub);
+    row.resize((size_t)H * sub);
+    for (long c = 0; c < (long)W * sub; ++c) {
+        const long xp = c - rc;
+        mpf_mul_ui(t, dx.v, (unsigned long)std::labs(xp));
+        if (xp < 0) mpf_neg(t, t);
+        mpf_div_ui(t, t, (unsigned long)sub);
+        col[(size_t)c] = mpf_get_ld(t);
+    }
+    for (long r = 0; r < (long)H * sub; ++r) {
+        const long yp = r - rr;
+        mpf_mul_ui(t, dy.v, (unsigned long)std::labs(yp));
+        if (yp < 0) mpf_neg(t, t);
+        mpf_div_ui(t, t, (unsigned long)sub);
+        row[(size_t)r] = mpf_get_ld(t);
     }
 }
 

@@ -84,6 +84,7 @@ struct Frame {
     bool dbl = false;
     bool escalated = false; // 1e150 < scale <= 1e280 frame moved to the floatexp path
     int ref_i = 0, ref_j = 0;
+    int ref_u = 0, ref_v = 0;      // the reference's sub-sample offsets (row, column)
     std::vector<double> z_m3;
     MxC sa_A[MX_SA_N] = {};
     MxC sa_B[MX_SA_N] = {};        // series of the derivative delta (EDE)
@@ -96,7 +97,12 @@ struct Frame {
     int n_refs = 0;
     // Build the reference at pixel (pi, pj) of the double path with everything a pass
     // against it needs (createRef(random=false) + buildBLA, MP.cpp:1750-1758, 4014-4187).
-    void set_pixel_reference(int pi, int pj);
+    // the reference at sample (pi, pj, u, v) (createRef, mandel_perturbation.cpp:
+    // 4014-4074); with_bla: build its BLA table (else none)
+    void set_pixel_reference(int pi, int pj, int u = 0, int v = 0, bool with_bla = true);
+    // dc per cell column and row of the (H sub) x (W sub) cell grid relative to the
+    // reference's cell (stepParallel, mandel_perturbation.cpp:2300-2312)
+    void cell_dc(std::vector<double>& col, std::vector<double>& row) const;
     // ref_pixel_n / ref_pixel_value are final after this (the floatexp path computes
     // an EDE reference pixel on a host thread while the frame's kernels run)
     void ref_pixel_wait();

@@ -207,6 +207,28 @@ static void grid_pixel_c(const mxo_grid* g, int i, int j, int u, int v, int sub,
     mpfr_clear(t);
 }
 
+/* The reference point of sample (i, j, u, v) as createRef forms it in mpf
+ * (MP.cpp:4057-4074): c0 + dx j + (+-dx |v|) / sub, c0 + dy i + (+-dy |u|) / sub. */
+static void grid_ref_c(const mxo_grid* g, int i, int j, int u, int v, int sub, mpfr_ptr cre, mpfr_ptr cim) {
+    mpf_t t, a;
+    mpf_init2(t, mpf_prec_of(g->prec));
+    mpf_init2(a, mpf_prec_of(g->prec));
+    const mpf_srcptr c0[2] = {g->fc0_re, g->fc0_im}, d[2] = {g->fdx, g->fdy};
+    const int k[2] = {j, i}, off[2] = {v, u};
+    for (int q = 0; q < 2; ++q) {
+        mpf_set(a, c0[q]);
+        mpf_mul_ui(t, d[q], (unsigned long)k[q]);
+        mpf_add(a, a, t);
+        mpf_mul_ui(t, d[q], (unsigned long)abs(off[q]));
+        if (off[q] < 0) mpf_neg(t, t);
+        mpf_div_ui(t, t, (unsigned long)sub);
+        mpf_add(a, a, t);
+        mpfr_set_f(q ? cim : cre, a, MPFR_RNDN);
+    }
+    mpf_clear(t);
+    mpf_clear(a);
+}
+
 /* exact geometric centre c0 + d*(n-1)/2 (MP.cpp:4024-4036) */
 static void grid_view_center(const mxo_grid* g, int W, int H, mpfr_ptr cre, mpfr_ptr cim) {
     if (g->as_mpf) {
@@ -2707,6 +2729,15 @@ typedef struct {
     int sa_it0, sa_order0;
     long long steps, bla_applies, bla_skipped, rebases;
     int ede;             /* exterior distance estimate output */
+    /* output cells per pixel side (0 or 1: none) and adaptive supersampling (ss): the
+     * (H sub) x (W sub) cells of getIndex (MP.cpp:4334-4340), written where a sample is
+     * computed; ss_pixels: pixels refined; ss_passes: reference passes over their
+     * sub-samples (MP.cpp:1886-2005) */
+    int sub, ss;
+    float* cells;
+    double* cell_val;
+    int32_t* cell_nesc;
+    int ss_pixels, ss_passes;
 } mxo_dframe;
 
 MXO_EXPORT int mxo_compute_dbl(mxo_dframe* f) {
@@ -2907,11 +2938,148 @@ MXO_EXPORT int mxo_compute_dbl(mxo_dframe* f) {
         n = 0;
         for (int q = 0; q < nr; ++q)
             if (rest[q] != best) idx[n++] = rest[q];
-        if (n == 0) break;
+        /* with sub-samples to follow, the last reference's BLA table is built even when
+         * no base pixel is left for it (MP.cpp:1750-1756) */
+        if (n == 0 && !(f->sub > 1 && f->ss)) break;
         MXO_SETUP_REF();
+        if (n == 0) break;
     }
     f->passes = passes;
     for (int p = 0; p < W * H; ++p) f->out[p] = (f->n_esc[p] == INT32_MIN) ? -10.f : (f->n_esc[p] < 0) ? -2.f : (float)f->val[p];
+    f->ss_pixels = 0;
+    f->ss_passes = 0;
+    if (f->sub > 1) {
+        const int S = f->sub;
+        for (int i = 0; i < H; ++i)
+            for (int j = 0; j < W; ++j) {
+                const size_t cc = ss_cell(W, S, i, j, 0, 0);
+                f->cells[cc] = f->out[i * W + j];
+                f->cell_val[cc] = f->val[i * W + j];
+                f->cell_nesc[cc] = f->n_esc[i * W + j];
+            }
+    }
+    if (f->sub > 1 && f->ss) {
+        /* adaptive supersampling on the double path (MP.cpp:1789-2005): the flagged
+         * pixels' other cells in the engine's std::set order (i, j, yi, xi), passes
+         * against the current reference -- the first with its BLA table, the later ones
+         * after a glitch re-reference with BLA off -- until every sub-sample resolves */
+        const int S = f->sub, h = S / 2, per = S * S - 1;
+        unsigned char* flags = (unsigned char*)malloc((size_t)W * H);
+        f->ss_pixels = ss_flag_pixels(f->out, W, H, mxit, ede, flags);
+        const int ns = f->ss_pixels * per;
+        int* sc = (int*)malloc(sizeof(int) * (size_t)(ns + 1));   /* cell column */
+        int* sr = (int*)malloc(sizeof(int) * (size_t)(ns + 1));   /* cell row */
+        int* pend = (int*)malloc(sizeof(int) * (size_t)(ns + 1));
+        int* left = (int*)malloc(sizeof(int) * (size_t)(ns + 1));
+        double* sgp = (double*)malloc(sizeof(double) * (size_t)(ns + 1));
+        double* ccol = (double*)malloc(sizeof(double) * (size_t)W * S);
+        double* crow = (double*)malloc(sizeof(double) * (size_t)H * S);
+        int m = 0;
+        for (int p = 0; p < W * H; ++p) {
+            if (!flags[p]) continue;
+            const int i = p / W, j = p % W;
+            for (int yi = -h; yi <= h; ++yi)
+                for (int xi = -h; xi <= h; ++xi) {
+                    if (!xi && !yi) continue;
+                    sc[m] = j * S + h + xi;
+                    sr[m] = i * S + h + yi;
+                    pend[m] = m;
+                    ++m;
+                }
+        }
+        /* the current reference's cell, and dc = trunc53(d * (cell - reference cell) / sub)
+         * per cell column and row (stepParallel, MP.cpp:2300-2312) */
+        int rc = ref_j * S + h, rr = ref_i * S + h;
+#define MXO_CELL_DC()                                                                                         \
+    do {                                                                                                      \
+        for (int q = 0; q < W * S; ++q) {                                                                     \
+            mpfr_mul_si(t, g.dx, q - rc, MPFR_RNDN);                                                          \
+            mpfr_div_ui(t, t, S, MPFR_RNDN);                                                                  \
+            ccol[q] = mpfr_to_ld_trunc(t);                                                                    \
+        }                                                                                                     \
+        for (int q = 0; q < H * S; ++q) {                                                                     \
+            mpfr_mul_si(t, g.dy, q - rr, MPFR_RNDN);                                                          \
+            mpfr_div_ui(t, t, S, MPFR_RNDN);                                                                  \
+            crow[q] = mpfr_to_ld_trunc(t);                                                                    \
+        }                                                                                                     \
+    } while (0)
+        MXO_CELL_DC();
+        int np = m;
+        while (np > 0) {
+            _Pragma("omp parallel for schedule(dynamic, 16) num_threads(f->threads) reduction(+ : steps, ap, sk, rb)")
+            for (int q = 0; q < np; ++q) {
+                const int k = pend[q];
+                mxo_c dc = {ccol[sc[k]], crow[sr[k]]};
+                mxo_c dz = sa_seed(&sa, dc);
+                mxo_c dd0 = {0.0, 0.0};
+                if (ede) dd0 = sa_horner(sa.B, sa.order, sa.delta, dc);
+                mxo_stats st = {0, 0, 0, 0, 0, 0, 0};
+                int ne;
+                double gpp;
+                const double v = pixel_double(&ctx, dc, dz, dd0, mxit, &ne, &gpp, &st);
+                sgp[k] = gpp;
+                if (ne != -3) {
+                    const size_t cc = (size_t)sr[k] * W * S + sc[k];
+                    f->cell_val[cc] = v;
+                    f->cell_nesc[cc] = ne;
+                    f->cells[cc] = ne < 0 ? -2.f : (float)v;
+                }
+                steps += st.steps;
+                ap += st.bla_applies;
+                sk += st.bla_skipped;
+                rb += st.rebases;
+            }
+            ++f->ss_passes;
+            int nl = 0;
+            for (int q = 0; q < np; ++q)
+                if (sgp[pend[q]] >= 0) left[nl++] = pend[q];
+            if (nl == 0) break;
+            /* next reference: the pending sub-sample with the smallest glitch metric,
+             * first in set order among equals (MP.cpp:2587-2594, 4046-4056) */
+            int best = left[0];
+            double mn = 1e100;
+            for (int q = 0; q < nl; ++q)
+                if (sgp[left[q]] < mn) {
+                    mn = sgp[left[q]];
+                    best = left[q];
+                }
+            const int bi = sr[best] / S, bj = sc[best] / S, bu = sr[best] % S - h, bv = sc[best] % S - h;
+            orbit_free(&o);
+            grid_ref_c(&g, bi, bj, bu, bv, S, rre, rim);
+            orbit_build_sa(&o, &sa, rre, rim, pb, mxit, escR, delta, ede);
+            {
+                int ne_;
+                const size_t cc = (size_t)sr[best] * W * S + sc[best];
+                const double v = ref_pixel_value(&o, mxit, ede ? 2 : 0, esc2, g.dx, &ne_);
+                f->cell_val[cc] = v;
+                f->cell_nesc[cc] = ne_;
+                f->cells[cc] = ne_ < 0 ? -2.f : (float)v;
+            }
+            ctx.zf = o.zf;
+            ctx.z_m3 = o.z_m3;
+            ctx.dfr = o.dfr;
+            ctx.dfi = o.dfi;
+            ctx.mx_ref_it = o.ref_it;
+            ctx.sa_it = sa.it;
+            ctx.bla = NULL; /* the base table belongs to the old reference (MP.cpp:2003) */
+            rc = sc[best];
+            rr = sr[best];
+            np = 0;
+            for (int q = 0; q < nl; ++q)
+                if (left[q] != best) pend[np++] = left[q];
+            if (np == 0) break;
+            MXO_CELL_DC();
+        }
+#undef MXO_CELL_DC
+        free(flags);
+        free(sc);
+        free(sr);
+        free(pend);
+        free(left);
+        free(sgp);
+        free(ccol);
+        free(crow);
+    }
     f->steps = steps;
     f->bla_applies = ap;
     f->bla_skipped = sk;

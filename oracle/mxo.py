@@ -63,6 +63,8 @@ class DFrame(C.Structure):
         ("passes", C.c_int), ("escalate", C.c_int), ("sa_it0", C.c_int), ("sa_order0", C.c_int),
         ("steps", C.c_longlong), ("bla_applies", C.c_longlong), ("bla_skipped", C.c_longlong),
         ("rebases", C.c_longlong), ("ede", C.c_int),
+        ("sub", C.c_int), ("ss", C.c_int), ("cells", C.POINTER(C.c_float)), ("cell_val", C.POINTER(C.c_double)),
+        ("cell_nesc", C.POINTER(C.c_int32)), ("ss_pixels", C.c_int), ("ss_passes", C.c_int),
     ]
 
 
@@ -178,9 +180,12 @@ def compute_fe(cre, cim, scale, W, H, mxit, c_method=0, escape_radius=1e8, prec_
 
 
 def compute_dbl(cre, cim, scale, W, H, mxit, escape_radius=1e8, prec_bits=None, use_bla=True, bla_minskip=8,
-                interior=True, threads=None, c_method=0):
+                interior=True, threads=None, c_method=0, sub=1):
     """Whole plain-double frame (Compute's double path: centre-pixel reference with its
-    series approximation, BLA, probe gate, passes re-referenced on glitches)."""
+    series approximation, BLA, probe gate, passes re-referenced on glitches).  With
+    sub > 1 the info dict carries the (H sub) x (W sub) cells ("cells", "cell_val",
+    "cell_nesc") and, when c_method has the super-sampling bit (1), the adaptive
+    sub-samples' passes ("ss_pixels", "ss_passes")."""
     f = DFrame()
     f.cre, f.cim, f.scale = cre.encode(), cim.encode(), scale.encode()
     f.W, f.H, f.mxit, f.escape_radius = W, H, mxit, escape_radius
@@ -194,9 +199,21 @@ def compute_dbl(cre, cim, scale, W, H, mxit, escape_radius=1e8, prec_bits=None, 
     f.val = val.ctypes.data_as(C.POINTER(C.c_double))
     f.n_esc = nesc.ctypes.data_as(C.POINTER(C.c_int32))
     f.out = out.ctypes.data_as(C.POINTER(C.c_float))
+    f.sub, f.ss = sub, int(sub > 1 and (c_method & 1) != 0)
+    if sub > 1:
+        ncell = W * H * sub * sub
+        cells = np.full(ncell, -10.0, np.float32)
+        cell_val = np.full(ncell, -10.0, np.float64)
+        cell_nesc = np.full(ncell, np.iinfo(np.int32).min, np.int32)
+        f.cells = cells.ctypes.data_as(C.POINTER(C.c_float))
+        f.cell_val = cell_val.ctypes.data_as(C.POINTER(C.c_double))
+        f.cell_nesc = cell_nesc.ctypes.data_as(C.POINTER(C.c_int32))
     lib().mxo_compute_dbl(C.byref(f))
     info = {k: getattr(f, k) for k in ("passes", "escalate", "sa_it0", "sa_order0", "steps", "bla_applies",
-                                        "bla_skipped", "rebases")}
+                                        "bla_skipped", "rebases", "ss_pixels", "ss_passes")}
+    if sub > 1:
+        info.update(cells=cells.reshape(H * sub, W * sub), cell_val=cell_val.reshape(H * sub, W * sub),
+                    cell_nesc=cell_nesc.reshape(H * sub, W * sub))
     return val.reshape(H, W), nesc.reshape(H, W), out.reshape(H, W), info
 
 

@@ -133,17 +133,36 @@ int main(int argc, char** argv) {
             for (float v : buf)
                 if (v == -10.f || v == 7.0f) return fail("shallow view left a pixel");
         }
+        // adaptive supersampling on the plain-double path (scale 1e100) runs on the
+        // device too: every pixel's centre cell is computed, refined pixels all sub cells
+        {
+            const int S = 3;
+            std::vector<float> buf((size_t)W * H * S * S, -10.f);
+            ComputeRequest r = q;
+            r.scale = mid.v;
+            r.coloringMethod = 1;
+            r.sub = S;
+            r.iterations = buf.data();
+            const int before = cpu->calls;
+            if (!be->compute(r) || cpu->calls != before || !be->lastComputeUsedGpuPath()) return fail("double-path supersampling");
+            for (int i = 0; i < H; ++i)
+                for (int j = 0; j < W; ++j) {
+                    int done = 0;
+                    for (int u = 0; u < S; ++u)
+                        for (int v = 0; v < S; ++v) done += buf[((size_t)i * S + u) * W * S + (size_t)j * S + v] != -10.f;
+                    if (buf[((size_t)i * S + 1) * W * S + (size_t)j * S + 1] == -10.f || (done != 1 && done != S * S))
+                        return fail("double-path supersampling cells");
+                }
+        }
         // requests off this path go to the CPU backend (compute_backend_d3d11.cpp:301-314):
-        // the Julia mode, an orbit-trap colouring, adaptive supersampling on the
-        // plain-double path
+        // the Julia mode, an orbit-trap colouring
         struct Off {
             const char* what;
             mpf_ptr scale;
             ComputeMode mode;
             int method, sub;
         } off[] = {{"julia", sc.v, ComputeMode::Julia, 0, 1},
-                   {"orbit trap", sc.v, ComputeMode::Mandelbrot, 32, 1},
-                   {"double-path supersampling", mid.v, ComputeMode::Mandelbrot, 1, 3}};
+                   {"orbit trap", sc.v, ComputeMode::Mandelbrot, 32, 1}};
         for (const Off& o : off) {
             std::vector<float> buf((size_t)W * H * o.sub * o.sub, -10.f);
             ComputeRequest r = q;

@@ -149,3 +149,24 @@ def test_orbit_in_lds_changes_nothing(ctx, name, ede):
     assert np.array_equal(n0, n1) and np.array_equal(v0.view(np.int64), v1.view(np.int64))
     assert np.array_equal(s0, s1)
     assert np.array_equal(np.sort(u0), np.sort(u1))
+
+
+@pytest.mark.parametrize("name,c_method", [("ticktock140", 1), ("mini106", 3), ("point31", 1)])
+def test_double_supersampled_frame_matches_oracle(ctx, name, c_method):
+    """Adaptive supersampling on the double path (mandel_perturbation.cpp:1789-2005):
+    the same cells refined, every cell's escape count equal to the oracle's and its
+    value within 1e-9; ticktock at 1e140 re-references at a sub-sample, the minibrot's
+    edge is refined under EDE."""
+    from test_oracle_dbl_ss import MINI106, TICKTOCK140
+
+    frames = {"ticktock140": TICKTOCK140 + (0.0, 0), "mini106": MINI106 + (0.0, 0), "point31": FRAMES["point31"]}
+    cre, cim, scale, W, H, mxit, R, pb = frames[name]
+    print(f"[dbl ss] {name} {W}x{H} sub 3 c_method={c_method}", flush=True)
+    it, val, nesc, info = ctx.compute(cre, cim, scale, W, H, mxit, escape_radius=R, prec_bits=pb, c_method=c_method, sub=3)
+    _, _, _, oi = mxo.compute_dbl(cre, cim, scale, W, H, mxit, escape_radius=R or 1e8, prec_bits=pb or None,
+                                  c_method=c_method, sub=3)
+    assert info["path"] == 1 and info["ss_pixels"] == oi["ss_pixels"] > 0
+    done = oi["cell_nesc"] != np.iinfo(np.int32).min
+    assert np.array_equal(it != -10, done)
+    _check(val[done], nesc[done], oi["cell_val"][done], oi["cell_nesc"][done], name)
+    assert np.array_equal(it[done], oi["cells"][done]) or np.abs(it[done] - oi["cells"][done]).max() < 1e-3

@@ -176,7 +176,6 @@ def test_escalated_frame_prepares_the_floatexp_reference():
 @pytest.mark.parametrize("sub,c_method,scale,rc", [
     (2, 1, cases.pow10(300), mxhip.MX_E_INVALID),          # even sub: past the engine's buffer
     (17, 1, cases.pow10(300), mxhip.MX_E_INVALID),
-    (3, 1, "40000000000", mxhip.MX_E_UNSUPPORTED),         # adaptive SS on the plain-double path
 ])
 def test_supersampling_requests_refused(sub, c_method, scale, rc):
     with pytest.raises(mxhip.MxError) as e:
@@ -187,6 +186,8 @@ def test_supersampling_requests_refused(sub, c_method, scale, rc):
 def test_supersampling_prepare_accepted():
     fv = mxhip.frame_prepare("0", "1", cases.pow10(300), 16, 8, 1000, c_method=1, sub=3)
     assert fv["path"] == 2
+    fv = mxhip.frame_prepare("0", "1", "40000000000", 16, 8, 1000, c_method=1, sub=3)
+    assert fv["path"] == 1
     fv = mxhip.frame_prepare("0", "1", "40000000000", 16, 8, 1000, c_method=1, sub=3, options=mxhip.MX_OPT_FLOATEXP)
     assert fv["path"] == 2
 
