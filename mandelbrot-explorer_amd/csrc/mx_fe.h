@@ -51,9 +51,18 @@ MX_HD Fe fe_renorm(double m, int e) {
     return fe_make(mm, e + ex);
 }
 
+// fe_renorm of a value that cannot be 0: the product or quotient of two non-zero
+// mantissas (|m| >= 0.5 or a stored constant such as 1), the square root of a positive
+// one.  frexp alone; identical results, without fe_renorm's zero test.
+MX_HD Fe fe_renorm_nz(double m, int e) {
+    int ex;
+    double mm = frexp(m, &ex);
+    return fe_make(mm, e + ex);
+}
+
 MX_HD Fe fe_mul(Fe a, Fe b) {
     if (a.m == 0.0 || b.m == 0.0) return fe_make(0.0, 0);
-    return fe_renorm(a.m * b.m, a.e + b.e);
+    return fe_renorm_nz(a.m * b.m, a.e + b.e);
 }
 
 MX_HD Fe fe_mul_d(Fe a, double b) {
@@ -76,7 +85,7 @@ MX_HD Fe fe_sub(Fe a, Fe b) { return fe_add(a, fe_neg(b)); }
 
 MX_HD Fe fe_div(Fe a, Fe b) {
     if (a.m == 0.0) return fe_make(0.0, 0);
-    return fe_renorm(a.m / b.m, a.e - b.e);
+    return fe_renorm_nz(a.m / b.m, a.e - b.e);
 }
 
 MX_HD bool fe_abs_less(Fe a, Fe b) {
@@ -94,7 +103,7 @@ MX_HD Fe fe_sqrt(Fe a) {
         m *= 2.0;
         --e;
     }
-    return fe_renorm(sqrt(m), e / 2);
+    return fe_renorm_nz(sqrt(m), e / 2);
 }
 
 // Exact 2^d for d in [-1022, 1023] built from the exponent bits.

@@ -264,6 +264,9 @@ MX_HD void step_j(const Px& p, double zmr, double zmi, double& a, double& b) {
 // One trip of the pixel loop.  Returns true when the sample is finished (value and
 // n_esc written to v/n).  n: escape iteration, -1 = reached mxit, -2 = detected interior.
 template <bool EDE, bool INTERIOR>
+MX_HD bool trip_step(const MxRefDev& R, const MxTileDev& T, Px& p, double& v, int& n, Counters& c, bool& s_changed);
+
+template <bool EDE, bool INTERIOR>
 MX_HD bool trip(const MxRefDev& R, GPtr<const int2> lev, const MxTileDev& T, Px& p, double& v, int& n, Counters& c) {
     const double ESC2 = T.esc2;
     const int reflen = R.mx_ref_it + 1;
@@ -290,6 +293,7 @@ MX_HD bool trip(const MxRefDev& R, GPtr<const int2> lev, const MxTileDev& T, Px&
             p.twin2r += p.twin2r;
         }
     }
+    bool applied = false;
     if ((T.flags & MX_F_BLA) && p.m >= 2) {
         int kbla = 0;
         const int skip = try_bla(R, lev, p.rm, p.S, p.S2, p.wr, p.wi, p.dr, p.di, per ? per : reflen, ESC2, kbla);
@@ -319,9 +323,6 @@ MX_HD bool trip(const MxRefDev& R, GPtr<const int2> lev, const MxTileDev& T, Px&
             c.applies++;
             c.skipped += skip;
             p.s = fe_to_double(p.S);
-            p.S2 = fe_mul(p.S, p.S);
-            p.dr = fe_to_double(fe_div(p.dcr, p.S));
-            p.di = fe_to_double(fe_div(p.dci, p.S));
             p.zsr = 1e30;
             p.zsi = 1e30;
             p.save_iter = p.iter;
@@ -329,9 +330,31 @@ MX_HD bool trip(const MxRefDev& R, GPtr<const int2> lev, const MxTileDev& T, Px&
             p.conf_P = 0;
             p.conf_giveup = 0;
             p.xv = 0;
-            return false;
+            applied = true;
         }
     }
+    // a trip that changes S leaves S^2 and d = dc / S to be recomputed once, in one place
+    // for every lane of the wave, after whichever branch changed it
+    bool s_changed = applied;
+    if (!applied && trip_step<EDE, INTERIOR>(R, T, p, v, n, c, s_changed)) return true;
+    if (s_changed) {
+        p.S2 = fe_mul(p.S, p.S);
+        p.dr = fe_to_double(fe_div(p.dcr, p.S));
+        p.di = fe_to_double(fe_div(p.dci, p.S));
+    }
+    return false;
+}
+
+// The step part of a trip (no BLA skip taken at this index).  Returns true when the
+// sample is finished; s_changed is set when S changed (S^2 and d are then left to the
+// caller).
+template <bool EDE, bool INTERIOR>
+MX_HD bool trip_step(const MxRefDev& R, const MxTileDev& T, Px& p, double& v, int& n, Counters& c, bool& s_changed) {
+    const double ESC2 = T.esc2;
+    const int reflen = R.mx_ref_it + 1;
+    const int per = R.period;
+    const int mend = per ? INT_MAX : reflen;
+    (void)reflen;
     // X_m and X_{m+1} come from the previous trip; X_{m+2} is fetched one trip ahead
     // (m <= mx_ref_it here, so rm+1 is in range; rm+2 only when m+1 <= mx_ref_it; a
     // periodic orbit carries copies of its first two entries after the last, so
@@ -383,9 +406,7 @@ MX_HD bool trip(const MxRefDev& R, GPtr<const int2> lev, const MxTileDev& T, Px&
         p.wr = r.wr;
         p.wi = r.wi;
         p.s = fe_to_double(p.S);
-        p.S2 = fe_mul(p.S, p.S);
-        p.dr = fe_to_double(fe_div(p.dcr, p.S));
-        p.di = fe_to_double(fe_div(p.dci, p.S));
+        s_changed = true;
     } else {
         p.wr = nwr;
         p.wi = nwi;
@@ -488,12 +509,10 @@ MX_HD bool trip(const MxRefDev& R, GPtr<const int2> lev, const MxTileDev& T, Px&
         const Fe wmag = fe_sqrt(fe_from(wmag2));
         p.S = fe_mul(p.S, wmag);
         p.s = fe_to_double(p.S);
-        p.S2 = fe_mul(p.S, p.S);
         const double inv = 1.0 / fe_to_double(wmag);
         p.wr *= inv;
         p.wi *= inv;
-        p.dr = fe_to_double(fe_div(p.dcr, p.S));
-        p.di = fe_to_double(fe_div(p.dci, p.S));
+        s_changed = true;
     }
     const double dzmag2 = p.s * p.s * (p.wr * p.wr + p.wi * p.wi);
     if (zrad < 1e-8 || zrad < dzmag2 || p.m >= mend) {
@@ -515,11 +534,9 @@ MX_HD bool trip(const MxRefDev& R, GPtr<const int2> lev, const MxTileDev& T, Px&
             } else {
                 p.S = Snew;
                 p.s = fe_to_double(p.S);
-                p.S2 = fe_mul(p.S, p.S);
                 p.wr = fe_to_double(fe_div(zrfe, p.S));
                 p.wi = fe_to_double(fe_div(zife, p.S));
-                p.dr = fe_to_double(fe_div(p.dcr, p.S));
-                p.di = fe_to_double(fe_div(p.dci, p.S));
+                s_changed = true;
             }
             p.m = 0;
             p.rm = -1;
