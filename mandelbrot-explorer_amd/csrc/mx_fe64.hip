@@ -264,7 +264,8 @@ MX_HD void step_j(const Px& p, double zmr, double zmi, double& a, double& b) {
 // One trip of the pixel loop.  Returns true when the sample is finished (value and
 // n_esc written to v/n).  n: escape iteration, -1 = reached mxit, -2 = detected interior.
 template <bool EDE, bool INTERIOR>
-MX_HD bool trip_step(const MxRefDev& R, const MxTileDev& T, Px& p, double& v, int& n, Counters& c, bool& s_changed);
+MX_HD bool trip_step(const MxRefDev& R, const MxTileDev& T, Px& p, double& v, int& n, Counters& c, bool& s_changed,
+                     bool& j_changed);
 
 template <bool EDE, bool INTERIOR>
 MX_HD bool trip(const MxRefDev& R, GPtr<const int2> lev, const MxTileDev& T, Px& p, double& v, int& n, Counters& c) {
@@ -293,7 +294,7 @@ MX_HD bool trip(const MxRefDev& R, GPtr<const int2> lev, const MxTileDev& T, Px&
             p.twin2r += p.twin2r;
         }
     }
-    bool applied = false;
+    bool applied = false, j_changed = false;
     if ((T.flags & MX_F_BLA) && p.m >= 2) {
         int kbla = 0;
         const int skip = try_bla(R, lev, p.rm, p.S, p.S2, p.wr, p.wi, p.dr, p.di, per ? per : reflen, ESC2, kbla);
@@ -313,7 +314,7 @@ MX_HD bool trip(const MxRefDev& R, GPtr<const int2> lev, const MxTileDev& T, Px&
                 } else {
                     p.jr = fe_to_double(fe_div(nJr, p.SJ));
                     p.ji = fe_to_double(fe_div(nJi, p.SJ));
-                    p.invSJd = 1.0 / fe_to_double(p.SJ);
+                    j_changed = true;
                 }
             }
             p.m += skip;
@@ -322,7 +323,6 @@ MX_HD bool trip(const MxRefDev& R, GPtr<const int2> lev, const MxTileDev& T, Px&
             if (per && p.rm >= per) p.rm -= per;
             c.applies++;
             c.skipped += skip;
-            p.s = fe_to_double(p.S);
             p.zsr = 1e30;
             p.zsi = 1e30;
             p.save_iter = p.iter;
@@ -333,23 +333,28 @@ MX_HD bool trip(const MxRefDev& R, GPtr<const int2> lev, const MxTileDev& T, Px&
             applied = true;
         }
     }
-    // a trip that changes S leaves S^2 and d = dc / S to be recomputed once, in one place
-    // for every lane of the wave, after whichever branch changed it
+    // a trip that changes S (or SJ) leaves what derives from it to be recomputed once, in
+    // one place for every lane of the wave, after whichever branch changed it: s, S^2
+    // and d = dc / S (1 / SJ in double)
     bool s_changed = applied;
-    if (!applied && trip_step<EDE, INTERIOR>(R, T, p, v, n, c, s_changed)) return true;
+    if (!applied && trip_step<EDE, INTERIOR>(R, T, p, v, n, c, s_changed, j_changed)) return true;
     if (s_changed) {
+        p.s = fe_to_double(p.S);
         p.S2 = fe_mul(p.S, p.S);
         p.dr = fe_to_double(fe_div(p.dcr, p.S));
         p.di = fe_to_double(fe_div(p.dci, p.S));
     }
+    if (EDE && j_changed) p.invSJd = 1.0 / fe_to_double(p.SJ);
     return false;
 }
 
 // The step part of a trip (no BLA skip taken at this index).  Returns true when the
-// sample is finished; s_changed is set when S changed (S^2 and d are then left to the
-// caller).
+// sample is finished; s_changed / j_changed are set when S / SJ changed (what derives
+// from them is then left to the caller; s itself is kept current here where the step
+// reads it).
 template <bool EDE, bool INTERIOR>
-MX_HD bool trip_step(const MxRefDev& R, const MxTileDev& T, Px& p, double& v, int& n, Counters& c, bool& s_changed) {
+MX_HD bool trip_step(const MxRefDev& R, const MxTileDev& T, Px& p, double& v, int& n, Counters& c, bool& s_changed,
+                     bool& j_changed) {
     const double ESC2 = T.esc2;
     const int reflen = R.mx_ref_it + 1;
     const int per = R.period;
@@ -435,7 +440,7 @@ MX_HD bool trip_step(const MxRefDev& R, const MxTileDev& T, Px& p, double& v, in
                 const double inv = 1.0 / fe_to_double(jmag);
                 p.jr *= inv;
                 p.ji *= inv;
-                p.invSJd = 1.0 / fe_to_double(p.SJ);
+                j_changed = true;
             }
         }
     }
@@ -533,7 +538,6 @@ MX_HD bool trip_step(const MxRefDev& R, const MxTileDev& T, Px& p, double& v, in
                 p.wr = p.wi = 0.0;
             } else {
                 p.S = Snew;
-                p.s = fe_to_double(p.S);
                 p.wr = fe_to_double(fe_div(zrfe, p.S));
                 p.wi = fe_to_double(fe_div(zife, p.S));
                 s_changed = true;
