@@ -20,6 +20,12 @@ beside `value`.
 Metric: nominal giga pixel-iterations per second (sum of escape counts, max_iter for
 interior samples: verify.cpp:261-273) and the frame time.
 
+Reported beside the headline (each leg can be skipped): the end-to-end frame through
+mx_compute, the plain-double path's 640x360 frame at scale 4e10 with the CPU oracle on the
+same frame, 96 batched uneven tiles over 12 boundary-walk locations in one persistent
+launch (tests/golden/c4_requests.json), and a 240-frame 1280x720 zoom to 1e-500 replayed
+as one HIP graph.
+
 Prints one JSON line.  --gpus N > 1 is refused: this build runs on one GPU.
 """
 import argparse
@@ -137,14 +143,15 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-double", action="store_true", help="skip the plain-double path (640x360 @ scale 4e10) leg")
     ap.add_argument("--no-e2e", action="store_true", help="skip the end-to-end mx_compute frame")
-    ap.add_argument("--c4", action="store_true", help="add the batched uneven-tiles leg (96 tiles, 12 references)")
+    ap.add_argument("--no-c4", dest="c4", action="store_false",
+                    help="skip the batched uneven-tiles leg (96 tiles, 12 references; run when its request file exists)")
     ap.add_argument("--no-tune", action="store_true", help="skip the autotuner (chunk / grid size / BLA cutoff)")
     ap.add_argument("--knobs", default="auto", help="JSON file of launch knobs per form: read instead of tuning when "
                     "it exists, written after tuning otherwise; 'auto' (default): profiles/knobs_<workload>.json, "
                     "the autotuner's committed result that the profiles under profiles/ were taken with")
     ap.add_argument("--copy-calibration", action="store_true", help="only copy 1 GiB device to device with the "
                     "library's copy kernel and exit (the reading that scales rocprofv3's FETCH_SIZE / WRITE_SIZE)")
-    ap.add_argument("--c5", action="store_true", help="add the zoom-sequence leg (hipGraph of 1280x720 frames)")
+    ap.add_argument("--no-c5", dest="c5", action="store_false", help="skip the zoom-sequence leg (hipGraph of 1280x720 frames)")
     ap.add_argument("--c5-frames", type=int, default=240, help="first N frames of the zoom sequence")
     args = ap.parse_args()
     if args.gpus != 1:
