@@ -355,14 +355,19 @@ def c4_leg(ctx, forms=("fe32", "fe64"), reps=1):
     grid = 1024
     t0 = time.time()
     locs = {}
+    # every location on the floatexp path with the centre pixel's full reference (the
+    # engine's MANDEL_FE=1 MANDEL_PERIODIC=0)
+    opts = mxhip.MX_OPT_FLOATEXP | mxhip.MX_OPT_NO_PERIODIC
     for l in spec["locations"]:
         mx = max(q["max_iter"] for q in spec["requests"] if q["location"] == l["seed"])
-        fv = mxhip.frame_prepare(l["re"], l["im"], "4" + "0" * l["depth"], grid, grid, mx)
+        fv = mxhip.frame_prepare(l["re"], l["im"], "4" + "0" * l["depth"], grid, grid, mx, options=opts)
         locs[l["seed"]] = (fv, ctx.upload(fv))
     prep_s = time.time() - t0
-    log(f"c4: 12 references prepared in {prep_s:.1f} s")
-    out = {"workload": "96 square tiles (64..1024) of 12 seeded walk views (1e-86..1e-276, 1024x1024 grid "
-                       "each), max_iter 1e4..1e7, smooth, BLA, interior detection",
+    log(f"c4: {len(locs)} references prepared in {prep_s:.1f} s")
+    depths = sorted(l["depth"] for l in spec["locations"])
+    out = {"workload": f"96 square tiles (64..1024) of {len(locs)} seeded walk views (widths 1e-{depths[0]}.."
+                       f"1e-{depths[-1]}, a 1024x1024 grid each), max_iter 1e4..1e7, smooth, BLA, interior "
+                       f"detection (tests/golden/c4_requests.json)",
            "requests": len(spec["requests"]), "samples": int(sum(q["side"] ** 2 for q in spec["requests"])),
            "host_prepare_s": prep_s}
     res = {}

@@ -11,7 +11,9 @@ Definition (the workload spec's batch, with the choices it leaves open fixed):
   request r renders the side x side tile of that grid at column x0 = draw mod
   (1025 - side) and row y0 = draw mod (1025 - side), so the requests at one location
   share one reference orbit (built to the largest max_iter among them), one BLA
-  table and one pair of dc tables.
+  table and one pair of dc tables; every location is rendered on the floatexp path
+  with the centre pixel's full reference (the engine's MANDEL_FE=1 MANDEL_PERIODIC=0),
+  smooth output, interior detection on.
 Writes tests/golden/c4_seed<k>.json (one location centre each) and, once all twelve
 exist, tests/golden/c4_requests.json.
 Usage: python tests/golden/make_c4.py [--plan] [--seeds 1,2,...] [--out DIR]

@@ -165,3 +165,68 @@ def test_zoom_graph_equals_frame_by_frame(ctx):
             p.destroy()
         for r in refs:
             r.release()
+
+
+C4 = os.path.join(GOLD, "c4_requests.json")
+
+
+@pytest.mark.skipif(not os.path.exists(C4), reason="tests/golden/c4_requests.json not generated")
+def test_batched_tiles_match_oracle(ctx):
+    """C4: the 96 requests' tiles (64..1024 square, max_iter 1e4..1e7) over the 12
+    walk locations, each location's reference, BLA table and dc tables built once on the
+    floatexp path with the centre pixel's full reference; one persistent launch of all
+    tiles (largest work first) equals the tiles launched one by one, and a 12x12 window of
+    each location's first request equals the oracle's per-pixel loop on the same
+    reference."""
+    import autotune
+
+    spec = json.load(open(C4))
+    assert len(spec["locations"]) == 12 and len(spec["requests"]) == 96
+    opts = mxhip.MX_OPT_FLOATEXP | mxhip.MX_OPT_NO_PERIODIC
+    locs, refs = {}, []
+    plans, subs = [], []
+    try:
+        for l in spec["locations"]:
+            mx = max(q["max_iter"] for q in spec["requests"] if q["location"] == l["seed"])
+            print(f"[c4] location {l['seed']} depth {l['depth']} max_iter {mx}", flush=True)
+            fv = mxhip.frame_prepare(l["re"], l["im"], "4" + "0" * l["depth"], 1024, 1024, mx, options=opts)
+            assert fv["path"] == 2
+            locs[l["seed"]] = (fv, ctx.upload(fv))
+            refs.append(locs[l["seed"]][1])
+        for q in spec["requests"]:
+            fv, ref = locs[q["location"]]
+            sub = autotune.subframe(fv, q["x0"], q["y0"], q["side"], q["side"])
+            t = ctx.tile(sub, form=mxhip.MX_FORM_FE64, interior=True, mxit=q["max_iter"])
+            t.skip_sample = sub["skip"]
+            plans.append(ctx.plan(ref, t))
+            subs.append(sub)
+        order = sorted(range(96), key=lambda k: -spec["requests"][k]["side"] ** 2 * spec["requests"][k]["max_iter"])
+        b = ctx.batch([plans[k] for k in order])
+        print("[c4] batch launch", flush=True)
+        b.launch()
+        b.wait()
+        batched = [p.read()[:2] for p in plans]
+        b.destroy()
+        for k, p in enumerate(plans):
+            p.launch()
+            p.wait()
+            nesc, val, _, _ = p.read()
+            assert np.array_equal(nesc, batched[k][0]) and np.array_equal(val, batched[k][1]), k
+        seen = set()
+        for k, q in enumerate(spec["requests"]):
+            if q["location"] in seen:
+                continue
+            seen.add(q["location"])
+            fv = locs[q["location"]][0]
+            side = q["side"]
+            w = 12
+            x0, y0 = (side - w) // 2, (side - w) // 3
+            idx = np.array([(y0 + i) * side + x0 + j for i in range(w) for j in range(w)])
+            idx = idx[idx != subs[k]["skip"]]
+            ov, on = _oracle_samples(dict(subs[k], W=side), idx, q["max_iter"], False)
+            _check(batched[k][1][idx], batched[k][0][idx], ov, on, f"c4 request {k}")
+    finally:
+        for p in plans:
+            p.destroy()
+        for r in refs:
+            r.release()
