@@ -342,7 +342,7 @@ def double_leg(ctx, no_cpu, reps=5):
     return out
 
 
-def c4_leg(ctx, forms=("fe32", "fe64"), reps=1):
+def c4_leg(ctx, forms=("fe64", "fe32"), reps=1):
     """96 uneven tiles over 12 boundary-walk locations (tests/golden/c4_requests.json):
     each location's reference, BLA table and dc tables are built once and resident;
     the tiles run as one persistent launch per form (largest work first), and the
@@ -365,7 +365,7 @@ def c4_leg(ctx, forms=("fe32", "fe64"), reps=1):
     prep_s = time.time() - t0
     log(f"c4: {len(locs)} references prepared in {prep_s:.1f} s")
     depths = sorted(l["depth"] for l in spec["locations"])
-    out = {"workload": f"96 square tiles (64..1024) of {len(locs)} seeded walk views (widths 1e-{depths[0]}.."
+    out = {"workload": f"{len(spec['requests'])} square tiles (64..1024) of {len(locs)} seeded walk views (widths 1e-{depths[0]}.."
                        f"1e-{depths[-1]}, a 1024x1024 grid each), max_iter 1e4..1e7, smooth, BLA, interior "
                        f"detection (tests/golden/c4_requests.json)",
            "requests": len(spec["requests"]), "samples": int(sum(q["side"] ** 2 for q in spec["requests"])),
@@ -441,7 +441,7 @@ def c5_frames(n=240, width_end_exp=500, mxit0=1000, mxit1=5_000_000):
     return out
 
 
-def c5_leg(ctx, nframes=240, forms=("fe32", "fe64")):
+def c5_leg(ctx, nframes=240, forms=("fe64", "fe32")):
     """Zoom animation: 1280x720 frames from width 4 to 1e-500 around the seed-42 walk
     centre, captured as one HIP graph per form.  Frames at scale <= 1e6 (the engine's
     shallow, non-perturbation path) are left out; every other frame runs in the
