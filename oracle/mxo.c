@@ -2131,9 +2131,37 @@ MXO_EXPORT void mxo_pixels_fe(const mxo_pixels_in* in, int n, const double* dcr_
     free(zim);
 }
 
-/* Brute-force pixel grid for a frame (ComputeDirect, MP.cpp:959-990): pixels on a
- * (step x step) grid; others untouched.  wp_bits = working precision. */
-MXO_EXPORT void mxo_direct(const char* cre, const char* cim, const char* scale, int W, int H, int prec_bits, int wp_bits, int mxit, int c_method, float escape_radius, int step, int threads, int track_d, double* val, int32_t* nesc) {
+/* c lies inside the main cardioid or the period-2 disc by at least 2^(8 - prec):
+ * q (q + x - 1/4) < y^2 / 4 with q = (x - 1/4)^2 + y^2, or (x + 1)^2 + y^2 < 1/16,
+ * evaluated in MPFR with that margin for its rounding, so a point near either boundary
+ * is never claimed.  Such a c has an attracting fixed point or 2-cycle: its orbit stays
+ * bounded and never escapes. */
+static int in_main_bulbs(mpfr_srcptr x, mpfr_srcptr y) {
+    const mpfr_prec_t p = mpfr_get_prec(x);
+    mpfr_t q, t, y2, m;
+    mpfr_inits2(p + 16, q, t, y2, m, (mpfr_ptr)0);
+    mpfr_set_ui_2exp(m, 1, 8 - (long)p, MPFR_RNDN);
+    mpfr_sqr(y2, y, MPFR_RNDN);
+    mpfr_sub_d(t, x, 0.25, MPFR_RNDN);
+    mpfr_sqr(q, t, MPFR_RNDN);
+    mpfr_add(q, q, y2, MPFR_RNDN);
+    mpfr_add(t, t, q, MPFR_RNDN);
+    mpfr_mul(t, t, q, MPFR_RNDN);  /* q (q + x - 1/4) */
+    mpfr_add(t, t, m, MPFR_RNDN);
+    mpfr_div_2ui(q, y2, 2, MPFR_RNDN);
+    int in = mpfr_less_p(t, q);
+    if (!in) {
+        mpfr_add_ui(t, x, 1, MPFR_RNDN);
+        mpfr_sqr(t, t, MPFR_RNDN);
+        mpfr_add(t, t, y2, MPFR_RNDN);
+        mpfr_add(t, t, m, MPFR_RNDN);
+        in = mpfr_cmp_d(t, 0.0625) < 0;
+    }
+    mpfr_clears(q, t, y2, m, (mpfr_ptr)0);
+    return in;
+}
+
+static void direct_grid(const char* cre, const char* cim, const char* scale, int W, int H, int prec_bits, int wp_bits, int mxit, int c_method, float escape_radius, int step, int threads, int track_d, int skip_bulbs, double* val, int32_t* nesc) {
     mxo_grid g;
     grid_init(&g, cre, cim, scale, W, H, 0, 0, prec_bits > wp_bits ? prec_bits : wp_bits, 0);
     const double esc2 = (double)(escape_radius * escape_radius);
@@ -2145,11 +2173,28 @@ MXO_EXPORT void mxo_direct(const char* cre, const char* cim, const char* scale, 
         mpfr_inits2(g.prec, a, b, (mpfr_ptr)0);
         grid_pixel_c(&g, i, j, 0, 0, 1, a, b);
         int ne;
-        val[i * W + j] = point_mpfr(a, b, wp_bits, mxit, (c_method & 2) != 0, track_d, esc2, g.dx, &ne);
+        if (skip_bulbs && in_main_bulbs(a, b)) {
+            val[i * W + j] = -2.0;
+            ne = -2;
+        } else
+            val[i * W + j] = point_mpfr(a, b, wp_bits, mxit, (c_method & 2) != 0, track_d, esc2, g.dx, &ne);
         nesc[i * W + j] = ne;
         mpfr_clears(a, b, (mpfr_ptr)0);
     }
     grid_clear(&g);
+}
+
+/* Brute-force pixel grid for a frame (ComputeDirect, MP.cpp:959-990): pixels on a
+ * (step x step) grid; others untouched.  wp_bits = working precision. */
+MXO_EXPORT void mxo_direct(const char* cre, const char* cim, const char* scale, int W, int H, int prec_bits, int wp_bits, int mxit, int c_method, float escape_radius, int step, int threads, int track_d, double* val, int32_t* nesc) {
+    direct_grid(cre, cim, scale, W, H, prec_bits, wp_bits, mxit, c_method, escape_radius, step, threads, track_d, 0, val, nesc);
+}
+
+/* The same grid, with the pixels inside the main cardioid or the period-2 disc marked
+ * interior (-2) without iterating: the escaping pixels and their values are those of
+ * mxo_direct.  For the boundary-walk probes, which only rank escaping pixels. */
+MXO_EXPORT void mxo_direct_skip_bulbs(const char* cre, const char* cim, const char* scale, int W, int H, int prec_bits, int wp_bits, int mxit, int c_method, float escape_radius, int step, int threads, int track_d, double* val, int32_t* nesc) {
+    direct_grid(cre, cim, scale, W, H, prec_bits, wp_bits, mxit, c_method, escape_radius, step, threads, track_d, 1, val, nesc);
 }
 
 /* Brute-force samples idx[0..n) (row-major in the W x H grid of the frame) */

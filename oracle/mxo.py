@@ -99,6 +99,7 @@ def lib():
         L.mxo_bla_tables.restype = C.c_int
         L.mxo_pixels_fe.argtypes = [C.POINTER(PixelsIn), C.c_int] + [C.c_void_p] * 7
         L.mxo_direct.argtypes = [C.c_char_p, C.c_char_p, C.c_char_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_float, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p]
+        L.mxo_direct_skip_bulbs.argtypes = L.mxo_direct.argtypes
         L.mxo_direct_list.argtypes = [C.c_char_p, C.c_char_p, C.c_char_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
                                       C.c_int, C.c_float, C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p]
         L.mxo_point.argtypes = [C.c_char_p, C.c_char_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, C.c_char_p, C.c_void_p]
@@ -444,12 +445,14 @@ def dbl_setup(cre, cim, scale, W, H, mxit, escape_radius=1e8, prec_bits=None, ed
 
 
 def direct(cre, cim, scale, W, H, mxit, c_method=0, escape_radius=1e8, step=1, prec_bits=None, wp_bits=None,
-           threads=None, track_d=True):
-    """Brute-force MPFR frame (ComputeDirect); unsampled pixels are NaN / 0."""
+           threads=None, track_d=True, skip_bulbs=False):
+    """Brute-force MPFR frame (ComputeDirect); unsampled pixels are NaN / 0.  skip_bulbs:
+    pixels inside the main cardioid or the period-2 disc are marked interior (-2, -2)
+    without iterating; every escaping pixel is unchanged."""
     pb = prec_bits or precision_bits(scale)
     val = np.full(W * H, np.nan)
     nesc = np.zeros(W * H, np.int32)
-    lib().mxo_direct(cre.encode(), cim.encode(), scale.encode(), W, H, pb, wp_bits or pb, mxit, c_method,
+    (lib().mxo_direct_skip_bulbs if skip_bulbs else lib().mxo_direct)(cre.encode(), cim.encode(), scale.encode(), W, H, pb, wp_bits or pb, mxit, c_method,
                      escape_radius, step, threads or os.cpu_count(), int(track_d), _p(val), _p(nesc))
     return val.reshape(H, W), nesc.reshape(H, W)
 

@@ -142,8 +142,10 @@ def walk(seed, depths, log=print, on_depth=None, resume=None, mxit_cap=MXIT_CAP,
                                                      threads=THREADS, periodic=rule == "doubling",
                                                      c_method=2 if rule == "de" else 0)
             else:
+                # pixels inside the main cardioid or the period-2 disc cannot escape: the
+                # probe marks them without iterating (every escaping pixel is unchanged)
                 val, nesc = mxo.direct(format(cx, "f"), format(cy, "f"), format(scale, "f"), W, H, mxit, prec_bits=prec, wp_bits=prec, threads=THREADS,
-                                       c_method=2 if rule == "de" else 0)
+                                       c_method=2 if rule == "de" else 0, skip_bulbs=True)
             flat = nesc.ravel()
             if rule == "de":
                 # escaping pixels ranked by their distance estimate (pixels), closest first
