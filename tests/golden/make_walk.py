@@ -7,7 +7,11 @@ splitmix64; each step renders a 64x64 probe of the view (square, the engine's gr
 scale = 4/w) with the oracle (full reference from the centre pixel), picks one probe
 pixel uniformly (splitmix64 mod k) among k candidates, recentres on it and halves w;
 MPFR precision of each probe = ceil(-log2 w) + 64 bits.  Centres are recorded at every
-requested depth; the decimal strings carry ceil(-log10 w) + 30 digits.
+requested depth; the decimal strings carry ceil(-log10 w) + 30 digits.  Probes at
+scale <= 1e6 (brute-force MPFR) mark the pixels inside the main cardioid or the
+period-2 disc as interior without iterating them: no rule ranks a non-escaping pixel by
+anything but "did not escape", so the walks are the same as without the shortcut
+(tests/test_oracle_walk_probe.py; C4's seed-6 location regenerated byte for byte).
 
 --rule spec (the workload definition as written): probe max_iter 1024; every pixel is a
   candidate, with count = its escape iteration, or 1024 when it did not escape;
