@@ -139,9 +139,13 @@ extern "C" int mx_ctx_create(int device, mx_ctx** out) {
     ctx->cu_count = prop.multiProcessorCount;
     ctx->lds_max = (int)prop.sharedMemPerBlock;
     if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) return fail(MX_E_HIP);
+    if (hipStreamCreateWithFlags(&ctx->side, hipStreamNonBlocking) != hipSuccess) return fail(MX_E_HIP);
     if (hipHostMalloc((void**)&ctx->cancel_host, sizeof(int), hipHostMallocMapped) != hipSuccess) return fail(MX_E_HIP);
     *ctx->cancel_host = 0;
     if (hipHostGetDevicePointer((void**)&ctx->cancel_dev, ctx->cancel_host, 0) != hipSuccess) return fail(MX_E_HIP);
+    if (hipHostMalloc((void**)&ctx->abort_host, sizeof(int), hipHostMallocMapped) != hipSuccess) return fail(MX_E_HIP);
+    *ctx->abort_host = 0;
+    if (hipHostGetDevicePointer((void**)&ctx->abort_dev, ctx->abort_host, 0) != hipSuccess) return fail(MX_E_HIP);
     // this device's copy of the no-alloc scratch area (the device is current)
     if (hipGetSymbolAddress((void**)&ctx->noalloc_scratch, HIP_SYMBOL(scratch)) != hipSuccess) {
         (void)hipGetLastError();
@@ -159,7 +163,12 @@ extern "C" void mx_ctx_destroy(mx_ctx* ctx) {
         (void)hipStreamSynchronize(ctx->stream);
         (void)hipStreamDestroy(ctx->stream);
     }
+    if (ctx->side) {
+        (void)hipStreamSynchronize(ctx->side);
+        (void)hipStreamDestroy(ctx->side);
+    }
     if (ctx->cancel_host) (void)hipHostFree(ctx->cancel_host);
+    if (ctx->abort_host) (void)hipHostFree(ctx->abort_host);
     delete ctx;
 }
 
@@ -170,11 +179,16 @@ extern "C" const char* mx_last_error(const mx_ctx* ctx) { return ctx ? ctx->err.
 extern "C" void mx_cancel(mx_ctx* ctx) {
     if (!ctx || !ctx->cancel_host) return;
     __atomic_store_n(ctx->cancel_host, 1, __ATOMIC_RELEASE);
+    // a pass on the side stream watches the abort flag: it stops as well
+    if (ctx->abort_host) __atomic_store_n(ctx->abort_host, 1, __ATOMIC_RELEASE);
 }
 
 extern "C" void mx_reset_cancellation(mx_ctx* ctx) {
     if (!ctx || !ctx->cancel_host) return;
-    if (!ctx->busy.load()) __atomic_store_n(ctx->cancel_host, 0, __ATOMIC_RELEASE);
+    if (!ctx->busy.load()) {
+        __atomic_store_n(ctx->cancel_host, 0, __ATOMIC_RELEASE);
+        if (ctx->abort_host) __atomic_store_n(ctx->abort_host, 0, __ATOMIC_RELEASE);
+    }
 }
 
 extern "C" int mx_ref_upload(mx_ctx* ctx, const mx_ref_desc* d, mx_ref** out) {
@@ -345,6 +359,7 @@ extern "C" int mx_plan_create(mx_ctx* ctx, const mx_ref* ref, const mx_tile* t, 
     auto* plan = new (std::nothrow) mx_plan();
     if (!plan) return MX_E_NOMEM;
     plan->ctx = ctx;
+    plan->stream = ctx->stream;
     plan->ref = ref;
     plan->form = t->form;
     const size_t W = sdc ? (size_t)t->n_samples : (size_t)t->width, H = sdc ? 1 : (size_t)t->height, n = W * H;
@@ -466,24 +481,24 @@ extern "C" int mx_plan_launch(mx_plan* plan) {
     mx_ctx* ctx = plan->ctx;
     MX_HIP(hipSetDevice(ctx->device));
     MxTileDev& T = plan->tile;
-    MX_HIP(hipEventRecord(plan->ev0, ctx->stream));
-    hipLaunchKernelGGL(mx_reset_kernel, dim3(1024), dim3(256), 0, ctx->stream, T.queue, T.stats, T.unres_count, T.val, T.nesc, T.out, plan->n);
+    MX_HIP(hipEventRecord(plan->ev0, plan->stream));
+    hipLaunchKernelGGL(mx_reset_kernel, dim3(1024), dim3(256), 0, plan->stream, T.queue, T.stats, T.unres_count, T.val, T.nesc, T.out, plan->n);
     MX_HIP(hipGetLastError());
     if (plan->form == MX_FORM_DOUBLE) {
         MxRefDev R = plan->ref->dev;
         R.minlevel = plan->minlevel;
-        MX_HIP(mx_launch_dbl(R, plan->ref->lev, T, plan->blocks, plan->threads, ctx->stream, plan->nstage));
+        MX_HIP(mx_launch_dbl(R, plan->ref->lev, T, plan->blocks, plan->threads, plan->stream, plan->nstage));
     } else {
-        MX_HIP(launch_jobs(plan->form, plan->job_dev, 1, plan->ede, plan->interior, plan->blocks, plan->threads, ctx->stream));
+        MX_HIP(launch_jobs(plan->form, plan->job_dev, 1, plan->ede, plan->interior, plan->blocks, plan->threads, plan->stream));
     }
-    MX_HIP(hipEventRecord(plan->ev1, ctx->stream));
+    MX_HIP(hipEventRecord(plan->ev1, plan->stream));
     return MX_OK;
 }
 
 extern "C" int mx_plan_wait(mx_plan* plan, double* kernel_ms) {
     if (!plan) return MX_E_INVALID;
     mx_ctx* ctx = plan->ctx;
-    MX_HIP(hipStreamSynchronize(ctx->stream));
+    MX_HIP(hipStreamSynchronize(plan->stream));
     if (kernel_ms) {
         float ms = 0.f;
         MX_HIP(hipEventElapsedTime(&ms, plan->ev0, plan->ev1));
@@ -500,11 +515,11 @@ extern "C" int mx_plan_read(mx_plan* plan, int32_t* n_esc, double* value, float*
     if (!plan) return MX_E_INVALID;
     mx_ctx* ctx = plan->ctx;
     MxTileDev& T = plan->tile;
-    if (n_esc) MX_HIP(hipMemcpyAsync(n_esc, T.nesc, 4 * plan->n, hipMemcpyDeviceToHost, ctx->stream));
-    if (value) MX_HIP(hipMemcpyAsync(value, T.val, 8 * plan->n, hipMemcpyDeviceToHost, ctx->stream));
-    if (out) MX_HIP(hipMemcpyAsync(out, T.out, 4 * plan->n, hipMemcpyDeviceToHost, ctx->stream));
-    if (stats4) MX_HIP(hipMemcpyAsync(stats4, T.stats, 32, hipMemcpyDeviceToHost, ctx->stream));
-    MX_HIP(hipStreamSynchronize(ctx->stream));
+    if (n_esc) MX_HIP(hipMemcpyAsync(n_esc, T.nesc, 4 * plan->n, hipMemcpyDeviceToHost, plan->stream));
+    if (value) MX_HIP(hipMemcpyAsync(value, T.val, 8 * plan->n, hipMemcpyDeviceToHost, plan->stream));
+    if (out) MX_HIP(hipMemcpyAsync(out, T.out, 4 * plan->n, hipMemcpyDeviceToHost, plan->stream));
+    if (stats4) MX_HIP(hipMemcpyAsync(stats4, T.stats, 32, hipMemcpyDeviceToHost, plan->stream));
+    MX_HIP(hipStreamSynchronize(plan->stream));
     return MX_OK;
 }
 
@@ -515,20 +530,20 @@ extern "C" int mx_plan_unresolved(mx_plan* plan, int32_t* idx, double* glitch, i
     *count = 0;
     if (!T.unres_count) return MX_OK;
     unsigned c = 0;
-    MX_HIP(hipMemcpyAsync(&c, T.unres_count, 4, hipMemcpyDeviceToHost, ctx->stream));
-    MX_HIP(hipStreamSynchronize(ctx->stream));
+    MX_HIP(hipMemcpyAsync(&c, T.unres_count, 4, hipMemcpyDeviceToHost, plan->stream));
+    MX_HIP(hipStreamSynchronize(plan->stream));
     *count = (int32_t)c;
     const size_t k = std::min((size_t)c, (size_t)cap);
-    if (k && idx) MX_HIP(hipMemcpyAsync(idx, T.unres_idx, 4 * k, hipMemcpyDeviceToHost, ctx->stream));
-    if (k && glitch) MX_HIP(hipMemcpyAsync(glitch, T.unres_gp, 8 * k, hipMemcpyDeviceToHost, ctx->stream));
-    MX_HIP(hipStreamSynchronize(ctx->stream));
+    if (k && idx) MX_HIP(hipMemcpyAsync(idx, T.unres_idx, 4 * k, hipMemcpyDeviceToHost, plan->stream));
+    if (k && glitch) MX_HIP(hipMemcpyAsync(glitch, T.unres_gp, 8 * k, hipMemcpyDeviceToHost, plan->stream));
+    MX_HIP(hipStreamSynchronize(plan->stream));
     return MX_OK;
 }
 
 extern "C" void mx_plan_destroy(mx_plan* plan) {
     if (!plan) return;
     (void)hipSetDevice(plan->ctx->device);
-    (void)hipStreamSynchronize(plan->ctx->stream);
+    if (plan->stream) (void)hipStreamSynchronize(plan->stream);
     if (plan->ev0) (void)hipEventDestroy(plan->ev0);
     if (plan->ev1) (void)hipEventDestroy(plan->ev1);
     if (plan->block) (void)hipFree(plan->block);
@@ -881,12 +896,11 @@ int upload_frame_ref(mx_ctx* ctx, const mx::Frame& F, mx_ref** ref) {
     return mx_ref_upload(ctx, &d, ref);
 }
 
-// One launch of the double kernel against `ref` over the grid (list == nullptr) or a
-// list of samples; results merged into nesc/val for the samples it resolved, the
-// unresolved ones returned with their glitch metric.
-int dbl_pass(mx_ctx* ctx, const std::vector<double>& dcol, const std::vector<double>& drow, const mx_ref* ref, mx_tile t,
-             const std::vector<int32_t>* list, int stride, std::vector<int32_t>& nesc, std::vector<double>& val,
-             std::vector<int32_t>& un_idx, std::vector<double>& un_gp, uint64_t st[4], double* kernel_ms) {
+// Creates and launches one pass of the double kernel against `ref` over the grid
+// (list == nullptr) or a list of samples.  side: on the context's side stream, stopped
+// by its abort flag instead of the cancellation flag.
+int dbl_pass_begin(mx_ctx* ctx, const std::vector<double>& dcol, const std::vector<double>& drow, const mx_ref* ref,
+                   mx_tile t, const std::vector<int32_t>* list, int stride, bool side, mx_plan** out) {
     t.stride = stride;
     t.dcol_d = dcol.data();
     t.drow_d = drow.data();
@@ -895,13 +909,38 @@ int dbl_pass(mx_ctx* ctx, const std::vector<double>& dcol, const std::vector<dou
     mx_plan* plan = nullptr;
     int rc = mx_plan_create(ctx, ref, &t, &plan);
     if (rc != MX_OK) return rc;
+    if (side) {
+        plan->stream = ctx->side;
+        plan->tile.cancel = ctx->abort_dev;
+    }
+    rc = mx_plan_launch(plan);
+    if (rc != MX_OK) {
+        mx_plan_destroy(plan);
+        return rc;
+    }
+    *out = plan;
+    return MX_OK;
+}
+
+// Stops a side-stream pass whose results are not wanted and releases it.
+void dbl_pass_discard(mx_ctx* ctx, mx_plan* plan) {
+    if (!plan) return;
+    __atomic_store_n(ctx->abort_host, 1, __ATOMIC_RELEASE);
+    (void)hipStreamSynchronize(plan->stream);
+    mx_plan_destroy(plan);
+    __atomic_store_n(ctx->abort_host, 0, __ATOMIC_RELEASE);
+}
+
+// Waits for a launched pass; results merged into nesc/val for the samples it resolved,
+// the unresolved ones returned with their glitch metric.  The plan stays alive.
+int dbl_pass_collect(mx_plan* plan, std::vector<int32_t>& nesc, std::vector<double>& val, std::vector<int32_t>& un_idx,
+                     std::vector<double>& un_gp, uint64_t st[4], double* kernel_ms) {
     double ms = 0.0;
-    const size_t n = (size_t)t.width * t.height;
+    const size_t n = plan->n;
     std::vector<int32_t> pn(n);
     std::vector<double> pv(n);
     uint64_t ps[4] = {0, 0, 0, 0};
-    rc = mx_plan_launch(plan);
-    if (rc == MX_OK) rc = mx_plan_wait(plan, &ms);
+    int rc = mx_plan_wait(plan, &ms);
     if (rc == MX_OK) rc = mx_plan_read(plan, pn.data(), pv.data(), nullptr, ps);
     int32_t cnt = 0;
     if (rc == MX_OK) rc = mx_plan_unresolved(plan, nullptr, nullptr, 0, &cnt);
@@ -910,7 +949,6 @@ int dbl_pass(mx_ctx* ctx, const std::vector<double>& dcol, const std::vector<dou
         un_gp.resize((size_t)cnt);
         rc = mx_plan_unresolved(plan, un_idx.data(), un_gp.data(), cnt, &cnt);
     }
-    mx_plan_destroy(plan);
     if (rc != MX_OK) return rc;
     for (size_t k = 0; k < n; ++k)
         if (pn[k] != INT_MIN) {
@@ -932,6 +970,18 @@ int dbl_pass(mx_ctx* ctx, const std::vector<double>& dcol, const std::vector<dou
     un_idx.swap(si);
     un_gp.swap(sg);
     return MX_OK;
+}
+
+// One pass, launched and collected on the context's stream.
+int dbl_pass(mx_ctx* ctx, const std::vector<double>& dcol, const std::vector<double>& drow, const mx_ref* ref, mx_tile t,
+             const std::vector<int32_t>* list, int stride, std::vector<int32_t>& nesc, std::vector<double>& val,
+             std::vector<int32_t>& un_idx, std::vector<double>& un_gp, uint64_t st[4], double* kernel_ms) {
+    mx_plan* plan = nullptr;
+    int rc = dbl_pass_begin(ctx, dcol, drow, ref, t, list, stride, false, &plan);
+    if (rc != MX_OK) return rc;
+    rc = dbl_pass_collect(plan, nesc, val, un_idx, un_gp, st, kernel_ms);
+    mx_plan_destroy(plan);
+    return rc;
 }
 
 // Mandel::Compute's plain-double path (mandel_perturbation.cpp:1668-1761): probe
@@ -974,29 +1024,52 @@ int compute_dbl(mx_ctx* ctx, mx::Frame& F, const mx_request* req, mx_frame_info*
     uint64_t st[4] = {0, 0, 0, 0};
     double probe_ms = 0.0, kernel_ms = 0.0;
     bool interior_used = F.interior;
+    // the reference pixel's value comes from its own orbit
+    size_t refpix = (size_t)F.ref_i * F.W + F.ref_j;
+    // the first pass as it runs when the probe finds no interior sample (interior
+    // detection off), started on the side stream beside the probe: the probe and the
+    // first pass are each bound by their longest chain of dependent steps, so side by
+    // side the frame waits for the longer one only.  Identical launch, identical
+    // results; stopped and discarded when the probe does find an interior sample.
+    mx_plan* spec = nullptr;
     if (F.interior) {
-        std::vector<int32_t> pn(n, INT_MIN);
-        std::vector<double> pv(n, -10.0);
-        uint64_t pst[4] = {0, 0, 0, 0};
-        rc = dbl_pass(ctx, F.dcol_d, F.drow_d, ref, t, nullptr, 8, pn, pv, un_idx, un_gp, pst, &probe_ms);
+        mx_tile ts = t;
+        ts.interior = 0;
+        ts.int_rep = 0;
+        ts.skip_sample = (int32_t)refpix;
+        rc = dbl_pass_begin(ctx, F.dcol_d, F.drow_d, ref, ts, nullptr, 1, true, &spec);
         if (rc != MX_OK) {
             mx_ref_release(ref);
             return rc;
         }
+        std::vector<int32_t> pn(n, INT_MIN);
+        std::vector<double> pv(n, -10.0);
+        uint64_t pst[4] = {0, 0, 0, 0};
+        mx_plan* probe = nullptr;
+        rc = dbl_pass_begin(ctx, F.dcol_d, F.drow_d, ref, t, nullptr, 8, false, &probe);
+        if (rc == MX_OK) rc = dbl_pass_collect(probe, pn, pv, un_idx, un_gp, pst, &probe_ms);
         int nint = 0;
-        for (int i = 0; i < F.H; i += 8)
+        for (int i = 0; rc == MX_OK && i < F.H; i += 8)
             for (int j = 0; j < F.W; j += 8) {
                 const int32_t v = pn[(size_t)i * F.W + j];
                 if (v == -1 || v == -2) ++nint;
             }
+        if (rc != MX_OK || nint > 0) {
+            // stop the side pass before anything waits on the whole device
+            dbl_pass_discard(ctx, spec);
+            spec = nullptr;
+        }
+        mx_plan_destroy(probe);
+        if (rc != MX_OK) {
+            mx_ref_release(ref);
+            return rc;
+        }
         if (nint == 0) {
             interior_used = false;
             t.interior = 0;
             t.int_rep = 0;
         }
     }
-    // the reference pixel's value comes from its own orbit
-    size_t refpix = (size_t)F.ref_i * F.W + F.ref_j;
     nesc[refpix] = F.ref_pixel_n;
     val[refpix] = F.ref_pixel_value;
     t.skip_sample = (int32_t)refpix;
@@ -1004,7 +1077,12 @@ int compute_dbl(mx_ctx* ctx, mx::Frame& F, const mx_request* req, mx_frame_info*
     std::vector<int32_t> list;
     bool first = true;
     while (true) {
-        rc = dbl_pass(ctx, F.dcol_d, F.drow_d, ref, t, first ? nullptr : &list, 1, nesc, val, un_idx, un_gp, st, &kernel_ms);
+        if (spec) {
+            rc = dbl_pass_collect(spec, nesc, val, un_idx, un_gp, st, &kernel_ms);
+            mx_plan_destroy(spec);
+            spec = nullptr;
+        } else
+            rc = dbl_pass(ctx, F.dcol_d, F.drow_d, ref, t, first ? nullptr : &list, 1, nesc, val, un_idx, un_gp, st, &kernel_ms);
         first = false;
         ++passes;
         if (rc != MX_OK) break;

@@ -145,8 +145,11 @@ struct mx_ctx {
     int lds_max = 0;                          // dynamic LDS bytes a workgroup may use
     unsigned char* noalloc_scratch = nullptr; // device address of this device's fixed scratch area
     hipStream_t stream = nullptr;
+    hipStream_t side = nullptr;     // a second stream, for a pass that runs beside another
     int* cancel_host = nullptr;     // pinned, host-mapped flag
     int* cancel_dev = nullptr;      // device view of the same flag
+    int* abort_host = nullptr;      // pinned, host-mapped: stops a pass on the side stream
+    int* abort_dev = nullptr;       // device view of the same flag
     std::atomic<bool> busy{false};
     std::string err;
     std::mutex err_mu;
@@ -181,6 +184,7 @@ struct mx_plan {
     int work_tiles = 0;         // 64-sample work items of one launch
     int grid_request = 0;       // caller's grid size (0: default)
     int nstage = 0;             // plain-double form: orbit entries staged in LDS
+    hipStream_t stream = nullptr; // the stream its launch, wait and reads use
 };
 
 struct mx_batch {

@@ -44,6 +44,31 @@ def test_double_frame_matches_oracle(ctx, name, c_method):
         assert info[k] == info_o[k], k
 
 
+# the period-106 minibrot of the ticktock view at scale 2e9: the 8x8-strided probe hits
+# its interior
+FRAMES["mini106"] = ("-1.4799461355795131264511335866615963781", "0.00090132482780272173726842509958825902",
+                     "2000000000", 64, 48, 20000, 0.0, 0)
+
+
+@pytest.mark.parametrize("name,interior_used", [("c1", 0), ("mini106", 1)])
+def test_first_pass_beside_the_probe(ctx, name, interior_used):
+    """The first pass starts on a second stream beside the interior-gate probe, with
+    interior detection off: kept when the probe finds no interior sample (c1), stopped
+    and discarded when it finds one (mini106), after which the pass runs again with
+    detection on.  Both frames equal the oracle."""
+    cre, cim, scale, W, H, mxit, R, pb = FRAMES[name]
+    print(f"[dbl beside probe] {name} {W}x{H} mxit={mxit}", flush=True)
+    it, val, nesc, info = ctx.compute(cre, cim, scale, W, H, mxit, escape_radius=R, prec_bits=pb)
+    val_o, nesc_o, out_o, info_o = mxo.compute_dbl(cre, cim, scale, W, H, mxit, escape_radius=R or 1e8, prec_bits=pb or None)
+    assert info["path"] == 1 and info["interior_used"] == interior_used and info["passes"] == info_o["passes"]
+    _check(val, nesc, val_o, nesc_o, name)
+    for k in ("steps", "bla_applies", "bla_skipped", "rebases"):
+        assert info[k] == info_o[k], k
+    # the context is reusable after a discarded side pass
+    it2, val2, nesc2, _ = ctx.compute(cre, cim, scale, W, H, mxit, escape_radius=R, prec_bits=pb)
+    assert np.array_equal(nesc2, nesc) and np.array_equal(val2, val)
+
+
 @pytest.mark.parametrize("name,golden,tol", [("point31", "point_7e31.f32", (12000.0, 200.0, 4000.0)),
                                              ("slowpoint", "slowpoint_7e8.f32", (90000.0, 300.0, 3000.0))])
 def test_double_frame_matches_reference_golden(ctx, name, golden, tol):
