@@ -16,9 +16,11 @@ Definition (the workload spec's batch, with the choices it leaves open fixed):
   smooth output, interior detection on.
 Writes tests/golden/c4_seed<k>.json (one location centre each) and, once all twelve
 exist, tests/golden/c4_requests.json.
-Usage: python tests/golden/make_c4.py [--plan] [--seeds 1,2,...] [--out DIR]
+Usage: python tests/golden/make_c4.py [--plan] [--seeds 1,2,...] [--partial] [--out DIR]
   --plan: print the draws only; --seeds: walk only these locations (the request file
-  is written once every location's centre exists); --out: as make_walk.py
+  is written once every location's centre exists); --partial: write the request file
+  from the locations that exist, with their requests only, the others listed under
+  "missing"; --out: as make_walk.py
 """
 import json
 import math
@@ -81,6 +83,13 @@ def main():
     if len(out["locations"]) == NLOC:
         json.dump(out, open(os.path.join(out_dir, "c4_requests.json"), "w"), indent=1)
         print("wrote c4_requests.json", flush=True)
+    elif "--partial" in args and out["locations"]:
+        # the locations whose walks are done, with their requests; the others listed
+        have = {l["seed"] for l in out["locations"]}
+        out["missing"] = sorted(l["seed"] for l in locs if l["seed"] not in have)
+        out["requests"] = [q for q in reqs if q["location"] in have]
+        json.dump(out, open(os.path.join(out_dir, "c4_requests.json"), "w"), indent=1)
+        print(f"wrote c4_requests.json with {len(have)} of {NLOC} locations", flush=True)
 
 
 if __name__ == "__main__":

@@ -181,7 +181,11 @@ def test_batched_tiles_match_oracle(ctx):
     import autotune
 
     spec = json.load(open(C4))
-    assert len(spec["locations"]) == 12 and len(spec["requests"]) == 96
+    # a request file written before every walk finished lists the absent locations
+    missing = spec.get("missing", [])
+    have = {l["seed"] for l in spec["locations"]}
+    assert len(have) == 12 - len(missing) and not have & set(missing)
+    assert len(spec["requests"]) == 8 * len(have) and all(q["location"] in have for q in spec["requests"])
     opts = mxhip.MX_OPT_FLOATEXP | mxhip.MX_OPT_NO_PERIODIC
     locs, refs = {}, []
     plans, subs = [], []
@@ -200,7 +204,7 @@ def test_batched_tiles_match_oracle(ctx):
             t.skip_sample = sub["skip"]
             plans.append(ctx.plan(ref, t))
             subs.append(sub)
-        order = sorted(range(96), key=lambda k: -spec["requests"][k]["side"] ** 2 * spec["requests"][k]["max_iter"])
+        order = sorted(range(len(plans)), key=lambda k: -spec["requests"][k]["side"] ** 2 * spec["requests"][k]["max_iter"])
         b = ctx.batch([plans[k] for k in order])
         print("[c4] batch launch", flush=True)
         b.launch()
