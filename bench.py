@@ -348,7 +348,8 @@ def double_leg(ctx, no_cpu, reps=5):
 
 
 def c4_leg(ctx, forms=("fe64", "fe32"), reps=1):
-    """96 uneven tiles over 12 boundary-walk locations (tests/golden/c4_requests.json):
+    """The uneven tiles of tests/golden/c4_requests.json (96 over 12 boundary-walk
+    locations; fewer when the file lists walks still missing):
     each location's reference, BLA table and dc tables are built once and resident;
     the tiles run as one persistent launch per form (largest work first), and the
     chosen form also tile by tile, one launch each, for the batching gain."""
